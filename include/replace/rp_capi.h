@@ -1,0 +1,248 @@
+/*
+ * rp_capi.h -- C ABI of the MI355X (gfx950) Nesterov global placer.
+ *
+ * This is the drop-in boundary underneath the replace::Replace C++ facade
+ * (include/replace/Replace.h).  Every entry point takes plain pointers and
+ * sizes, never C++ or torch types, so any host language can bind it
+ * (see INTEGRATION.md for the ctypes / C++ bindings).
+ *
+ * Reference interface being replaced (reference = The-OpenROAD-Project/RePlAce):
+ *   rp_create           <- PlacerBase::PlacerBase + PlacerBase::init   src/placerBase.cpp:592-758
+ *                          (OpenDB traversal replaced by the host SoA rp_netlist below)
+ *   rp_initial_place    <- Replace::doInitialPlace                     src/replace.cpp:73-88
+ *                          InitialPlace::doBicgstabPlace               src/initialPlace.cpp:56-105
+ *   rp_nesterov_place   <- Replace::doNesterovPlace                    src/replace.cpp:90-128
+ *                          NesterovBase::init, NesterovPlace::init     src/nesterovBase.cpp:916-1062,
+ *                                                                      src/nesterovPlace.cpp:64-175
+ *                          NesterovPlace::doNesterovPlace              src/nesterovPlace.cpp:275-472
+ *   rp_get_positions    <- NesterovPlace::updateDb / dbInst::setLocation src/nesterovPlace.cpp:585-595
+ *   rp_set_positions    <- incremental mode: positions already in the DB  src/initialPlace.cpp:67-69
+ *   rp_destroy          <- Replace::~Replace / reset                   src/replace.cpp:35-65
+ *
+ * Error convention.  The reference prints "[ERROR] msg (REPL-code)" and
+ * calls exit(code) (src/logger.cpp:31-36).  This ABI never exits: it returns
+ *   0            success
+ *   1..4         the reference's REPL error codes with the same meaning
+ *                (1 negative filler area / utilisation > 100.1 %,
+ *                 2 diverged during Nesterov init, 3 back-tracking exhausted,
+ *                 4 overflow rebound with HPWL growth)
+ *   < 0          HIP / resource / argument errors; rp_last_error() has text.
+ * The facade maps 1..4 back to the reference message and exit code.
+ */
+#ifndef REPLACE_RP_CAPI_H
+#define REPLACE_RP_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RP_OK 0
+#define RP_ERR_FILLER 1      /* REPL-1 */
+#define RP_ERR_INIT_DIVERGED 2 /* REPL-2 */
+#define RP_ERR_BACKTRACK 3   /* REPL-3 */
+#define RP_ERR_REBOUND 4     /* REPL-4 */
+#define RP_ERR_ARG (-1)
+#define RP_ERR_HIP (-2)
+#define RP_ERR_NOMEM (-3)
+#define RP_ERR_STATE (-4)
+#define RP_ERR_NO_DEVICE (-5)
+
+/*
+ * Host SoA netlist.  Borrowed for the duration of rp_create only; the
+ * library copies what it needs.  Units are database units (DBU).
+ *
+ * Mirrors what PlacerBase::init reads from OpenDB (src/placerBase.cpp:605-758):
+ *  - rows: the core area is the bounding box of all rows
+ *    (getCoreRectFromDb, src/placerBase.cpp:961-976) and rows drive the
+ *    fragmented-row dummy instances (src/placerBase.cpp:760-846).
+ *  - insts: every dbInst in block order.  inst_box is the placement bbox
+ *    (location + master size).  inst_fixed = LOCKED/FIRM/COVER status.
+ *  - nets / pins: signal nets only (POWER/GROUND/RESET skipped by the
+ *    caller, src/placerBase.cpp:686-693), pins net-major: a net's ITerms
+ *    first, then its BTerms (src/placerBase.cpp:702-713).
+ *  - pin_inst[p] >= 0: ITerm of that instance; pin_off[2p..2p+1] is the
+ *    offset of the pin-shape bbox centre from the master centre
+ *    (Pin::updateCoordi(dbITerm*), src/placerBase.cpp:330-376).
+ *  - pin_inst[p] == -1: top-level port (BTerm); pin_box[4p..4p+3] is the
+ *    bbox of its BPins, or (INT_MAX, INT_MAX, INT_MIN, INT_MIN) when the
+ *    port is unplaced (src/placerBase.cpp:381-406).  pin_box may be NULL
+ *    when the netlist has no BTerms.
+ *  - inst_pin_start/inst_pins (optional, NULL = net order): per-instance
+ *    ITerm order (dbInst::getITerms, src/placerBase.cpp:729-743).
+ */
+typedef struct rp_netlist {
+  int32_t dbu_per_micron;
+  int32_t die_lx, die_ly, die_ux, die_uy;
+  int32_t site_w, site_h;
+  int32_t num_rows;
+  const int32_t* row_box;       /* 4 * num_rows */
+  int32_t num_insts;
+  const int32_t* inst_box;      /* 4 * num_insts */
+  const uint8_t* inst_fixed;    /* num_insts */
+  int32_t num_nets;
+  const int32_t* net_pin_start; /* num_nets + 1 */
+  int32_t num_pins;
+  const int32_t* pin_inst;      /* num_pins */
+  const int32_t* pin_off;       /* 2 * num_pins */
+  const int32_t* pin_box;       /* 4 * num_pins or NULL */
+  const int32_t* inst_pin_start;/* num_insts + 1 or NULL */
+  const int32_t* inst_pins;     /* inst_pin_start[num_insts] or NULL */
+} rp_netlist;
+
+/*
+ * Every Replace setter (include/replace/Replace.h:39-61) plus the constants
+ * the reference keeps without a setter (src/nesterovPlace.cpp:19-29,
+ * src/nesterovBase.cpp:875-880).  rp_params_default() gives the values of
+ * the Replace constructor (src/replace.cpp:13-33).
+ */
+typedef struct rp_params {
+  int32_t initial_place_max_iter;         /* 20   */
+  int32_t initial_place_min_diff_length;  /* 1500 */
+  int32_t initial_place_max_solver_iter;  /* 100  */
+  int32_t initial_place_max_fanout;       /* 200  */
+  float initial_place_net_weight_scale;   /* 800  */
+  int32_t nesterov_place_max_iter;        /* 2000 */
+  int32_t bin_grid_cnt_x;                 /* 0 = automatic */
+  int32_t bin_grid_cnt_y;                 /* 0 = automatic */
+  float target_overflow;                  /* 0.1  */
+  float target_density;                   /* 1.0  */
+  float init_density_penalty;             /* 8e-5 */
+  float init_wire_length_coef;            /* 0.25 */
+  float min_phi_coef;                     /* 0.95 */
+  float max_phi_coef;                     /* 1.05 */
+  float reference_hpwl;                   /* 446000000 */
+  int32_t incremental_place_mode;         /* 0 */
+  int32_t verbose;                        /* 0 */
+  /* constants without a reference setter */
+  int32_t max_back_track;                 /* 10 */
+  float min_preconditioner;               /* 1.0 */
+  float initial_prev_coordi_update_coef;  /* 100 */
+  float min_wire_length_force_bar;        /* -300 */
+} rp_params;
+
+/* Per-iteration record of the Nesterov loop: the values the reference prints
+ * in "[NesterovSolve] Iter: i overflow: o HPWL: h" (src/nesterovPlace.cpp:421-423). */
+typedef struct rp_iter_record {
+  float overflow;
+  float step_length;
+  float density_penalty;
+  int32_t num_back_track;
+  int64_t hpwl;
+} rp_iter_record;
+
+typedef struct rp_nesterov_stats {
+  int32_t iterations;     /* outer iterations run (the i+1 of the last one) */
+  int32_t evals;          /* gradient evaluations including the two init evals */
+  int32_t finished;       /* 1 when the overflow target was reached */
+  int32_t code;           /* 0 or REPL code 2/3/4 */
+  float final_overflow;
+  float final_step_length;
+  float final_density_penalty;
+  int64_t final_hpwl;     /* pin-centre HPWL of the last accepted iterate */
+  double seconds;         /* device time of the loop, init included */
+} rp_nesterov_stats;
+
+typedef struct rp_ip_stats {
+  int32_t iterations;
+  float last_error;
+  int64_t last_hpwl;     /* inst-bbox HPWL printed at the last iteration */
+  double seconds;
+} rp_ip_stats;
+
+typedef struct rp_ctx rp_ctx;
+
+void rp_params_default(rp_params* params);
+
+/* PlacerBase::init: instance classification, areas, fragmented-row dummy
+ * instances, pins.  Returns 1 (REPL exit 1) when utilisation >= 100.1 %.
+ * Device selection: the current HIP device. */
+int rp_create(const rp_netlist* netlist, const rp_params* params, rp_ctx** out);
+
+/* Replace::doInitialPlace (B2B net model + BiCGSTAB on the device). */
+int rp_initial_place(rp_ctx* ctx, rp_ip_stats* stats);
+
+/* Replace::doNesterovPlace: NesterovBase + NesterovPlace init and the loop.
+ * Returns 0 or REPL code 1..4; positions are updated in every case, as the
+ * reference calls updateDb before exiting (src/nesterovPlace.cpp:465-471). */
+int rp_nesterov_place(rp_ctx* ctx, rp_nesterov_stats* stats);
+
+/* Per-iteration records of the last rp_nesterov_place call. */
+int rp_get_iter_records(rp_ctx* ctx, rp_iter_record* out, int32_t capacity, int32_t* count);
+
+/* Lower-left corners of all num_insts instances (as the DB would hold them). */
+int rp_get_positions(rp_ctx* ctx, int32_t* lx, int32_t* ly);
+/* Move instances (incremental mode / resume); lx, ly for all num_insts. */
+int rp_set_positions(rp_ctx* ctx, const int32_t* lx, const int32_t* ly);
+
+void rp_destroy(rp_ctx* ctx);
+const char* rp_last_error(rp_ctx* ctx);
+
+/* ------------------------------------------------------------------------
+ * Stage-level entry points.  They expose single pieces of one gradient
+ * evaluation (src/nesterovPlace.cpp:338-377) so parity can be checked one
+ * stage at a time against the same state.  Valid after rp_nesterov_setup.
+ * ---------------------------------------------------------------------- */
+
+/* Geometry of the Nesterov state built by NesterovBase::init. */
+typedef struct rp_nb_info {
+  int32_t num_gcells, num_fillers, num_place_insts;
+  int32_t num_gnets, num_gpins;
+  int32_t bin_cnt_x, bin_cnt_y, bin_size_x, bin_size_y;
+  int32_t core_lx, core_ly, core_ux, core_uy;
+  int32_t filler_dx, filler_dy;
+  int64_t std_insts_area, macro_insts_area, place_insts_area, nonplace_insts_area;
+  float target_density;
+} rp_nb_info;
+
+/* NesterovBase::init only (gcells, fillers, bin grid, density sizes). */
+int rp_nesterov_setup(rp_ctx* ctx, rp_nb_info* info);
+
+/* Per-gcell constants: density box half sizes, density scale, cell size,
+ * initial density centre (after updateDensityCoordiLayoutInside). */
+int rp_get_gcells(rp_ctx* ctx, int32_t* dhalf_x, int32_t* dhalf_y, float* scale,
+                  int32_t* dx, int32_t* dy, float* init_cx, float* init_cy, uint8_t* kind);
+
+/* Bin nonPlace areas (int64, B). */
+int rp_get_nonplace(rp_ctx* ctx, int64_t* nonplace);
+
+/* One gradient evaluation at density centres (cx, cy) (float, per gcell),
+ * density penalty lambda and wire-length coefficients.  Any output pointer
+ * may be NULL.  Outputs are host buffers:
+ *   bin_inst, bin_filler : int64[B]   scattered areas (bins y-major, j*Nx+i)
+ *   density, phi, ex, ey : float[B]
+ *   overflow_area        : int64[1]
+ *   net_sums             : float[8*Nn] (minX, xminX, maxX, xmaxX, minY, yminY, maxY, ymaxY)
+ *   wl_grad, dens_grad, sum_grad : float[2*Nc] (x, y interleaved)
+ *   sums                 : double[3] (sum|wl|, sum|dens|, gradSum)
+ *   hpwl                 : int64[1]
+ */
+typedef struct rp_eval_out {
+  int64_t* bin_inst;
+  int64_t* bin_filler;
+  float* density;
+  float* phi;
+  float* ex;
+  float* ey;
+  int64_t* overflow_area;
+  float* net_sums;
+  int32_t* net_box;
+  float* wl_grad;
+  float* dens_grad;
+  float* sum_grad;
+  double* sums;
+  int64_t* hpwl;
+} rp_eval_out;
+
+int rp_eval_gradient(rp_ctx* ctx, const float* cx, const float* cy, float density_penalty,
+                     float wl_coef_x, float wl_coef_y, rp_eval_out* out);
+
+/* Poisson solve alone: density[B] (y-major) -> phi, ex, ey (y-major). */
+int rp_eval_poisson(rp_ctx* ctx, const float* density, float* phi, float* ex, float* ey);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* REPLACE_RP_CAPI_H */

@@ -1,0 +1,179 @@
+"""ctypes binding of the product C ABI (include/replace/rp_capi.h).
+
+``libreplace_amd.so`` (built by replace_amd/build.py with hipcc for gfx950)
+is the only compute path: if it is missing this module raises, there is no
+CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+from .netlist import (Netlist, RpEvalOut, RpIpStats, RpIterRecord, RpNbInfo, RpNesterovStats,
+                      RpParams, params_default)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libreplace_amd.so")
+
+_lib = None
+
+
+class ReplaceError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"rp error {code}: {msg}")
+        self.code = code
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is not built; run python -c 'import __graft_entry__ as g; g.build()'")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.rp_params_default.argtypes = [ctypes.POINTER(RpParams)]
+        L.rp_create.argtypes = [vp, ctypes.POINTER(RpParams), ctypes.POINTER(vp)]
+        L.rp_initial_place.argtypes = [vp, ctypes.POINTER(RpIpStats)]
+        L.rp_nesterov_place.argtypes = [vp, ctypes.POINTER(RpNesterovStats)]
+        L.rp_get_iter_records.argtypes = [vp, ctypes.POINTER(RpIterRecord), ctypes.c_int32,
+                                          ctypes.POINTER(ctypes.c_int32)]
+        L.rp_get_positions.argtypes = [vp, vp, vp]
+        L.rp_set_positions.argtypes = [vp, vp, vp]
+        L.rp_destroy.argtypes = [vp]
+        L.rp_last_error.restype = ctypes.c_char_p
+        L.rp_last_error.argtypes = [vp]
+        L.rp_nesterov_setup.argtypes = [vp, ctypes.POINTER(RpNbInfo)]
+        L.rp_get_gcells.argtypes = [vp] + [vp] * 8
+        L.rp_get_nonplace.argtypes = [vp, vp]
+        L.rp_eval_gradient.argtypes = [vp, vp, vp, ctypes.c_float, ctypes.c_float,
+                                       ctypes.c_float, ctypes.POINTER(RpEvalOut)]
+        L.rp_eval_poisson.argtypes = [vp, vp, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def alloc_eval_out(info: RpNbInfo):
+    """Host buffers for every ``rp_eval_out`` field, plus the C view."""
+    nb = info.bin_cnt_x * info.bin_cnt_y
+    ng, nn = info.num_gcells, info.num_gnets
+    bufs = dict(
+        bin_inst=np.zeros(nb, np.int64), bin_filler=np.zeros(nb, np.int64),
+        density=np.zeros(nb, np.float32), phi=np.zeros(nb, np.float32),
+        ex=np.zeros(nb, np.float32), ey=np.zeros(nb, np.float32),
+        overflow_area=np.zeros(1, np.int64), net_sums=np.zeros(8 * nn, np.float32),
+        net_box=np.zeros(4 * nn, np.int32), wl_grad=np.zeros(2 * ng, np.float32),
+        dens_grad=np.zeros(2 * ng, np.float32), sum_grad=np.zeros(2 * ng, np.float32),
+        sums=np.zeros(3, np.float64), hpwl=np.zeros(1, np.int64),
+    )
+    out = RpEvalOut()
+    for name, ctype in RpEvalOut._fields_:
+        setattr(out, name, bufs[name].ctypes.data_as(ctype))
+    return out, bufs
+
+
+class Placer:
+    """One placement context on the GPU (owns an ``rp_ctx``)."""
+
+    def __init__(self, netlist: Netlist, params: Optional[RpParams] = None):
+        self._nl = netlist
+        self._cnl = netlist.to_c()
+        self.params = params if params is not None else params_default()
+        h = ctypes.c_void_p()
+        code = lib().rp_create(ctypes.byref(self._cnl), ctypes.byref(self.params),
+                               ctypes.byref(h))
+        self.h = h
+        self.create_code = code
+        if code < 0:
+            raise ReplaceError(code, self.last_error())
+        self.info: Optional[RpNbInfo] = None
+
+    def last_error(self) -> str:
+        m = lib().rp_last_error(self.h)
+        return m.decode() if m else ""
+
+    def _check(self, r: int) -> int:
+        if r < 0:
+            raise ReplaceError(r, self.last_error())
+        return r
+
+    def close(self) -> None:
+        if self.h:
+            lib().rp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def initial_place(self) -> RpIpStats:
+        st = RpIpStats()
+        self._check(lib().rp_initial_place(self.h, ctypes.byref(st)))
+        return st
+
+    def setup(self) -> RpNbInfo:
+        info = RpNbInfo()
+        r = self._check(lib().rp_nesterov_setup(self.h, ctypes.byref(info)))
+        if r:
+            raise ReplaceError(r, "nesterov setup")
+        self.info = info
+        return info
+
+    def gcells(self):
+        n = self.info.num_gcells
+        d = dict(dhalf_x=np.zeros(n, np.int32), dhalf_y=np.zeros(n, np.int32),
+                 scale=np.zeros(n, np.float32), dx=np.zeros(n, np.int32), dy=np.zeros(n, np.int32),
+                 init_cx=np.zeros(n, np.float32), init_cy=np.zeros(n, np.float32),
+                 kind=np.zeros(n, np.uint8))
+        self._check(lib().rp_get_gcells(self.h, *(_p(d[k]) for k in (
+            "dhalf_x", "dhalf_y", "scale", "dx", "dy", "init_cx", "init_cy", "kind"))))
+        return d
+
+    def nonplace(self):
+        out = np.zeros(self.info.bin_cnt_x * self.info.bin_cnt_y, np.int64)
+        self._check(lib().rp_get_nonplace(self.h, _p(out)))
+        return out
+
+    def eval_gradient(self, cx, cy, lam, wlx, wly):
+        out, bufs = alloc_eval_out(self.info)
+        cx = np.ascontiguousarray(cx, np.float32)
+        cy = np.ascontiguousarray(cy, np.float32)
+        self._check(lib().rp_eval_gradient(self.h, _p(cx), _p(cy), lam, wlx, wly,
+                                           ctypes.byref(out)))
+        return bufs
+
+    def eval_poisson(self, density):
+        nb = self.info.bin_cnt_x * self.info.bin_cnt_y
+        d = np.ascontiguousarray(density, np.float32).reshape(-1)
+        outs = [np.zeros(nb, np.float32) for _ in range(3)]
+        self._check(lib().rp_eval_poisson(self.h, _p(d), *(_p(o) for o in outs)))
+        return outs
+
+    def nesterov_place(self):
+        st = RpNesterovStats()
+        code = self._check(lib().rp_nesterov_place(self.h, ctypes.byref(st)))
+        return code, st
+
+    def records(self):
+        cnt = ctypes.c_int32(0)
+        self._check(lib().rp_get_iter_records(self.h, None, 0, ctypes.byref(cnt)))
+        arr = (RpIterRecord * max(cnt.value, 1))()
+        self._check(lib().rp_get_iter_records(self.h, arr, cnt.value, ctypes.byref(cnt)))
+        return [(r.overflow, r.hpwl, r.step_length, r.density_penalty, r.num_back_track)
+                for r in arr[: cnt.value]]
+
+    def positions(self):
+        n = self._nl.num_insts
+        lx = np.zeros(n, np.int32)
+        ly = np.zeros(n, np.int32)
+        self._check(lib().rp_get_positions(self.h, _p(lx), _p(ly)))
+        return lx, ly

@@ -1,0 +1,127 @@
+"""Writes the golden fixtures under tests/golden/ (run in the development
+container, where the reference tree exists; the fixtures are data only).
+
+  <design>.npz        netlists of the reference's regression designs
+                      (test/<design>.def + test/nangate45.lef), read by
+                      replace_amd.lefdef
+  traces.json         the reference's golden log lines (test/<design>.ok):
+                      flags of test/<design>.tcl, the [InitialPlace] and
+                      [NesterovSolve] lines, finish overflow, error code and
+                      the [INFO] values the tests check
+  mt19937.json        first draws of std::mt19937(0) from libstdc++ (the
+                      engine initFillerGCells uses, src/nesterovBase.cpp:1139)
+  poisson_ref.npz     inputs and outputs of the reference's own FFT Poisson
+                      solve (src/fft.cpp, Ooura fftsg), built into
+                      oracle/_ref by oracle/Makefile
+
+Usage: python tests/golden/make_fixtures.py [/root/reference]
+"""
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from replace_amd.lefdef import read_def, read_lef  # noqa: E402
+
+# global_placement flags of test/<design>.tcl mapped onto rp_params fields
+# (src/replace.tcl:8-97: density default 0.7, -skip_initial_place -> 0 IP
+# iterations, -incremental -> incremental mode and 0 IP iterations).
+DESIGNS = {
+    "simple01": dict(target_density=0.7, init_density_penalty=0.01, initial_place_max_iter=0),
+    "simple02": dict(target_density=0.6, init_density_penalty=0.01, initial_place_max_iter=0),
+    "simple03": dict(target_density=0.7),
+    "simple04": dict(target_density=0.7),
+    "ar01": dict(target_density=0.6, init_density_penalty=0.01, initial_place_max_iter=0),
+    "ar02": dict(target_density=0.6, init_density_penalty=0.01, initial_place_max_iter=0),
+    "core01": dict(target_density=0.6, init_density_penalty=0.01, initial_place_max_iter=0),
+    "incremental01": dict(target_density=0.7, init_density_penalty=0.1, initial_place_max_iter=0,
+                          incremental_place_mode=1),
+    "diverge01": dict(target_density=0.7, init_density_penalty=100.0, initial_place_max_iter=0),
+}
+
+
+def parse_ok(path):
+    ip, nes, info = [], [], {}
+    finished = None
+    code = 0
+    for line in open(path):
+        line = line.rstrip("\n")
+        m = re.match(r"\[InitialPlace\]  Iter: (\d+) CG Error: (\S+) HPWL: (\d+)", line)
+        if m:
+            ip.append([int(m.group(1)), m.group(2), int(m.group(3))])
+            continue
+        m = re.match(r"\[NesterovSolve\] Iter: (\d+) overflow: (\S+) HPWL: (\d+)", line)
+        if m:
+            nes.append([int(m.group(1)), m.group(2), int(m.group(3))])
+            continue
+        m = re.match(r"\[NesterovSolve\] Finished with Overflow: (\S+)", line)
+        if m:
+            finished = m.group(1)
+            continue
+        m = re.match(r"\[INFO\] (\S+(?: \S+)?) = (.*)", line)
+        if m:
+            info[m.group(1)] = m.group(2)
+        m = re.search(r"\(REPL-(\d+)\)", line)
+        if m:
+            code = int(m.group(1))
+    return dict(ip=ip, nesterov=nes, finished=finished, code=code, info=info)
+
+
+MT_SRC = r"""
+#include <random>
+#include <cstdio>
+int main() { std::mt19937 r(0); for (int i = 0; i < 2000; i++) printf("%lu\n", (unsigned long)r()); }
+"""
+
+
+def main():
+    ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    test = os.path.join(ref, "test")
+    lef = read_lef(os.path.join(test, "nangate45.lef"))
+    traces = {}
+    for name, flags in DESIGNS.items():
+        nl = read_def(os.path.join(test, name + ".def"), lef, name=name)
+        nl.save(os.path.join(HERE, name + ".npz"))
+        t = parse_ok(os.path.join(test, name + ".ok"))
+        t["flags"] = flags
+        traces[name] = t
+    with open(os.path.join(HERE, "traces.json"), "w") as f:
+        json.dump(traces, f, indent=1)
+
+    with tempfile.TemporaryDirectory() as td:
+        src = os.path.join(td, "mt.cpp")
+        exe = os.path.join(td, "mt")
+        open(src, "w").write(MT_SRC)
+        subprocess.check_call(["g++", "-O1", "-o", exe, src])
+        vals = [int(v) for v in subprocess.check_output([exe]).split()]
+    with open(os.path.join(HERE, "mt19937.json"), "w") as f:
+        json.dump({"seed": 0, "draws": vals}, f)
+
+    import oracle
+    oracle.build()
+    if oracle.ref_available():
+        rng = np.random.default_rng(1234)
+        cases = {}
+        for k, (nx, ny, bsx, bsy) in enumerate([(16, 16, 3872, 3850), (32, 64, 3515, 5075),
+                                                 (128, 128, 2890, 2890), (64, 32, 5368, 3325)]):
+            rho = rng.uniform(0.0, 2.0, size=(ny, nx)).astype(np.float32)
+            phi, ex, ey = oracle.poisson_reference(rho, bsx, bsy)
+            cases[f"c{k}_dims"] = np.array([nx, ny, bsx, bsy], np.int32)
+            cases[f"c{k}_rho"] = rho
+            cases[f"c{k}_phi"] = phi
+            cases[f"c{k}_ex"] = ex
+            cases[f"c{k}_ey"] = ey
+        np.savez_compressed(os.path.join(HERE, "poisson_ref.npz"), **cases)
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()
