@@ -168,6 +168,10 @@ int rp_initial_place(rp_ctx* ctx, rp_ip_stats* stats);
  * reference calls updateDb before exiting (src/nesterovPlace.cpp:465-471). */
 int rp_nesterov_place(rp_ctx* ctx, rp_nesterov_stats* stats);
 
+/* [InitialPlace] lines of the last rp_initial_place call: CG error and the
+ * inst-bbox HPWL printed per iteration (src/initialPlace.cpp:88-90). */
+int rp_get_ip_records(rp_ctx* ctx, float* err, int64_t* hpwl, int32_t capacity, int32_t* count);
+
 /* Per-iteration records of the last rp_nesterov_place call. */
 int rp_get_iter_records(rp_ctx* ctx, rp_iter_record* out, int32_t capacity, int32_t* count);
 

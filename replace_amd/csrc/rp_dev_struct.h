@@ -1,0 +1,60 @@
+// rp_dev_struct.h -- kernel argument block shared by rp_kernels.hip and rp_host.cpp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rp_device.h"
+
+// Device pointers and sizes of one placement problem (layout in rp_device.h).
+struct RpDev {
+  int nc, nn, np, nbx, nby, bsx, bsy;
+  int core_lx, core_ly, core_ux, core_uy;
+  float td;
+  float bar;          // minWireLengthForceBar
+  float min_precond;  // minPreconditioner
+  const int2* dhalf;
+  const float* scale;
+  const uint8_t* kind;
+  const float* area;
+  const float* npin;
+  float2* slp;
+  float2* sumg;
+  float2* co;
+  int2* ic;
+  const int* cp_start;
+  const int* cp_net;
+  const int2* cp_off;
+  const int* nstart;
+  const int* pcell;
+  const int2* ppos;
+  int4* nbox;
+  float4* nsum;
+  unsigned long long* bin_inst;
+  unsigned long long* bin_filler;
+  const long long* bin_nonplace;
+  float* rho;
+  float2* field;
+  float* phi;
+  float* t;
+  float* tt;
+  float* y1t;
+  float* y2t;
+  float* y1;
+  float* y2;
+  const float2* twx;   // e^{-2 pi i k / Nx}, k < Nx/2
+  const float2* tw2x;  // e^{-i pi k / (2 Nx)}, k < Nx
+  const float2* twy;
+  const float2* tw2y;
+  const float* wx;
+  const float* wx2;
+  const float* wy;
+  const float* wy2;
+  double* part_grad;
+  double* part_bin;
+  long long* part_net;
+  RpState* st;
+  RpIterRec* rec;
+  float2* dbg_wl;
+  float2* dbg_dens;
+};

@@ -1,0 +1,1430 @@
+// rp_host.cpp -- host side of the C ABI (include/replace/rp_capi.h).
+//
+// One-time work runs on the host (it is O(N) and cheap next to the loop):
+//   PlacerBase::init derivations         (reference src/placerBase.cpp:605-846)
+//   B2B matrix assembly of InitialPlace  (src/initialPlace.cpp:128-378)
+//   NesterovBase::init: fillers, bins,
+//     density sizes                      (src/nesterovBase.cpp:631-741, 916-1155)
+// Everything per iteration runs on the device: BiCGSTAB (SpMV + FP64 dots),
+// and the whole Nesterov loop (gradient evaluations, back-tracking, schedules
+// and the stop test) driven by the controller kernel in rp_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/replace/rp_capi.h"
+#include "rp_device.h"
+
+#include "rp_dev_struct.h"
+
+hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi);
+hipError_t rp_launch_ctrl_init(const RpDev& d, hipStream_t s, int stage);
+hipError_t rp_launch_scatter(const RpDev& d, hipStream_t s);
+hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int from_rho);
+hipError_t rp_launch_wa_grad(const RpDev& d, hipStream_t s);
+hipError_t rp_launch_spmv(int n, const int* rowp, const int* col, const float* val, const float* x, float* y,
+                          hipStream_t s);
+hipError_t rp_launch_dot3(int n, const float* a0, const float* b0, const float* a1, const float* b1,
+                          const float* a2, const float* b2, double* parts, int nparts, double* out3,
+                          hipStream_t s);
+hipError_t rp_launch_bicg_p(int n, float* p, const float* r, const float* v, double beta, double w,
+                            hipStream_t s);
+hipError_t rp_launch_bicg_xr(int n, float* x, float* r, const float* p, const float* sv, const float* t,
+                             double alpha, double w, hipStream_t s);
+hipError_t rp_launch_axpy(int n, float* y, const float* a, double alpha, const float* b, hipStream_t s);
+
+namespace {
+
+constexpr long double kSqrt2 = 1.414213562373095048801L;  // src/nesterovBase.cpp:13
+constexpr long double kPi = 3.141592653589793238462L;     // src/fft.cpp:9
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+}  // namespace
+
+struct rp_ctx {
+  rp_params prm;
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // ---- PlacerBase ----
+  int core_lx = 0, core_ly = 0, core_ux = 0, core_uy = 0;
+  int site_w = 0, site_h = 0;
+  int n_real = 0;
+  std::vector<int> ilx, ily, iux, iuy, mw, mh, dblx, dbly;  // incl. dummies
+  std::vector<uint8_t> fixed;
+  std::vector<int> place, inst2place, nonplace;
+  int64_t place_area = 0, nonplace_area = 0, macro_area = 0, std_area = 0;
+  int n_pin = 0, n_net = 0;
+  std::vector<int> net_start, pin_inst, pin_offx, pin_offy, pin_cx, pin_cy;
+  std::vector<int> pin_box;  // 4 per pin (BTerms)
+  std::vector<int> ipin_start, ipins;
+  // ---- initial place stats ----
+  std::vector<float> ip_err;
+  std::vector<int64_t> ip_hpwl;
+  // ---- NesterovBase ----
+  bool nb_ready = false;
+  int n_g = 0, n_gi = 0, n_gf = 0;
+  int filler_dx = 0, filler_dy = 0;
+  int nbx = 0, nby = 0, bsx = 0, bsy = 0;
+  float td = 0;
+  std::vector<int> g_inst;
+  std::vector<uint8_t> g_kind;
+  std::vector<int> g_dx, g_dy, g_dhx, g_dhy;
+  std::vector<float> g_scale, g_icx, g_icy;
+  std::vector<int64_t> nonplace_bins;
+  // ---- device ----
+  DevBuf<int2> d_dhalf;
+  DevBuf<float> d_scale, d_area, d_npin;
+  DevBuf<uint8_t> d_kind;
+  DevBuf<float2> d_slp, d_sumg, d_co;
+  DevBuf<int2> d_ic;
+  DevBuf<int> d_cp_start, d_cp_net, d_nstart, d_pcell;
+  DevBuf<int2> d_cp_off, d_ppos;
+  DevBuf<int4> d_nbox;
+  DevBuf<float4> d_nsum;
+  DevBuf<unsigned long long> d_bin_inst, d_bin_filler;
+  DevBuf<long long> d_bin_nonplace;
+  DevBuf<float> d_rho, d_phi, d_t, d_tt, d_y1t, d_y2t, d_y1, d_y2;
+  DevBuf<float2> d_field;
+  DevBuf<float2> d_twx, d_tw2x, d_twy, d_tw2y;
+  DevBuf<float> d_wx, d_wx2, d_wy, d_wy2;
+  DevBuf<double> d_part_grad, d_part_bin;
+  DevBuf<long long> d_part_net;
+  DevBuf<RpState> d_state;
+  DevBuf<RpIterRec> d_rec;
+  DevBuf<float2> d_dbg_wl, d_dbg_dens;
+  RpState* h_state = nullptr;  // pinned
+  std::vector<int> net_perm;   // device net id -> reference net id
+  RpDev dev;
+  // ---- results ----
+  std::vector<RpIterRec> records;
+  rp_nesterov_stats last_stats;
+  bool placed = false;
+};
+
+static int fail(rp_ctx* c, int code, const char* fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return code;
+}
+
+#define HIPCHK(c, x)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) return fail((c), RP_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+static hipError_t dalloc(DevBuf<T>& b, size_t n) {
+  b.release();
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)&b.p, n * sizeof(T));
+  if (e == hipSuccess) b.n = n;
+  return e;
+}
+
+template <typename T>
+static hipError_t dupload(DevBuf<T>& b, const std::vector<T>& v, hipStream_t s) {
+  hipError_t e = dalloc(b, v.size());
+  if (e != hipSuccess || v.empty()) return e;
+  return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+static inline int idiv_upper(int r, int s) {
+  // ceil-like bin upper index of getMinMaxIdx / getDensityMinMaxIdx
+  int fm = r >= s ? r % s : r;
+  return (fm == 0) ? r / s : r / s + 1;
+}
+
+// ===========================================================================
+// PlacerBase
+// ===========================================================================
+extern "C" void rp_params_default(rp_params* p) {
+  // Replace::Replace, src/replace.cpp:13-33 and NesterovPlaceVars / NesterovBaseVars constants
+  p->initial_place_max_iter = 20;
+  p->initial_place_min_diff_length = 1500;
+  p->initial_place_max_solver_iter = 100;
+  p->initial_place_max_fanout = 200;
+  p->initial_place_net_weight_scale = 800.0f;
+  p->nesterov_place_max_iter = 2000;
+  p->bin_grid_cnt_x = 0;
+  p->bin_grid_cnt_y = 0;
+  p->target_overflow = 0.1f;
+  p->target_density = 1.0f;
+  p->init_density_penalty = 0.00008f;
+  p->init_wire_length_coef = 0.25f;
+  p->min_phi_coef = 0.95f;
+  p->max_phi_coef = 1.05f;
+  p->reference_hpwl = 446000000.0f;
+  p->incremental_place_mode = 0;
+  p->verbose = 0;
+  p->max_back_track = 10;
+  p->min_preconditioner = 1.0f;
+  p->initial_prev_coordi_update_coef = 100.0f;
+  p->min_wire_length_force_bar = -300.0f;
+}
+
+static int inst_cx(const rp_ctx* c, int i) { return (c->ilx[i] + c->iux[i]) / 2; }
+static int inst_cy(const rp_ctx* c, int i) { return (c->ily[i] + c->iuy[i]) / 2; }
+
+static void inst_update_pins(rp_ctx* c, int i) {
+  // Pin::updateLocation, src/placerBase.cpp:408-412
+  for (int k = c->ipin_start[i]; k < c->ipin_start[i + 1]; k++) {
+    int q = c->ipins[k];
+    c->pin_cx[q] = inst_cx(c, i) + c->pin_offx[q];
+    c->pin_cy[q] = inst_cy(c, i) + c->pin_offy[q];
+  }
+}
+
+static void inst_set_center(rp_ctx* c, int i, int x, int y) {
+  // Instance::setCenterLocation, src/placerBase.cpp:127-140 (odd sizes shrink by one)
+  int hx = (c->iux[i] - c->ilx[i]) / 2, hy = (c->iuy[i] - c->ily[i]) / 2;
+  c->ilx[i] = x - hx;
+  c->ily[i] = y - hy;
+  c->iux[i] = x + hx;
+  c->iuy[i] = y + hy;
+  inst_update_pins(c, i);
+}
+
+static std::pair<int, int> site_range(int ll, int uu, int core, int size, int lo, int hi) {
+  // getMinMaxIdx, src/placerBase.cpp:984-993
+  int a = (ll - core) / size;
+  int b = idiv_upper(uu - core, size);
+  return {std::max(lo, a), std::min(hi, b)};
+}
+
+extern "C" int rp_create(const rp_netlist* nl, const rp_params* prm, rp_ctx** out) {
+  if (!out) return RP_ERR_ARG;
+  *out = nullptr;
+  if (!nl || !prm || nl->num_rows <= 0 || nl->site_w <= 0 || nl->site_h <= 0) return RP_ERR_ARG;
+  std::unique_ptr<rp_ctx> c(new rp_ctx());
+  c->prm = *prm;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    *out = c.release();
+    return fail(*out, RP_ERR_NO_DEVICE, "no HIP device visible");
+  }
+  (void)hipGetDevice(&c->device);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    *out = c.release();
+    return fail(*out, RP_ERR_HIP, "hipStreamCreate failed");
+  }
+  if (hipHostMalloc((void**)&c->h_state, sizeof(RpState), hipHostMallocDefault) != hipSuccess) {
+    *out = c.release();
+    return fail(*out, RP_ERR_HIP, "hipHostMalloc failed");
+  }
+  // core = bbox of rows (src/placerBase.cpp:961-976)
+  int minX = INT_MAX, minY = INT_MAX, maxX = INT_MIN, maxY = INT_MIN;
+  for (int r = 0; r < nl->num_rows; r++) {
+    minX = std::min(nl->row_box[4 * r + 0], minX);
+    minY = std::min(nl->row_box[4 * r + 1], minY);
+    maxX = std::max(nl->row_box[4 * r + 2], maxX);
+    maxY = std::max(nl->row_box[4 * r + 3], maxY);
+  }
+  c->core_lx = minX;
+  c->core_ly = minY;
+  c->core_ux = maxX;
+  c->core_uy = maxY;
+  c->site_w = nl->site_w;
+  c->site_h = nl->site_h;
+  const int nr = nl->num_insts;
+  // fragmented-row dummy instances (src/placerBase.cpp:760-846)
+  int scx = (c->core_ux - c->core_lx) / c->site_w;
+  int scy = (c->core_uy - c->core_ly) / c->site_h;
+  std::vector<uint8_t> grid((size_t)std::max(scx, 0) * std::max(scy, 0), 0);
+  for (int r = 0; r < nl->num_rows; r++) {
+    auto px = site_range(nl->row_box[4 * r], nl->row_box[4 * r + 2], c->core_lx, c->site_w, 0, scx);
+    auto py = site_range(nl->row_box[4 * r + 1], nl->row_box[4 * r + 3], c->core_ly, c->site_h, 0, scy);
+    for (int i = px.first; i < px.second; i++)
+      for (int j = py.first; j < py.second; j++) grid[(size_t)j * scx + i] = 1;
+  }
+  for (int i = 0; i < nr; i++) {
+    if (!nl->inst_fixed[i]) continue;
+    auto px = site_range(nl->inst_box[4 * i], nl->inst_box[4 * i + 2], c->core_lx, c->site_w, 0, scx);
+    auto py = site_range(nl->inst_box[4 * i + 1], nl->inst_box[4 * i + 3], c->core_ly, c->site_h, 0, scy);
+    for (int a = px.first; a < px.second; a++)
+      for (int b = py.first; b < py.second; b++) grid[(size_t)b * scx + a] = 2;
+  }
+  std::vector<std::array<int, 4>> dummies;
+  for (int j = 0; j < scy; j++) {
+    for (int i = 0; i < scx; i++) {
+      if (grid[(size_t)j * scx + i] != 0) continue;
+      int sx = i;
+      while (i < scx && grid[(size_t)j * scx + i] == 0) i++;
+      dummies.push_back({c->core_lx + c->site_w * sx, c->core_ly + c->site_h * j, c->core_lx + c->site_w * i,
+                         c->core_ly + c->site_h * (j + 1)});
+    }
+  }
+  grid.clear();
+  grid.shrink_to_fit();
+  c->n_real = nr;
+  int ni = nr + (int)dummies.size();
+  c->ilx.resize(ni);
+  c->ily.resize(ni);
+  c->iux.resize(ni);
+  c->iuy.resize(ni);
+  c->mw.assign(ni, 0);
+  c->mh.assign(ni, 0);
+  c->dblx.assign(ni, 0);
+  c->dbly.assign(ni, 0);
+  c->fixed.assign(ni, 1);
+  c->inst2place.assign(ni, -1);
+  for (int i = 0; i < nr; i++) {
+    c->ilx[i] = nl->inst_box[4 * i];
+    c->ily[i] = nl->inst_box[4 * i + 1];
+    c->iux[i] = nl->inst_box[4 * i + 2];
+    c->iuy[i] = nl->inst_box[4 * i + 3];
+    c->mw[i] = c->iux[i] - c->ilx[i];
+    c->mh[i] = c->iuy[i] - c->ily[i];
+    c->dblx[i] = c->ilx[i];
+    c->dbly[i] = c->ily[i];
+    c->fixed[i] = nl->inst_fixed[i] ? 1 : 0;
+  }
+  for (size_t k = 0; k < dummies.size(); k++) {
+    int i = nr + (int)k;
+    c->ilx[i] = dummies[k][0];
+    c->ily[i] = dummies[k][1];
+    c->iux[i] = dummies[k][2];
+    c->iuy[i] = dummies[k][3];
+  }
+  // classification and areas (src/placerBase.cpp:639-680)
+  for (int i = 0; i < ni; i++) {
+    int64_t a = (int64_t)(c->iux[i] - c->ilx[i]) * (int64_t)(c->iuy[i] - c->ily[i]);
+    if (i >= nr) {
+      c->nonplace.push_back(i);
+      c->nonplace_area += a;
+    } else if (c->fixed[i]) {
+      int rlx = std::max(c->core_lx, c->ilx[i]), rly = std::max(c->core_ly, c->ily[i]);
+      int rux = std::min(c->core_ux, c->iux[i]), ruy = std::min(c->core_uy, c->iuy[i]);
+      if (!(rlx >= rux || rly >= ruy)) {
+        c->nonplace.push_back(i);
+        c->nonplace_area += (int64_t)(rux - rlx) * (int64_t)(ruy - rly);
+      }
+    } else {
+      c->inst2place[i] = (int)c->place.size();
+      c->place.push_back(i);
+      c->place_area += a;
+      if (c->iuy[i] - c->ily[i] > c->site_h * 6)
+        c->macro_area += a;
+      else
+        c->std_area += a;
+    }
+  }
+  // pins (src/placerBase.cpp:330-406, 702-713)
+  c->n_net = nl->num_nets;
+  c->n_pin = nl->num_pins;
+  const int np = c->n_pin;
+  c->net_start.assign(nl->net_pin_start, nl->net_pin_start + c->n_net + 1);
+  c->pin_inst.resize(np);
+  c->pin_offx.assign(np, 0);
+  c->pin_offy.assign(np, 0);
+  c->pin_cx.resize(np);
+  c->pin_cy.resize(np);
+  c->pin_box.assign((size_t)4 * np, 0);
+  for (int q = 0; q < np; q++) {
+    int i = nl->pin_inst[q];
+    if (i >= nr || i < -1) return fail(*out = c.release(), RP_ERR_ARG, "pin %d: bad instance %d", q, i);
+    c->pin_inst[q] = i;
+    if (i >= 0) {
+      c->pin_offx[q] = nl->pin_off[2 * q];
+      c->pin_offy[q] = nl->pin_off[2 * q + 1];
+      c->pin_cx[q] = c->ilx[i] + c->mw[i] / 2 + c->pin_offx[q];
+      c->pin_cy[q] = c->ily[i] + c->mh[i] / 2 + c->pin_offy[q];
+    } else {
+      int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
+      if (nl->pin_box) {
+        lx = nl->pin_box[4 * q];
+        ly = nl->pin_box[4 * q + 1];
+        ux = nl->pin_box[4 * q + 2];
+        uy = nl->pin_box[4 * q + 3];
+      }
+      c->pin_box[4 * (size_t)q] = lx;
+      c->pin_box[4 * (size_t)q + 1] = ly;
+      c->pin_box[4 * (size_t)q + 2] = ux;
+      c->pin_box[4 * (size_t)q + 3] = uy;
+      c->pin_cx[q] = (int)(((int64_t)lx + ux) / 2);
+      c->pin_cy[q] = (int)(((int64_t)ly + uy) / 2);
+    }
+  }
+  // per-instance pin order (src/placerBase.cpp:728-743)
+  c->ipin_start.assign(ni + 1, 0);
+  if (nl->inst_pin_start && nl->inst_pins) {
+    for (int i = 0; i <= nr; i++) c->ipin_start[i] = nl->inst_pin_start[i];
+    for (int i = nr + 1; i <= ni; i++) c->ipin_start[i] = c->ipin_start[nr];
+    c->ipins.assign(nl->inst_pins, nl->inst_pins + nl->inst_pin_start[nr]);
+  } else {
+    std::vector<int> cnt(ni + 1, 0);
+    for (int q = 0; q < np; q++)
+      if (c->pin_inst[q] >= 0) cnt[c->pin_inst[q] + 1]++;
+    for (int i = 0; i < ni; i++) cnt[i + 1] += cnt[i];
+    c->ipin_start = cnt;
+    c->ipins.resize(cnt[ni]);
+    for (int q = 0; q < np; q++)
+      if (c->pin_inst[q] >= 0) c->ipins[cnt[c->pin_inst[q]]++] = q;
+  }
+  *out = c.release();
+  // utilisation check of PlacerBase::printInfo (src/placerBase.cpp:936-956)
+  rp_ctx* cc = *out;
+  int64_t coreArea = (int64_t)(cc->core_ux - cc->core_lx) * (int64_t)(cc->core_uy - cc->core_ly);
+  float util = (float)cc->place_area / (float)(coreArea - cc->nonplace_area) * 100;
+  if (util >= 100.1) return fail(cc, RP_ERR_FILLER, "Util exceeds 100%%: %f", util);
+  return RP_OK;
+}
+
+extern "C" const char* rp_last_error(rp_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" void rp_destroy(rp_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->h_state) (void)hipHostFree(c->h_state);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+// inst-bbox HPWL over DB boxes and BPin boxes (PlacerBase::hpwl, src/placerBase.cpp:868-876, 480-500)
+static int64_t inst_hpwl(const rp_ctx* c) {
+  int64_t h = 0;
+  for (int n = 0; n < c->n_net; n++) {
+    int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
+    for (int q = c->net_start[n]; q < c->net_start[n + 1]; q++) {
+      int i = c->pin_inst[q];
+      if (i < 0) continue;
+      lx = std::min(c->dblx[i], lx);
+      ly = std::min(c->dbly[i], ly);
+      ux = std::max(c->dblx[i] + c->mw[i], ux);
+      uy = std::max(c->dbly[i] + c->mh[i], uy);
+    }
+    for (int q = c->net_start[n]; q < c->net_start[n + 1]; q++) {
+      if (c->pin_inst[q] >= 0 || c->pin_box[4 * (size_t)q] == INT_MAX) continue;
+      lx = std::min(c->pin_box[4 * (size_t)q], lx);
+      ly = std::min(c->pin_box[4 * (size_t)q + 1], ly);
+      ux = std::max(c->pin_box[4 * (size_t)q + 2], ux);
+      uy = std::max(c->pin_box[4 * (size_t)q + 3], uy);
+    }
+    h += (int64_t)((int)((unsigned)ux - (unsigned)lx) + (int)((unsigned)uy - (unsigned)ly));
+  }
+  return h;
+}
+
+// ===========================================================================
+// InitialPlace: host B2B assembly, device BiCGSTAB
+// ===========================================================================
+namespace {
+
+struct Csr {
+  int n = 0;
+  std::vector<int> rowp, col;
+  std::vector<float> val;
+};
+
+struct Trip {
+  int r, c;
+  float v;
+};
+
+// setFromTriplets: duplicates summed in triplet order (per row, by column).
+void build_csr(int n, const std::vector<Trip>& t, Csr& m) {
+  m.n = n;
+  std::vector<int> cnt(n + 1, 0);
+  for (const Trip& x : t) cnt[x.r + 1]++;
+  for (int r = 0; r < n; r++) cnt[r + 1] += cnt[r];
+  std::vector<int> order(t.size());
+  std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
+  for (size_t k = 0; k < t.size(); k++) order[fillp[t[k].r]++] = (int)k;
+  m.rowp.assign(n + 1, 0);
+  m.col.clear();
+  m.val.clear();
+  m.col.reserve(t.size());
+  m.val.reserve(t.size());
+  std::vector<int> tmp;
+  for (int r = 0; r < n; r++) {
+    tmp.assign(order.begin() + cnt[r], order.begin() + cnt[r + 1]);
+    std::stable_sort(tmp.begin(), tmp.end(), [&](int a, int b) { return t[a].c < t[b].c; });
+    for (size_t k = 0; k < tmp.size();) {
+      int cc = t[tmp[k]].c;
+      float v = 0.0f;
+      while (k < tmp.size() && t[tmp[k]].c == cc) v += t[tmp[k++]].v;
+      m.col.push_back(cc);
+      m.val.push_back(v);
+    }
+    m.rowp[r + 1] = (int)m.col.size();
+  }
+}
+
+struct DevSolver {
+  int n = 0, nparts = 0;
+  DevBuf<int> rowp, col;
+  DevBuf<float> val, b, x, r, r0, v, p, s, t;
+  DevBuf<double> parts, dots;
+  double* h_dots = nullptr;
+};
+
+}  // namespace
+
+static int dev_dots(rp_ctx* c, DevSolver& S, const float* a0, const float* b0, const float* a1, const float* b1,
+                    const float* a2, const float* b2, double out[3]) {
+  HIPCHK(c, rp_launch_dot3(S.n, a0, b0, a1, b1, a2, b2, S.parts.p, S.nparts, S.dots.p, c->stream));
+  HIPCHK(c, hipMemcpyAsync(S.h_dots, S.dots.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  out[0] = S.h_dots[0];
+  out[1] = S.h_dots[1];
+  out[2] = S.h_dots[2];
+  return 0;
+}
+
+// BiCGSTAB, identity preconditioner (published algorithm, van der Vorst 1992,
+// in the variant that restarts when rho collapses); reference call site
+// src/initialPlace.cpp:78-86.  x (device) holds the warm start.
+static int dev_bicgstab(rp_ctx* c, DevSolver& S, const Csr& A, const std::vector<float>& b, std::vector<float>& x,
+                        int max_iter, float tol, float* err_out) {
+  int n = A.n;
+  hipStream_t st = c->stream;
+  HIPCHK(c, dupload(S.rowp, A.rowp, st));
+  HIPCHK(c, dupload(S.col, A.col, st));
+  HIPCHK(c, dupload(S.val, A.val, st));
+  HIPCHK(c, hipMemcpyAsync(S.b.p, b.data(), n * sizeof(float), hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(S.x.p, x.data(), n * sizeof(float), hipMemcpyHostToDevice, st));
+  // r = b - A x ; r0 = r
+  HIPCHK(c, rp_launch_spmv(n, S.rowp.p, S.col.p, S.val.p, S.x.p, S.t.p, st));
+  HIPCHK(c, rp_launch_axpy(n, S.r.p, S.b.p, -1.0, S.t.p, st));
+  HIPCHK(c, hipMemcpyAsync(S.r0.p, S.r.p, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  HIPCHK(c, hipMemsetAsync(S.v.p, 0, n * sizeof(float), st));
+  HIPCHK(c, hipMemsetAsync(S.p.p, 0, n * sizeof(float), st));
+  double d[3];
+  if (dev_dots(c, S, S.r0.p, S.r0.p, S.b.p, S.b.p, nullptr, nullptr, d)) return RP_ERR_HIP;
+  double r0_sq = d[0], rhs_sq = d[1];
+  if (rhs_sq == 0.0) {
+    std::fill(x.begin(), x.end(), 0.0f);
+    *err_out = 0.0f;
+    return 0;
+  }
+  double rho = 1.0, alpha = 1.0, w = 1.0;
+  double tol2 = (double)tol * (double)tol * rhs_sq;
+  const double eps = 1.1920928955078125e-07;
+  double eps2 = eps * eps;
+  int i = 0, restarts = 0;
+  if (dev_dots(c, S, S.r.p, S.r.p, S.r0.p, S.r.p, nullptr, nullptr, d)) return RP_ERR_HIP;
+  double rr = d[0], r0r = d[1];
+  while (rr > tol2 && i < max_iter) {
+    double rho_old = rho;
+    rho = r0r;
+    if (std::fabs(rho) < eps2 * r0_sq) {
+      HIPCHK(c, rp_launch_spmv(n, S.rowp.p, S.col.p, S.val.p, S.x.p, S.t.p, st));
+      HIPCHK(c, rp_launch_axpy(n, S.r.p, S.b.p, -1.0, S.t.p, st));
+      HIPCHK(c, hipMemcpyAsync(S.r0.p, S.r.p, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+      if (dev_dots(c, S, S.r.p, S.r.p, nullptr, nullptr, nullptr, nullptr, d)) return RP_ERR_HIP;
+      rho = r0_sq = d[0];
+      if (restarts++ == 0) i = 0;
+    }
+    double beta = (rho / rho_old) * (alpha / w);
+    HIPCHK(c, rp_launch_bicg_p(n, S.p.p, S.r.p, S.v.p, beta, w, st));
+    HIPCHK(c, rp_launch_spmv(n, S.rowp.p, S.col.p, S.val.p, S.p.p, S.v.p, st));
+    if (dev_dots(c, S, S.r0.p, S.v.p, nullptr, nullptr, nullptr, nullptr, d)) return RP_ERR_HIP;
+    alpha = rho / d[0];
+    HIPCHK(c, rp_launch_axpy(n, S.s.p, S.r.p, -alpha, S.v.p, st));
+    HIPCHK(c, rp_launch_spmv(n, S.rowp.p, S.col.p, S.val.p, S.s.p, S.t.p, st));
+    if (dev_dots(c, S, S.t.p, S.t.p, S.t.p, S.s.p, nullptr, nullptr, d)) return RP_ERR_HIP;
+    w = d[0] > 0.0 ? d[1] / d[0] : 0.0;
+    HIPCHK(c, rp_launch_bicg_xr(n, S.x.p, S.r.p, S.p.p, S.s.p, S.t.p, alpha, w, st));
+    i++;
+    if (dev_dots(c, S, S.r.p, S.r.p, S.r0.p, S.r.p, nullptr, nullptr, d)) return RP_ERR_HIP;
+    rr = d[0];
+    r0r = d[1];
+  }
+  *err_out = (float)std::sqrt(rr / rhs_sq);
+  HIPCHK(c, hipMemcpyAsync(x.data(), S.x.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  return 0;
+}
+
+extern "C" int rp_initial_place(rp_ctx* c, rp_ip_stats* stats) {
+  if (!c) return RP_ERR_ARG;
+  auto t0 = std::chrono::steady_clock::now();
+  const rp_params& p = c->prm;
+  const int n = (int)c->place.size();
+  c->ip_err.clear();
+  c->ip_hpwl.clear();
+  // placeInstsCenter (src/initialPlace.cpp:108-115)
+  if (!p.incremental_place_mode) {
+    int ccx = (c->core_lx + c->core_ux) / 2, ccy = (c->core_ly + c->core_uy) / 2;
+    for (int k = 0; k < n; k++) inst_set_center(c, c->place[k], ccx, ccy);
+  }
+  if (p.initial_place_max_iter > 0 && n > 0) {
+    DevSolver S;
+    S.n = n;
+    S.nparts = std::max(1, std::min(1024, (n + RP_BLOCK - 1) / RP_BLOCK));
+    HIPCHK(c, dalloc(S.b, n));
+    HIPCHK(c, dalloc(S.x, n));
+    HIPCHK(c, dalloc(S.r, n));
+    HIPCHK(c, dalloc(S.r0, n));
+    HIPCHK(c, dalloc(S.v, n));
+    HIPCHK(c, dalloc(S.p, n));
+    HIPCHK(c, dalloc(S.s, n));
+    HIPCHK(c, dalloc(S.t, n));
+    HIPCHK(c, dalloc(S.parts, (size_t)3 * S.nparts));
+    HIPCHK(c, dalloc(S.dots, 3));
+    HIPCHK(c, hipHostMalloc((void**)&S.h_dots, 3 * sizeof(double), hipHostMallocDefault));
+    std::vector<uint8_t> mm(c->n_pin);
+    std::vector<float> bx(n), by(n), xx(n), xy(n);
+    std::vector<Trip> tx, ty;
+    int rc = 0;
+    for (int it = 1; it <= p.initial_place_max_iter; it++) {
+      // updatePinInfo, src/initialPlace.cpp:128-182 (first strict extreme wins)
+      std::fill(mm.begin(), mm.end(), 0);
+      for (int e = 0; e < c->n_net; e++) {
+        int pminx = -1, pmaxx = -1, pminy = -1, pmaxy = -1;
+        int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
+        for (int q = c->net_start[e]; q < c->net_start[e + 1]; q++) {
+          if (lx > c->pin_cx[q]) {
+            if (pminx >= 0) mm[pminx] &= (uint8_t)~1u;
+            lx = c->pin_cx[q];
+            pminx = q;
+            mm[q] |= 1u;
+          }
+          if (ux < c->pin_cx[q]) {
+            if (pmaxx >= 0) mm[pmaxx] &= (uint8_t)~2u;
+            ux = c->pin_cx[q];
+            pmaxx = q;
+            mm[q] |= 2u;
+          }
+          if (ly > c->pin_cy[q]) {
+            if (pminy >= 0) mm[pminy] &= (uint8_t)~4u;
+            ly = c->pin_cy[q];
+            pminy = q;
+            mm[q] |= 4u;
+          }
+          if (uy < c->pin_cy[q]) {
+            if (pmaxy >= 0) mm[pmaxy] &= (uint8_t)~8u;
+            uy = c->pin_cy[q];
+            pmaxy = q;
+            mm[q] |= 8u;
+          }
+        }
+      }
+      // createSparseMatrix, src/initialPlace.cpp:185-378
+      for (int k = 0; k < n; k++) {
+        int i = c->place[k];
+        xx[k] = (float)inst_cx(c, i);
+        xy[k] = (float)inst_cy(c, i);
+        bx[k] = by[k] = 0.0f;
+      }
+      tx.clear();
+      ty.clear();
+      const float minDiff = (float)p.initial_place_min_diff_length;
+      for (int e = 0; e < c->n_net; e++) {
+        int s = c->net_start[e], deg = c->net_start[e + 1] - s;
+        if (deg <= 1 || deg >= p.initial_place_max_fanout) continue;
+        float netWeight = p.initial_place_net_weight_scale / (float)(deg - 1);
+        for (int a1 = 0; a1 < deg; a1++) {
+          int q1 = s + a1;
+          int i1 = c->pin_inst[q1];
+          int k1 = i1 >= 0 ? c->inst2place[i1] : -1;
+          for (int a2 = 0; a2 <= a1; a2++) {
+            int q2 = s + a2;
+            int i2 = c->pin_inst[q2];
+            if (i1 == i2) continue;
+            int k2 = i2 >= 0 ? c->inst2place[i2] : -1;
+            if ((mm[q1] & 3u) || (mm[q2] & 3u)) {
+              int diff = std::abs(c->pin_cx[q1] - c->pin_cx[q2]);
+              float w = diff > p.initial_place_min_diff_length ? netWeight / (float)diff : netWeight / minDiff;
+              if (k1 >= 0 && k2 >= 0) {
+                tx.push_back({k1, k1, w});
+                tx.push_back({k2, k2, w});
+                tx.push_back({k1, k2, -w});
+                tx.push_back({k2, k1, -w});
+                bx[k1] += -w * (float)((c->pin_cx[q1] - inst_cx(c, i1)) - (c->pin_cx[q2] - inst_cx(c, i2)));
+                bx[k2] += -w * (float)((c->pin_cx[q2] - inst_cx(c, i2)) - (c->pin_cx[q1] - inst_cx(c, i1)));
+              } else if (k1 < 0 && k2 >= 0) {
+                tx.push_back({k2, k2, w});
+                bx[k2] += w * (float)(c->pin_cx[q1] - (c->pin_cx[q2] - inst_cx(c, i2)));
+              } else if (k1 >= 0 && k2 < 0) {
+                tx.push_back({k1, k1, w});
+                bx[k1] += w * (float)(c->pin_cx[q2] - (c->pin_cx[q1] - inst_cx(c, i1)));
+              }
+            }
+            if ((mm[q1] & 12u) || (mm[q2] & 12u)) {
+              int diff = std::abs(c->pin_cy[q1] - c->pin_cy[q2]);
+              float w = diff > p.initial_place_min_diff_length ? netWeight / (float)diff : netWeight / minDiff;
+              if (k1 >= 0 && k2 >= 0) {
+                ty.push_back({k1, k1, w});
+                ty.push_back({k2, k2, w});
+                ty.push_back({k1, k2, -w});
+                ty.push_back({k2, k1, -w});
+                by[k1] += -w * (float)((c->pin_cy[q1] - inst_cy(c, i1)) - (c->pin_cy[q2] - inst_cy(c, i2)));
+                by[k2] += -w * (float)((c->pin_cy[q2] - inst_cy(c, i2)) - (c->pin_cy[q1] - inst_cy(c, i1)));
+              } else if (k1 < 0 && k2 >= 0) {
+                ty.push_back({k2, k2, w});
+                by[k2] += w * (float)(c->pin_cy[q1] - (c->pin_cy[q2] - inst_cy(c, i2)));
+              } else if (k1 >= 0 && k2 < 0) {
+                ty.push_back({k1, k1, w});
+                by[k1] += w * (float)(c->pin_cy[q2] - (c->pin_cy[q1] - inst_cy(c, i1)));
+              }
+            }
+          }
+        }
+      }
+      Csr Ax, Ay;
+      build_csr(n, tx, Ax);
+      build_csr(n, ty, Ay);
+      const float tol = 1.1920928955078125e-07f;  // NumTraits<float>::epsilon()
+      float ex = 0, ey = 0;
+      rc = dev_bicgstab(c, S, Ax, bx, xx, p.initial_place_max_solver_iter, tol, &ex);
+      if (rc) break;
+      rc = dev_bicgstab(c, S, Ay, by, xy, p.initial_place_max_solver_iter, tol, &ey);
+      if (rc) break;
+      float err = std::max(ex, ey);
+      c->ip_err.push_back(err);
+      c->ip_hpwl.push_back(inst_hpwl(c));  // printed before updateCoordi (src/initialPlace.cpp:88-91)
+      // updateCoordi, src/initialPlace.cpp:380-385
+      for (int k = 0; k < n; k++) {
+        int i = c->place[k];
+        inst_set_center(c, i, (int)xx[k], (int)xy[k]);
+        c->dblx[i] = c->ilx[i];
+        c->dbly[i] = c->ily[i];
+      }
+      if (err <= 1e-5 && it >= 5) break;
+    }
+    (void)hipHostFree(S.h_dots);
+    if (rc) return rc;
+  }
+  c->nb_ready = false;
+  if (stats) {
+    stats->iterations = (int)c->ip_err.size();
+    stats->last_error = c->ip_err.empty() ? 0.0f : c->ip_err.back();
+    stats->last_hpwl = c->ip_hpwl.empty() ? 0 : c->ip_hpwl.back();
+    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return RP_OK;
+}
+
+extern "C" int rp_get_ip_records(rp_ctx* c, float* err, int64_t* hpwl, int32_t cap, int32_t* count) {
+  if (!c || !count) return RP_ERR_ARG;
+  *count = (int32_t)c->ip_err.size();
+  for (int k = 0; k < std::min<int>(cap, *count); k++) {
+    if (err) err[k] = c->ip_err[k];
+    if (hpwl) hpwl[k] = c->ip_hpwl[k];
+  }
+  return RP_OK;
+}
+
+// ===========================================================================
+// NesterovBase::init and device upload
+// ===========================================================================
+static void make_twiddles(int n, std::vector<float2>& fft, std::vector<float2>& dct) {
+  fft.resize(std::max(1, n / 2));
+  dct.resize(n);
+  for (int k = 0; k < n / 2; k++) {
+    double a = -2.0 * M_PI * (double)k / (double)n;
+    fft[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  for (int k = 0; k < n; k++) {
+    double a = M_PI * (double)k / (2.0 * (double)n);
+    dct[k] = make_float2((float)std::cos(a), (float)-std::sin(a));
+  }
+}
+
+static bool is_pow2(int v) { return v >= 4 && v <= 2048 && (v & (v - 1)) == 0; }
+
+extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
+  if (!c) return RP_ERR_ARG;
+  const rp_params& p = c->prm;
+  const int ngi = (int)c->place.size();
+  if (ngi == 0) return fail(c, RP_ERR_ARG, "no movable instances");
+  // fillers (initFillerGCells, src/nesterovBase.cpp:1066-1155)
+  std::vector<int> dxs(ngi), dys(ngi);
+  for (int k = 0; k < ngi; k++) {
+    int i = c->place[k];
+    dxs[k] = c->iux[i] - c->ilx[i];
+    dys[k] = c->iuy[i] - c->ily[i];
+  }
+  std::sort(dxs.begin(), dxs.end());
+  std::sort(dys.begin(), dys.end());
+  int minIdx = (int)((double)ngi * 0.05), maxIdx = (int)((double)ngi * 0.95);
+  int64_t dxSum = 0, dySum = 0;
+  for (int k = minIdx; k < maxIdx; k++) {
+    dxSum += dxs[k];
+    dySum += dys[k];
+  }
+  int avgDx = maxIdx > minIdx ? (int)(dxSum / (maxIdx - minIdx)) : 0;
+  int avgDy = maxIdx > minIdx ? (int)(dySum / (maxIdx - minIdx)) : 0;
+  int cdx = c->core_ux - c->core_lx, cdy = c->core_uy - c->core_ly;
+  int64_t coreArea = (int64_t)cdx * (int64_t)cdy;
+  int64_t white = coreArea - c->nonplace_area;
+  int64_t movable = (int64_t)((float)white * p.target_density);
+  int64_t totalFiller = movable - c->std_area - (int64_t)((float)c->macro_area * p.target_density);
+  if (totalFiller < 0)
+    return fail(c, RP_ERR_FILLER,
+                "Filler area is negative!!\n Please put higher target density or \n Re-floorplan to have enough "
+                "coreArea\n");
+  int fillerCnt = (avgDx * avgDy) > 0 ? (int)(totalFiller / (int64_t)(avgDx * avgDy)) : 0;
+  c->filler_dx = avgDx;
+  c->filler_dy = avgDy;
+  c->n_gi = ngi;
+  c->n_gf = fillerCnt;
+  c->n_g = ngi + fillerCnt;
+  const int ng = c->n_g;
+  std::vector<int> glx(ng), gly(ng), gux(ng), guy(ng);
+  c->g_inst.assign(ng, -1);
+  c->g_kind.assign(ng, 2);
+  for (int k = 0; k < ngi; k++) {
+    int i = c->place[k];
+    c->g_inst[k] = i;
+    glx[k] = c->ilx[i];
+    gly[k] = c->ily[i];
+    gux[k] = c->iux[i];
+    guy[k] = c->iuy[i];
+    c->g_kind[k] = (c->iuy[i] - c->ily[i]) > c->site_h * 6 ? 1 : 0;
+  }
+  std::mt19937 rv(0);
+  for (int k = 0; k < fillerCnt; k++) {
+    auto rx = rv();
+    auto ry = rv();
+    int cx = (int)(rx % (unsigned long)(long)cdx + (unsigned long)(long)c->core_lx);
+    int cy = (int)(ry % (unsigned long)(long)cdy + (unsigned long)(long)c->core_ly);
+    int g = ngi + k;
+    glx[g] = cx - avgDx / 2;
+    gly[g] = cy - avgDy / 2;
+    gux[g] = cx + avgDx / 2;
+    guy[g] = cy + avgDy / 2;
+  }
+  // bin grid (BinGrid::initBins, src/nesterovBase.cpp:631-717)
+  c->td = p.target_density;
+  int64_t totalBinArea = (int64_t)cdx * (int64_t)cdy;
+  int64_t avgArea = c->place_area / (int64_t)ngi;
+  int64_t idealBinArea = (int64_t)std::round((float)avgArea / c->td);
+  int idealBinCnt = idealBinArea > 0 ? (int)(totalBinArea / idealBinArea) : 0;
+  int found = 2;
+  for (found = 2; found <= 1024; found *= 2)
+    if (found * found <= idealBinCnt && 4 * found * found > idealBinCnt) break;
+  c->nbx = p.bin_grid_cnt_x ? p.bin_grid_cnt_x : found;
+  c->nby = p.bin_grid_cnt_y ? p.bin_grid_cnt_y : found;
+  if (!is_pow2(c->nbx) || !is_pow2(c->nby))
+    return fail(c, RP_ERR_ARG, "bin grid %dx%d: each side must be a power of two in [4, 2048]", c->nbx, c->nby);
+  c->bsx = (int)std::ceil((float)cdx / (float)c->nbx);
+  c->bsy = (int)std::ceil((float)cdy / (float)c->nby);
+  // the kernels assume the regular layout the reference's loop produces
+  if ((int64_t)(c->nbx - 1) * c->bsx >= cdx || (int64_t)(c->nby - 1) * c->bsy >= cdy)
+    return fail(c, RP_ERR_ARG, "bin grid %dx%d does not tile the core regularly", c->nbx, c->nby);
+  const int nb = c->nbx * c->nby;
+  // nonPlace areas (updateBinsNonPlaceArea, src/nesterovBase.cpp:719-741)
+  c->nonplace_bins.assign(nb, 0);
+  for (int i : c->nonplace) {
+    auto px = site_range(c->ilx[i], c->iux[i], c->core_lx, c->bsx, 0, c->nbx);
+    auto py = site_range(c->ily[i], c->iuy[i], c->core_ly, c->bsy, 0, c->nby);
+    for (int a = px.first; a < px.second; a++) {
+      int blx = c->core_lx + a * c->bsx, bux = std::min(blx + c->bsx, c->core_ux);
+      for (int b = py.first; b < py.second; b++) {
+        int bly = c->core_ly + b * c->bsy, buy = std::min(bly + c->bsy, c->core_uy);
+        int rlx = std::max(blx, c->ilx[i]), rly = std::max(bly, c->ily[i]);
+        int rux = std::min(bux, c->iux[i]), ruy = std::min(buy, c->iuy[i]);
+        int64_t ov = (rlx >= rux || rly >= ruy) ? 0 : (int64_t)(rux - rlx) * (int64_t)(ruy - rly);
+        c->nonplace_bins[(size_t)b * c->nbx + a] += (int64_t)((float)ov * c->td);
+      }
+    }
+  }
+  // density size and scale (src/nesterovBase.cpp:1033-1061), then the
+  // layout clamp of NesterovPlace::init (src/nesterovBase.cpp:1245-1268)
+  c->g_dx.resize(ng);
+  c->g_dy.resize(ng);
+  c->g_dhx.resize(ng);
+  c->g_dhy.resize(ng);
+  c->g_scale.resize(ng);
+  c->g_icx.resize(ng);
+  c->g_icy.resize(ng);
+  for (int g = 0; g < ng; g++) {
+    int gdx = gux[g] - glx[g], gdy = guy[g] - gly[g];
+    c->g_dx[g] = gdx;
+    c->g_dy[g] = gdy;
+    float scaleX, scaleY, dsx, dsy;
+    if ((long double)gdx < kSqrt2 * (long double)c->bsx) {
+      scaleX = (float)gdx / (float)(kSqrt2 * (long double)c->bsx);
+      dsx = (float)(kSqrt2 * (long double)(float)c->bsx);
+    } else {
+      scaleX = 1.0f;
+      dsx = (float)gdx;
+    }
+    if ((long double)gdy < kSqrt2 * (long double)c->bsy) {
+      scaleY = (float)gdy / (float)(kSqrt2 * (long double)c->bsy);
+      dsy = (float)(kSqrt2 * (long double)(float)c->bsy);
+    } else {
+      scaleY = 1.0f;
+      dsy = (float)gdy;
+    }
+    int ddx = (int)dsx, ddy = (int)dsy;
+    int ccx = (gux[g] + glx[g]) / 2, ccy = (guy[g] + gly[g]) / 2;
+    int dlx = ccx - ddx / 2, dly = ccy - ddy / 2;
+    int dux = ccx + ddx / 2, duy = ccy + ddy / 2;
+    c->g_scale[g] = scaleX * scaleY;
+    int wdx = dux - dlx, wdy = duy - dly;
+    float tlx = (float)dlx, tly = (float)dly;
+    if (tlx < (float)c->core_lx) tlx = (float)c->core_lx;
+    if (tly < (float)c->core_ly) tly = (float)c->core_ly;
+    if (tlx + (float)wdx > (float)c->core_ux) tlx = (float)(c->core_ux - wdx);
+    if (tly + (float)wdy > (float)c->core_uy) tly = (float)(c->core_uy - wdy);
+    int nlx = (int)tlx, nly = (int)tly;
+    c->g_dhx[g] = wdx / 2;
+    c->g_dhy[g] = wdy / 2;
+    c->g_icx[g] = (float)((nlx + nlx + wdx) / 2);
+    c->g_icy[g] = (float)((nly + nly + wdy) / 2);
+  }
+
+  // ---------------- device upload ----------------
+  hipStream_t s = c->stream;
+  std::vector<int2> dhalf(ng);
+  std::vector<float> area(ng), npin(ng);
+  for (int g = 0; g < ng; g++) {
+    dhalf[g] = make_int2(c->g_dhx[g], c->g_dhy[g]);
+    area[g] = (float)c->g_dx[g] * (float)c->g_dy[g];
+    int i = c->g_inst[g];
+    npin[g] = i >= 0 ? (float)(c->ipin_start[i + 1] - c->ipin_start[i]) : 0.0f;
+  }
+  // nets renumbered by degree (stable): equal work per wavefront
+  const int nn = c->n_net, npn = c->n_pin;
+  c->net_perm.resize(nn);
+  std::iota(c->net_perm.begin(), c->net_perm.end(), 0);
+  std::stable_sort(c->net_perm.begin(), c->net_perm.end(), [&](int a, int b) {
+    return (c->net_start[a + 1] - c->net_start[a]) < (c->net_start[b + 1] - c->net_start[b]);
+  });
+  std::vector<int> net_new(nn), nstart(nn + 1, 0), pin_new(npn);
+  for (int k = 0; k < nn; k++) net_new[c->net_perm[k]] = k;
+  for (int k = 0; k < nn; k++) {
+    int e = c->net_perm[k];
+    nstart[k + 1] = nstart[k] + (c->net_start[e + 1] - c->net_start[e]);
+  }
+  std::vector<int> pcell(npn);
+  std::vector<int2> ppos(npn);
+  std::vector<int> pin_net(npn);
+  for (int k = 0; k < nn; k++) {
+    int e = c->net_perm[k];
+    for (int q = c->net_start[e], m = nstart[k]; q < c->net_start[e + 1]; q++, m++) {
+      pin_new[q] = m;
+      pin_net[q] = k;
+      int i = c->pin_inst[q];
+      int g = i >= 0 ? c->inst2place[i] : -1;
+      pcell[m] = g;
+      ppos[m] = g >= 0 ? make_int2(c->pin_offx[q], c->pin_offy[q]) : make_int2(c->pin_cx[q], c->pin_cy[q]);
+    }
+  }
+  std::vector<int> cp_start(ng + 1, 0), cp_net;
+  std::vector<int2> cp_off;
+  for (int g = 0; g < ng; g++) {
+    int i = c->g_inst[g];
+    if (i >= 0) {
+      for (int k = c->ipin_start[i]; k < c->ipin_start[i + 1]; k++) {
+        int q = c->ipins[k];
+        cp_net.push_back(pin_net[q]);
+        cp_off.push_back(make_int2(c->pin_offx[q], c->pin_offy[q]));
+      }
+    }
+    cp_start[g + 1] = (int)cp_net.size();
+  }
+  std::vector<float2> twx, tw2x, twy, tw2y;
+  make_twiddles(c->nbx, twx, tw2x);
+  make_twiddles(c->nby, twy, tw2y);
+  // FFT::init twiddles with the reference's long double rounding (src/fft.cpp:78-90)
+  std::vector<float> wx(c->nbx), wx2(c->nbx), wy(c->nby), wy2(c->nby);
+  for (int i = 0; i < c->nbx; i++) {
+    wx[i] = (float)(kPi * (long double)(float)i / (long double)(float)c->nbx);
+    wx2[i] = wx[i] * wx[i];
+  }
+  for (int j = 0; j < c->nby; j++) {
+    wy[j] = (float)(kPi * (long double)(float)j / (long double)(float)c->nby * (long double)(float)c->bsy /
+                    (long double)(float)c->bsx);
+    wy2[j] = wy[j] * wy[j];
+  }
+  HIPCHK(c, dupload(c->d_dhalf, dhalf, s));
+  HIPCHK(c, dupload(c->d_scale, c->g_scale, s));
+  HIPCHK(c, dupload(c->d_kind, c->g_kind, s));
+  HIPCHK(c, dupload(c->d_area, area, s));
+  HIPCHK(c, dupload(c->d_npin, npin, s));
+  HIPCHK(c, dalloc(c->d_slp, (size_t)3 * ng));
+  HIPCHK(c, dalloc(c->d_sumg, (size_t)3 * ng));
+  HIPCHK(c, dalloc(c->d_co, (size_t)2 * ng));
+  HIPCHK(c, dalloc(c->d_ic, ng));
+  HIPCHK(c, hipMemsetAsync(c->d_sumg.p, 0, sizeof(float2) * 3 * ng, s));
+  HIPCHK(c, dupload(c->d_cp_start, cp_start, s));
+  HIPCHK(c, dupload(c->d_cp_net, cp_net, s));
+  HIPCHK(c, dupload(c->d_cp_off, cp_off, s));
+  HIPCHK(c, dupload(c->d_nstart, nstart, s));
+  HIPCHK(c, dupload(c->d_pcell, pcell, s));
+  HIPCHK(c, dupload(c->d_ppos, ppos, s));
+  HIPCHK(c, dalloc(c->d_nbox, nn));
+  HIPCHK(c, dalloc(c->d_nsum, (size_t)2 * nn));
+  HIPCHK(c, dalloc(c->d_bin_inst, nb));
+  HIPCHK(c, dalloc(c->d_bin_filler, nb));
+  HIPCHK(c, hipMemsetAsync(c->d_bin_inst.p, 0, sizeof(unsigned long long) * nb, s));
+  HIPCHK(c, hipMemsetAsync(c->d_bin_filler.p, 0, sizeof(unsigned long long) * nb, s));
+  std::vector<long long> npl(c->nonplace_bins.begin(), c->nonplace_bins.end());
+  HIPCHK(c, dupload(c->d_bin_nonplace, npl, s));
+  HIPCHK(c, dalloc(c->d_rho, nb));
+  HIPCHK(c, dalloc(c->d_phi, nb));
+  HIPCHK(c, dalloc(c->d_field, nb));
+  HIPCHK(c, dalloc(c->d_t, nb));
+  HIPCHK(c, dalloc(c->d_tt, nb));
+  HIPCHK(c, dalloc(c->d_y1t, nb));
+  HIPCHK(c, dalloc(c->d_y2t, nb));
+  HIPCHK(c, dalloc(c->d_y1, nb));
+  HIPCHK(c, dalloc(c->d_y2, nb));
+  HIPCHK(c, dupload(c->d_twx, twx, s));
+  HIPCHK(c, dupload(c->d_tw2x, tw2x, s));
+  HIPCHK(c, dupload(c->d_twy, twy, s));
+  HIPCHK(c, dupload(c->d_tw2y, tw2y, s));
+  HIPCHK(c, dupload(c->d_wx, wx, s));
+  HIPCHK(c, dupload(c->d_wx2, wx2, s));
+  HIPCHK(c, dupload(c->d_wy, wy, s));
+  HIPCHK(c, dupload(c->d_wy2, wy2, s));
+  int npg = (ng + RP_BLOCK - 1) / RP_BLOCK, npb = c->nby / 2, npn2 = (nn + RP_BLOCK - 1) / RP_BLOCK;
+  HIPCHK(c, dalloc(c->d_part_grad, (size_t)RP_P_COUNT * npg));
+  HIPCHK(c, dalloc(c->d_part_bin, npb));
+  HIPCHK(c, dalloc(c->d_part_net, std::max(1, npn2)));
+  HIPCHK(c, hipMemsetAsync(c->d_part_net.p, 0, sizeof(long long) * std::max(1, npn2), s));
+  HIPCHK(c, dalloc(c->d_state, 1));
+  HIPCHK(c, dalloc(c->d_rec, std::max(1, p.nesterov_place_max_iter)));
+  HIPCHK(c, dalloc(c->d_dbg_wl, ng));
+  HIPCHK(c, dalloc(c->d_dbg_dens, ng));
+  HIPCHK(c, hipStreamSynchronize(s));
+
+  RpDev& d = c->dev;
+  memset(&d, 0, sizeof d);
+  d.nc = ng;
+  d.nn = nn;
+  d.np = npn;
+  d.nbx = c->nbx;
+  d.nby = c->nby;
+  d.bsx = c->bsx;
+  d.bsy = c->bsy;
+  d.core_lx = c->core_lx;
+  d.core_ly = c->core_ly;
+  d.core_ux = c->core_ux;
+  d.core_uy = c->core_uy;
+  d.td = c->td;
+  d.bar = p.min_wire_length_force_bar;
+  d.min_precond = p.min_preconditioner;
+  d.dhalf = c->d_dhalf.p;
+  d.scale = c->d_scale.p;
+  d.kind = c->d_kind.p;
+  d.area = c->d_area.p;
+  d.npin = c->d_npin.p;
+  d.slp = c->d_slp.p;
+  d.sumg = c->d_sumg.p;
+  d.co = c->d_co.p;
+  d.ic = c->d_ic.p;
+  d.cp_start = c->d_cp_start.p;
+  d.cp_net = c->d_cp_net.p;
+  d.cp_off = c->d_cp_off.p;
+  d.nstart = c->d_nstart.p;
+  d.pcell = c->d_pcell.p;
+  d.ppos = c->d_ppos.p;
+  d.nbox = c->d_nbox.p;
+  d.nsum = c->d_nsum.p;
+  d.bin_inst = c->d_bin_inst.p;
+  d.bin_filler = c->d_bin_filler.p;
+  d.bin_nonplace = c->d_bin_nonplace.p;
+  d.rho = c->d_rho.p;
+  d.field = c->d_field.p;
+  d.phi = c->d_phi.p;
+  d.t = c->d_t.p;
+  d.tt = c->d_tt.p;
+  d.y1t = c->d_y1t.p;
+  d.y2t = c->d_y2t.p;
+  d.y1 = c->d_y1.p;
+  d.y2 = c->d_y2.p;
+  d.twx = c->d_twx.p;
+  d.tw2x = c->d_tw2x.p;
+  d.twy = c->d_twy.p;
+  d.tw2y = c->d_tw2y.p;
+  d.wx = c->d_wx.p;
+  d.wx2 = c->d_wx2.p;
+  d.wy = c->d_wy.p;
+  d.wy2 = c->d_wy2.p;
+  d.part_grad = c->d_part_grad.p;
+  d.part_bin = c->d_part_bin.p;
+  d.part_net = c->d_part_net.p;
+  d.st = c->d_state.p;
+  d.rec = c->d_rec.p;
+  d.dbg_wl = nullptr;
+  d.dbg_dens = nullptr;
+
+  RpState& S = *c->h_state;
+  memset(&S, 0, sizeof S);
+  S.num_cells = ng;
+  S.max_iter = p.nesterov_place_max_iter;
+  S.max_back_track = p.max_back_track;
+  S.num_partials_grad = npg;
+  S.num_partials_bin = npb;
+  S.num_partials_net = npn2;
+  S.target_overflow = p.target_overflow;
+  S.min_phi_coef = p.min_phi_coef;
+  S.reference_hpwl = p.reference_hpwl;
+  S.init_density_penalty = p.init_density_penalty;
+  S.init_wl_coef = p.init_wire_length_coef;
+  S.area_denom = (float)c->std_area + (float)c->macro_area * c->td;  // int64 + float -> float
+  S.bin_size_sum = (float)(c->bsx + c->bsy);
+  S.initial_prev_coef = p.initial_prev_coordi_update_coef;
+  S.max_phi_coef = p.max_phi_coef;
+  S.ring.cur = 0;
+  S.ring.next = 1;
+  S.ring.prev = 2;
+  S.ring.co_cur = 0;
+  S.ring.co_next = 1;
+  c->nb_ready = true;
+  if (info) {
+    memset(info, 0, sizeof *info);
+    info->num_gcells = ng;
+    info->num_fillers = fillerCnt;
+    info->num_place_insts = ngi;
+    info->num_gnets = nn;
+    info->num_gpins = npn;
+    info->bin_cnt_x = c->nbx;
+    info->bin_cnt_y = c->nby;
+    info->bin_size_x = c->bsx;
+    info->bin_size_y = c->bsy;
+    info->core_lx = c->core_lx;
+    info->core_ly = c->core_ly;
+    info->core_ux = c->core_ux;
+    info->core_uy = c->core_uy;
+    info->filler_dx = avgDx;
+    info->filler_dy = avgDy;
+    info->std_insts_area = c->std_area;
+    info->macro_insts_area = c->macro_area;
+    info->place_insts_area = c->place_area;
+    info->nonplace_insts_area = c->nonplace_area;
+    info->target_density = c->td;
+  }
+  return RP_OK;
+}
+
+extern "C" int rp_get_gcells(rp_ctx* c, int32_t* dhx, int32_t* dhy, float* scale, int32_t* dx, int32_t* dy,
+                             float* icx, float* icy, uint8_t* kind) {
+  if (!c || !c->nb_ready) return RP_ERR_STATE;
+  for (int g = 0; g < c->n_g; g++) {
+    if (dhx) dhx[g] = c->g_dhx[g];
+    if (dhy) dhy[g] = c->g_dhy[g];
+    if (scale) scale[g] = c->g_scale[g];
+    if (dx) dx[g] = c->g_dx[g];
+    if (dy) dy[g] = c->g_dy[g];
+    if (icx) icx[g] = c->g_icx[g];
+    if (icy) icy[g] = c->g_icy[g];
+    if (kind) kind[g] = c->g_kind[g];
+  }
+  return RP_OK;
+}
+
+extern "C" int rp_get_nonplace(rp_ctx* c, int64_t* out) {
+  if (!c || !c->nb_ready || !out) return RP_ERR_STATE;
+  memcpy(out, c->nonplace_bins.data(), sizeof(int64_t) * c->nonplace_bins.size());
+  return RP_OK;
+}
+
+// ===========================================================================
+// stage-level evaluation (tests) and Poisson-only entry
+// ===========================================================================
+extern "C" int rp_eval_gradient(rp_ctx* c, const float* cx, const float* cy, float lam, float wlx, float wly,
+                                rp_eval_out* o) {
+  if (!c || !c->nb_ready || !cx || !cy || !o) return RP_ERR_STATE;
+  const int ng = c->n_g, nb = c->nbx * c->nby, nn = c->n_net;
+  hipStream_t s = c->stream;
+  RpState saved = *c->h_state;
+  RpState S = saved;
+  S.phase = 0;
+  S.mode = 1;
+  S.ring.cur = 1;
+  S.ring.next = 1;
+  S.dst = 1;
+  S.density_penalty = lam;
+  S.wl_coef_x = wlx;
+  S.wl_coef_y = wly;
+  std::vector<float2> xy(ng);
+  for (int g = 0; g < ng; g++) xy[g] = make_float2(cx[g], cy[g]);
+  HIPCHK(c, hipMemcpyAsync(c->d_slp.p + (size_t)ng, xy.data(), sizeof(float2) * ng, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(c->d_state.p, &S, sizeof S, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemsetAsync(c->d_bin_inst.p, 0, sizeof(unsigned long long) * nb, s));
+  HIPCHK(c, hipMemsetAsync(c->d_bin_filler.p, 0, sizeof(unsigned long long) * nb, s));
+  RpDev d = c->dev;
+  d.dbg_wl = c->d_dbg_wl.p;
+  d.dbg_dens = c->d_dbg_dens.p;
+  HIPCHK(c, rp_launch_scatter(d, s));
+  if (o->bin_inst)
+    HIPCHK(c, hipMemcpyAsync(o->bin_inst, c->d_bin_inst.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, s));
+  if (o->bin_filler)
+    HIPCHK(c, hipMemcpyAsync(o->bin_filler, c->d_bin_filler.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, rp_launch_poisson(d, s, 1, 0));
+  HIPCHK(c, rp_launch_wa_grad(d, s));
+  std::vector<float2> field(nb), sg(ng), wl(ng), dn(ng);
+  std::vector<double> pg((size_t)RP_P_COUNT * c->h_state->num_partials_grad), pb(c->h_state->num_partials_bin);
+  std::vector<long long> pn(std::max(1, c->h_state->num_partials_net));
+  HIPCHK(c, hipMemcpyAsync(field.data(), c->d_field.p, sizeof(float2) * nb, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(sg.data(), c->d_sumg.p + (size_t)ng, sizeof(float2) * ng, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(wl.data(), c->d_dbg_wl.p, sizeof(float2) * ng, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(dn.data(), c->d_dbg_dens.p, sizeof(float2) * ng, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(pg.data(), c->d_part_grad.p, sizeof(double) * pg.size(), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(pb.data(), c->d_part_bin.p, sizeof(double) * pb.size(), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(pn.data(), c->d_part_net.p, sizeof(long long) * pn.size(), hipMemcpyDeviceToHost, s));
+  if (o->density)
+    HIPCHK(c, hipMemcpyAsync(o->density, c->d_rho.p, sizeof(float) * nb, hipMemcpyDeviceToHost, s));
+  if (o->phi) HIPCHK(c, hipMemcpyAsync(o->phi, c->d_phi.p, sizeof(float) * nb, hipMemcpyDeviceToHost, s));
+  std::vector<int4> nbox(nn);
+  std::vector<float4> nsum((size_t)2 * nn);
+  HIPCHK(c, hipMemcpyAsync(nbox.data(), c->d_nbox.p, sizeof(int4) * nn, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(nsum.data(), c->d_nsum.p, sizeof(float4) * 2 * nn, hipMemcpyDeviceToHost, s));
+  // restore the controller state
+  HIPCHK(c, hipMemcpyAsync(c->d_state.p, &saved, sizeof saved, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  for (int b = 0; b < nb; b++) {
+    if (o->ex) o->ex[b] = field[b].x;
+    if (o->ey) o->ey[b] = field[b].y;
+  }
+  for (int g = 0; g < ng; g++) {
+    if (o->sum_grad) {
+      o->sum_grad[2 * g] = sg[g].x;
+      o->sum_grad[2 * g + 1] = sg[g].y;
+    }
+    if (o->wl_grad) {
+      o->wl_grad[2 * g] = wl[g].x;
+      o->wl_grad[2 * g + 1] = wl[g].y;
+    }
+    if (o->dens_grad) {
+      o->dens_grad[2 * g] = dn[g].x;
+      o->dens_grad[2 * g + 1] = dn[g].y;
+    }
+  }
+  double sums[3] = {0, 0, 0};
+  for (int k = 0; k < c->h_state->num_partials_grad; k++) {
+    sums[0] += pg[(size_t)k * RP_P_COUNT + RP_P_WL];
+    sums[1] += pg[(size_t)k * RP_P_COUNT + RP_P_DENS];
+    sums[2] += pg[(size_t)k * RP_P_COUNT + RP_P_GRAD];
+  }
+  if (o->sums) memcpy(o->sums, sums, sizeof sums);
+  if (o->overflow_area) {
+    double ov = 0;
+    for (double v : pb) ov += v;
+    *o->overflow_area = (int64_t)ov;
+  }
+  if (o->hpwl) {
+    long long h = 0;
+    for (long long v : pn) h += v;
+    *o->hpwl = h;
+  }
+  // nets back in reference order
+  for (int k = 0; k < nn; k++) {
+    int e = c->net_perm[k];
+    if (o->net_sums) {
+      float* q = o->net_sums + (size_t)8 * e;
+      q[0] = nsum[2 * k].x;
+      q[1] = nsum[2 * k].y;
+      q[2] = nsum[2 * k].z;
+      q[3] = nsum[2 * k].w;
+      q[4] = nsum[2 * k + 1].x;
+      q[5] = nsum[2 * k + 1].y;
+      q[6] = nsum[2 * k + 1].z;
+      q[7] = nsum[2 * k + 1].w;
+    }
+    if (o->net_box) {
+      int32_t* q = o->net_box + (size_t)4 * e;
+      q[0] = nbox[k].x;
+      q[1] = nbox[k].y;
+      q[2] = nbox[k].z;
+      q[3] = nbox[k].w;
+    }
+  }
+  return RP_OK;
+}
+
+extern "C" int rp_eval_poisson(rp_ctx* c, const float* density, float* phi, float* ex, float* ey) {
+  if (!c || !c->nb_ready || !density) return RP_ERR_STATE;
+  const int nb = c->nbx * c->nby;
+  hipStream_t s = c->stream;
+  RpState saved = *c->h_state;
+  RpState S = saved;
+  S.phase = 0;
+  HIPCHK(c, hipMemcpyAsync(c->d_state.p, &S, sizeof S, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(c->d_rho.p, density, sizeof(float) * nb, hipMemcpyHostToDevice, s));
+  HIPCHK(c, rp_launch_poisson(c->dev, s, 1, 1));
+  std::vector<float2> field(nb);
+  HIPCHK(c, hipMemcpyAsync(field.data(), c->d_field.p, sizeof(float2) * nb, hipMemcpyDeviceToHost, s));
+  if (phi) HIPCHK(c, hipMemcpyAsync(phi, c->d_phi.p, sizeof(float) * nb, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(c->d_state.p, &saved, sizeof saved, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  for (int b = 0; b < nb; b++) {
+    if (ex) ex[b] = field[b].x;
+    if (ey) ey[b] = field[b].y;
+  }
+  return RP_OK;
+}
+
+// ===========================================================================
+// Nesterov loop
+// ===========================================================================
+static int slots_per_poll(const rp_ctx* c) {
+  const char* e = getenv("RP_SLOTS_PER_POLL");
+  if (e && atoi(e) > 0) return atoi(e);
+  int ng = c->n_g;
+  return ng >= 200000 ? 8 : 32;
+}
+
+extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
+  if (!c) return RP_ERR_ARG;
+  if (!c->nb_ready) {
+    int r = rp_nesterov_setup(c, nullptr);
+    if (r) return r;
+  }
+  hipStream_t s = c->stream;
+  const int ng = c->n_g;
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  // NesterovPlace::init (src/nesterovPlace.cpp:64-175)
+  RpState& S = *c->h_state;
+  S.phase = 0;
+  S.code = 0;
+  S.evals = 0;
+  S.diverged = 0;
+  S.mode = 1;
+  S.ring.cur = 0;
+  S.ring.next = 1;
+  S.ring.prev = 2;
+  S.ring.co_cur = 0;
+  S.ring.co_next = 1;
+  S.dst = S.ring.cur;
+  S.density_penalty = 0.0f;  // densityPenalty_(0) during both init evaluations
+  S.max_phi_coef = c->prm.max_phi_coef;
+  S.step_length = 0.0f;
+  S.coeff = 0.0f;
+  std::vector<float2> xy(ng);
+  for (int g = 0; g < ng; g++) xy[g] = make_float2(c->g_icx[g], c->g_icy[g]);
+  HIPCHK(c, hipEventRecord(e0, s));
+  for (int slot : {0, 2}) {  // curSLP = prevSLP = initial density centres
+    HIPCHK(c, hipMemcpyAsync(c->d_slp.p + (size_t)slot * ng, xy.data(), sizeof(float2) * ng, hipMemcpyHostToDevice, s));
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_co.p, xy.data(), sizeof(float2) * ng, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(c->d_state.p, &S, sizeof S, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemsetAsync(c->d_bin_inst.p, 0, sizeof(unsigned long long) * c->nbx * c->nby, s));
+  HIPCHK(c, hipMemsetAsync(c->d_bin_filler.p, 0, sizeof(unsigned long long) * c->nbx * c->nby, s));
+  const RpDev& d = c->dev;
+  // eval A at curSLP with lambda 0
+  HIPCHK(c, rp_launch_scatter(d, s));
+  HIPCHK(c, rp_launch_poisson(d, s, 0, 0));
+  HIPCHK(c, rp_launch_ctrl_init(d, s, 0));
+  HIPCHK(c, rp_launch_wa_grad(d, s));
+  HIPCHK(c, rp_launch_ctrl_init(d, s, 1));
+  // eval B at prevSLP = clamp(curSLP + 100 * g)
+  HIPCHK(c, rp_launch_scatter(d, s));
+  HIPCHK(c, rp_launch_poisson(d, s, 0, 0));
+  HIPCHK(c, rp_launch_wa_grad(d, s));
+  HIPCHK(c, rp_launch_ctrl_init(d, s, 2));
+  // main loop: queue evaluation slots, poll the state between batches
+  const int K = slots_per_poll(c);
+  RpState* hs = c->h_state;
+  hipEvent_t polled;
+  HIPCHK(c, hipEventCreateWithFlags(&polled, hipEventDisableTiming));
+  int max_batches = (c->prm.nesterov_place_max_iter * std::max(1, c->prm.max_back_track)) / K + 4;
+  for (int b = 0; b < max_batches; b++) {
+    for (int k = 0; k < K; k++) HIPCHK(c, rp_launch_eval(d, s, 0));
+    HIPCHK(c, hipMemcpyAsync(hs, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipEventRecord(polled, s));
+    HIPCHK(c, hipEventSynchronize(polled));
+    if (hs->phase != 0) break;
+  }
+  HIPCHK(c, hipEventRecord(e1, s));
+  HIPCHK(c, hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(polled);
+  HIPCHK(c, hipMemcpy(hs, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost));
+  int nrec = hs->iter;
+  c->records.resize(nrec);
+  if (nrec > 0) HIPCHK(c, hipMemcpy(c->records.data(), c->d_rec.p, sizeof(RpIterRec) * nrec, hipMemcpyDeviceToHost));
+  // updateDb (src/nesterovPlace.cpp:585-595): LL = dC - dD/2 of the last evaluated iterate
+  std::vector<int2> ic(ng);
+  HIPCHK(c, hipMemcpy(ic.data(), c->d_ic.p, sizeof(int2) * ng, hipMemcpyDeviceToHost));
+  for (int g = 0; g < c->n_gi; g++) {
+    int i = c->g_inst[g];
+    c->dblx[i] = ic[g].x - c->g_dhx[g];
+    c->dbly[i] = ic[g].y - c->g_dhy[g];
+  }
+  c->placed = true;
+  int code = hs->phase == 2 ? hs->code : 0;
+  if (hs->phase == 2 && hs->code == 2) code = 2;
+  rp_nesterov_stats& out = c->last_stats;
+  out.iterations = hs->iter;
+  out.evals = hs->evals;
+  out.finished = hs->phase == 1;
+  out.code = code;
+  out.final_overflow = hs->sum_overflow;
+  out.final_step_length = hs->step_length;
+  out.final_density_penalty = hs->density_penalty;
+  out.final_hpwl = hs->prev_hpwl;
+  out.seconds = ms * 1e-3;
+  if (st) *st = out;
+  return code;
+}
+
+extern "C" int rp_get_iter_records(rp_ctx* c, rp_iter_record* out, int32_t cap, int32_t* count) {
+  if (!c || !count) return RP_ERR_ARG;
+  *count = (int32_t)c->records.size();
+  int n = std::min<int>(cap, *count);
+  for (int k = 0; k < n && out; k++) {
+    out[k].overflow = c->records[k].overflow;
+    out[k].step_length = c->records[k].step_length;
+    out[k].density_penalty = c->records[k].density_penalty;
+    out[k].num_back_track = c->records[k].num_back_track;
+    out[k].hpwl = c->records[k].hpwl;
+  }
+  return RP_OK;
+}
+
+extern "C" int rp_get_positions(rp_ctx* c, int32_t* lx, int32_t* ly) {
+  if (!c || !lx || !ly) return RP_ERR_ARG;
+  for (int i = 0; i < c->n_real; i++) {
+    lx[i] = c->dblx[i];
+    ly[i] = c->dbly[i];
+  }
+  return RP_OK;
+}
+
+extern "C" int rp_set_positions(rp_ctx* c, const int32_t* lx, const int32_t* ly) {
+  if (!c || !lx || !ly) return RP_ERR_ARG;
+  for (int i = 0; i < c->n_real; i++) {
+    int w = c->iux[i] - c->ilx[i], h = c->iuy[i] - c->ily[i];
+    c->ilx[i] = lx[i];
+    c->ily[i] = ly[i];
+    c->iux[i] = lx[i] + w;
+    c->iuy[i] = ly[i] + h;
+    c->dblx[i] = lx[i];
+    c->dbly[i] = ly[i];
+    inst_update_pins(c, i);
+  }
+  c->nb_ready = false;
+  return RP_OK;
+}

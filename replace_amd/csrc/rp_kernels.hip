@@ -1,0 +1,1049 @@
+// rp_kernels.hip -- gfx950 kernels of one Nesterov gradient evaluation and
+// its on-device controller.
+//
+// One evaluation (reference src/nesterovPlace.cpp:338-377) is the launch
+// sequence
+//   k_step_scatter      step + clamp + int centre + density scatter (A14, A5, A6)
+//   k_binfin_rowdct     bin density, overflow partials, DCT-II along x (A6 tail, A8)
+//   k_transpose         (Ny x Nx) -> (Nx x Ny)
+//   k_col_spectral      DCT-II along y, spectral divide, DCT-III / DST-III along y (A8)
+//   k_transpose2        both y-inverse results back to (Ny x Nx)
+//   k_row_inverse       DST-III / DCT-III along x -> field (Ex, Ey) (A8/A9)
+//   k_wa                net bbox, WA exponent sums, HPWL partials (A10, A16)
+//   k_grad              WL gradient, field gather, preconditioner, partial sums (A11-A13, A15)
+//   k_control_main      reductions + back-tracking / schedules / stop test (A15, A16, A18)
+// Every kernel reads the controller state first and returns when the run
+// has stopped, so the host can queue evaluation slots (or replay a graph of
+// them) without knowing how many back-tracking trials an iteration takes.
+//
+// Arithmetic follows the reference's float / double promotions and must be
+// built with -ffp-contract=off (no FMA contraction).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <limits.h>
+
+#include "rp_device.h"
+
+#include "rp_dev_struct.h"
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// fastExp, reference src/nesterovBase.cpp:1568-1582
+__device__ __forceinline__ float fast_exp(float a) {
+  a = (float)(1.0 + (double)a / 1024.0);
+  a = a * a; a = a * a; a = a * a; a = a * a; a = a * a;
+  a = a * a; a = a * a; a = a * a; a = a * a; a = a * a;
+  return a;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Fixed-order block sum (deterministic); result valid in thread 0.
+__device__ double block_sum_d(double v, double* sh) {
+  v = wave_sum_d(v);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) r += sh[k];
+  __syncthreads();
+  return r;
+}
+
+__device__ long long block_sum_ll(long long v, long long* sh) {
+  v = wave_sum_ll(v);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  long long r = 0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) r += sh[k];
+  __syncthreads();
+  return r;
+}
+
+// Bin geometry of BinGrid::initBins (src/nesterovBase.cpp:674-711): regular
+// bins of binSize, the last column / row truncated at the core edge.
+__device__ __forceinline__ void bin_x(const RpDev& d, int i, int& lx, int& ux) {
+  lx = d.core_lx + i * d.bsx;
+  ux = min(lx + d.bsx, d.core_ux);
+}
+__device__ __forceinline__ void bin_y(const RpDev& d, int j, int& ly, int& uy) {
+  ly = d.core_ly + j * d.bsy;
+  uy = min(ly + d.bsy, d.core_uy);
+}
+
+// getDensityMinMaxIdxX/Y (src/nesterovBase.cpp:822-841), clamped to the grid
+__device__ __forceinline__ void idx_range(int lo_c, int hi_c, int core_l, int bs, int cnt, int& a,
+                                          int& b) {
+  a = (lo_c - core_l) / bs;
+  int r = hi_c - core_l;
+  int fm = r >= bs ? r % bs : r;
+  b = (fm == 0) ? r / bs : r / bs + 1;
+  a = max(a, 0);
+  b = min(b, cnt);
+}
+
+// getDensityCoordiLayoutInsideX/Y (src/nesterovBase.cpp:1270-1297)
+__device__ __forceinline__ float clamp_in(float c, int h, int lo, int hi) {
+  float v = c;
+  if (c - (float)h < (float)lo) v = (float)(lo + h);
+  if (c + (float)h > (float)hi) v = (float)(hi - h);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// radix-2 Stockham FFT in LDS (natural order in, natural order out).
+// dir = -1 forward (e^{-i}), +1 inverse (e^{+i}), unnormalised.
+// Returns the buffer holding the result.
+// ---------------------------------------------------------------------------
+__device__ float2* lds_fft(float2* a, float2* b, int n, const float2* __restrict__ tw, int dir) {
+  float2* in = a;
+  float2* out = b;
+  int half = n >> 1;
+  for (int ns = 1; ns < n; ns <<= 1) {
+    int tstride = n / (2 * ns);
+    for (int j = threadIdx.x; j < half; j += blockDim.x) {
+      int k = j & (ns - 1);
+      float2 u = in[j];
+      float2 v = in[j + half];
+      float2 w = tw[k * tstride];
+      if (dir > 0) w.y = -w.y;
+      float2 tv = cmul(v, w);
+      int dst = ((j - k) << 1) + k;
+      out[dst] = make_float2(u.x + tv.x, u.y + tv.y);
+      out[dst + ns] = make_float2(u.x - tv.x, u.y - tv.y);
+    }
+    __syncthreads();
+    float2* s = in;
+    in = out;
+    out = s;
+  }
+  return in;
+}
+
+// Makhoul input reorder index: x[i] goes to v[pos(i)]
+__device__ __forceinline__ int mk_pos(int i, int n) { return (i & 1) ? (n - 1 - (i >> 1)) : (i >> 1); }
+
+// After a forward FFT of z = v_a + i v_b (each Makhoul-reordered), output
+// the two DCT-II spectra: X[k] = sum_n x[n] cos(pi (n+1/2) k / N).
+__device__ __forceinline__ void dct2_post(const float2* Z, int n, int k, const float2* __restrict__ tw2,
+                                          float& xa, float& xb) {
+  float2 zk = Z[k];
+  float2 zm = Z[(n - k) & (n - 1)];
+  zm.y = -zm.y;  // conj
+  float2 va = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
+  float dr = zk.x - zm.x, di = zk.y - zm.y;
+  float2 vb = make_float2(0.5f * di, -0.5f * dr);
+  float2 w = tw2[k];  // (cos t, -sin t), t = pi k / 2N
+  xa = va.x * w.x - va.y * w.y;
+  xb = vb.x * w.x - vb.y * w.y;
+}
+
+// Pre-twiddle of the inverse Makhoul (DCT-III, Ooura "IDCT" without scale):
+// W[k] = 1/2 e^{i pi k/2N} (a[k] - i a[N-k]) (k >= 1), W[0] = a[0];
+// returns W_a + i W_b for two real coefficient rows read through getters.
+__device__ __forceinline__ float2 dct3_pre(float a_k, float a_nk, float b_k, float b_nk, int k,
+                                           const float2* __restrict__ tw2) {
+  float2 wa, wb;
+  if (k == 0) {
+    wa = make_float2(a_k, 0.0f);
+    wb = make_float2(b_k, 0.0f);
+  } else {
+    float2 w = tw2[k];
+    float c = w.x, s = -w.y;  // e^{+i t} = (c, s)
+    wa = make_float2(0.5f * (a_k * c + a_nk * s), 0.5f * (a_k * s - a_nk * c));
+    wb = make_float2(0.5f * (b_k * c + b_nk * s), 0.5f * (b_k * s - b_nk * c));
+  }
+  return make_float2(wa.x - wb.y, wa.y + wb.x);
+}
+
+// ---------------------------------------------------------------------------
+// 1. step + clamp + integer centre + density scatter
+// ---------------------------------------------------------------------------
+#define AGG_SLOTS 1024
+
+__device__ __forceinline__ int agg_slot(int b) { return (int)(((unsigned)b * 2654435761u) >> 22); }
+
+__global__ void __launch_bounds__(RP_BLOCK) k_step_scatter(RpDev d) {
+  __shared__ int s_key[AGG_SLOTS];
+  __shared__ unsigned long long s_inst[AGG_SLOTS];
+  __shared__ unsigned long long s_fill[AGG_SLOTS];
+  const RpState* st = d.st;
+  if (st->phase != 0) return;
+  for (int k = threadIdx.x; k < AGG_SLOTS; k += blockDim.x) {
+    s_key[k] = -1;
+    s_inst[k] = 0;
+    s_fill[k] = 0;
+  }
+  __syncthreads();
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < d.nc) {
+    int2 h = d.dhalf[c];
+    int dst = st->dst;
+    float2 p;
+    if (st->mode == 0) {
+      int cur = st->ring.cur;
+      float2 x = d.slp[(size_t)cur * d.nc + c];
+      float2 g = d.sumg[(size_t)cur * d.nc + c];
+      float2 o = d.co[(size_t)st->ring.co_cur * d.nc + c];
+      float sl = st->step_length, cf = st->coeff;
+      // src/nesterovPlace.cpp:339-361 (no FMA: built with -ffp-contract=off)
+      float nx = x.x + sl * g.x;
+      float ny = x.y + sl * g.y;
+      float sx = nx + cf * (nx - o.x);
+      float sy = ny + cf * (ny - o.y);
+      float2 nco = make_float2(clamp_in(nx, h.x, d.core_lx, d.core_ux), clamp_in(ny, h.y, d.core_ly, d.core_uy));
+      p = make_float2(clamp_in(sx, h.x, d.core_lx, d.core_ux), clamp_in(sy, h.y, d.core_ly, d.core_uy));
+      d.co[(size_t)st->ring.co_next * d.nc + c] = nco;
+      d.slp[(size_t)dst * d.nc + c] = p;
+    } else {
+      p = d.slp[(size_t)dst * d.nc + c];
+    }
+    // setDensityCenterLocation(int, int): float -> int truncation (src/nesterovBase.cpp:1203)
+    int2 icc = make_int2((int)p.x, (int)p.y);
+    d.ic[c] = icc;
+    int dlx = icc.x - h.x, dux = icc.x + h.x, dly = icc.y - h.y, duy = icc.y + h.y;
+    int x0, x1, y0, y1;
+    idx_range(dlx, dux, d.core_lx, d.bsx, d.nbx, x0, x1);
+    idx_range(dly, duy, d.core_ly, d.bsy, d.nby, y0, y1);
+    float sc = d.scale[c];
+    uint8_t kd = d.kind[c];
+    for (int i = x0; i < x1; i++) {
+      int blx, bux;
+      bin_x(d, i, blx, bux);
+      int w = min(bux, dux) - max(blx, dlx);
+      if (w <= 0) continue;
+      for (int j = y0; j < y1; j++) {
+        int bly, buy;
+        bin_y(d, j, bly, buy);
+        int hh = min(buy, duy) - max(bly, dly);
+        if (hh <= 0) continue;
+        // getOverlapDensityArea * densityScale (* td for macros), truncated to int64
+        float v = (float)w * (float)hh;
+        v = v * sc;
+        if (kd == 1) v = v * d.td;
+        unsigned long long a = (unsigned long long)(long long)v;
+        if (a == 0) continue;
+        int b = j * d.nbx + i;
+        int slot = agg_slot(b);
+        int old = atomicCAS(&s_key[slot], -1, b);
+        if (old == -1 || old == b) {
+          atomicAdd(kd == 2 ? &s_fill[slot] : &s_inst[slot], a);
+        } else {
+          atomicAdd(kd == 2 ? &d.bin_filler[b] : &d.bin_inst[b], a);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < AGG_SLOTS; k += blockDim.x) {
+    int b = s_key[k];
+    if (b >= 0) {
+      if (s_inst[k]) atomicAdd(&d.bin_inst[b], s_inst[k]);
+      if (s_fill[k]) atomicAdd(&d.bin_filler[b], s_fill[k]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2. bin density + overflow + DCT-II along x (two rows per workgroup)
+// ---------------------------------------------------------------------------
+__global__ void k_binfin_rowdct(RpDev d, int from_rho) {
+  extern __shared__ float2 smem[];
+  __shared__ double sh_red[RP_BLOCK / 64];
+  if (d.st->phase != 0) return;
+  const int n = d.nbx;
+  float2* A = smem;
+  float2* B = smem + n;
+  int ja = 2 * blockIdx.x, jb = ja + 1;
+  double ov_sum = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    int blx, bux;
+    bin_x(d, i, blx, bux);
+    float r2[2];
+    for (int r = 0; r < 2; r++) {
+      int j = ja + r;
+      int bly, buy;
+      bin_y(d, j, bly, buy);
+      size_t b = (size_t)j * n + i;
+      if (from_rho) {  // Poisson-only entry: density given
+        r2[r] = d.rho[b];
+        continue;
+      }
+      long long inst = (long long)d.bin_inst[b];
+      long long fil = (long long)d.bin_filler[b];
+      long long np = d.bin_nonplace[b];
+      d.bin_inst[b] = 0ull;
+      d.bin_filler[b] = 0ull;
+      long long bin_area = (long long)(bux - blx) * (long long)(buy - bly);
+      float at = (float)bin_area * d.td;
+      // src/nesterovBase.cpp:806-816
+      float rho = ((float)inst + (float)fil + (float)np) / at;
+      float ov = (float)inst + (float)np - at;
+      ov = (0.0f < ov) ? ov : 0.0f;
+      ov_sum += (double)ov;
+      d.rho[b] = rho;
+      r2[r] = rho;
+    }
+    A[mk_pos(i, n)] = make_float2(r2[0], r2[1]);
+  }
+  double tot = block_sum_d(ov_sum, sh_red);
+  if (threadIdx.x == 0) d.part_bin[blockIdx.x] = tot;
+  // block_sum_d synchronised the LDS writes above
+  float2* Z = lds_fft(A, B, n, d.twx, -1);
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    float xa, xb;
+    dct2_post(Z, n, k, d.tw2x, xa, xb);
+    d.t[(size_t)ja * n + k] = xa;
+    d.t[(size_t)jb * n + k] = xb;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3. tiled transposes
+// ---------------------------------------------------------------------------
+__global__ void k_transpose(const float* __restrict__ src, float* __restrict__ dst, int rows, int cols,
+                            const RpState* st) {
+  __shared__ float tile[32][33];
+  if (st->phase != 0) return;
+  int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  int tx = threadIdx.x, ty = threadIdx.y;
+  for (int k = 0; k < 32; k += 8) {
+    int r = by + ty + k, c = bx + tx;
+    if (r < rows && c < cols) tile[ty + k][tx] = src[(size_t)r * cols + c];
+  }
+  __syncthreads();
+  for (int k = 0; k < 32; k += 8) {
+    int r = bx + ty + k, c = by + tx;
+    if (r < cols && c < rows) dst[(size_t)r * rows + c] = tile[tx][ty + k];
+  }
+}
+
+__global__ void k_transpose2(const float* __restrict__ s1, float* __restrict__ d1, const float* __restrict__ s2,
+                             float* __restrict__ d2, int rows, int cols, const RpState* st) {
+  __shared__ float tile[32][33];
+  if (st->phase != 0) return;
+  const float* src = blockIdx.z ? s2 : s1;
+  float* dst = blockIdx.z ? d2 : d1;
+  int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  int tx = threadIdx.x, ty = threadIdx.y;
+  for (int k = 0; k < 32; k += 8) {
+    int r = by + ty + k, c = bx + tx;
+    if (r < rows && c < cols) tile[ty + k][tx] = src[(size_t)r * cols + c];
+  }
+  __syncthreads();
+  for (int k = 0; k < 32; k += 8) {
+    int r = bx + ty + k, c = by + tx;
+    if (r < cols && c < rows) dst[(size_t)r * rows + c] = tile[tx][ty + k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 4. column pass on the transposed spectrum rows (u fixed, v along the row):
+// DCT-II along y, FFT::doFFT's scaling and spectral divide
+// (src/fft.cpp:117-166), then DCT-III (phi) and DST-III (Ey) along y.
+// ---------------------------------------------------------------------------
+__global__ void k_col_spectral(RpDev d) {
+  extern __shared__ float2 smem[];
+  if (d.st->phase != 0) return;
+  const int n = d.nby;
+  float2* A = smem;
+  float2* B = smem + n;
+  float* pa = (float*)(smem + 2 * n);  // phi hat rows a, b; Ey hat rows a, b
+  float* pb = pa + n;
+  float* ea = pb + n;
+  float* eb = ea + n;
+  int ua = 2 * blockIdx.x, ub = ua + 1;
+  for (int v = threadIdx.x; v < n; v += blockDim.x)
+    A[mk_pos(v, n)] = make_float2(d.tt[(size_t)ua * n + v], d.tt[(size_t)ub * n + v]);
+  __syncthreads();
+  float2* Z = lds_fft(A, B, n, d.twy, -1);
+  const float scale4 = (float)(4.0 / (double)d.nbx / (double)n);
+  for (int v = threadIdx.x; v < n; v += blockDim.x) {
+    float xa, xb;
+    dct2_post(Z, n, v, d.tw2y, xa, xb);
+    float wy = d.wy[v], wy2 = d.wy2[v];
+    float f[2] = {xa, xb};
+    int us[2] = {ua, ub};
+    for (int r = 0; r < 2; r++) {
+      int u = us[r];
+      float F = f[r];
+      if (v == 0) F *= 0.5f;
+      if (u == 0) F *= 0.5f;
+      F = F * scale4;
+      float ph = 0.0f, ey = 0.0f;
+      if (!(u == 0 && v == 0)) {
+        ph = F / (d.wx2[u] + wy2);
+        ey = ph * wy;
+      }
+      (r ? pb : pa)[v] = ph;
+      (r ? eb : ea)[v] = ey;
+    }
+  }
+  __syncthreads();
+  // DCT-III of phi hat (rows a, b) -> y1t
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    int nk = (n - k) & (n - 1);
+    float ank = k ? pa[nk] : 0.0f, bnk = k ? pb[nk] : 0.0f;
+    A[k] = dct3_pre(pa[k], ank, pb[k], bnk, k, d.tw2y);
+  }
+  __syncthreads();
+  Z = lds_fft(A, B, n, d.twy, +1);
+  for (int m = threadIdx.x; m < n; m += blockDim.x) {
+    // y[2q] = w[q], y[2q+1] = w[N-1-q]
+    float2 z = Z[(m & 1) ? (n - 1 - (m >> 1)) : (m >> 1)];
+    d.y1t[(size_t)ua * n + m] = z.x;
+    d.y1t[(size_t)ub * n + m] = z.y;
+  }
+  __syncthreads();
+  // DST-III of Ey hat: b'[0] = a[0] (the A[N] slot), b'[j] = a[N-j]; sign (-1)^k
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    int nk = (n - k) & (n - 1);
+    float ak = k ? ea[n - k] : ea[0];
+    float bk = k ? eb[n - k] : eb[0];
+    float ank = k ? (nk ? ea[n - nk] : ea[0]) : 0.0f;
+    float bnk = k ? (nk ? eb[n - nk] : eb[0]) : 0.0f;
+    A[k] = dct3_pre(ak, ank, bk, bnk, k, d.tw2y);
+  }
+  __syncthreads();
+  Z = lds_fft(A, B, n, d.twy, +1);
+  for (int m = threadIdx.x; m < n; m += blockDim.x) {
+    float2 z = Z[(m & 1) ? (n - 1 - (m >> 1)) : (m >> 1)];
+    float sg = (m & 1) ? -1.0f : 1.0f;
+    d.y2t[(size_t)ua * n + m] = sg * z.x;
+    d.y2t[(size_t)ub * n + m] = sg * z.y;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 5. inverse along x: Ex = DST-III_x(wx * Y1), Ey = DCT-III_x(Y2), phi optional
+// ---------------------------------------------------------------------------
+__global__ void k_row_inverse(RpDev d, int want_phi) {
+  extern __shared__ float2 smem[];
+  if (d.st->phase != 0) return;
+  const int n = d.nbx;
+  float2* A = smem;
+  float2* B = smem + n;
+  float* ra = (float*)(smem + 2 * n);
+  float* rb = ra + n;
+  int ja = 2 * blockIdx.x, jb = ja + 1;
+  // Ex: DST-III of (wx[u] * Y1[j][u])
+  for (int u = threadIdx.x; u < n; u += blockDim.x) {
+    ra[u] = d.wx[u] * d.y1[(size_t)ja * n + u];
+    rb[u] = d.wx[u] * d.y1[(size_t)jb * n + u];
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    int nk = (n - k) & (n - 1);
+    float ak = k ? ra[n - k] : ra[0];
+    float bk = k ? rb[n - k] : rb[0];
+    float ank = k ? (nk ? ra[n - nk] : ra[0]) : 0.0f;
+    float bnk = k ? (nk ? rb[n - nk] : rb[0]) : 0.0f;
+    A[k] = dct3_pre(ak, ank, bk, bnk, k, d.tw2x);
+  }
+  __syncthreads();
+  float2* Z = lds_fft(A, B, n, d.twx, +1);
+  for (int m = threadIdx.x; m < n; m += blockDim.x) {
+    float2 z = Z[(m & 1) ? (n - 1 - (m >> 1)) : (m >> 1)];
+    float sg = (m & 1) ? -1.0f : 1.0f;
+    ra[m] = sg * z.x;  // Ex row a (ra no longer needed as input)
+    rb[m] = sg * z.y;
+  }
+  __syncthreads();
+  // Ey: DCT-III of Y2 rows
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    int nk = (n - k) & (n - 1);
+    const float* qa = d.y2 + (size_t)ja * n;
+    const float* qb = d.y2 + (size_t)jb * n;
+    A[k] = dct3_pre(qa[k], k ? qa[nk] : 0.0f, qb[k], k ? qb[nk] : 0.0f, k, d.tw2x);
+  }
+  __syncthreads();
+  Z = lds_fft(A, B, n, d.twx, +1);
+  for (int m = threadIdx.x; m < n; m += blockDim.x) {
+    float2 z = Z[(m & 1) ? (n - 1 - (m >> 1)) : (m >> 1)];
+    d.field[(size_t)ja * n + m] = make_float2(ra[m], z.x);
+    d.field[(size_t)jb * n + m] = make_float2(rb[m], z.y);
+  }
+  if (!want_phi) return;
+  __syncthreads();
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    int nk = (n - k) & (n - 1);
+    const float* qa = d.y1 + (size_t)ja * n;
+    const float* qb = d.y1 + (size_t)jb * n;
+    A[k] = dct3_pre(qa[k], k ? qa[nk] : 0.0f, qb[k], k ? qb[nk] : 0.0f, k, d.tw2x);
+  }
+  __syncthreads();
+  Z = lds_fft(A, B, n, d.twx, +1);
+  for (int m = threadIdx.x; m < n; m += blockDim.x) {
+    float2 z = Z[(m & 1) ? (n - 1 - (m >> 1)) : (m >> 1)];
+    d.phi[(size_t)ja * n + m] = z.x;
+    d.phi[(size_t)jb * n + m] = z.y;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 6. WA exponent sums per net (src/nesterovBase.cpp:1304-1360) + HPWL
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int2 pin_pos(const RpDev& d, int q) {
+  int c = d.pcell[q];
+  int2 o = d.ppos[q];
+  if (c >= 0) {
+    int2 b = d.ic[c];
+    return make_int2(b.x + o.x, b.y + o.y);
+  }
+  return o;
+}
+
+__global__ void __launch_bounds__(RP_BLOCK) k_wa(RpDev d) {
+  __shared__ long long sh_red[RP_BLOCK / 64];
+  const RpState* st = d.st;
+  if (st->phase != 0) return;
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  long long h = 0;
+  if (e < d.nn) {
+    float cx = st->wl_coef_x, cy = st->wl_coef_y;
+    int s = d.nstart[e], t = d.nstart[e + 1];
+    int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
+    for (int q = s; q < t; q++) {
+      int2 p = pin_pos(d, q);
+      lx = min(p.x, lx);
+      ly = min(p.y, ly);
+      ux = max(p.x, ux);
+      uy = max(p.y, uy);
+    }
+    float s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0;
+    for (int q = s; q < t; q++) {
+      int2 p = pin_pos(d, q);
+      float eMinX = (float)(lx - p.x) * cx;
+      float eMaxX = (float)(p.x - ux) * cx;
+      float eMinY = (float)(ly - p.y) * cy;
+      float eMaxY = (float)(p.y - uy) * cy;
+      if (eMinX > d.bar) {
+        float v = fast_exp(eMinX);
+        s0 += v;
+        s1 += (float)p.x * v;
+      }
+      if (eMaxX > d.bar) {
+        float v = fast_exp(eMaxX);
+        s2 += v;
+        s3 += (float)p.x * v;
+      }
+      if (eMinY > d.bar) {
+        float v = fast_exp(eMinY);
+        s4 += v;
+        s5 += (float)p.y * v;
+      }
+      if (eMaxY > d.bar) {
+        float v = fast_exp(eMaxY);
+        s6 += v;
+        s7 += (float)p.y * v;
+      }
+    }
+    d.nbox[e] = make_int4(lx, ly, ux, uy);
+    d.nsum[2 * (size_t)e] = make_float4(s0, s1, s2, s3);
+    d.nsum[2 * (size_t)e + 1] = make_float4(s4, s5, s6, s7);
+    if (t > s) h = (long long)((ux - lx) + (uy - ly));
+  }
+  long long tot = block_sum_ll(h, sh_red);
+  if (threadIdx.x == 0) d.part_net[blockIdx.x] = tot;
+}
+
+// ---------------------------------------------------------------------------
+// 7. gradients per gcell (src/nesterovPlace.cpp:206-273)
+// ---------------------------------------------------------------------------
+// getWireLengthGradientPinWA for one pin and one axis (src/nesterovBase.cpp:1383-1440):
+// returns gMin - gMax with the exponentials recomputed as the WA pass stored them.
+__device__ __forceinline__ float wl_axis(int p, int lo, int hi, float coef, float S0, float XS0, float S1,
+                                         float XS1, float bar) {
+  float gmin = 0.0f, gmax = 0.0f;
+  float e0 = (float)(lo - p) * coef;
+  if (e0 > bar) {
+    float ex = fast_exp(e0);
+    gmin = (float)(((double)S0 * ((double)ex * (1.0 - (double)(coef * (float)p))) + (double)(coef * ex * XS0)) /
+                   (double)(S0 * S0));
+  }
+  float e1 = (float)(p - hi) * coef;
+  if (e1 > bar) {
+    float ex = fast_exp(e1);
+    gmax = (float)(((double)S1 * ((double)ex * (1.0 + (double)(coef * (float)p))) - (double)(coef * ex * XS1)) /
+                   (double)(S1 * S1));
+  }
+  return gmin - gmax;
+}
+
+__global__ void __launch_bounds__(RP_BLOCK) k_grad(RpDev d) {
+  __shared__ double sh_red[RP_BLOCK / 64];
+  const RpState* st = d.st;
+  if (st->phase != 0) return;
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double v_wl = 0.0, v_dens = 0.0, v_grad = 0.0, v_dc = 0.0, v_dg = 0.0;
+  if (c < d.nc) {
+    float cx = st->wl_coef_x, cy = st->wl_coef_y;
+    float lam = st->density_penalty;
+    int dst = st->dst, ref = st->ring.cur;
+    int2 icc = d.ic[c];
+    // wire length
+    float gx = 0.0f, gy = 0.0f;
+    int k0 = d.cp_start[c], k1 = d.cp_start[c + 1];
+    for (int k = k0; k < k1; k++) {
+      int e = d.cp_net[k];
+      int2 o = d.cp_off[k];
+      int px = icc.x + o.x, py = icc.y + o.y;
+      int4 bx = d.nbox[e];
+      float4 sxv = d.nsum[2 * (size_t)e];
+      float4 syv = d.nsum[2 * (size_t)e + 1];
+      float tx = wl_axis(px, bx.x, bx.z, cx, sxv.x, sxv.y, sxv.z, sxv.w, d.bar);
+      float ty = wl_axis(py, bx.y, bx.w, cy, syv.x, syv.y, syv.z, syv.w, d.bar);
+      gx += tx;
+      gy += ty;
+    }
+    // field gather (getDensityGradient, src/nesterovBase.cpp:1458-1478)
+    int2 h = d.dhalf[c];
+    int dlx = icc.x - h.x, dux = icc.x + h.x, dly = icc.y - h.y, duy = icc.y + h.y;
+    int x0, x1, y0, y1;
+    idx_range(dlx, dux, d.core_lx, d.bsx, d.nbx, x0, x1);
+    idx_range(dly, duy, d.core_ly, d.bsy, d.nby, y0, y1);
+    float sc = d.scale[c];
+    float fx = 0.0f, fy = 0.0f;
+    for (int i = x0; i < x1; i++) {
+      int blx, bux;
+      bin_x(d, i, blx, bux);
+      int w = min(bux, dux) - max(blx, dlx);
+      for (int j = y0; j < y1; j++) {
+        int bly, buy;
+        bin_y(d, j, bly, buy);
+        int hh = min(buy, duy) - max(bly, dly);
+        float ov = (w <= 0 || hh <= 0) ? 0.0f : (float)w * (float)hh;
+        ov = ov * sc;
+        float2 f = d.field[(size_t)j * d.nbx + i];
+        fx += ov * f.x;
+        fy += ov * f.y;
+      }
+    }
+    // sum gradient and preconditioner
+    float sx = gx + lam * fx;
+    float sy = gy + lam * fy;
+    float np = d.npin[c], ar = d.area[c];
+    float ppx = np + lam * ar;
+    float ppy = np + lam * ar;
+    if (ppx <= d.min_precond) ppx = d.min_precond;
+    if (ppy <= d.min_precond) ppy = d.min_precond;
+    sx /= ppx;
+    sy /= ppy;
+    d.sumg[(size_t)dst * d.nc + c] = make_float2(sx, sy);
+    if (d.dbg_wl) d.dbg_wl[c] = make_float2(gx, gy);
+    if (d.dbg_dens) d.dbg_dens[c] = make_float2(fx, fy);
+    v_wl = (double)fabsf(gx) + (double)fabsf(gy);
+    v_dens = (double)fabsf(fx) + (double)fabsf(fy);
+    v_grad = (double)(fabsf(sx) + fabsf(sy));
+    // getDistance terms (src/nesterovPlace.cpp:598-607)
+    float2 a = d.slp[(size_t)ref * d.nc + c];
+    float2 b = d.slp[(size_t)dst * d.nc + c];
+    float2 ga = d.sumg[(size_t)ref * d.nc + c];
+    float ddx = a.x - b.x, ddy = a.y - b.y;
+    float gdx = ga.x - sx, gdy = ga.y - sy;
+    v_dc = (double)(ddx * ddx) + (double)(ddy * ddy);
+    v_dg = (double)(gdx * gdx) + (double)(gdy * gdy);
+  }
+  double r;
+  r = block_sum_d(v_wl, sh_red);
+  if (threadIdx.x == 0) d.part_grad[(size_t)blockIdx.x * RP_P_COUNT + RP_P_WL] = r;
+  r = block_sum_d(v_dens, sh_red);
+  if (threadIdx.x == 0) d.part_grad[(size_t)blockIdx.x * RP_P_COUNT + RP_P_DENS] = r;
+  r = block_sum_d(v_grad, sh_red);
+  if (threadIdx.x == 0) d.part_grad[(size_t)blockIdx.x * RP_P_COUNT + RP_P_GRAD] = r;
+  r = block_sum_d(v_dc, sh_red);
+  if (threadIdx.x == 0) d.part_grad[(size_t)blockIdx.x * RP_P_COUNT + RP_P_DCOORD] = r;
+  r = block_sum_d(v_dg, sh_red);
+  if (threadIdx.x == 0) d.part_grad[(size_t)blockIdx.x * RP_P_COUNT + RP_P_DGRAD] = r;
+}
+
+// ---------------------------------------------------------------------------
+// 8. controller
+// ---------------------------------------------------------------------------
+// Fixed-order reduction of `n` partials with stride `stride` (1 workgroup).
+__device__ double reduce_parts_d(const double* p, int n, int stride, int off, double* sh) {
+  double v = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) v += p[(size_t)k * stride + off];
+  return block_sum_d(v, sh);
+}
+
+__device__ long long reduce_parts_ll(const long long* p, int n, long long* sh) {
+  long long v = 0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) v += p[k];
+  return block_sum_ll(v, sh);
+}
+
+struct Totals {
+  double wl, dens, grad, dc, dg, ovf;
+  long long hpwl;
+};
+
+__device__ void reduce_all(const RpDev& d, Totals& t) {
+  __shared__ double shd[RP_BLOCK / 64];
+  __shared__ long long shl[RP_BLOCK / 64];
+  const RpState* st = d.st;
+  t.wl = reduce_parts_d(d.part_grad, st->num_partials_grad, RP_P_COUNT, RP_P_WL, shd);
+  t.dens = reduce_parts_d(d.part_grad, st->num_partials_grad, RP_P_COUNT, RP_P_DENS, shd);
+  t.grad = reduce_parts_d(d.part_grad, st->num_partials_grad, RP_P_COUNT, RP_P_GRAD, shd);
+  t.dc = reduce_parts_d(d.part_grad, st->num_partials_grad, RP_P_COUNT, RP_P_DCOORD, shd);
+  t.dg = reduce_parts_d(d.part_grad, st->num_partials_grad, RP_P_COUNT, RP_P_DGRAD, shd);
+  t.ovf = reduce_parts_d(d.part_bin, st->num_partials_bin, 1, 0, shd);
+  t.hpwl = reduce_parts_ll(d.part_net, st->num_partials_net, shl);
+}
+
+// updateWireLengthCoef, src/nesterovPlace.cpp:474-490
+__device__ void update_wl_coef(RpState* s, float overflow) {
+  float c;
+  if (overflow > 1.0)
+    c = 0.1f;
+  else if (overflow < 0.1)
+    c = 10.0f;
+  else
+    c = (float)(1.0 / pow(10.0, ((double)overflow - 0.1) * 20 / 9.0 - 1.0));
+  s->wl_coef_x = c * s->base_wl_coef;
+  s->wl_coef_y = c * s->base_wl_coef;
+}
+
+__device__ float overflow_of(const RpState* s, long long area) { return (float)area / s->area_denom; }
+
+// getStepLength with FP64 sums: float(sqrt(sum / (2N))) per distance
+__device__ float step_of(const RpState* s, double dc, double dg) {
+  double n2 = 2.0 * (double)s->num_cells;
+  float cd = (float)sqrt(dc / n2);
+  float gd = (float)sqrt(dg / n2);
+  return cd / gd;
+}
+
+__device__ bool bad_sum(double v) {
+  float f = (float)v;
+  return isnan(f) || isinf(f);
+}
+
+__device__ void begin_iteration(RpState* s) {
+  // src/nesterovPlace.cpp:318-326
+  float prevA = s->cur_a;
+  s->cur_a = (float)((1.0 + sqrt(4.0 * (double)prevA * (double)prevA + 1.0)) * 0.5);
+  s->coeff = (float)(((double)prevA - 1.0) / (double)s->cur_a);
+  s->bt = 0;
+  s->mode = 0;
+  s->dst = s->ring.next;
+}
+
+// stage: 0 = after init eval A's Poisson solve (before WA)
+//        1 = after init eval A's gradients
+//        2 = after init eval B's gradients
+__global__ void k_control_init(RpDev d, int stage) {
+  Totals t;
+  reduce_all(d, t);
+  if (threadIdx.x != 0) return;
+  RpState* s = d.st;
+  if (s->phase != 0) return;
+  long long ovf_area = (long long)t.ovf;
+  if (stage == 0) {
+    // NesterovPlace::init, src/nesterovPlace.cpp:106-119
+    s->base_wl_coef = (float)((double)s->init_wl_coef / ((double)s->bin_size_sum * 0.5));
+    s->overflow_area = ovf_area;
+    s->sum_overflow = overflow_of(s, ovf_area);
+    update_wl_coef(s, s->sum_overflow);
+    return;
+  }
+  s->evals++;
+  s->wl_grad_sum = t.wl;
+  s->dens_grad_sum = t.dens;
+  if (bad_sum(t.wl) || bad_sum(t.dens)) {
+    s->phase = 2;
+    s->code = 2;  // diverged in init -> REPL-2 (src/nesterovPlace.cpp:280-283)
+    s->diverged = 1;
+    return;
+  }
+  if (stage == 1) {
+    s->prev_hpwl = t.hpwl;  // src/nesterovPlace.cpp:98-99
+    // updateInitialPrevSLPCoordi as a step of length initialPrevCoordiUpdateCoef
+    s->mode = 0;
+    s->dst = s->ring.prev;
+    s->step_length = s->initial_prev_coef;
+    s->coeff = 0.0f;
+    return;
+  }
+  // stage 2: src/nesterovPlace.cpp:156-170
+  s->density_penalty = ((float)t.wl / (float)t.dens) * s->init_density_penalty;
+  s->overflow_area = ovf_area;
+  s->sum_overflow = overflow_of(s, ovf_area);
+  s->step_length = step_of(s, t.dc, t.dg);
+  s->cur_a = 1.0f;
+  s->min_sum_overflow = 1e30f;
+  s->hpwl_with_min_sum_overflow = 1e30f;
+  s->is_max_phi_coef_changed = 0;
+  s->iter = 0;
+  begin_iteration(s);
+}
+
+__global__ void k_control_main(RpDev d) {
+  if (d.st->phase != 0) return;
+  Totals t;
+  reduce_all(d, t);
+  if (threadIdx.x != 0) return;
+  RpState* s = d.st;
+  s->evals++;
+  s->wl_grad_sum = t.wl;
+  s->dens_grad_sum = t.dens;
+  s->grad_sum = t.grad;
+  s->coord_dist2 = t.dc;
+  s->grad_dist2 = t.dg;
+  s->overflow_area_d = t.ovf;
+  s->hpwl = t.hpwl;
+  bool div = bad_sum(t.wl) || bad_sum(t.dens);
+  bool accepted = false;
+  if (!div) {
+    float ns = step_of(s, t.dc, t.dg);
+    s->last_new_step = ns;
+    if ((double)ns > (double)s->step_length * 0.95) {
+      s->step_length = ns;
+      accepted = true;
+    } else {
+      s->step_length = ns;
+      s->bt++;
+    }
+  }
+  if (!div && !accepted && s->bt < s->max_back_track) return;  // next slot re-evaluates
+  // end of the back-tracking loop: src/nesterovPlace.cpp:392-413
+  if (!s->is_max_phi_coef_changed && s->sum_overflow < 0.35f) {
+    s->is_max_phi_coef_changed = 1;
+    s->max_phi_coef = (float)((double)s->max_phi_coef * 0.99);
+  }
+  if (!div && !accepted) {
+    s->code = 3;
+    s->phase = 2;
+    s->diverged = 1;
+    return;
+  }
+  if (div) {
+    s->code = 0;
+    s->phase = 2;
+    s->diverged = 1;
+    return;
+  }
+  // updateNextIter, src/nesterovPlace.cpp:514-553
+  RpRing r = s->ring;
+  s->ring.prev = r.cur;
+  s->ring.cur = r.next;
+  s->ring.next = r.prev;
+  s->ring.co_cur = r.co_next;
+  s->ring.co_next = r.co_cur;
+  long long ovf_area = (long long)t.ovf;
+  s->overflow_area = ovf_area;
+  s->sum_overflow = overflow_of(s, ovf_area);
+  update_wl_coef(s, s->sum_overflow);
+  long long hpwl = t.hpwl;
+  float scaled = (float)(hpwl - s->prev_hpwl) / s->reference_hpwl;
+  float coef = (scaled < 0) ? s->max_phi_coef
+                            : (float)((double)s->max_phi_coef * pow((double)s->max_phi_coef, (double)scaled * -1.0));
+  coef = (s->min_phi_coef < coef) ? coef : s->min_phi_coef;  // std::max(minPhiCoef, coef)
+  s->prev_hpwl = hpwl;
+  s->density_penalty *= coef;
+  int i = s->iter;
+  RpIterRec rec;
+  rec.overflow = s->sum_overflow;
+  rec.step_length = s->step_length;
+  rec.density_penalty = s->density_penalty;
+  rec.num_back_track = s->bt + 1;
+  rec.hpwl = hpwl;
+  d.rec[i] = rec;
+  s->iter = i + 1;
+  // src/nesterovPlace.cpp:437-462
+  if (s->min_sum_overflow > s->sum_overflow) {
+    s->min_sum_overflow = s->sum_overflow;
+    s->hpwl_with_min_sum_overflow = (float)s->prev_hpwl;
+  }
+  if (s->sum_overflow < 0.3f && s->sum_overflow - s->min_sum_overflow >= 0.02f &&
+      s->hpwl_with_min_sum_overflow * 1.2f < (float)s->prev_hpwl) {
+    s->code = 4;
+    s->phase = 2;
+    s->diverged = 1;
+    return;
+  }
+  if (i > 50 && s->sum_overflow <= s->target_overflow) {
+    s->phase = 1;
+    return;
+  }
+  if (s->iter >= s->max_iter) {
+    s->phase = 3;
+    return;
+  }
+  begin_iteration(s);
+}
+
+// ---------------------------------------------------------------------------
+// initial placement: CSR SpMV and BiCGSTAB vector kernels (FP64 dots)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(RP_BLOCK) k_spmv(int n, const int* __restrict__ rowp, const int* __restrict__ col,
+                                                   const float* __restrict__ val, const float* __restrict__ x,
+                                                   float* __restrict__ y) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double s = 0.0;
+  for (int k = rowp[r]; k < rowp[r + 1]; k++) s += (double)val[k] * (double)x[col[k]];
+  y[r] = (float)s;
+}
+
+// partial dot products: out[blk*3 + q] for pairs (a0,b0) (a1,b1) (a2,b2); unused pairs NULL
+__global__ void __launch_bounds__(RP_BLOCK) k_dot3(int n, const float* a0, const float* b0, const float* a1,
+                                                   const float* b1, const float* a2, const float* b2,
+                                                   double* out) {
+  __shared__ double sh[RP_BLOCK / 64];
+  double v0 = 0, v1 = 0, v2 = 0;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    if (a0) v0 += (double)a0[k] * (double)b0[k];
+    if (a1) v1 += (double)a1[k] * (double)b1[k];
+    if (a2) v2 += (double)a2[k] * (double)b2[k];
+  }
+  double r0 = block_sum_d(v0, sh);
+  double r1 = block_sum_d(v1, sh);
+  double r2 = block_sum_d(v2, sh);
+  if (threadIdx.x == 0) {
+    out[blockIdx.x * 3 + 0] = r0;
+    out[blockIdx.x * 3 + 1] = r1;
+    out[blockIdx.x * 3 + 2] = r2;
+  }
+}
+
+__global__ void k_dot_final(int nparts, const double* parts, double* out3) {
+  __shared__ double sh[RP_BLOCK / 64];
+  for (int q = 0; q < 3; q++) {
+    double v = 0;
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) v += parts[k * 3 + q];
+    double r = block_sum_d(v, sh);
+    if (threadIdx.x == 0) out3[q] = r;
+  }
+}
+
+// y = a + alpha * b  (double coefficients, float vectors)
+__global__ void k_axpy(int n, float* y, const float* a, double alpha, const float* b) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) y[k] = (float)((double)a[k] + alpha * (double)b[k]);
+}
+
+// p = r + beta * (p - w * v)
+__global__ void k_bicg_p(int n, float* p, const float* r, const float* v, double beta, double w) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = (float)((double)r[k] + beta * ((double)p[k] - w * (double)v[k]));
+}
+
+// x += alpha * p + w * s ; r = s - w * t
+__global__ void k_bicg_xr(int n, float* x, float* r, const float* p, const float* s, const float* t, double alpha,
+                          double w) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) {
+    x[k] = (float)((double)x[k] + alpha * (double)p[k] + w * (double)s[k]);
+    r[k] = (float)((double)s[k] - w * (double)t[k]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host-callable launch wrappers (C++ linkage, used by rp_host.cpp)
+// ---------------------------------------------------------------------------
+static int fft_threads(int n) {
+  int t = n / 2;
+  if (t > RP_BLOCK) t = RP_BLOCK;
+  if (t < 64) t = 64;
+  return t;
+}
+
+hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi) {
+  int nb_c = (d.nc + RP_BLOCK - 1) / RP_BLOCK;
+  int nb_n = (d.nn + RP_BLOCK - 1) / RP_BLOCK;
+  hipLaunchKernelGGL(k_step_scatter, dim3(nb_c), dim3(RP_BLOCK), 0, s, d);
+  size_t shx = sizeof(float2) * 2 * d.nbx;
+  hipLaunchKernelGGL(k_binfin_rowdct, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shx, s, d, 0);
+  hipLaunchKernelGGL(k_transpose, dim3((d.nbx + 31) / 32, (d.nby + 31) / 32), dim3(32, 8), 0, s,
+                     (const float*)d.t, d.tt, d.nby, d.nbx, (const RpState*)d.st);
+  size_t shy = sizeof(float2) * 2 * d.nby + sizeof(float) * 4 * d.nby;
+  hipLaunchKernelGGL(k_col_spectral, dim3(d.nbx / 2), dim3(fft_threads(d.nby)), shy, s, d);
+  hipLaunchKernelGGL(k_transpose2, dim3((d.nby + 31) / 32, (d.nbx + 31) / 32, 2), dim3(32, 8), 0, s,
+                     (const float*)d.y1t, d.y1, (const float*)d.y2t, d.y2, d.nbx, d.nby, (const RpState*)d.st);
+  size_t shr = sizeof(float2) * 2 * d.nbx + sizeof(float) * 2 * d.nbx;
+  hipLaunchKernelGGL(k_row_inverse, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shr, s, d, want_phi);
+  hipLaunchKernelGGL(k_wa, dim3(nb_n), dim3(RP_BLOCK), 0, s, d);
+  hipLaunchKernelGGL(k_grad, dim3(nb_c), dim3(RP_BLOCK), 0, s, d);
+  hipLaunchKernelGGL(k_control_main, dim3(1), dim3(RP_BLOCK), 0, s, d);
+  return hipGetLastError();
+}
+
+hipError_t rp_launch_ctrl_init(const RpDev& d, hipStream_t s, int stage) {
+  hipLaunchKernelGGL(k_control_init, dim3(1), dim3(RP_BLOCK), 0, s, d, stage);
+  return hipGetLastError();
+}
+
+// Split stages for the stage-level test entry points.
+hipError_t rp_launch_scatter(const RpDev& d, hipStream_t s) {
+  hipLaunchKernelGGL(k_step_scatter, dim3((d.nc + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, d);
+  return hipGetLastError();
+}
+
+hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int from_rho) {
+  size_t shx = sizeof(float2) * 2 * d.nbx;
+  hipLaunchKernelGGL(k_binfin_rowdct, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shx, s, d, from_rho);
+  hipLaunchKernelGGL(k_transpose, dim3((d.nbx + 31) / 32, (d.nby + 31) / 32), dim3(32, 8), 0, s,
+                     (const float*)d.t, d.tt, d.nby, d.nbx, (const RpState*)d.st);
+  size_t shy = sizeof(float2) * 2 * d.nby + sizeof(float) * 4 * d.nby;
+  hipLaunchKernelGGL(k_col_spectral, dim3(d.nbx / 2), dim3(fft_threads(d.nby)), shy, s, d);
+  hipLaunchKernelGGL(k_transpose2, dim3((d.nby + 31) / 32, (d.nbx + 31) / 32, 2), dim3(32, 8), 0, s,
+                     (const float*)d.y1t, d.y1, (const float*)d.y2t, d.y2, d.nbx, d.nby, (const RpState*)d.st);
+  size_t shr = sizeof(float2) * 2 * d.nbx + sizeof(float) * 2 * d.nbx;
+  hipLaunchKernelGGL(k_row_inverse, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shr, s, d, want_phi);
+  return hipGetLastError();
+}
+
+hipError_t rp_launch_wa_grad(const RpDev& d, hipStream_t s) {
+  hipLaunchKernelGGL(k_wa, dim3((d.nn + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, d);
+  hipLaunchKernelGGL(k_grad, dim3((d.nc + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, d);
+  return hipGetLastError();
+}
+
+hipError_t rp_launch_spmv(int n, const int* rowp, const int* col, const float* val, const float* x, float* y,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_spmv, dim3((n + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, n, rowp, col, val, x, y);
+  return hipGetLastError();
+}
+
+hipError_t rp_launch_dot3(int n, const float* a0, const float* b0, const float* a1, const float* b1,
+                          const float* a2, const float* b2, double* parts, int nparts, double* out3,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_dot3, dim3(nparts), dim3(RP_BLOCK), 0, s, n, a0, b0, a1, b1, a2, b2, parts);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RP_BLOCK), 0, s, nparts, (const double*)parts, out3);
+  return hipGetLastError();
+}
+
+hipError_t rp_launch_axpy(int n, float* y, const float* a, double alpha, const float* b, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy, dim3((n + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, n, y, a, alpha, b);
+  return hipGetLastError();
+}
+
+hipError_t rp_launch_bicg_p(int n, float* p, const float* r, const float* v, double beta, double w,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_bicg_p, dim3((n + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, n, p, r, v, beta, w);
+  return hipGetLastError();
+}
+
+hipError_t rp_launch_bicg_xr(int n, float* x, float* r, const float* p, const float* sv, const float* t,
+                             double alpha, double w, hipStream_t s) {
+  hipLaunchKernelGGL(k_bicg_xr, dim3((n + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, n, x, r, p, sv, t,
+                     alpha, w);
+  return hipGetLastError();
+}
