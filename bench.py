@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of the placement hot path on one MI355X.
+
+Metric: seconds from the start of doNesterovPlace (netlist resident in HBM,
+after the initial place) to the first iteration with overflow <= 0.10, on
+the 1M-cell workload (1,000,000 movable cells, 1,050,000 nets, ~4.2M pins,
+40 fixed macros, 1024 x 1024 bins, target density 0.70, FP32 with int64
+density atomics).  One step = one full Nesterov placement to overflow 0.10.
+
+Besides the contract fields the JSON line carries
+  roofline      the dominant kernel's algorithmic bytes per launch / its mean
+                launch time (HIP events on the library stream, timed steps)
+  cpu_baseline  the oracle (single-threaded CPU restatement, the reference's
+                own Ooura FFT when oracle/_ref is built) pinned to one core:
+                seconds per gradient evaluation on a bounded sample x the
+                GPU run's evaluation count
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "s to overflow≤0.10 at 1M cells; served placements/s + p99 at ≤0.5% HPWL"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(nc, npin, nn, nb):
+    """Bytes one launch of each kernel must move (inputs read once, outputs
+    written once; see DESIGN.md, per-kernel table)."""
+    return {
+        "step_scatter": 64 * nc + 16 * nb,
+        "binfin_rowdct": 28 * nb + 4 * nb,
+        "transpose": 8 * nb,
+        "col_spectral": 12 * nb,
+        "transpose2": 16 * nb,
+        "row_inverse": 16 * nb,
+        "wa": 20 * npin + 52 * nn,
+        "grad": 52 * nc + 60 * npin + 8 * nb,
+        "control": 8 * 5 * ((nc + 255) // 256),
+    }
+
+
+def cpu_baseline(workload, gpu_evals, sample_evals, core):
+    code = f"""
+import json, os, sys, time
+sys.path.insert(0, {ROOT!r})
+os.sched_setaffinity(0, {{{core}}})
+import oracle
+from replace_amd import netgen
+nl, p = netgen.workload({workload!r})
+p.initial_place_max_iter = 0
+o = oracle.Oracle(nl, p, poisson='auto')
+o.initial_place()
+o.setup()
+t0 = time.perf_counter()
+o.nesterov_init()
+t1 = time.perf_counter()
+o.nesterov_iterate({sample_evals})
+t2 = time.perf_counter()
+c, st = o.nesterov_iterate(0)
+print(json.dumps(dict(init_s=t1 - t0, iter_s=t2 - t1, evals=st.evals, poisson=o.poisson)))
+"""
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=900)
+    if out.returncode != 0:
+        raise RuntimeError(out.stderr[-2000:])
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    loop_evals = r["evals"] - 2
+    per_eval = r["iter_s"] / max(1, loop_evals)
+    est = r["init_s"] + per_eval * max(0, gpu_evals - 2)
+    cpu = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {
+        "value": round(est, 3),
+        "unit": "s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle {workload}: init (2 evals) {r['init_s']:.2f}s + {loop_evals} loop evals "
+                   f"{r['iter_s']:.2f}s = {per_eval:.3f}s/eval on 1 core ({cpu}); value = init + "
+                   f"s/eval x {gpu_evals - 2} GPU loop evals; Poisson: {r['poisson']}"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1"])
+    ap.add_argument("--cpu-sample-evals", type=int, default=None)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build places one instance per GPU and "
+              "has no multi-GPU path (see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    torch.cuda.set_device(0)
+    from replace_amd import netgen
+    from replace_amd.capi import Placer
+
+    t = time.perf_counter()
+    nl, p = netgen.workload(args.workload)
+    gen_s = time.perf_counter() - t
+    placer = Placer(nl, p)
+    t = time.perf_counter()
+    ip = placer.initial_place()
+    ip_s = time.perf_counter() - t
+    info = placer.setup()
+    for _ in range(args.warmup):
+        placer.nesterov_place()
+    placer.set_kernel_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    runs = []
+    for _ in range(args.steps):
+        code, st = placer.nesterov_place()
+        runs.append((code, st.iterations, st.evals, st.final_overflow, st.final_hpwl, st.seconds,
+                     st.finished))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) / args.steps * 1e3
+    kt = placer.kernel_timing()
+    nb = info.bin_cnt_x * info.bin_cnt_y
+    abytes = algorithmic_bytes(info.num_gcells, info.num_gpins, info.num_gnets, nb)
+    kernels = {}
+    for k, (tot, n) in kt.items():
+        avg = tot / max(1, n)
+        kernels[k] = {"avg_us": round(avg * 1e3, 2), "launches": n,
+                      "GBs": round(abytes[k] / (avg * 1e-3) / 1e9, 1) if avg > 0 else None}
+    dom = max(kt, key=lambda k: kt[k][0])
+    avg_ms = kt[dom][0] / max(1, kt[dom][1])
+    achieved = abytes[dom] / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"{dom}_{args.workload}_traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+    code, iters, evals, ovf, hpwl, dev_s, fin = runs[-1]
+    result = {
+        "metric": METRIC,
+        "value": round(ms_per_step / 1e3, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": ("1M-cell plain placement: 1,000,000 movable cells, 1,050,000 nets, 40 fixed "
+                         "macros, 1024x1024 bins, td 0.70, stop at overflow 0.10")
+            if args.workload == "c1" else
+            "20k-cell plain placement: 20,000 cells, 21,000 nets, 128x128 bins, td 0.70",
+            "cells": int(info.num_place_insts), "fillers": int(info.num_fillers),
+            "nets": int(info.num_gnets), "pins": int(info.num_gpins),
+            "bins": f"{info.bin_cnt_x}x{info.bin_cnt_y}", "parallelism": "none (1 GPU)",
+        },
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "algorithmic_bytes_per_launch": abytes[dom],
+                     "avg_launch_us": round(avg_ms * 1e3, 2)},
+        "run": {"code": code, "finished": fin, "iterations": iters, "evals": evals,
+                "final_overflow": round(ovf, 5), "final_hpwl": hpwl, "device_s": round(dev_s, 4),
+                "initial_place_s": round(ip_s, 3), "initial_place_iters": ip.iterations,
+                "netgen_s": round(gen_s, 2)},
+        "kernels": kernels,
+    }
+    if not args.no_cpu_baseline:
+        n = args.cpu_sample_evals or (8 if args.workload == "c1" else 150)
+        try:
+            result["cpu_baseline"] = cpu_baseline(args.workload, evals, n, core=0)
+        except Exception as e:  # reported, never fatal for the GPU number
+            result["cpu_baseline"] = {"value": None, "unit": "s", "cores": 1, "kind": "port",
+                                      "sample": f"failed: {e}"[:300]}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

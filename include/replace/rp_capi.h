@@ -183,6 +183,16 @@ int rp_set_positions(rp_ctx* ctx, const int32_t* lx, const int32_t* ly);
 void rp_destroy(rp_ctx* ctx);
 const char* rp_last_error(rp_ctx* ctx);
 
+/* Per-kernel timing of the evaluation slots (no reference counterpart: the
+ * reference has no timers).  When enabled, rp_nesterov_place brackets every
+ * kernel of every executed slot with HIP events on the context's stream and
+ * accumulates the elapsed time.  rp_get_kernel_timing returns, for the 9
+ * kernels of a slot in launch order (step+scatter, bin finalize + row DCT,
+ * transpose, column spectral pass, transpose, row inverse, WA, gradient,
+ * controller), the summed milliseconds and the number of executed slots. */
+int rp_set_kernel_timing(rp_ctx* ctx, int enable);
+int rp_get_kernel_timing(rp_ctx* ctx, double* total_ms_per_kernel, int64_t* launches);
+
 /* ------------------------------------------------------------------------
  * Stage-level entry points.  They expose single pieces of one gradient
  * evaluation (src/nesterovPlace.cpp:338-377) so parity can be checked one

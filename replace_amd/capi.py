@@ -52,6 +52,9 @@ def lib() -> ctypes.CDLL:
         L.rp_eval_gradient.argtypes = [vp, vp, vp, ctypes.c_float, ctypes.c_float,
                                        ctypes.c_float, ctypes.POINTER(RpEvalOut)]
         L.rp_eval_poisson.argtypes = [vp, vp, vp, vp, vp]
+        L.rp_set_kernel_timing.argtypes = [vp, ctypes.c_int]
+        L.rp_get_kernel_timing.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int64)]
+        L.rp_get_ip_records.argtypes = [vp, vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
         _lib = L
     return _lib
 
@@ -170,6 +173,27 @@ class Placer:
         self._check(lib().rp_get_iter_records(self.h, arr, cnt.value, ctypes.byref(cnt)))
         return [(r.overflow, r.hpwl, r.step_length, r.density_penalty, r.num_back_track)
                 for r in arr[: cnt.value]]
+
+    KERNELS = ("step_scatter", "binfin_rowdct", "transpose", "col_spectral", "transpose2",
+               "row_inverse", "wa", "grad", "control")
+
+    def set_kernel_timing(self, on: bool) -> None:
+        self._check(lib().rp_set_kernel_timing(self.h, 1 if on else 0))
+
+    def kernel_timing(self):
+        """{kernel: (total_ms, launches)} of the slots run since timing was enabled."""
+        tot = np.zeros(len(self.KERNELS), np.float64)
+        n = ctypes.c_int64(0)
+        self._check(lib().rp_get_kernel_timing(self.h, _p(tot), ctypes.byref(n)))
+        return {k: (float(t), int(n.value)) for k, t in zip(self.KERNELS, tot)}
+
+    def ip_records(self):
+        n = ctypes.c_int32(0)
+        self._check(lib().rp_get_ip_records(self.h, None, None, 0, ctypes.byref(n)))
+        err = np.zeros(max(1, n.value), np.float32)
+        hp = np.zeros(max(1, n.value), np.int64)
+        self._check(lib().rp_get_ip_records(self.h, _p(err), _p(hp), n.value, ctypes.byref(n)))
+        return err[: n.value], hp[: n.value]
 
     def positions(self):
         n = self._nl.num_insts

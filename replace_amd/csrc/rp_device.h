@@ -29,7 +29,21 @@
 #include <stdint.h>
 
 #define RP_BLOCK 256
-#define RP_MAX_FFT 4096
+#define RP_MAX_FFT 2048
+
+// Kernels of one evaluation slot, in launch order (per-kernel timing index).
+enum {
+  RP_K_STEP_SCATTER = 0,
+  RP_K_BINFIN_ROWDCT,
+  RP_K_TRANSPOSE,
+  RP_K_COL_SPECTRAL,
+  RP_K_TRANSPOSE2,
+  RP_K_ROW_INVERSE,
+  RP_K_WA,
+  RP_K_GRAD,
+  RP_K_CONTROL,
+  RP_NUM_EVAL_KERNELS
+};
 
 // Ring slots of the SLP / gradient buffers.
 struct RpRing {

@@ -29,7 +29,7 @@
 
 #include "rp_dev_struct.h"
 
-hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi);
+hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_t* ev);
 hipError_t rp_launch_ctrl_init(const RpDev& d, hipStream_t s, int stage);
 hipError_t rp_launch_scatter(const RpDev& d, hipStream_t s);
 hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int from_rho);
@@ -119,6 +119,10 @@ struct rp_ctx {
   RpState* h_state = nullptr;  // pinned
   std::vector<int> net_perm;   // device net id -> reference net id
   RpDev dev;
+  // ---- per-kernel timing (optional) ----
+  bool timing = false;
+  double ktime_ms[RP_NUM_EVAL_KERNELS] = {0};
+  long long kcount = 0;
   // ---- results ----
   std::vector<RpIterRec> records;
   rp_nesterov_stats last_stats;
@@ -1347,13 +1351,37 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
   hipEvent_t polled;
   HIPCHK(c, hipEventCreateWithFlags(&polled, hipEventDisableTiming));
   int max_batches = (c->prm.nesterov_place_max_iter * std::max(1, c->prm.max_back_track)) / K + 4;
+  std::vector<hipEvent_t> kev;
+  if (c->timing) {
+    kev.resize((size_t)K * (RP_NUM_EVAL_KERNELS + 1));
+    for (auto& e : kev) HIPCHK(c, hipEventCreate(&e));
+  }
+  HIPCHK(c, hipMemcpyAsync(hs, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  int evals_before = hs->evals;
   for (int b = 0; b < max_batches; b++) {
-    for (int k = 0; k < K; k++) HIPCHK(c, rp_launch_eval(d, s, 0));
+    for (int k = 0; k < K; k++)
+      HIPCHK(c, rp_launch_eval(d, s, 0, c->timing ? &kev[(size_t)k * (RP_NUM_EVAL_KERNELS + 1)] : nullptr));
     HIPCHK(c, hipMemcpyAsync(hs, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipEventRecord(polled, s));
     HIPCHK(c, hipEventSynchronize(polled));
+    if (c->timing) {
+      // only slots that evaluated (the ones after the stop are no-ops)
+      int ran = std::min(K, hs->evals - evals_before);
+      for (int k = 0; k < ran; k++) {
+        for (int q = 0; q < RP_NUM_EVAL_KERNELS; q++) {
+          float ms = 0;
+          hipEvent_t* ev = &kev[(size_t)k * (RP_NUM_EVAL_KERNELS + 1)];
+          HIPCHK(c, hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+          c->ktime_ms[q] += ms;
+        }
+      }
+      c->kcount += std::max(0, ran);
+    }
+    evals_before = hs->evals;
     if (hs->phase != 0) break;
   }
+  for (auto& e : kev) (void)hipEventDestroy(e);
   HIPCHK(c, hipEventRecord(e1, s));
   HIPCHK(c, hipEventSynchronize(e1));
   float ms = 0;
@@ -1426,5 +1454,20 @@ extern "C" int rp_set_positions(rp_ctx* c, const int32_t* lx, const int32_t* ly)
     inst_update_pins(c, i);
   }
   c->nb_ready = false;
+  return RP_OK;
+}
+
+extern "C" int rp_set_kernel_timing(rp_ctx* c, int enable) {
+  if (!c) return RP_ERR_ARG;
+  c->timing = enable != 0;
+  for (double& v : c->ktime_ms) v = 0.0;
+  c->kcount = 0;
+  return RP_OK;
+}
+
+extern "C" int rp_get_kernel_timing(rp_ctx* c, double* total_ms, int64_t* launches) {
+  if (!c || !total_ms || !launches) return RP_ERR_ARG;
+  for (int q = 0; q < RP_NUM_EVAL_KERNELS; q++) total_ms[q] = c->ktime_ms[q];
+  *launches = c->kcount;
   return RP_OK;
 }

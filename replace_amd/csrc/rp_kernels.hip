@@ -965,23 +965,42 @@ static int fft_threads(int n) {
   return t;
 }
 
-hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi) {
+// One evaluation slot.  ev (optional) holds RP_NUM_EVAL_KERNELS + 1 events
+// recorded around each kernel (per-kernel timing on this stream).
+hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_t* ev) {
   int nb_c = (d.nc + RP_BLOCK - 1) / RP_BLOCK;
   int nb_n = (d.nn + RP_BLOCK - 1) / RP_BLOCK;
+  int k = 0;
+#define RP_MARK()                            \
+  do {                                       \
+    if (ev) (void)hipEventRecord(ev[k], s);  \
+    k++;                                     \
+  } while (0)
+  RP_MARK();
   hipLaunchKernelGGL(k_step_scatter, dim3(nb_c), dim3(RP_BLOCK), 0, s, d);
+  RP_MARK();
   size_t shx = sizeof(float2) * 2 * d.nbx;
   hipLaunchKernelGGL(k_binfin_rowdct, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shx, s, d, 0);
+  RP_MARK();
   hipLaunchKernelGGL(k_transpose, dim3((d.nbx + 31) / 32, (d.nby + 31) / 32), dim3(32, 8), 0, s,
                      (const float*)d.t, d.tt, d.nby, d.nbx, (const RpState*)d.st);
+  RP_MARK();
   size_t shy = sizeof(float2) * 2 * d.nby + sizeof(float) * 4 * d.nby;
   hipLaunchKernelGGL(k_col_spectral, dim3(d.nbx / 2), dim3(fft_threads(d.nby)), shy, s, d);
+  RP_MARK();
   hipLaunchKernelGGL(k_transpose2, dim3((d.nby + 31) / 32, (d.nbx + 31) / 32, 2), dim3(32, 8), 0, s,
                      (const float*)d.y1t, d.y1, (const float*)d.y2t, d.y2, d.nbx, d.nby, (const RpState*)d.st);
+  RP_MARK();
   size_t shr = sizeof(float2) * 2 * d.nbx + sizeof(float) * 2 * d.nbx;
   hipLaunchKernelGGL(k_row_inverse, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shr, s, d, want_phi);
+  RP_MARK();
   hipLaunchKernelGGL(k_wa, dim3(nb_n), dim3(RP_BLOCK), 0, s, d);
+  RP_MARK();
   hipLaunchKernelGGL(k_grad, dim3(nb_c), dim3(RP_BLOCK), 0, s, d);
+  RP_MARK();
   hipLaunchKernelGGL(k_control_main, dim3(1), dim3(RP_BLOCK), 0, s, d);
+  RP_MARK();
+#undef RP_MARK
   return hipGetLastError();
 }
 
