@@ -23,8 +23,8 @@ struct RpDev {
   float2* co;
   int2* ic;
   const int* cp_start;
-  const int* cp_net;
-  const int2* cp_off;
+  const int* cp_pin;   // cell's pins (reference ITerm order) -> device pin index
+  float2* pgrad;       // per-pin WA gradient gMin - gMax, device pin order
   const int* nstart;
   const int* pcell;
   const int2* ppos;

@@ -29,6 +29,7 @@
 #include <stdint.h>
 
 #define RP_BLOCK 256
+#define RP_CTRL_BLOCK 1024  // controller workgroup (fixed-order reductions)
 #define RP_MAX_FFT 2048
 
 // Kernels of one evaluation slot, in launch order (per-kernel timing index).

@@ -33,7 +33,7 @@ hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_
 hipError_t rp_launch_ctrl_init(const RpDev& d, hipStream_t s, int stage);
 hipError_t rp_launch_scatter(const RpDev& d, hipStream_t s);
 hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int from_rho);
-hipError_t rp_launch_wa_grad(const RpDev& d, hipStream_t s);
+hipError_t rp_launch_wa_grad(const RpDev& d, hipStream_t s, int store_nets);
 hipError_t rp_launch_spmv(int n, const int* rowp, const int* col, const float* val, const float* x, float* y,
                           hipStream_t s);
 hipError_t rp_launch_dot3(int n, const float* a0, const float* b0, const float* a1, const float* b1,
@@ -101,8 +101,9 @@ struct rp_ctx {
   DevBuf<uint8_t> d_kind;
   DevBuf<float2> d_slp, d_sumg, d_co;
   DevBuf<int2> d_ic;
-  DevBuf<int> d_cp_start, d_cp_net, d_nstart, d_pcell;
-  DevBuf<int2> d_cp_off, d_ppos;
+  DevBuf<int> d_cp_start, d_cp_pin, d_nstart, d_pcell;
+  DevBuf<int2> d_ppos;
+  DevBuf<float2> d_pgrad;
   DevBuf<int4> d_nbox;
   DevBuf<float4> d_nsum;
   DevBuf<unsigned long long> d_bin_inst, d_bin_filler;
@@ -118,6 +119,8 @@ struct rp_ctx {
   DevBuf<float2> d_dbg_wl, d_dbg_dens;
   RpState* h_state = nullptr;  // pinned
   std::vector<int> net_perm;   // device net id -> reference net id
+  std::vector<int> gperm;      // device gcell index -> reference gcell index
+  std::vector<int> ginv;       // reference gcell index -> device gcell index
   RpDev dev;
   // ---- per-kernel timing (optional) ----
   bool timing = false;
@@ -456,32 +459,53 @@ struct Trip {
 };
 
 // setFromTriplets: duplicates summed in triplet order (per row, by column).
+// Stable counting sort by row, then a stable in-place insertion sort of each
+// (short) row by column; equal (row, col) entries are added in triplet order.
 void build_csr(int n, const std::vector<Trip>& t, Csr& m) {
   m.n = n;
-  std::vector<int> cnt(n + 1, 0);
-  for (const Trip& x : t) cnt[x.r + 1]++;
-  for (int r = 0; r < n; r++) cnt[r + 1] += cnt[r];
-  std::vector<int> order(t.size());
-  std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
-  for (size_t k = 0; k < t.size(); k++) order[fillp[t[k].r]++] = (int)k;
-  m.rowp.assign(n + 1, 0);
-  m.col.clear();
-  m.val.clear();
-  m.col.reserve(t.size());
-  m.val.reserve(t.size());
-  std::vector<int> tmp;
-  for (int r = 0; r < n; r++) {
-    tmp.assign(order.begin() + cnt[r], order.begin() + cnt[r + 1]);
-    std::stable_sort(tmp.begin(), tmp.end(), [&](int a, int b) { return t[a].c < t[b].c; });
-    for (size_t k = 0; k < tmp.size();) {
-      int cc = t[tmp[k]].c;
-      float v = 0.0f;
-      while (k < tmp.size() && t[tmp[k]].c == cc) v += t[tmp[k++]].v;
-      m.col.push_back(cc);
-      m.val.push_back(v);
+  std::vector<int> start(n + 1, 0);
+  for (const Trip& x : t) start[x.r + 1]++;
+  for (int r = 0; r < n; r++) start[r + 1] += start[r];
+  std::vector<int> tc(t.size());
+  std::vector<float> tv(t.size());
+  {
+    std::vector<int> fillp(start.begin(), start.end() - 1);
+    for (const Trip& x : t) {
+      int at = fillp[x.r]++;
+      tc[at] = x.c;
+      tv[at] = x.v;
     }
-    m.rowp[r + 1] = (int)m.col.size();
   }
+  m.rowp.assign(n + 1, 0);
+  m.col.resize(t.size());
+  m.val.resize(t.size());
+  int nnz = 0;
+  for (int r = 0; r < n; r++) {
+    int a = start[r], b = start[r + 1];
+    for (int k = a + 1; k < b; k++) {  // stable insertion sort by column
+      int ck = tc[k];
+      float vk = tv[k];
+      int j = k - 1;
+      while (j >= a && tc[j] > ck) {
+        tc[j + 1] = tc[j];
+        tv[j + 1] = tv[j];
+        j--;
+      }
+      tc[j + 1] = ck;
+      tv[j + 1] = vk;
+    }
+    for (int k = a; k < b;) {
+      int cc = tc[k];
+      float v = 0.0f;
+      while (k < b && tc[k] == cc) v += tv[k++];
+      m.col[nnz] = cc;
+      m.val[nnz] = v;
+      nnz++;
+    }
+    m.rowp[r + 1] = nnz;
+  }
+  m.col.resize(nnz);
+  m.val.resize(nnz);
 }
 
 struct DevSolver {
@@ -601,7 +625,12 @@ extern "C" int rp_initial_place(rp_ctx* c, rp_ip_stats* stats) {
     std::vector<float> bx(n), by(n), xx(n), xy(n);
     std::vector<Trip> tx, ty;
     int rc = 0;
+    const bool phase_timing = getenv("RP_IP_TIMING") != nullptr;
+    double t_pins = 0, t_pairs = 0, t_csr = 0, t_solve = 0;
+    using clk = std::chrono::steady_clock;
+    auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
     for (int it = 1; it <= p.initial_place_max_iter; it++) {
+      auto tp0 = clk::now();
       // updatePinInfo, src/initialPlace.cpp:128-182 (first strict extreme wins)
       std::fill(mm.begin(), mm.end(), 0);
       for (int e = 0; e < c->n_net; e++) {
@@ -634,6 +663,8 @@ extern "C" int rp_initial_place(rp_ctx* c, rp_ip_stats* stats) {
           }
         }
       }
+      t_pins += since(tp0);
+      auto tp1 = clk::now();
       // createSparseMatrix, src/initialPlace.cpp:185-378
       for (int k = 0; k < n; k++) {
         int i = c->place[k];
@@ -696,15 +727,20 @@ extern "C" int rp_initial_place(rp_ctx* c, rp_ip_stats* stats) {
           }
         }
       }
+      t_pairs += since(tp1);
+      auto tp2 = clk::now();
       Csr Ax, Ay;
       build_csr(n, tx, Ax);
       build_csr(n, ty, Ay);
+      t_csr += since(tp2);
+      auto tp3 = clk::now();
       const float tol = 1.1920928955078125e-07f;  // NumTraits<float>::epsilon()
       float ex = 0, ey = 0;
       rc = dev_bicgstab(c, S, Ax, bx, xx, p.initial_place_max_solver_iter, tol, &ex);
       if (rc) break;
       rc = dev_bicgstab(c, S, Ay, by, xy, p.initial_place_max_solver_iter, tol, &ey);
       if (rc) break;
+      t_solve += since(tp3);
       float err = std::max(ex, ey);
       c->ip_err.push_back(err);
       c->ip_hpwl.push_back(inst_hpwl(c));  // printed before updateCoordi (src/initialPlace.cpp:88-91)
@@ -718,6 +754,9 @@ extern "C" int rp_initial_place(rp_ctx* c, rp_ip_stats* stats) {
       if (err <= 1e-5 && it >= 5) break;
     }
     (void)hipHostFree(S.h_dots);
+    if (phase_timing)
+      fprintf(stderr, "[rp] initial place phases: pin info %.3fs, B2B pairs %.3fs, CSR %.3fs, BiCGSTAB %.3fs\n",
+              t_pins, t_pairs, t_csr, t_solve);
     if (rc) return rc;
   }
   c->nb_ready = false;
@@ -903,53 +942,130 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
 
   // ---------------- device upload ----------------
   hipStream_t s = c->stream;
-  std::vector<int2> dhalf(ng);
-  std::vector<float> area(ng), npin(ng);
-  for (int g = 0; g < ng; g++) {
-    dhalf[g] = make_int2(c->g_dhx[g], c->g_dhy[g]);
-    area[g] = (float)c->g_dx[g] * (float)c->g_dy[g];
-    int i = c->g_inst[g];
-    npin[g] = i >= 0 ? (float)(c->ipin_start[i + 1] - c->ipin_start[i]) : 0.0f;
-  }
-  // nets renumbered by degree (stable): equal work per wavefront
+  // ---- locality order of the device arrays ----
+  // Movable cells are numbered in breadth-first order over the netlist
+  // (cells that share nets get nearby indices, so the scatter's per-block
+  // aggregation and the pin gathers of the WA / gradient kernels touch
+  // nearby memory); fillers keep their order after them.  Nets are grouped
+  // by degree bucket (equal work per wavefront) and ordered by the lowest
+  // new index of their cells.  Pin order inside a net and inside a cell is
+  // the reference order, so every per-net and per-cell float sum keeps its
+  // reference summation order.
   const int nn = c->n_net, npn = c->n_pin;
+  {
+    std::vector<int> cell_net_start(ngi + 1, 0), cell_nets;
+    for (int g = 0; g < ngi; g++) {
+      int i = c->g_inst[g];
+      cell_net_start[g + 1] = cell_net_start[g] + (c->ipin_start[i + 1] - c->ipin_start[i]);
+    }
+    std::vector<int> pin_net_ref(npn);
+    for (int e = 0; e < nn; e++)
+      for (int q = c->net_start[e]; q < c->net_start[e + 1]; q++) pin_net_ref[q] = e;
+    cell_nets.resize(cell_net_start[ngi]);
+    for (int g = 0; g < ngi; g++) {
+      int i = c->g_inst[g], o = cell_net_start[g];
+      for (int k = c->ipin_start[i]; k < c->ipin_start[i + 1]; k++) cell_nets[o++] = pin_net_ref[c->ipins[k]];
+    }
+    std::vector<uint8_t> cell_seen(ngi, 0), net_seen(nn, 0);
+    c->gperm.clear();
+    c->gperm.reserve(ng);
+    std::vector<int> queue;
+    queue.reserve(ngi);
+    for (int root = 0; root < ngi; root++) {
+      if (cell_seen[root]) continue;
+      size_t head = queue.size();
+      queue.push_back(root);
+      cell_seen[root] = 1;
+      while (head < queue.size()) {
+        int g = queue[head++];
+        c->gperm.push_back(g);
+        for (int k = cell_net_start[g]; k < cell_net_start[g + 1]; k++) {
+          int e = cell_nets[k];
+          if (net_seen[e]) continue;
+          net_seen[e] = 1;
+          for (int q = c->net_start[e]; q < c->net_start[e + 1]; q++) {
+            int i = c->pin_inst[q];
+            int h = i >= 0 ? c->inst2place[i] : -1;
+            if (h >= 0 && !cell_seen[h]) {
+              cell_seen[h] = 1;
+              queue.push_back(h);
+            }
+          }
+        }
+      }
+    }
+    for (int g = ngi; g < ng; g++) c->gperm.push_back(g);
+    c->ginv.assign(ng, 0);
+    for (int k = 0; k < ng; k++) c->ginv[c->gperm[k]] = k;
+  }
+  std::vector<int2> dhalf(ng);
+  std::vector<float> area(ng), npin(ng), scale(ng);
+  std::vector<uint8_t> kind(ng);
+  for (int k = 0; k < ng; k++) {
+    int g = c->gperm[k];
+    dhalf[k] = make_int2(c->g_dhx[g], c->g_dhy[g]);
+    area[k] = (float)c->g_dx[g] * (float)c->g_dy[g];
+    int i = c->g_inst[g];
+    npin[k] = i >= 0 ? (float)(c->ipin_start[i + 1] - c->ipin_start[i]) : 0.0f;
+    scale[k] = c->g_scale[g];
+    kind[k] = c->g_kind[g];
+  }
+  auto bucket = [](int deg) {
+    int b = 0;
+    while (b < 12 && (2 << b) < deg) b++;  // 1-2, 3-4, 5-8, 9-16, ...
+    return b;
+  };
+  std::vector<int> net_key(nn, INT_MAX);
+  for (int e = 0; e < nn; e++)
+    for (int q = c->net_start[e]; q < c->net_start[e + 1]; q++) {
+      int i = c->pin_inst[q];
+      int g = i >= 0 ? c->inst2place[i] : -1;
+      if (g >= 0) net_key[e] = std::min(net_key[e], c->ginv[g]);
+    }
+  // nets in locality order, cut into one contiguous chunk per XCD (matching
+  // the kernel's XCD-contiguous block ranges, see xcd_block), each chunk
+  // sorted by degree bucket so every XCD gets the same mix of net sizes and
+  // every wavefront nets of similar size
   c->net_perm.resize(nn);
   std::iota(c->net_perm.begin(), c->net_perm.end(), 0);
-  std::stable_sort(c->net_perm.begin(), c->net_perm.end(), [&](int a, int b) {
-    return (c->net_start[a + 1] - c->net_start[a]) < (c->net_start[b + 1] - c->net_start[b]);
-  });
-  std::vector<int> net_new(nn), nstart(nn + 1, 0), pin_new(npn);
-  for (int k = 0; k < nn; k++) net_new[c->net_perm[k]] = k;
+  std::stable_sort(c->net_perm.begin(), c->net_perm.end(),
+                   [&](int a, int b) { return net_key[a] < net_key[b]; });
+  {
+    const int nblk = (nn + RP_BLOCK - 1) / RP_BLOCK, kXcd = 8;
+    int per = nblk / kXcd, rem = nblk % kXcd, at = 0;
+    for (int x = 0; x < kXcd && at < nn; x++) {
+      int blocks = per + (x < rem ? 1 : 0);
+      int end = std::min(nn, at + blocks * RP_BLOCK);
+      std::stable_sort(c->net_perm.begin() + at, c->net_perm.begin() + end, [&](int a, int b) {
+        return bucket(c->net_start[a + 1] - c->net_start[a]) < bucket(c->net_start[b + 1] - c->net_start[b]);
+      });
+      at = end;
+    }
+  }
+  std::vector<int> nstart(nn + 1, 0);
   for (int k = 0; k < nn; k++) {
     int e = c->net_perm[k];
     nstart[k + 1] = nstart[k] + (c->net_start[e + 1] - c->net_start[e]);
   }
   std::vector<int> pcell(npn);
   std::vector<int2> ppos(npn);
-  std::vector<int> pin_net(npn);
+  std::vector<int> pin_dev(npn);
   for (int k = 0; k < nn; k++) {
     int e = c->net_perm[k];
     for (int q = c->net_start[e], m = nstart[k]; q < c->net_start[e + 1]; q++, m++) {
-      pin_new[q] = m;
-      pin_net[q] = k;
+      pin_dev[q] = m;
       int i = c->pin_inst[q];
       int g = i >= 0 ? c->inst2place[i] : -1;
-      pcell[m] = g;
+      pcell[m] = g >= 0 ? c->ginv[g] : -1;
       ppos[m] = g >= 0 ? make_int2(c->pin_offx[q], c->pin_offy[q]) : make_int2(c->pin_cx[q], c->pin_cy[q]);
     }
   }
-  std::vector<int> cp_start(ng + 1, 0), cp_net;
-  std::vector<int2> cp_off;
-  for (int g = 0; g < ng; g++) {
-    int i = c->g_inst[g];
-    if (i >= 0) {
-      for (int k = c->ipin_start[i]; k < c->ipin_start[i + 1]; k++) {
-        int q = c->ipins[k];
-        cp_net.push_back(pin_net[q]);
-        cp_off.push_back(make_int2(c->pin_offx[q], c->pin_offy[q]));
-      }
-    }
-    cp_start[g + 1] = (int)cp_net.size();
+  std::vector<int> cp_start(ng + 1, 0), cp_pin;
+  for (int k = 0; k < ng; k++) {
+    int i = c->g_inst[c->gperm[k]];
+    if (i >= 0)
+      for (int j = c->ipin_start[i]; j < c->ipin_start[i + 1]; j++) cp_pin.push_back(pin_dev[c->ipins[j]]);
+    cp_start[k + 1] = (int)cp_pin.size();
   }
   std::vector<float2> twx, tw2x, twy, tw2y;
   make_twiddles(c->nbx, twx, tw2x);
@@ -966,8 +1082,8 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
     wy2[j] = wy[j] * wy[j];
   }
   HIPCHK(c, dupload(c->d_dhalf, dhalf, s));
-  HIPCHK(c, dupload(c->d_scale, c->g_scale, s));
-  HIPCHK(c, dupload(c->d_kind, c->g_kind, s));
+  HIPCHK(c, dupload(c->d_scale, scale, s));
+  HIPCHK(c, dupload(c->d_kind, kind, s));
   HIPCHK(c, dupload(c->d_area, area, s));
   HIPCHK(c, dupload(c->d_npin, npin, s));
   HIPCHK(c, dalloc(c->d_slp, (size_t)3 * ng));
@@ -976,8 +1092,8 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   HIPCHK(c, dalloc(c->d_ic, ng));
   HIPCHK(c, hipMemsetAsync(c->d_sumg.p, 0, sizeof(float2) * 3 * ng, s));
   HIPCHK(c, dupload(c->d_cp_start, cp_start, s));
-  HIPCHK(c, dupload(c->d_cp_net, cp_net, s));
-  HIPCHK(c, dupload(c->d_cp_off, cp_off, s));
+  HIPCHK(c, dupload(c->d_cp_pin, cp_pin, s));
+  HIPCHK(c, dalloc(c->d_pgrad, std::max(1, npn)));
   HIPCHK(c, dupload(c->d_nstart, nstart, s));
   HIPCHK(c, dupload(c->d_pcell, pcell, s));
   HIPCHK(c, dupload(c->d_ppos, ppos, s));
@@ -1043,8 +1159,8 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   d.co = c->d_co.p;
   d.ic = c->d_ic.p;
   d.cp_start = c->d_cp_start.p;
-  d.cp_net = c->d_cp_net.p;
-  d.cp_off = c->d_cp_off.p;
+  d.cp_pin = c->d_cp_pin.p;
+  d.pgrad = c->d_pgrad.p;
   d.nstart = c->d_nstart.p;
   d.pcell = c->d_pcell.p;
   d.ppos = c->d_ppos.p;
@@ -1168,7 +1284,7 @@ extern "C" int rp_eval_gradient(rp_ctx* c, const float* cx, const float* cy, flo
   S.wl_coef_x = wlx;
   S.wl_coef_y = wly;
   std::vector<float2> xy(ng);
-  for (int g = 0; g < ng; g++) xy[g] = make_float2(cx[g], cy[g]);
+  for (int k = 0; k < ng; k++) xy[k] = make_float2(cx[c->gperm[k]], cy[c->gperm[k]]);
   HIPCHK(c, hipMemcpyAsync(c->d_slp.p + (size_t)ng, xy.data(), sizeof(float2) * ng, hipMemcpyHostToDevice, s));
   HIPCHK(c, hipMemcpyAsync(c->d_state.p, &S, sizeof S, hipMemcpyHostToDevice, s));
   HIPCHK(c, hipMemsetAsync(c->d_bin_inst.p, 0, sizeof(unsigned long long) * nb, s));
@@ -1182,7 +1298,7 @@ extern "C" int rp_eval_gradient(rp_ctx* c, const float* cx, const float* cy, flo
   if (o->bin_filler)
     HIPCHK(c, hipMemcpyAsync(o->bin_filler, c->d_bin_filler.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, s));
   HIPCHK(c, rp_launch_poisson(d, s, 1, 0));
-  HIPCHK(c, rp_launch_wa_grad(d, s));
+  HIPCHK(c, rp_launch_wa_grad(d, s, 1));
   std::vector<float2> field(nb), sg(ng), wl(ng), dn(ng);
   std::vector<double> pg((size_t)RP_P_COUNT * c->h_state->num_partials_grad), pb(c->h_state->num_partials_bin);
   std::vector<long long> pn(std::max(1, c->h_state->num_partials_net));
@@ -1207,18 +1323,19 @@ extern "C" int rp_eval_gradient(rp_ctx* c, const float* cx, const float* cy, flo
     if (o->ex) o->ex[b] = field[b].x;
     if (o->ey) o->ey[b] = field[b].y;
   }
-  for (int g = 0; g < ng; g++) {
+  for (int k = 0; k < ng; k++) {
+    int g = c->gperm[k];
     if (o->sum_grad) {
-      o->sum_grad[2 * g] = sg[g].x;
-      o->sum_grad[2 * g + 1] = sg[g].y;
+      o->sum_grad[2 * g] = sg[k].x;
+      o->sum_grad[2 * g + 1] = sg[k].y;
     }
     if (o->wl_grad) {
-      o->wl_grad[2 * g] = wl[g].x;
-      o->wl_grad[2 * g + 1] = wl[g].y;
+      o->wl_grad[2 * g] = wl[k].x;
+      o->wl_grad[2 * g + 1] = wl[k].y;
     }
     if (o->dens_grad) {
-      o->dens_grad[2 * g] = dn[g].x;
-      o->dens_grad[2 * g + 1] = dn[g].y;
+      o->dens_grad[2 * g] = dn[k].x;
+      o->dens_grad[2 * g + 1] = dn[k].y;
     }
   }
   double sums[3] = {0, 0, 0};
@@ -1324,7 +1441,7 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
   S.step_length = 0.0f;
   S.coeff = 0.0f;
   std::vector<float2> xy(ng);
-  for (int g = 0; g < ng; g++) xy[g] = make_float2(c->g_icx[g], c->g_icy[g]);
+  for (int k = 0; k < ng; k++) xy[k] = make_float2(c->g_icx[c->gperm[k]], c->g_icy[c->gperm[k]]);
   HIPCHK(c, hipEventRecord(e0, s));
   for (int slot : {0, 2}) {  // curSLP = prevSLP = initial density centres
     HIPCHK(c, hipMemcpyAsync(c->d_slp.p + (size_t)slot * ng, xy.data(), sizeof(float2) * ng, hipMemcpyHostToDevice, s));
@@ -1338,12 +1455,12 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
   HIPCHK(c, rp_launch_scatter(d, s));
   HIPCHK(c, rp_launch_poisson(d, s, 0, 0));
   HIPCHK(c, rp_launch_ctrl_init(d, s, 0));
-  HIPCHK(c, rp_launch_wa_grad(d, s));
+  HIPCHK(c, rp_launch_wa_grad(d, s, 0));
   HIPCHK(c, rp_launch_ctrl_init(d, s, 1));
   // eval B at prevSLP = clamp(curSLP + 100 * g)
   HIPCHK(c, rp_launch_scatter(d, s));
   HIPCHK(c, rp_launch_poisson(d, s, 0, 0));
-  HIPCHK(c, rp_launch_wa_grad(d, s));
+  HIPCHK(c, rp_launch_wa_grad(d, s, 0));
   HIPCHK(c, rp_launch_ctrl_init(d, s, 2));
   // main loop: queue evaluation slots, poll the state between batches
   const int K = slots_per_poll(c);
@@ -1398,8 +1515,9 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
   HIPCHK(c, hipMemcpy(ic.data(), c->d_ic.p, sizeof(int2) * ng, hipMemcpyDeviceToHost));
   for (int g = 0; g < c->n_gi; g++) {
     int i = c->g_inst[g];
-    c->dblx[i] = ic[g].x - c->g_dhx[g];
-    c->dbly[i] = ic[g].y - c->g_dhy[g];
+    int k = c->ginv[g];
+    c->dblx[i] = ic[k].x - c->g_dhx[g];
+    c->dbly[i] = ic[k].y - c->g_dhy[g];
   }
   c->placed = true;
   int code = hs->phase == 2 ? hs->code : 0;

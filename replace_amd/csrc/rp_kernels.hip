@@ -106,6 +106,18 @@ __device__ __forceinline__ float clamp_in(float c, int h, int lo, int hi) {
   return v;
 }
 
+// XCD-aware block order.  MI355X deals workgroups round-robin to its 8 XCDs,
+// each with a private L2.  Hardware block b runs on XCD b % 8; this maps it
+// to logical block (b % 8) * per + min(b % 8, rem) + b / 8, so every XCD
+// walks a contiguous range of logical blocks (neighbouring cells / nets in
+// the locality order share its L2).  A bijection on [0, nblocks).
+#define RP_NUM_XCD 8
+__device__ __forceinline__ int xcd_block(int b, int nblocks) {
+  int per = nblocks / RP_NUM_XCD, rem = nblocks % RP_NUM_XCD;
+  int x = b % RP_NUM_XCD, j = b / RP_NUM_XCD;
+  return x * per + min(x, rem) + j;
+}
+
 // ---------------------------------------------------------------------------
 // radix-2 Stockham FFT in LDS (natural order in, natural order out).
 // dir = -1 forward (e^{-i}), +1 inverse (e^{+i}), unnormalised.
@@ -175,9 +187,16 @@ __device__ __forceinline__ float2 dct3_pre(float a_k, float a_nk, float b_k, flo
 // ---------------------------------------------------------------------------
 // 1. step + clamp + integer centre + density scatter
 // ---------------------------------------------------------------------------
-#define AGG_SLOTS 1024
+// LDS aggregation: each workgroup scatters RP_SCATTER_CELLS consecutive
+// cells (locality order, so mostly neighbours) into a hashed table of
+// AGG_SLOTS bins in LDS and flushes one int64 atomic per touched bin.
+// Colliding bins fall back to direct global atomics.  Integer sums are
+// order independent, so the result is exact and deterministic.
+#define AGG_SLOTS 4096
+#define SCATTER_PER_THREAD 4
+#define RP_SCATTER_CELLS (RP_BLOCK * SCATTER_PER_THREAD)
 
-__device__ __forceinline__ int agg_slot(int b) { return (int)(((unsigned)b * 2654435761u) >> 22); }
+__device__ __forceinline__ int agg_slot(int b) { return (int)(((unsigned)b * 2654435761u) >> 20); }
 
 __global__ void __launch_bounds__(RP_BLOCK) k_step_scatter(RpDev d) {
   __shared__ int s_key[AGG_SLOTS];
@@ -191,12 +210,15 @@ __global__ void __launch_bounds__(RP_BLOCK) k_step_scatter(RpDev d) {
     s_fill[k] = 0;
   }
   __syncthreads();
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < d.nc) {
+  const int dst = st->dst;
+  const int mode = st->mode;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  for (int r = 0; r < SCATTER_PER_THREAD; r++) {
+    int c = blk * RP_SCATTER_CELLS + r * RP_BLOCK + threadIdx.x;
+    if (c >= d.nc) break;
     int2 h = d.dhalf[c];
-    int dst = st->dst;
     float2 p;
-    if (st->mode == 0) {
+    if (mode == 0) {
       int cur = st->ring.cur;
       float2 x = d.slp[(size_t)cur * d.nc + c];
       float2 g = d.sumg[(size_t)cur * d.nc + c];
@@ -509,62 +531,6 @@ __device__ __forceinline__ int2 pin_pos(const RpDev& d, int q) {
   return o;
 }
 
-__global__ void __launch_bounds__(RP_BLOCK) k_wa(RpDev d) {
-  __shared__ long long sh_red[RP_BLOCK / 64];
-  const RpState* st = d.st;
-  if (st->phase != 0) return;
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  long long h = 0;
-  if (e < d.nn) {
-    float cx = st->wl_coef_x, cy = st->wl_coef_y;
-    int s = d.nstart[e], t = d.nstart[e + 1];
-    int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
-    for (int q = s; q < t; q++) {
-      int2 p = pin_pos(d, q);
-      lx = min(p.x, lx);
-      ly = min(p.y, ly);
-      ux = max(p.x, ux);
-      uy = max(p.y, uy);
-    }
-    float s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0;
-    for (int q = s; q < t; q++) {
-      int2 p = pin_pos(d, q);
-      float eMinX = (float)(lx - p.x) * cx;
-      float eMaxX = (float)(p.x - ux) * cx;
-      float eMinY = (float)(ly - p.y) * cy;
-      float eMaxY = (float)(p.y - uy) * cy;
-      if (eMinX > d.bar) {
-        float v = fast_exp(eMinX);
-        s0 += v;
-        s1 += (float)p.x * v;
-      }
-      if (eMaxX > d.bar) {
-        float v = fast_exp(eMaxX);
-        s2 += v;
-        s3 += (float)p.x * v;
-      }
-      if (eMinY > d.bar) {
-        float v = fast_exp(eMinY);
-        s4 += v;
-        s5 += (float)p.y * v;
-      }
-      if (eMaxY > d.bar) {
-        float v = fast_exp(eMaxY);
-        s6 += v;
-        s7 += (float)p.y * v;
-      }
-    }
-    d.nbox[e] = make_int4(lx, ly, ux, uy);
-    d.nsum[2 * (size_t)e] = make_float4(s0, s1, s2, s3);
-    d.nsum[2 * (size_t)e + 1] = make_float4(s4, s5, s6, s7);
-    if (t > s) h = (long long)((ux - lx) + (uy - ly));
-  }
-  long long tot = block_sum_ll(h, sh_red);
-  if (threadIdx.x == 0) d.part_net[blockIdx.x] = tot;
-}
-
-// ---------------------------------------------------------------------------
-// 7. gradients per gcell (src/nesterovPlace.cpp:206-273)
 // ---------------------------------------------------------------------------
 // getWireLengthGradientPinWA for one pin and one axis (src/nesterovBase.cpp:1383-1440):
 // returns gMin - gMax with the exponentials recomputed as the WA pass stored them.
@@ -586,31 +552,164 @@ __device__ __forceinline__ float wl_axis(int p, int lo, int hi, float coef, floa
   return gmin - gmax;
 }
 
+
+#define WA_CHUNK 8
+
+// Pin bbox update and WA exponent accumulation for one pin, in pin order
+// (src/nesterovBase.cpp:1316-1356).
+struct WaSums {
+  float s0, s1, s2, s3, s4, s5, s6, s7;
+};
+
+__device__ __forceinline__ void wa_pin(int2 p, int lx, int ly, int ux, int uy, float cx, float cy, float bar,
+                                       WaSums& w) {
+  float eMinX = (float)(lx - p.x) * cx;
+  float eMaxX = (float)(p.x - ux) * cx;
+  float eMinY = (float)(ly - p.y) * cy;
+  float eMaxY = (float)(p.y - uy) * cy;
+  if (eMinX > bar) {
+    float v = fast_exp(eMinX);
+    w.s0 += v;
+    w.s1 += (float)p.x * v;
+  }
+  if (eMaxX > bar) {
+    float v = fast_exp(eMaxX);
+    w.s2 += v;
+    w.s3 += (float)p.x * v;
+  }
+  if (eMinY > bar) {
+    float v = fast_exp(eMinY);
+    w.s4 += v;
+    w.s5 += (float)p.y * v;
+  }
+  if (eMaxY > bar) {
+    float v = fast_exp(eMaxY);
+    w.s6 += v;
+    w.s7 += (float)p.y * v;
+  }
+}
+
+// One thread per net.  Pin positions are gathered WA_CHUNK at a time with
+// independent loads (the first chunk stays in registers for the later
+// passes).  Pass 1: pin bbox; pass 2: the eight exponent sums in pin order;
+// pass 3: each pin's WA gradient gMin - gMax (getWireLengthGradientPinWA,
+// src/nesterovBase.cpp:1383-1440) from the net's final sums, stored per pin
+// so the cell gradient kernel gathers 8 bytes per pin.  The net bbox and
+// sums are stored only when store_nets is set (stage-level checks).
+__global__ void __launch_bounds__(RP_BLOCK) k_wa(RpDev d, int store_nets) {
+  __shared__ long long sh_red[RP_BLOCK / 64];
+  const RpState* st = d.st;
+  if (st->phase != 0) return;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  int e = blk * blockDim.x + threadIdx.x;
+  long long h = 0;
+  if (e < d.nn) {
+    float cx = st->wl_coef_x, cy = st->wl_coef_y;
+    int s = d.nstart[e], t = d.nstart[e + 1];
+    int deg = t - s;
+    int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
+    int2 pos[WA_CHUNK];
+#pragma unroll
+    for (int k = 0; k < WA_CHUNK; k++)
+      if (k < deg) pos[k] = pin_pos(d, s + k);
+#pragma unroll
+    for (int k = 0; k < WA_CHUNK; k++)
+      if (k < deg) {
+        lx = min(pos[k].x, lx);
+        ly = min(pos[k].y, ly);
+        ux = max(pos[k].x, ux);
+        uy = max(pos[k].y, uy);
+      }
+    for (int b = s + WA_CHUNK; b < t; b += WA_CHUNK) {
+      int2 pp[WA_CHUNK];
+#pragma unroll
+      for (int k = 0; k < WA_CHUNK; k++)
+        if (b + k < t) pp[k] = pin_pos(d, b + k);
+#pragma unroll
+      for (int k = 0; k < WA_CHUNK; k++)
+        if (b + k < t) {
+          lx = min(pp[k].x, lx);
+          ly = min(pp[k].y, ly);
+          ux = max(pp[k].x, ux);
+          uy = max(pp[k].y, uy);
+        }
+    }
+    WaSums w = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < WA_CHUNK; k++)
+      if (k < deg) wa_pin(pos[k], lx, ly, ux, uy, cx, cy, d.bar, w);
+    for (int b = s + WA_CHUNK; b < t; b += WA_CHUNK) {
+      int2 pp[WA_CHUNK];
+#pragma unroll
+      for (int k = 0; k < WA_CHUNK; k++)
+        if (b + k < t) pp[k] = pin_pos(d, b + k);
+#pragma unroll
+      for (int k = 0; k < WA_CHUNK; k++)
+        if (b + k < t) wa_pin(pp[k], lx, ly, ux, uy, cx, cy, d.bar, w);
+    }
+#pragma unroll
+    for (int k = 0; k < WA_CHUNK; k++)
+      if (k < deg)
+        d.pgrad[s + k] = make_float2(wl_axis(pos[k].x, lx, ux, cx, w.s0, w.s1, w.s2, w.s3, d.bar),
+                                     wl_axis(pos[k].y, ly, uy, cy, w.s4, w.s5, w.s6, w.s7, d.bar));
+    for (int b = s + WA_CHUNK; b < t; b += WA_CHUNK) {
+      int2 pp[WA_CHUNK];
+#pragma unroll
+      for (int k = 0; k < WA_CHUNK; k++)
+        if (b + k < t) pp[k] = pin_pos(d, b + k);
+#pragma unroll
+      for (int k = 0; k < WA_CHUNK; k++)
+        if (b + k < t)
+          d.pgrad[b + k] = make_float2(wl_axis(pp[k].x, lx, ux, cx, w.s0, w.s1, w.s2, w.s3, d.bar),
+                                       wl_axis(pp[k].y, ly, uy, cy, w.s4, w.s5, w.s6, w.s7, d.bar));
+    }
+    if (store_nets) {
+      d.nbox[e] = make_int4(lx, ly, ux, uy);
+      d.nsum[2 * (size_t)e] = make_float4(w.s0, w.s1, w.s2, w.s3);
+      d.nsum[2 * (size_t)e + 1] = make_float4(w.s4, w.s5, w.s6, w.s7);
+    }
+    if (t > s) h = (long long)((ux - lx) + (uy - ly));
+  }
+  long long tot = block_sum_ll(h, sh_red);
+  if (threadIdx.x == 0) d.part_net[blk] = tot;
+}
+
+// ---------------------------------------------------------------------------
+// 7. gradients per gcell (src/nesterovPlace.cpp:206-273)
+// The controller runs as its own one-workgroup kernel after this one: a
+// last-block-done fusion needs an agent-scope release per block, which on
+// MI355X writes back the XCD's L2 and costs far more than one launch.
 __global__ void __launch_bounds__(RP_BLOCK) k_grad(RpDev d) {
   __shared__ double sh_red[RP_BLOCK / 64];
   const RpState* st = d.st;
   if (st->phase != 0) return;
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  int c = blk * blockDim.x + threadIdx.x;
   double v_wl = 0.0, v_dens = 0.0, v_grad = 0.0, v_dc = 0.0, v_dg = 0.0;
   if (c < d.nc) {
-    float cx = st->wl_coef_x, cy = st->wl_coef_y;
     float lam = st->density_penalty;
     int dst = st->dst, ref = st->ring.cur;
     int2 icc = d.ic[c];
     // wire length
     float gx = 0.0f, gy = 0.0f;
     int k0 = d.cp_start[c], k1 = d.cp_start[c + 1];
-    for (int k = k0; k < k1; k++) {
-      int e = d.cp_net[k];
-      int2 o = d.cp_off[k];
-      int px = icc.x + o.x, py = icc.y + o.y;
-      int4 bx = d.nbox[e];
-      float4 sxv = d.nsum[2 * (size_t)e];
-      float4 syv = d.nsum[2 * (size_t)e + 1];
-      float tx = wl_axis(px, bx.x, bx.z, cx, sxv.x, sxv.y, sxv.z, sxv.w, d.bar);
-      float ty = wl_axis(py, bx.y, bx.w, cy, syv.x, syv.y, syv.z, syv.w, d.bar);
-      gx += tx;
-      gy += ty;
+    // per-pin WA gradients (written by k_wa), summed in the cell's pin order
+    // (getWireLengthGradientWA, src/nesterovBase.cpp:1363-1375)
+    for (int k = k0; k < k1; k += 4) {
+      int qq[4];
+      float2 pg[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (k + j < k1) qq[j] = d.cp_pin[k + j];
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (k + j < k1) pg[j] = d.pgrad[qq[j]];
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (k + j < k1) {
+          gx += pg[j].x;
+          gy += pg[j].y;
+        }
     }
     // field gather (getDensityGradient, src/nesterovBase.cpp:1458-1478)
     int2 h = d.dhalf[c];
@@ -662,21 +761,22 @@ __global__ void __launch_bounds__(RP_BLOCK) k_grad(RpDev d) {
   }
   double r;
   r = block_sum_d(v_wl, sh_red);
-  if (threadIdx.x == 0) d.part_grad[(size_t)blockIdx.x * RP_P_COUNT + RP_P_WL] = r;
+  if (threadIdx.x == 0) d.part_grad[(size_t)blk * RP_P_COUNT + RP_P_WL] = r;
   r = block_sum_d(v_dens, sh_red);
-  if (threadIdx.x == 0) d.part_grad[(size_t)blockIdx.x * RP_P_COUNT + RP_P_DENS] = r;
+  if (threadIdx.x == 0) d.part_grad[(size_t)blk * RP_P_COUNT + RP_P_DENS] = r;
   r = block_sum_d(v_grad, sh_red);
-  if (threadIdx.x == 0) d.part_grad[(size_t)blockIdx.x * RP_P_COUNT + RP_P_GRAD] = r;
+  if (threadIdx.x == 0) d.part_grad[(size_t)blk * RP_P_COUNT + RP_P_GRAD] = r;
   r = block_sum_d(v_dc, sh_red);
-  if (threadIdx.x == 0) d.part_grad[(size_t)blockIdx.x * RP_P_COUNT + RP_P_DCOORD] = r;
+  if (threadIdx.x == 0) d.part_grad[(size_t)blk * RP_P_COUNT + RP_P_DCOORD] = r;
   r = block_sum_d(v_dg, sh_red);
-  if (threadIdx.x == 0) d.part_grad[(size_t)blockIdx.x * RP_P_COUNT + RP_P_DGRAD] = r;
+  if (threadIdx.x == 0) d.part_grad[(size_t)blk * RP_P_COUNT + RP_P_DGRAD] = r;
 }
 
 // ---------------------------------------------------------------------------
 // 8. controller
 // ---------------------------------------------------------------------------
 // Fixed-order reduction of `n` partials with stride `stride` (1 workgroup).
+// Fixed-order reductions of the per-block partials (one workgroup).
 __device__ double reduce_parts_d(const double* p, int n, int stride, int off, double* sh) {
   double v = 0.0;
   for (int k = threadIdx.x; k < n; k += blockDim.x) v += p[(size_t)k * stride + off];
@@ -695,14 +795,22 @@ struct Totals {
 };
 
 __device__ void reduce_all(const RpDev& d, Totals& t) {
-  __shared__ double shd[RP_BLOCK / 64];
-  __shared__ long long shl[RP_BLOCK / 64];
+  __shared__ double shd[RP_CTRL_BLOCK / 64];
+  __shared__ long long shl[RP_CTRL_BLOCK / 64];
   const RpState* st = d.st;
-  t.wl = reduce_parts_d(d.part_grad, st->num_partials_grad, RP_P_COUNT, RP_P_WL, shd);
-  t.dens = reduce_parts_d(d.part_grad, st->num_partials_grad, RP_P_COUNT, RP_P_DENS, shd);
-  t.grad = reduce_parts_d(d.part_grad, st->num_partials_grad, RP_P_COUNT, RP_P_GRAD, shd);
-  t.dc = reduce_parts_d(d.part_grad, st->num_partials_grad, RP_P_COUNT, RP_P_DCOORD, shd);
-  t.dg = reduce_parts_d(d.part_grad, st->num_partials_grad, RP_P_COUNT, RP_P_DGRAD, shd);
+  // one pass over the gradient partials: each thread sums a fixed subset
+  double v[RP_P_COUNT] = {0, 0, 0, 0, 0};
+  const int ng = st->num_partials_grad;
+  for (int k = threadIdx.x; k < ng; k += blockDim.x) {
+    const double* q = d.part_grad + (size_t)k * RP_P_COUNT;
+#pragma unroll
+    for (int j = 0; j < RP_P_COUNT; j++) v[j] += q[j];
+  }
+  t.wl = block_sum_d(v[RP_P_WL], shd);
+  t.dens = block_sum_d(v[RP_P_DENS], shd);
+  t.grad = block_sum_d(v[RP_P_GRAD], shd);
+  t.dc = block_sum_d(v[RP_P_DCOORD], shd);
+  t.dg = block_sum_d(v[RP_P_DGRAD], shd);
   t.ovf = reduce_parts_d(d.part_bin, st->num_partials_bin, 1, 0, shd);
   t.hpwl = reduce_parts_ll(d.part_net, st->num_partials_net, shl);
 }
@@ -748,7 +856,7 @@ __device__ void begin_iteration(RpState* s) {
 // stage: 0 = after init eval A's Poisson solve (before WA)
 //        1 = after init eval A's gradients
 //        2 = after init eval B's gradients
-__global__ void k_control_init(RpDev d, int stage) {
+__global__ void __launch_bounds__(RP_CTRL_BLOCK) k_control_init(RpDev d, int stage) {
   Totals t;
   reduce_all(d, t);
   if (threadIdx.x != 0) return;
@@ -794,7 +902,7 @@ __global__ void k_control_init(RpDev d, int stage) {
   begin_iteration(s);
 }
 
-__global__ void k_control_main(RpDev d) {
+__global__ void __launch_bounds__(RP_CTRL_BLOCK) k_control_main(RpDev d) {
   if (d.st->phase != 0) return;
   Totals t;
   reduce_all(d, t);
@@ -969,6 +1077,7 @@ static int fft_threads(int n) {
 // recorded around each kernel (per-kernel timing on this stream).
 hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_t* ev) {
   int nb_c = (d.nc + RP_BLOCK - 1) / RP_BLOCK;
+  int nb_s = (d.nc + RP_SCATTER_CELLS - 1) / RP_SCATTER_CELLS;
   int nb_n = (d.nn + RP_BLOCK - 1) / RP_BLOCK;
   int k = 0;
 #define RP_MARK()                            \
@@ -977,7 +1086,7 @@ hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_
     k++;                                     \
   } while (0)
   RP_MARK();
-  hipLaunchKernelGGL(k_step_scatter, dim3(nb_c), dim3(RP_BLOCK), 0, s, d);
+  hipLaunchKernelGGL(k_step_scatter, dim3(nb_s), dim3(RP_BLOCK), 0, s, d);
   RP_MARK();
   size_t shx = sizeof(float2) * 2 * d.nbx;
   hipLaunchKernelGGL(k_binfin_rowdct, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shx, s, d, 0);
@@ -994,24 +1103,25 @@ hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_
   size_t shr = sizeof(float2) * 2 * d.nbx + sizeof(float) * 2 * d.nbx;
   hipLaunchKernelGGL(k_row_inverse, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shr, s, d, want_phi);
   RP_MARK();
-  hipLaunchKernelGGL(k_wa, dim3(nb_n), dim3(RP_BLOCK), 0, s, d);
+  hipLaunchKernelGGL(k_wa, dim3(nb_n), dim3(RP_BLOCK), 0, s, d, 0);
   RP_MARK();
   hipLaunchKernelGGL(k_grad, dim3(nb_c), dim3(RP_BLOCK), 0, s, d);
   RP_MARK();
-  hipLaunchKernelGGL(k_control_main, dim3(1), dim3(RP_BLOCK), 0, s, d);
+  hipLaunchKernelGGL(k_control_main, dim3(1), dim3(RP_CTRL_BLOCK), 0, s, d);
   RP_MARK();
 #undef RP_MARK
   return hipGetLastError();
 }
 
 hipError_t rp_launch_ctrl_init(const RpDev& d, hipStream_t s, int stage) {
-  hipLaunchKernelGGL(k_control_init, dim3(1), dim3(RP_BLOCK), 0, s, d, stage);
+  hipLaunchKernelGGL(k_control_init, dim3(1), dim3(RP_CTRL_BLOCK), 0, s, d, stage);
   return hipGetLastError();
 }
 
 // Split stages for the stage-level test entry points.
 hipError_t rp_launch_scatter(const RpDev& d, hipStream_t s) {
-  hipLaunchKernelGGL(k_step_scatter, dim3((d.nc + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, d);
+  hipLaunchKernelGGL(k_step_scatter, dim3((d.nc + RP_SCATTER_CELLS - 1) / RP_SCATTER_CELLS), dim3(RP_BLOCK), 0, s,
+                     d);
   return hipGetLastError();
 }
 
@@ -1029,8 +1139,8 @@ hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int fr
   return hipGetLastError();
 }
 
-hipError_t rp_launch_wa_grad(const RpDev& d, hipStream_t s) {
-  hipLaunchKernelGGL(k_wa, dim3((d.nn + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, d);
+hipError_t rp_launch_wa_grad(const RpDev& d, hipStream_t s, int store_nets) {
+  hipLaunchKernelGGL(k_wa, dim3((d.nn + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, d, store_nets);
   hipLaunchKernelGGL(k_grad, dim3((d.nc + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, d);
   return hipGetLastError();
 }
