@@ -31,17 +31,30 @@ HBM_PEAK_GBS = 8000.0
 
 
 def algorithmic_bytes(nc, npin, nn, nb):
-    """Bytes one launch of each kernel must move (inputs read once, outputs
-    written once; see DESIGN.md, per-kernel table)."""
+    """Bytes one launch of each kernel must move: its inputs read once and
+    its outputs written once (per-kernel table).
+      step_scatter   64 B/cell (read cur SLP, gradient, coordinate, half
+                     size, scale, kind; write next coordinate, next SLP,
+                     integer centre) + 16 B/bin (two int64 accumulators)
+      binfin_rowdct  48 B/bin (read inst, filler, nonPlace int64; write rho,
+                     the row transform, and the zeroed accumulators)
+      transpose       8 B/bin; col_spectral 12 B/bin; transpose2 16 B/bin;
+      row_inverse    16 B/bin (read Y1, Y2; write Ex, Ey)
+      wa             28 B/pin (cell id, offset, gathered centre, per-pin
+                     gradient) + 4 B/net
+      grad           64 B/cell + 12 B/pin (pin index, per-pin gradient)
+                     + 8 B/bin (field)
+      control        5 doubles per gradient block
+    """
     return {
         "step_scatter": 64 * nc + 16 * nb,
-        "binfin_rowdct": 28 * nb + 4 * nb,
+        "binfin_rowdct": 48 * nb,
         "transpose": 8 * nb,
         "col_spectral": 12 * nb,
         "transpose2": 16 * nb,
         "row_inverse": 16 * nb,
-        "wa": 20 * npin + 52 * nn,
-        "grad": 52 * nc + 60 * npin + 8 * nb,
+        "wa": 28 * npin + 4 * nn,
+        "grad": 64 * nc + 12 * npin + 8 * nb,
         "control": 8 * 5 * ((nc + 255) // 256),
     }
 
