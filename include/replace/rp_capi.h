@@ -255,6 +255,54 @@ int rp_eval_gradient(rp_ctx* ctx, const float* cx, const float* cy, float densit
 /* Poisson solve alone: density[B] (y-major) -> phi, ex, ey (y-major). */
 int rp_eval_poisson(rp_ctx* ctx, const float* density, float* phi, float* ex, float* ey);
 
+/* ------------------------------------------------------------------------
+ * Placement service (no reference counterpart: the reference places one
+ * design per process).  Many jobs share batched evaluation steps on one GPU:
+ * jobs are admitted in arrival order against an HBM budget, set up on host
+ * worker threads, run as continuous batches (each step evaluates every
+ * running job once through per-kernel block tables, captured in a HIP graph)
+ * and retired as they finish.  Each job's result is bit-identical to running
+ * it alone with rp_nesterov_place.  Used for the batched and served forms.
+ * ---------------------------------------------------------------------- */
+typedef struct rp_server rp_server;
+
+typedef struct rp_server_opts {
+  int64_t hbm_budget_bytes;  /* admission budget for job device memory */
+  int32_t max_active;        /* jobs set up or running at once */
+  int32_t setup_threads;     /* host workers for the per-job setup */
+  int32_t steps_per_poll;    /* batched steps per graph replay */
+  int32_t use_graph;         /* 1: replay a captured HIP graph */
+} rp_server_opts;
+
+typedef struct rp_job_result {
+  rp_nesterov_stats stats;
+  int32_t code;              /* 0 or REPL code, < 0 on setup errors */
+  int32_t done;
+  double arrival_s, admit_s, start_s, done_s;  /* seconds from rp_server_run start */
+  int64_t device_bytes;
+} rp_job_result;
+
+typedef struct rp_server_stats {
+  int64_t steps;             /* batched evaluation steps issued */
+  int32_t rebuilds;          /* block-table / graph rebuilds */
+  double wall_s;
+} rp_server_stats;
+
+void rp_server_opts_default(rp_server_opts* opts);
+int rp_server_create(const rp_server_opts* opts, rp_server** out);
+/* Deep-copies the netlist; the job becomes eligible arrival_s seconds after
+ * rp_server_run starts (0 = immediately). */
+int rp_server_submit(rp_server* s, const rp_netlist* netlist, const rp_params* params, double arrival_s,
+                     int32_t* job);
+/* Runs until every submitted job has finished. */
+int rp_server_run(rp_server* s);
+int rp_server_job(rp_server* s, int32_t job, rp_job_result* out);
+int rp_server_job_positions(rp_server* s, int32_t job, int32_t* lx, int32_t* ly);
+int rp_server_job_records(rp_server* s, int32_t job, rp_iter_record* out, int32_t capacity, int32_t* count);
+int rp_server_get_stats(rp_server* s, rp_server_stats* out);
+const char* rp_server_last_error(rp_server* s);
+void rp_server_destroy(rp_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,8 +12,8 @@ from typing import Optional
 
 import numpy as np
 
-from .netlist import (Netlist, RpEvalOut, RpIpStats, RpIterRecord, RpNbInfo, RpNesterovStats,
-                      RpParams, params_default)
+from .netlist import (Netlist, RpEvalOut, RpIpStats, RpIterRecord, RpJobResult, RpNbInfo,
+                      RpNesterovStats, RpParams, RpServerOpts, RpServerStats, params_default)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libreplace_amd.so")
@@ -55,6 +55,19 @@ def lib() -> ctypes.CDLL:
         L.rp_set_kernel_timing.argtypes = [vp, ctypes.c_int]
         L.rp_get_kernel_timing.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int64)]
         L.rp_get_ip_records.argtypes = [vp, vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+        L.rp_server_opts_default.argtypes = [ctypes.POINTER(RpServerOpts)]
+        L.rp_server_create.argtypes = [ctypes.POINTER(RpServerOpts), ctypes.POINTER(vp)]
+        L.rp_server_submit.argtypes = [vp, vp, ctypes.POINTER(RpParams), ctypes.c_double,
+                                       ctypes.POINTER(ctypes.c_int32)]
+        L.rp_server_run.argtypes = [vp]
+        L.rp_server_job.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(RpJobResult)]
+        L.rp_server_job_positions.argtypes = [vp, ctypes.c_int32, vp, vp]
+        L.rp_server_job_records.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(RpIterRecord),
+                                            ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+        L.rp_server_get_stats.argtypes = [vp, ctypes.POINTER(RpServerStats)]
+        L.rp_server_last_error.restype = ctypes.c_char_p
+        L.rp_server_last_error.argtypes = [vp]
+        L.rp_server_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -201,3 +214,81 @@ class Placer:
         ly = np.zeros(n, np.int32)
         self._check(lib().rp_get_positions(self.h, _p(lx), _p(ly)))
         return lx, ly
+
+
+class Server:
+    """Placement service: many jobs in continuous batches on one GPU."""
+
+    def __init__(self, hbm_budget_bytes=None, max_active=None, setup_threads=None,
+                 steps_per_poll=None, use_graph=None):
+        o = RpServerOpts()
+        lib().rp_server_opts_default(ctypes.byref(o))
+        for k, v in dict(hbm_budget_bytes=hbm_budget_bytes, max_active=max_active,
+                         setup_threads=setup_threads, steps_per_poll=steps_per_poll,
+                         use_graph=use_graph).items():
+            if v is not None:
+                setattr(o, k, int(v))
+        self.opts = o
+        h = ctypes.c_void_p()
+        r = lib().rp_server_create(ctypes.byref(o), ctypes.byref(h))
+        self.h = h
+        if r < 0:
+            raise ReplaceError(r, self.last_error())
+        self._nl = {}
+
+    def last_error(self) -> str:
+        m = lib().rp_server_last_error(self.h)
+        return m.decode() if m else ""
+
+    def submit(self, netlist: Netlist, params: RpParams, arrival_s: float = 0.0) -> int:
+        jid = ctypes.c_int32(0)
+        cnl = netlist.to_c()
+        r = lib().rp_server_submit(self.h, ctypes.byref(cnl), ctypes.byref(params),
+                                   float(arrival_s), ctypes.byref(jid))
+        if r < 0:
+            raise ReplaceError(r, self.last_error())
+        self._nl[jid.value] = netlist
+        return jid.value
+
+    def run(self) -> None:
+        r = lib().rp_server_run(self.h)
+        if r < 0:
+            raise ReplaceError(r, self.last_error())
+
+    def result(self, job: int) -> RpJobResult:
+        out = RpJobResult()
+        lib().rp_server_job(self.h, job, ctypes.byref(out))
+        return out
+
+    def positions(self, job: int):
+        n = self._nl[job].num_insts
+        lx = np.zeros(n, np.int32)
+        ly = np.zeros(n, np.int32)
+        r = lib().rp_server_job_positions(self.h, job, _p(lx), _p(ly))
+        if r < 0:
+            raise ReplaceError(r, self.last_error())
+        return lx, ly
+
+    def records(self, job: int):
+        cnt = ctypes.c_int32(0)
+        lib().rp_server_job_records(self.h, job, None, 0, ctypes.byref(cnt))
+        arr = (RpIterRecord * max(cnt.value, 1))()
+        lib().rp_server_job_records(self.h, job, arr, cnt.value, ctypes.byref(cnt))
+        return [(r.overflow, r.hpwl, r.step_length, r.density_penalty, r.num_back_track)
+                for r in arr[: cnt.value]]
+
+    def stats(self) -> RpServerStats:
+        st = RpServerStats()
+        lib().rp_server_get_stats(self.h, ctypes.byref(st))
+        return st
+
+    def close(self) -> None:
+        if self.h:
+            lib().rp_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,494 @@
+// rp_batch.cpp -- placement service on one GPU: continuous batching of many
+// placement jobs through shared evaluation steps.
+//
+// No reference counterpart: the reference places one design per process
+// (src/replace.cpp:73-128).  Each job here is exactly the reference's
+// placement (the same per-instance kernels and controller as rp_nesterov_place);
+// what is shared is the launch: every batched step runs one evaluation of
+// every running job through per-kernel block tables (rp_launch_eval_batch),
+// captured in a HIP graph.
+//
+// Scheduling loop (rp_server_run):
+//   1. jobs whose arrival time has passed are admitted in arrival order while
+//      their estimated device footprint fits the HBM budget and a slot is free;
+//      admitted jobs are set up (PlacerBase / NesterovBase derivations and
+//      upload) on host worker threads while the GPU keeps running;
+//   2. set-up jobs start: their NesterovPlace::init launches go on the
+//      service stream and they join the block tables;
+//   3. the graph of `steps_per_poll` batched steps is replayed;
+//   4. the phase word of every running job is gathered; finished jobs are
+//      collected (stats, records, positions), their memory is released and
+//      the budget is returned; the tables are rebuilt.
+// A job's arithmetic does not depend on which other jobs run beside it, so
+// its result is bit-identical alone and in any mix.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+
+#include "rp_ctx.h"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+struct Job {
+  int id = 0;
+  double arrival = 0;
+  std::vector<int32_t> rows, insts, nstart, pinst, poff, pbox, ips, ipn;
+  std::vector<uint8_t> fixed;
+  rp_netlist nl{};
+  rp_params prm{};
+  rp_ctx* ctx = nullptr;
+  enum { QUEUED, SETUP, READY, RUNNING, DONE, FAILED } status = QUEUED;
+  int code = 0;
+  std::string err;
+  double admit_s = -1, start_s = -1, done_s = -1;
+  size_t reserved = 0;
+  rp_nesterov_stats stats{};
+  std::vector<int32_t> lx, ly;
+  std::vector<rp_iter_record> rec;
+};
+
+// Device footprint estimate before setup (bytes per gcell / pin / net / bin of
+// the arrays in rp_device.h); replaced by the measured size after setup.
+size_t estimate_bytes(const rp_netlist& nl, const rp_params& p) {
+  size_t nc = (size_t)nl.num_insts, np = (size_t)nl.num_pins, nn = (size_t)nl.num_nets;
+  size_t bx = p.bin_grid_cnt_x > 0 ? p.bin_grid_cnt_x : 1024;
+  size_t by = p.bin_grid_cnt_y > 0 ? p.bin_grid_cnt_y : 1024;
+  return 120 * nc + 24 * np + 56 * nn + 64 * bx * by + (1 << 20);
+}
+
+// Block table with XCD-aware placement: hardware block b runs on XCD b % 8.
+// Small jobs go whole onto the least-loaded XCD (their working set shares one
+// L2); a job with more than 1/8 of all blocks is split into 8 contiguous
+// chunks exactly as xcd_block() splits a single-instance launch.
+void build_table(const std::vector<int>& counts, std::vector<int2>& tab) {
+  const int X = 8;
+  long total = 0;
+  for (int n : counts) total += n;
+  std::vector<std::vector<int2>> lists(X);
+  std::vector<long> load(X, 0);
+  std::vector<int> order(counts.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return counts[a] > counts[b]; });
+  for (int i : order) {
+    int n = counts[i];
+    if (n <= 0) continue;
+    if ((long)n * X > total && n >= X) {
+      int per = n / X, rem = n % X;
+      for (int x = 0; x < X; x++) {
+        int lo = x * per + std::min(x, rem), cnt = per + (x < rem ? 1 : 0);
+        for (int j = 0; j < cnt; j++) lists[x].push_back(make_int2(i, lo + j));
+        load[x] += cnt;
+      }
+    } else {
+      int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      for (int j = 0; j < n; j++) lists[x].push_back(make_int2(i, j));
+      load[x] += n;
+    }
+  }
+  size_t L = 0;
+  for (auto& l : lists) L = std::max(L, l.size());
+  tab.assign(L * X, make_int2(-1, 0));
+  for (int x = 0; x < X; x++)
+    for (size_t j = 0; j < lists[x].size(); j++) tab[j * X + x] = lists[x][j];
+}
+
+}  // namespace
+
+struct rp_server {
+  rp_server_opts opt{};
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<std::unique_ptr<Job>> jobs;
+  // tables of the running set
+  DevBuf<RpDev> d_devs;
+  DevBuf<int2> d_tab[RP_NUM_EVAL_KERNELS];
+  DevBuf<int> d_phase;
+  int* h_phase = nullptr;
+  size_t h_phase_cap = 0;
+  RpBatchLaunch launch{};
+  std::vector<int> running;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  // budget
+  size_t reserved = 0;
+  // setup workers
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<int> setup_queue;
+  bool stop = false;
+  std::vector<std::thread> workers;
+  rp_server_stats stats{};
+};
+
+static int sfail(rp_server* s, int code, const char* msg) {
+  s->err = msg;
+  return code;
+}
+
+#define SCHK(s, x)                                                         \
+  do {                                                                     \
+    hipError_t e_ = (x);                                                   \
+    if (e_ != hipSuccess) {                                                \
+      (s)->err = std::string(#x) + ": " + hipGetErrorString(e_);           \
+      return RP_ERR_HIP;                                                   \
+    }                                                                      \
+  } while (0)
+
+extern "C" void rp_server_opts_default(rp_server_opts* o) {
+  o->hbm_budget_bytes = (int64_t)200 << 30;
+  o->max_active = 512;
+  o->setup_threads = 8;
+  o->steps_per_poll = 8;
+  o->use_graph = 1;
+}
+
+static void setup_worker(rp_server* s) {
+  (void)hipSetDevice(s->device);
+  for (;;) {
+    int id;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv.wait(lk, [&] { return s->stop || !s->setup_queue.empty(); });
+      if (s->stop && s->setup_queue.empty()) return;
+      id = s->setup_queue.front();
+      s->setup_queue.pop_front();
+    }
+    Job& j = *s->jobs[id];
+    rp_ctx* c = nullptr;
+    int r = rp_create(&j.nl, &j.prm, &c);
+    if (r == 0) r = rp_initial_place(c, nullptr);
+    if (r == 0) r = rp_nesterov_setup(c, nullptr);
+    std::lock_guard<std::mutex> lk(s->mu);
+    j.ctx = c;
+    if (r != 0) {
+      j.code = r;
+      j.err = c ? rp_last_error(c) : "rp_create failed";
+      j.status = Job::FAILED;
+    } else {
+      j.status = Job::READY;
+    }
+  }
+}
+
+extern "C" int rp_server_create(const rp_server_opts* opt, rp_server** out) {
+  if (!out) return RP_ERR_ARG;
+  std::unique_ptr<rp_server> s(new rp_server());
+  if (opt)
+    s->opt = *opt;
+  else
+    rp_server_opts_default(&s->opt);
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    *out = s.release();
+    return sfail(*out, RP_ERR_NO_DEVICE, "no HIP device visible");
+  }
+  (void)hipGetDevice(&s->device);
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    *out = s.release();
+    return sfail(*out, RP_ERR_HIP, "hipStreamCreate failed");
+  }
+  int nt = std::max(1, s->opt.setup_threads);
+  rp_server* raw = s.get();
+  for (int k = 0; k < nt; k++) s->workers.emplace_back(setup_worker, raw);
+  *out = s.release();
+  return RP_OK;
+}
+
+extern "C" int rp_server_submit(rp_server* s, const rp_netlist* nl, const rp_params* p, double arrival_s,
+                                int32_t* job) {
+  if (!s || !nl || !p) return RP_ERR_ARG;
+  std::unique_ptr<Job> j(new Job());
+  j->id = (int)s->jobs.size();
+  j->arrival = arrival_s;
+  j->prm = *p;
+  auto cp = [](std::vector<int32_t>& v, const int32_t* a, size_t n) {
+    if (a) v.assign(a, a + n);
+  };
+  cp(j->rows, nl->row_box, 4 * (size_t)nl->num_rows);
+  cp(j->insts, nl->inst_box, 4 * (size_t)nl->num_insts);
+  j->fixed.assign(nl->inst_fixed, nl->inst_fixed + nl->num_insts);
+  cp(j->nstart, nl->net_pin_start, (size_t)nl->num_nets + 1);
+  cp(j->pinst, nl->pin_inst, (size_t)nl->num_pins);
+  cp(j->poff, nl->pin_off, 2 * (size_t)nl->num_pins);
+  cp(j->pbox, nl->pin_box, 4 * (size_t)nl->num_pins);
+  if (nl->inst_pin_start && nl->inst_pins) {
+    cp(j->ips, nl->inst_pin_start, (size_t)nl->num_insts + 1);
+    cp(j->ipn, nl->inst_pins, (size_t)nl->inst_pin_start[nl->num_insts]);
+  }
+  j->nl = *nl;
+  j->nl.row_box = j->rows.data();
+  j->nl.inst_box = j->insts.data();
+  j->nl.inst_fixed = j->fixed.data();
+  j->nl.net_pin_start = j->nstart.data();
+  j->nl.pin_inst = j->pinst.data();
+  j->nl.pin_off = j->poff.data();
+  j->nl.pin_box = nl->pin_box ? j->pbox.data() : nullptr;
+  j->nl.inst_pin_start = j->ips.empty() ? nullptr : j->ips.data();
+  j->nl.inst_pins = j->ips.empty() ? nullptr : j->ipn.data();
+  if (job) *job = j->id;
+  s->jobs.push_back(std::move(j));
+  return RP_OK;
+}
+
+static int rebuild(rp_server* s) {
+  const int n = (int)s->running.size();
+  std::vector<RpDev> devs(n);
+  std::vector<int> cnt[RP_NUM_EVAL_KERNELS];
+  size_t lb = 0, lc = 0, lr = 0;
+  for (int i = 0; i < n; i++) {
+    const RpDev& d = s->jobs[s->running[i]]->ctx->dev;
+    devs[i] = d;
+    int tx = (d.nbx + 31) / 32, ty = (d.nby + 31) / 32;
+    cnt[RP_K_STEP_SCATTER].push_back((d.nc + RP_SCATTER_CELLS - 1) / RP_SCATTER_CELLS);
+    cnt[RP_K_BINFIN_ROWDCT].push_back(d.nby / 2);
+    cnt[RP_K_TRANSPOSE].push_back(tx * ty);
+    cnt[RP_K_COL_SPECTRAL].push_back(d.nbx / 2);
+    cnt[RP_K_TRANSPOSE2].push_back(2 * tx * ty);
+    cnt[RP_K_ROW_INVERSE].push_back(d.nby / 2);
+    cnt[RP_K_WA].push_back((d.nn + RP_BLOCK - 1) / RP_BLOCK);
+    cnt[RP_K_GRAD].push_back((d.nc + RP_BLOCK - 1) / RP_BLOCK);
+    cnt[RP_K_CONTROL].push_back(1);
+    lb = std::max(lb, sizeof(float2) * 2 * d.nbx);
+    lc = std::max(lc, sizeof(float2) * 2 * d.nby + sizeof(float) * 4 * d.nby);
+    lr = std::max(lr, sizeof(float2) * 2 * d.nbx + sizeof(float) * 2 * d.nbx);
+  }
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->gexec = nullptr;
+  s->graph = nullptr;
+  SCHK(s, hipStreamSynchronize(s->stream));
+  if (s->d_devs.n < (size_t)std::max(1, n)) SCHK(s, dalloc(s->d_devs, std::max(1, 2 * n)));
+  if (n) SCHK(s, hipMemcpyAsync(s->d_devs.p, devs.data(), sizeof(RpDev) * n, hipMemcpyHostToDevice, s->stream));
+  RpBatchLaunch& L = s->launch;
+  L.devs = s->d_devs.p;
+  std::vector<int2> tabs[RP_NUM_EVAL_KERNELS];  // host sources live until the sync below
+  for (int k = 0; k < RP_NUM_EVAL_KERNELS; k++) {
+    std::vector<int2>& tab = tabs[k];
+    if (k == RP_K_CONTROL) {
+      for (int i = 0; i < n; i++) tab.push_back(make_int2(i, 0));
+    } else {
+      build_table(cnt[k], tab);
+    }
+    if (s->d_tab[k].n < std::max<size_t>(1, tab.size())) SCHK(s, dalloc(s->d_tab[k], std::max<size_t>(1, 2 * tab.size())));
+    if (!tab.empty())
+      SCHK(s, hipMemcpyAsync(s->d_tab[k].p, tab.data(), sizeof(int2) * tab.size(), hipMemcpyHostToDevice, s->stream));
+    L.tab[k] = s->d_tab[k].p;
+    L.nblk[k] = (int)tab.size();
+  }
+  L.lds_binfin = lb;
+  L.lds_col = lc;
+  L.lds_rinv = lr;
+  if (s->d_phase.n < (size_t)std::max(1, n)) SCHK(s, dalloc(s->d_phase, std::max(1, 2 * n)));
+  if (s->h_phase_cap < (size_t)std::max(1, n)) {
+    if (s->h_phase) (void)hipHostFree(s->h_phase);
+    s->h_phase_cap = (size_t)std::max(1, 2 * n);
+    SCHK(s, hipHostMalloc((void**)&s->h_phase, sizeof(int) * s->h_phase_cap, hipHostMallocDefault));
+  }
+  SCHK(s, hipStreamSynchronize(s->stream));
+  if (s->opt.use_graph && n > 0) {
+    SCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
+    for (int k = 0; k < std::max(1, s->opt.steps_per_poll); k++) {
+      hipError_t e = rp_launch_eval_batch(s->launch, s->stream);
+      if (e != hipSuccess) {
+        hipGraph_t g;
+        (void)hipStreamEndCapture(s->stream, &g);
+        s->err = hipGetErrorString(e);
+        return RP_ERR_HIP;
+      }
+    }
+    SCHK(s, hipStreamEndCapture(s->stream, &s->graph));
+    SCHK(s, hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
+  }
+  s->stats.rebuilds++;
+  return RP_OK;
+}
+
+extern "C" int rp_server_run(rp_server* s) {
+  if (!s) return RP_ERR_ARG;
+  const auto t0 = clk::now();
+  auto now_s = [&] { return std::chrono::duration<double>(clk::now() - t0).count(); };
+  std::vector<int> order(s->jobs.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return s->jobs[a]->arrival < s->jobs[b]->arrival; });
+  const size_t budget = (size_t)std::max<int64_t>(0, s->opt.hbm_budget_bytes);
+  size_t next = 0, done = 0, in_flight = 0;
+  size_t total = s->jobs.size();
+  bool dirty = false;
+  while (done < total) {
+    double t = now_s();
+    // 1. admission in arrival order (FIFO, no overtaking)
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      while (next < order.size()) {
+        Job& j = *s->jobs[order[next]];
+        if (j.arrival > t) break;
+        size_t est = estimate_bytes(j.nl, j.prm);
+        if (in_flight > 0 && (s->reserved + est > budget || (int)in_flight >= s->opt.max_active)) break;
+        j.reserved = est;
+        s->reserved += est;
+        j.status = Job::SETUP;
+        j.admit_s = t;
+        in_flight++;
+        s->setup_queue.push_back(j.id);
+        next++;
+      }
+    }
+    s->cv.notify_all();
+    // 2. start jobs whose setup finished
+    {
+      std::vector<int> ready, failed;
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (auto& jp : s->jobs) {
+          if (jp->status == Job::READY) ready.push_back(jp->id);
+          if (jp->status == Job::FAILED && jp->done_s < 0) failed.push_back(jp->id);
+        }
+      }
+      for (int id : failed) {
+        Job& j = *s->jobs[id];
+        j.done_s = now_s();
+        if (j.ctx) rp_destroy(j.ctx);
+        j.ctx = nullptr;
+        s->reserved -= j.reserved;
+        in_flight--;
+        done++;
+      }
+      for (int id : ready) {
+        Job& j = *s->jobs[id];
+        size_t actual = j.ctx->device_bytes();
+        s->reserved = s->reserved - j.reserved + actual;
+        j.reserved = actual;
+        int r = rp_nesterov_begin(j.ctx, s->stream);
+        if (r) {
+          j.code = r;
+          j.err = rp_last_error(j.ctx);
+          j.status = Job::FAILED;
+          continue;
+        }
+        j.status = Job::RUNNING;
+        j.start_s = now_s();
+        s->running.push_back(id);
+        dirty = true;
+      }
+    }
+    if (s->running.empty()) {
+      std::this_thread::sleep_for(std::chrono::microseconds(100));
+      continue;
+    }
+    if (dirty) {
+      int r = rebuild(s);
+      if (r) return r;
+      dirty = false;
+    }
+    // 3. batched steps
+    const int K = std::max(1, s->opt.steps_per_poll);
+    if (s->gexec) {
+      SCHK(s, hipGraphLaunch(s->gexec, s->stream));
+    } else {
+      for (int k = 0; k < K; k++) SCHK(s, rp_launch_eval_batch(s->launch, s->stream));
+    }
+    s->stats.steps += K;
+    // 4. phases; retire finished jobs
+    const int n = (int)s->running.size();
+    SCHK(s, rp_launch_gather_phase(s->d_devs.p, n, s->d_phase.p, s->stream));
+    SCHK(s, hipMemcpyAsync(s->h_phase, s->d_phase.p, sizeof(int) * n, hipMemcpyDeviceToHost, s->stream));
+    SCHK(s, hipStreamSynchronize(s->stream));
+    std::vector<int> keep;
+    for (int i = 0; i < n; i++) {
+      Job& j = *s->jobs[s->running[i]];
+      if (s->h_phase[i] == 0) {
+        keep.push_back(j.id);
+        continue;
+      }
+      double td = now_s();
+      j.code = rp_nesterov_collect(j.ctx, td - j.start_s);
+      j.stats = j.ctx->last_stats;
+      j.rec.resize(j.ctx->records.size());
+      for (size_t k = 0; k < j.rec.size(); k++) {
+        const RpIterRec& r = j.ctx->records[k];
+        j.rec[k] = rp_iter_record{r.overflow, r.step_length, r.density_penalty, r.num_back_track, r.hpwl};
+      }
+      j.lx.resize(j.nl.num_insts);
+      j.ly.resize(j.nl.num_insts);
+      rp_get_positions(j.ctx, j.lx.data(), j.ly.data());
+      rp_destroy(j.ctx);
+      j.ctx = nullptr;
+      s->reserved -= j.reserved;
+      j.done_s = td;
+      j.status = Job::DONE;
+      in_flight--;
+      done++;
+      dirty = true;
+    }
+    s->running.swap(keep);
+  }
+  s->stats.wall_s = now_s();
+  return RP_OK;
+}
+
+extern "C" int rp_server_job(rp_server* s, int32_t id, rp_job_result* out) {
+  if (!s || !out || id < 0 || id >= (int)s->jobs.size()) return RP_ERR_ARG;
+  const Job& j = *s->jobs[id];
+  memset(out, 0, sizeof *out);
+  out->stats = j.stats;
+  out->code = j.code;
+  out->done = j.status == Job::DONE;
+  out->arrival_s = j.arrival;
+  out->admit_s = j.admit_s;
+  out->start_s = j.start_s;
+  out->done_s = j.done_s;
+  out->device_bytes = (int64_t)j.reserved;
+  return RP_OK;
+}
+
+extern "C" int rp_server_job_positions(rp_server* s, int32_t id, int32_t* lx, int32_t* ly) {
+  if (!s || id < 0 || id >= (int)s->jobs.size() || !lx || !ly) return RP_ERR_ARG;
+  const Job& j = *s->jobs[id];
+  if (j.status != Job::DONE) return RP_ERR_STATE;
+  std::copy(j.lx.begin(), j.lx.end(), lx);
+  std::copy(j.ly.begin(), j.ly.end(), ly);
+  return RP_OK;
+}
+
+extern "C" int rp_server_job_records(rp_server* s, int32_t id, rp_iter_record* out, int32_t cap, int32_t* count) {
+  if (!s || id < 0 || id >= (int)s->jobs.size() || !count) return RP_ERR_ARG;
+  const Job& j = *s->jobs[id];
+  *count = (int32_t)j.rec.size();
+  for (int k = 0; k < std::min<int>(cap, *count) && out; k++) out[k] = j.rec[k];
+  return RP_OK;
+}
+
+extern "C" int rp_server_get_stats(rp_server* s, rp_server_stats* out) {
+  if (!s || !out) return RP_ERR_ARG;
+  *out = s->stats;
+  return RP_OK;
+}
+
+extern "C" const char* rp_server_last_error(rp_server* s) { return s ? s->err.c_str() : "null server"; }
+
+extern "C" void rp_server_destroy(rp_server* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv.notify_all();
+  for (auto& t : s->workers) t.join();
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (auto& j : s->jobs)
+    if (j->ctx) rp_destroy(j->ctx);
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  if (s->h_phase) (void)hipHostFree(s->h_phase);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}

@@ -1,0 +1,154 @@
+// rp_ctx.h -- the placement context behind rp_ctx* (private to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/replace/rp_capi.h"
+#include "rp_dev_struct.h"
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct rp_ctx {
+  rp_params prm;
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // ---- PlacerBase ----
+  int core_lx = 0, core_ly = 0, core_ux = 0, core_uy = 0;
+  int site_w = 0, site_h = 0;
+  int n_real = 0;
+  std::vector<int> ilx, ily, iux, iuy, mw, mh, dblx, dbly;  // incl. dummies
+  std::vector<uint8_t> fixed;
+  std::vector<int> place, inst2place, nonplace;
+  int64_t place_area = 0, nonplace_area = 0, macro_area = 0, std_area = 0;
+  int n_pin = 0, n_net = 0;
+  std::vector<int> net_start, pin_inst, pin_offx, pin_offy, pin_cx, pin_cy;
+  std::vector<int> pin_box;  // 4 per pin (BTerms)
+  std::vector<int> ipin_start, ipins;
+  // ---- initial place stats ----
+  std::vector<float> ip_err;
+  std::vector<int64_t> ip_hpwl;
+  // ---- NesterovBase ----
+  bool nb_ready = false;
+  int n_g = 0, n_gi = 0, n_gf = 0;
+  int filler_dx = 0, filler_dy = 0;
+  int nbx = 0, nby = 0, bsx = 0, bsy = 0;
+  float td = 0;
+  std::vector<int> g_inst;
+  std::vector<uint8_t> g_kind;
+  std::vector<int> g_dx, g_dy, g_dhx, g_dhy;
+  std::vector<float> g_scale, g_icx, g_icy;
+  std::vector<int64_t> nonplace_bins;
+  // ---- device ----
+  DevBuf<int2> d_dhalf;
+  DevBuf<float> d_scale, d_area, d_npin;
+  DevBuf<uint8_t> d_kind;
+  DevBuf<float2> d_slp, d_sumg, d_co;
+  DevBuf<int2> d_ic;
+  DevBuf<int> d_cp_start, d_cp_pin, d_nstart, d_pcell;
+  DevBuf<int2> d_ppos;
+  DevBuf<float2> d_pgrad;
+  DevBuf<int4> d_nbox;
+  DevBuf<float4> d_nsum;
+  DevBuf<unsigned long long> d_bin_inst, d_bin_filler;
+  DevBuf<long long> d_bin_nonplace;
+  DevBuf<float> d_rho, d_phi, d_t, d_tt, d_y1t, d_y2t, d_y1, d_y2;
+  DevBuf<float2> d_field;
+  DevBuf<float2> d_twx, d_tw2x, d_twy, d_tw2y;
+  DevBuf<float> d_wx, d_wx2, d_wy, d_wy2;
+  DevBuf<double> d_part_grad, d_part_bin;
+  DevBuf<long long> d_part_net;
+  DevBuf<RpState> d_state;
+  DevBuf<RpIterRec> d_rec;
+  DevBuf<float2> d_dbg_wl, d_dbg_dens;
+  RpState* h_state = nullptr;  // pinned
+  std::vector<float2> init_xy; // initial centres (device order), kept alive for async upload
+  std::vector<int> net_perm;   // device net id -> reference net id
+  std::vector<int> gperm;      // device gcell index -> reference gcell index
+  std::vector<int> ginv;       // reference gcell index -> device gcell index
+  RpDev dev;
+  // ---- per-kernel timing (optional) ----
+  bool timing = false;
+  double ktime_ms[RP_NUM_EVAL_KERNELS] = {0};
+  long long kcount = 0;
+  // ---- results ----
+  std::vector<RpIterRec> records;
+  rp_nesterov_stats last_stats;
+  bool placed = false;
+
+  // device bytes held by this context (HBM admission accounting)
+  size_t device_bytes() const {
+    size_t b = 0;
+    auto add = [&b](const auto& buf) { b += buf.n * sizeof(*buf.p); };
+    add(d_dhalf); add(d_scale); add(d_area); add(d_npin); add(d_kind); add(d_slp); add(d_sumg); add(d_co);
+    add(d_ic); add(d_cp_start); add(d_cp_pin); add(d_nstart); add(d_pcell); add(d_ppos); add(d_pgrad);
+    add(d_nbox); add(d_nsum); add(d_bin_inst); add(d_bin_filler); add(d_bin_nonplace); add(d_rho); add(d_phi);
+    add(d_t); add(d_tt); add(d_y1t); add(d_y2t); add(d_y1); add(d_y2); add(d_field); add(d_twx); add(d_tw2x);
+    add(d_twy); add(d_tw2y); add(d_wx); add(d_wx2); add(d_wy); add(d_wy2); add(d_part_grad); add(d_part_bin);
+    add(d_part_net); add(d_state); add(d_rec); add(d_dbg_wl); add(d_dbg_dens);
+    return b;
+  }
+};
+
+static inline int fail(rp_ctx* c, int code, const char* fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return code;
+}
+
+#define HIPCHK(c, x)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) return fail((c), RP_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+static inline hipError_t dalloc(DevBuf<T>& b, size_t n) {
+  b.release();
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)&b.p, n * sizeof(T));
+  if (e == hipSuccess) b.n = n;
+  return e;
+}
+
+template <typename T>
+static inline hipError_t dupload(DevBuf<T>& b, const std::vector<T>& v, hipStream_t s) {
+  hipError_t e = dalloc(b, v.size());
+  if (e != hipSuccess || v.empty()) return e;
+  return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+
+// launch wrappers (rp_kernels.hip)
+hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_t* ev);
+hipError_t rp_launch_ctrl_init(const RpDev& d, hipStream_t s, int stage);
+hipError_t rp_launch_scatter(const RpDev& d, hipStream_t s);
+hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int from_rho);
+hipError_t rp_launch_wa_grad(const RpDev& d, hipStream_t s, int store_nets);
+hipError_t rp_launch_eval_batch(const RpBatchLaunch& b, hipStream_t s);
+hipError_t rp_launch_gather_phase(const RpDev* devs, int n, int* out, hipStream_t s);
+
+// context-level steps shared by rp_nesterov_place and the batch engine
+int rp_nesterov_begin(rp_ctx* c, hipStream_t s);
+int rp_nesterov_collect(rp_ctx* c, double seconds);

@@ -58,3 +58,12 @@ struct RpDev {
   float2* dbg_wl;
   float2* dbg_dens;
 };
+
+// One batched evaluation step over many instances (kb_* kernels): per kernel
+// a block table of (instance, logical block) entries (instance -1 = idle).
+struct RpBatchLaunch {
+  const RpDev* devs;
+  const int2* tab[RP_NUM_EVAL_KERNELS];
+  int nblk[RP_NUM_EVAL_KERNELS];
+  size_t lds_binfin, lds_col, lds_rinv;
+};

@@ -30,6 +30,8 @@
 
 #define RP_BLOCK 256
 #define RP_CTRL_BLOCK 1024  // controller workgroup (fixed-order reductions)
+#define RP_SCATTER_PER_THREAD 4
+#define RP_SCATTER_CELLS (RP_BLOCK * RP_SCATTER_PER_THREAD)  // cells per scatter workgroup
 #define RP_MAX_FFT 2048
 
 // Kernels of one evaluation slot, in launch order (per-kernel timing index).

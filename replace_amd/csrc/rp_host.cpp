@@ -45,126 +45,12 @@ hipError_t rp_launch_bicg_xr(int n, float* x, float* r, const float* p, const fl
                              double alpha, double w, hipStream_t s);
 hipError_t rp_launch_axpy(int n, float* y, const float* a, double alpha, const float* b, hipStream_t s);
 
-namespace {
+#include "rp_ctx.h"
 
+namespace {
 constexpr long double kSqrt2 = 1.414213562373095048801L;  // src/nesterovBase.cpp:13
 constexpr long double kPi = 3.141592653589793238462L;     // src/fft.cpp:9
-
-template <typename T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  ~DevBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-};
-
 }  // namespace
-
-struct rp_ctx {
-  rp_params prm;
-  std::string err;
-  int device = 0;
-  hipStream_t stream = nullptr;
-  // ---- PlacerBase ----
-  int core_lx = 0, core_ly = 0, core_ux = 0, core_uy = 0;
-  int site_w = 0, site_h = 0;
-  int n_real = 0;
-  std::vector<int> ilx, ily, iux, iuy, mw, mh, dblx, dbly;  // incl. dummies
-  std::vector<uint8_t> fixed;
-  std::vector<int> place, inst2place, nonplace;
-  int64_t place_area = 0, nonplace_area = 0, macro_area = 0, std_area = 0;
-  int n_pin = 0, n_net = 0;
-  std::vector<int> net_start, pin_inst, pin_offx, pin_offy, pin_cx, pin_cy;
-  std::vector<int> pin_box;  // 4 per pin (BTerms)
-  std::vector<int> ipin_start, ipins;
-  // ---- initial place stats ----
-  std::vector<float> ip_err;
-  std::vector<int64_t> ip_hpwl;
-  // ---- NesterovBase ----
-  bool nb_ready = false;
-  int n_g = 0, n_gi = 0, n_gf = 0;
-  int filler_dx = 0, filler_dy = 0;
-  int nbx = 0, nby = 0, bsx = 0, bsy = 0;
-  float td = 0;
-  std::vector<int> g_inst;
-  std::vector<uint8_t> g_kind;
-  std::vector<int> g_dx, g_dy, g_dhx, g_dhy;
-  std::vector<float> g_scale, g_icx, g_icy;
-  std::vector<int64_t> nonplace_bins;
-  // ---- device ----
-  DevBuf<int2> d_dhalf;
-  DevBuf<float> d_scale, d_area, d_npin;
-  DevBuf<uint8_t> d_kind;
-  DevBuf<float2> d_slp, d_sumg, d_co;
-  DevBuf<int2> d_ic;
-  DevBuf<int> d_cp_start, d_cp_pin, d_nstart, d_pcell;
-  DevBuf<int2> d_ppos;
-  DevBuf<float2> d_pgrad;
-  DevBuf<int4> d_nbox;
-  DevBuf<float4> d_nsum;
-  DevBuf<unsigned long long> d_bin_inst, d_bin_filler;
-  DevBuf<long long> d_bin_nonplace;
-  DevBuf<float> d_rho, d_phi, d_t, d_tt, d_y1t, d_y2t, d_y1, d_y2;
-  DevBuf<float2> d_field;
-  DevBuf<float2> d_twx, d_tw2x, d_twy, d_tw2y;
-  DevBuf<float> d_wx, d_wx2, d_wy, d_wy2;
-  DevBuf<double> d_part_grad, d_part_bin;
-  DevBuf<long long> d_part_net;
-  DevBuf<RpState> d_state;
-  DevBuf<RpIterRec> d_rec;
-  DevBuf<float2> d_dbg_wl, d_dbg_dens;
-  RpState* h_state = nullptr;  // pinned
-  std::vector<int> net_perm;   // device net id -> reference net id
-  std::vector<int> gperm;      // device gcell index -> reference gcell index
-  std::vector<int> ginv;       // reference gcell index -> device gcell index
-  RpDev dev;
-  // ---- per-kernel timing (optional) ----
-  bool timing = false;
-  double ktime_ms[RP_NUM_EVAL_KERNELS] = {0};
-  long long kcount = 0;
-  // ---- results ----
-  std::vector<RpIterRec> records;
-  rp_nesterov_stats last_stats;
-  bool placed = false;
-};
-
-static int fail(rp_ctx* c, int code, const char* fmt, ...) {
-  if (c) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    c->err = buf;
-  }
-  return code;
-}
-
-#define HIPCHK(c, x)                                                                     \
-  do {                                                                                   \
-    hipError_t e_ = (x);                                                                 \
-    if (e_ != hipSuccess) return fail((c), RP_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
-  } while (0)
-
-template <typename T>
-static hipError_t dalloc(DevBuf<T>& b, size_t n) {
-  b.release();
-  if (n == 0) n = 1;
-  hipError_t e = hipMalloc((void**)&b.p, n * sizeof(T));
-  if (e == hipSuccess) b.n = n;
-  return e;
-}
-
-template <typename T>
-static hipError_t dupload(DevBuf<T>& b, const std::vector<T>& v, hipStream_t s) {
-  hipError_t e = dalloc(b, v.size());
-  if (e != hipSuccess || v.empty()) return e;
-  return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
-}
 
 static inline int idiv_upper(int r, int s) {
   // ceil-like bin upper index of getMinMaxIdx / getDensityMinMaxIdx
@@ -1412,18 +1298,15 @@ static int slots_per_poll(const rp_ctx* c) {
   return ng >= 200000 ? 8 : 32;
 }
 
-extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
-  if (!c) return RP_ERR_ARG;
+// NesterovPlace::init (src/nesterovPlace.cpp:64-175) as device launches on
+// stream s: state and initial centres uploaded, the two lambda-0
+// evaluations and their controller stages queued.  No host wait.
+int rp_nesterov_begin(rp_ctx* c, hipStream_t s) {
   if (!c->nb_ready) {
     int r = rp_nesterov_setup(c, nullptr);
     if (r) return r;
   }
-  hipStream_t s = c->stream;
   const int ng = c->n_g;
-  hipEvent_t e0, e1;
-  HIPCHK(c, hipEventCreate(&e0));
-  HIPCHK(c, hipEventCreate(&e1));
-  // NesterovPlace::init (src/nesterovPlace.cpp:64-175)
   RpState& S = *c->h_state;
   S.phase = 0;
   S.code = 0;
@@ -1440,13 +1323,13 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
   S.max_phi_coef = c->prm.max_phi_coef;
   S.step_length = 0.0f;
   S.coeff = 0.0f;
-  std::vector<float2> xy(ng);
-  for (int k = 0; k < ng; k++) xy[k] = make_float2(c->g_icx[c->gperm[k]], c->g_icy[c->gperm[k]]);
-  HIPCHK(c, hipEventRecord(e0, s));
-  for (int slot : {0, 2}) {  // curSLP = prevSLP = initial density centres
-    HIPCHK(c, hipMemcpyAsync(c->d_slp.p + (size_t)slot * ng, xy.data(), sizeof(float2) * ng, hipMemcpyHostToDevice, s));
-  }
-  HIPCHK(c, hipMemcpyAsync(c->d_co.p, xy.data(), sizeof(float2) * ng, hipMemcpyHostToDevice, s));
+  S.iter = 0;
+  c->init_xy.resize(ng);
+  for (int k = 0; k < ng; k++) c->init_xy[k] = make_float2(c->g_icx[c->gperm[k]], c->g_icy[c->gperm[k]]);
+  for (int slot : {0, 2})  // curSLP = prevSLP = initial density centres
+    HIPCHK(c, hipMemcpyAsync(c->d_slp.p + (size_t)slot * ng, c->init_xy.data(), sizeof(float2) * ng,
+                             hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(c->d_co.p, c->init_xy.data(), sizeof(float2) * ng, hipMemcpyHostToDevice, s));
   HIPCHK(c, hipMemcpyAsync(c->d_state.p, &S, sizeof S, hipMemcpyHostToDevice, s));
   HIPCHK(c, hipMemsetAsync(c->d_bin_inst.p, 0, sizeof(unsigned long long) * c->nbx * c->nby, s));
   HIPCHK(c, hipMemsetAsync(c->d_bin_filler.p, 0, sizeof(unsigned long long) * c->nbx * c->nby, s));
@@ -1462,6 +1345,52 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
   HIPCHK(c, rp_launch_poisson(d, s, 0, 0));
   HIPCHK(c, rp_launch_wa_grad(d, s, 0));
   HIPCHK(c, rp_launch_ctrl_init(d, s, 2));
+  return RP_OK;
+}
+
+// Read back the final state, records and positions after the loop stopped
+// (updateDb, src/nesterovPlace.cpp:585-595: LL = dC - dD/2 of the last
+// evaluated iterate).  Returns the REPL code.
+int rp_nesterov_collect(rp_ctx* c, double seconds) {
+  const int ng = c->n_g;
+  RpState* hs = c->h_state;
+  HIPCHK(c, hipMemcpy(hs, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost));
+  int nrec = hs->iter;
+  c->records.resize(nrec);
+  if (nrec > 0) HIPCHK(c, hipMemcpy(c->records.data(), c->d_rec.p, sizeof(RpIterRec) * nrec, hipMemcpyDeviceToHost));
+  std::vector<int2> ic(ng);
+  HIPCHK(c, hipMemcpy(ic.data(), c->d_ic.p, sizeof(int2) * ng, hipMemcpyDeviceToHost));
+  for (int g = 0; g < c->n_gi; g++) {
+    int i = c->g_inst[g];
+    int k = c->ginv[g];
+    c->dblx[i] = ic[k].x - c->g_dhx[g];
+    c->dbly[i] = ic[k].y - c->g_dhy[g];
+  }
+  c->placed = true;
+  int code = hs->phase == 2 ? hs->code : 0;
+  rp_nesterov_stats& out = c->last_stats;
+  out.iterations = hs->iter;
+  out.evals = hs->evals;
+  out.finished = hs->phase == 1;
+  out.code = code;
+  out.final_overflow = hs->sum_overflow;
+  out.final_step_length = hs->step_length;
+  out.final_density_penalty = hs->density_penalty;
+  out.final_hpwl = hs->prev_hpwl;
+  out.seconds = seconds;
+  return code;
+}
+
+extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
+  if (!c) return RP_ERR_ARG;
+  hipStream_t s = c->stream;
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  HIPCHK(c, hipEventRecord(e0, s));
+  int r = rp_nesterov_begin(c, s);
+  if (r) return r;
+  const RpDev& d = c->dev;
   // main loop: queue evaluation slots, poll the state between batches
   const int K = slots_per_poll(c);
   RpState* hs = c->h_state;
@@ -1476,7 +1405,7 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
   HIPCHK(c, hipMemcpyAsync(hs, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
   int evals_before = hs->evals;
-  for (int b = 0; b < max_batches; b++) {
+  for (int b = 0; b < max_batches && hs->phase == 0; b++) {
     for (int k = 0; k < K; k++)
       HIPCHK(c, rp_launch_eval(d, s, 0, c->timing ? &kev[(size_t)k * (RP_NUM_EVAL_KERNELS + 1)] : nullptr));
     HIPCHK(c, hipMemcpyAsync(hs, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost, s));
@@ -1496,7 +1425,6 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
       c->kcount += std::max(0, ran);
     }
     evals_before = hs->evals;
-    if (hs->phase != 0) break;
   }
   for (auto& e : kev) (void)hipEventDestroy(e);
   HIPCHK(c, hipEventRecord(e1, s));
@@ -1506,33 +1434,8 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipEventDestroy(polled);
-  HIPCHK(c, hipMemcpy(hs, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost));
-  int nrec = hs->iter;
-  c->records.resize(nrec);
-  if (nrec > 0) HIPCHK(c, hipMemcpy(c->records.data(), c->d_rec.p, sizeof(RpIterRec) * nrec, hipMemcpyDeviceToHost));
-  // updateDb (src/nesterovPlace.cpp:585-595): LL = dC - dD/2 of the last evaluated iterate
-  std::vector<int2> ic(ng);
-  HIPCHK(c, hipMemcpy(ic.data(), c->d_ic.p, sizeof(int2) * ng, hipMemcpyDeviceToHost));
-  for (int g = 0; g < c->n_gi; g++) {
-    int i = c->g_inst[g];
-    int k = c->ginv[g];
-    c->dblx[i] = ic[k].x - c->g_dhx[g];
-    c->dbly[i] = ic[k].y - c->g_dhy[g];
-  }
-  c->placed = true;
-  int code = hs->phase == 2 ? hs->code : 0;
-  if (hs->phase == 2 && hs->code == 2) code = 2;
-  rp_nesterov_stats& out = c->last_stats;
-  out.iterations = hs->iter;
-  out.evals = hs->evals;
-  out.finished = hs->phase == 1;
-  out.code = code;
-  out.final_overflow = hs->sum_overflow;
-  out.final_step_length = hs->step_length;
-  out.final_density_penalty = hs->density_penalty;
-  out.final_hpwl = hs->prev_hpwl;
-  out.seconds = ms * 1e-3;
-  if (st) *st = out;
+  int code = rp_nesterov_collect(c, ms * 1e-3);
+  if (st) *st = c->last_stats;
   return code;
 }
 

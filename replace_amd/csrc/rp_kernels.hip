@@ -193,12 +193,10 @@ __device__ __forceinline__ float2 dct3_pre(float a_k, float a_nk, float b_k, flo
 // Colliding bins fall back to direct global atomics.  Integer sums are
 // order independent, so the result is exact and deterministic.
 #define AGG_SLOTS 4096
-#define SCATTER_PER_THREAD 4
-#define RP_SCATTER_CELLS (RP_BLOCK * SCATTER_PER_THREAD)
 
 __device__ __forceinline__ int agg_slot(int b) { return (int)(((unsigned)b * 2654435761u) >> 20); }
 
-__global__ void __launch_bounds__(RP_BLOCK) k_step_scatter(RpDev d) {
+__device__ void scatter_body(const RpDev& d, int blk) {
   __shared__ int s_key[AGG_SLOTS];
   __shared__ unsigned long long s_inst[AGG_SLOTS];
   __shared__ unsigned long long s_fill[AGG_SLOTS];
@@ -212,8 +210,7 @@ __global__ void __launch_bounds__(RP_BLOCK) k_step_scatter(RpDev d) {
   __syncthreads();
   const int dst = st->dst;
   const int mode = st->mode;
-  const int blk = xcd_block(blockIdx.x, gridDim.x);
-  for (int r = 0; r < SCATTER_PER_THREAD; r++) {
+  for (int r = 0; r < RP_SCATTER_PER_THREAD; r++) {
     int c = blk * RP_SCATTER_CELLS + r * RP_BLOCK + threadIdx.x;
     if (c >= d.nc) break;
     int2 h = d.dhalf[c];
@@ -285,14 +282,14 @@ __global__ void __launch_bounds__(RP_BLOCK) k_step_scatter(RpDev d) {
 // ---------------------------------------------------------------------------
 // 2. bin density + overflow + DCT-II along x (two rows per workgroup)
 // ---------------------------------------------------------------------------
-__global__ void k_binfin_rowdct(RpDev d, int from_rho) {
+__device__ void binfin_body(const RpDev& d, int blk, int from_rho) {
   extern __shared__ float2 smem[];
   __shared__ double sh_red[RP_BLOCK / 64];
   if (d.st->phase != 0) return;
   const int n = d.nbx;
   float2* A = smem;
   float2* B = smem + n;
-  int ja = 2 * blockIdx.x, jb = ja + 1;
+  int ja = 2 * blk, jb = ja + 1;
   double ov_sum = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     int blx, bux;
@@ -325,7 +322,7 @@ __global__ void k_binfin_rowdct(RpDev d, int from_rho) {
     A[mk_pos(i, n)] = make_float2(r2[0], r2[1]);
   }
   double tot = block_sum_d(ov_sum, sh_red);
-  if (threadIdx.x == 0) d.part_bin[blockIdx.x] = tot;
+  if (threadIdx.x == 0) d.part_bin[blk] = tot;
   // block_sum_d synchronised the LDS writes above
   float2* Z = lds_fft(A, B, n, d.twx, -1);
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
@@ -339,30 +336,12 @@ __global__ void k_binfin_rowdct(RpDev d, int from_rho) {
 // ---------------------------------------------------------------------------
 // 3. tiled transposes
 // ---------------------------------------------------------------------------
-__global__ void k_transpose(const float* __restrict__ src, float* __restrict__ dst, int rows, int cols,
-                            const RpState* st) {
+// 32x32 tile (tx0, ty0) of a rows x cols -> cols x rows transpose
+__device__ void transpose_body(const float* __restrict__ src, float* __restrict__ dst, int rows, int cols,
+                               const RpState* st, int tile_x, int tile_y) {
   __shared__ float tile[32][33];
   if (st->phase != 0) return;
-  int bx = blockIdx.x * 32, by = blockIdx.y * 32;
-  int tx = threadIdx.x, ty = threadIdx.y;
-  for (int k = 0; k < 32; k += 8) {
-    int r = by + ty + k, c = bx + tx;
-    if (r < rows && c < cols) tile[ty + k][tx] = src[(size_t)r * cols + c];
-  }
-  __syncthreads();
-  for (int k = 0; k < 32; k += 8) {
-    int r = bx + ty + k, c = by + tx;
-    if (r < cols && c < rows) dst[(size_t)r * rows + c] = tile[tx][ty + k];
-  }
-}
-
-__global__ void k_transpose2(const float* __restrict__ s1, float* __restrict__ d1, const float* __restrict__ s2,
-                             float* __restrict__ d2, int rows, int cols, const RpState* st) {
-  __shared__ float tile[32][33];
-  if (st->phase != 0) return;
-  const float* src = blockIdx.z ? s2 : s1;
-  float* dst = blockIdx.z ? d2 : d1;
-  int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  int bx = tile_x * 32, by = tile_y * 32;
   int tx = threadIdx.x, ty = threadIdx.y;
   for (int k = 0; k < 32; k += 8) {
     int r = by + ty + k, c = bx + tx;
@@ -380,7 +359,7 @@ __global__ void k_transpose2(const float* __restrict__ s1, float* __restrict__ d
 // DCT-II along y, FFT::doFFT's scaling and spectral divide
 // (src/fft.cpp:117-166), then DCT-III (phi) and DST-III (Ey) along y.
 // ---------------------------------------------------------------------------
-__global__ void k_col_spectral(RpDev d) {
+__device__ void col_body(const RpDev& d, int blk) {
   extern __shared__ float2 smem[];
   if (d.st->phase != 0) return;
   const int n = d.nby;
@@ -390,7 +369,7 @@ __global__ void k_col_spectral(RpDev d) {
   float* pb = pa + n;
   float* ea = pb + n;
   float* eb = ea + n;
-  int ua = 2 * blockIdx.x, ub = ua + 1;
+  int ua = 2 * blk, ub = ua + 1;
   for (int v = threadIdx.x; v < n; v += blockDim.x)
     A[mk_pos(v, n)] = make_float2(d.tt[(size_t)ua * n + v], d.tt[(size_t)ub * n + v]);
   __syncthreads();
@@ -455,7 +434,7 @@ __global__ void k_col_spectral(RpDev d) {
 // ---------------------------------------------------------------------------
 // 5. inverse along x: Ex = DST-III_x(wx * Y1), Ey = DCT-III_x(Y2), phi optional
 // ---------------------------------------------------------------------------
-__global__ void k_row_inverse(RpDev d, int want_phi) {
+__device__ void rowinv_body(const RpDev& d, int blk, int want_phi) {
   extern __shared__ float2 smem[];
   if (d.st->phase != 0) return;
   const int n = d.nbx;
@@ -463,7 +442,7 @@ __global__ void k_row_inverse(RpDev d, int want_phi) {
   float2* B = smem + n;
   float* ra = (float*)(smem + 2 * n);
   float* rb = ra + n;
-  int ja = 2 * blockIdx.x, jb = ja + 1;
+  int ja = 2 * blk, jb = ja + 1;
   // Ex: DST-III of (wx[u] * Y1[j][u])
   for (int u = threadIdx.x; u < n; u += blockDim.x) {
     ra[u] = d.wx[u] * d.y1[(size_t)ja * n + u];
@@ -596,11 +575,10 @@ __device__ __forceinline__ void wa_pin(int2 p, int lx, int ly, int ux, int uy, f
 // src/nesterovBase.cpp:1383-1440) from the net's final sums, stored per pin
 // so the cell gradient kernel gathers 8 bytes per pin.  The net bbox and
 // sums are stored only when store_nets is set (stage-level checks).
-__global__ void __launch_bounds__(RP_BLOCK) k_wa(RpDev d, int store_nets) {
+__device__ void wa_body(const RpDev& d, int blk, int store_nets) {
   __shared__ long long sh_red[RP_BLOCK / 64];
   const RpState* st = d.st;
   if (st->phase != 0) return;
-  const int blk = xcd_block(blockIdx.x, gridDim.x);
   int e = blk * blockDim.x + threadIdx.x;
   long long h = 0;
   if (e < d.nn) {
@@ -679,11 +657,10 @@ __global__ void __launch_bounds__(RP_BLOCK) k_wa(RpDev d, int store_nets) {
 // The controller runs as its own one-workgroup kernel after this one: a
 // last-block-done fusion needs an agent-scope release per block, which on
 // MI355X writes back the XCD's L2 and costs far more than one launch.
-__global__ void __launch_bounds__(RP_BLOCK) k_grad(RpDev d) {
+__device__ void grad_body(const RpDev& d, int blk) {
   __shared__ double sh_red[RP_BLOCK / 64];
   const RpState* st = d.st;
   if (st->phase != 0) return;
-  const int blk = xcd_block(blockIdx.x, gridDim.x);
   int c = blk * blockDim.x + threadIdx.x;
   double v_wl = 0.0, v_dens = 0.0, v_grad = 0.0, v_dc = 0.0, v_dg = 0.0;
   if (c < d.nc) {
@@ -902,7 +879,7 @@ __global__ void __launch_bounds__(RP_CTRL_BLOCK) k_control_init(RpDev d, int sta
   begin_iteration(s);
 }
 
-__global__ void __launch_bounds__(RP_CTRL_BLOCK) k_control_main(RpDev d) {
+__device__ void control_main_body(const RpDev& d) {
   if (d.st->phase != 0) return;
   Totals t;
   reduce_all(d, t);
@@ -997,6 +974,89 @@ __global__ void __launch_bounds__(RP_CTRL_BLOCK) k_control_main(RpDev d) {
   begin_iteration(s);
 }
 
+
+// ---------------------------------------------------------------------------
+// kernels: single-instance wrappers (XCD-aware block order) and batched
+// wrappers.  A batched launch covers many placement instances: entry b of
+// its block table names (instance, logical block) and the instance's device
+// pointers come from devs[instance].  Each instance sees exactly the logical
+// blocks, block size and reduction order of its single-instance launch, so
+// its result is bit-identical alone or in a batch.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(RP_BLOCK) k_step_scatter(RpDev d) {
+  scatter_body(d, xcd_block(blockIdx.x, gridDim.x));
+}
+__global__ void __launch_bounds__(RP_BLOCK) k_binfin_rowdct(RpDev d, int from_rho) {
+  binfin_body(d, blockIdx.x, from_rho);
+}
+__global__ void __launch_bounds__(RP_BLOCK) k_transpose(const float* __restrict__ src, float* __restrict__ dst,
+                                                        int rows, int cols, const RpState* st) {
+  transpose_body(src, dst, rows, cols, st, blockIdx.x, blockIdx.y);
+}
+__global__ void __launch_bounds__(RP_BLOCK) k_transpose2(RpDev d) {
+  if (blockIdx.z == 0)
+    transpose_body(d.y1t, d.y1, d.nbx, d.nby, d.st, blockIdx.x, blockIdx.y);
+  else
+    transpose_body(d.y2t, d.y2, d.nbx, d.nby, d.st, blockIdx.x, blockIdx.y);
+}
+__global__ void __launch_bounds__(RP_BLOCK) k_col_spectral(RpDev d) { col_body(d, blockIdx.x); }
+__global__ void __launch_bounds__(RP_BLOCK) k_row_inverse(RpDev d, int want_phi) {
+  rowinv_body(d, blockIdx.x, want_phi);
+}
+__global__ void __launch_bounds__(RP_BLOCK) k_wa(RpDev d, int store_nets) {
+  wa_body(d, xcd_block(blockIdx.x, gridDim.x), store_nets);
+}
+__global__ void __launch_bounds__(RP_BLOCK) k_grad(RpDev d) { grad_body(d, xcd_block(blockIdx.x, gridDim.x)); }
+__global__ void __launch_bounds__(RP_CTRL_BLOCK) k_control_main(RpDev d) { control_main_body(d); }
+
+__global__ void __launch_bounds__(RP_BLOCK) kb_step_scatter(const RpDev* devs, const int2* tab) {
+  int2 e = tab[blockIdx.x];
+  if (e.x >= 0) scatter_body(devs[e.x], e.y);
+}
+__global__ void __launch_bounds__(RP_BLOCK) kb_binfin_rowdct(const RpDev* devs, const int2* tab) {
+  int2 e = tab[blockIdx.x];
+  if (e.x >= 0) binfin_body(devs[e.x], e.y, 0);
+}
+// transposes: local index -> (tile_x, tile_y[, which])
+__global__ void __launch_bounds__(RP_BLOCK) kb_transpose(const RpDev* devs, const int2* tab) {
+  int2 e = tab[blockIdx.x];
+  if (e.x < 0) return;
+  const RpDev& d = devs[e.x];
+  int tx = (d.nbx + 31) / 32;
+  transpose_body(d.t, d.tt, d.nby, d.nbx, d.st, e.y % tx, e.y / tx);
+}
+__global__ void __launch_bounds__(RP_BLOCK) kb_transpose2(const RpDev* devs, const int2* tab) {
+  int2 e = tab[blockIdx.x];
+  if (e.x < 0) return;
+  const RpDev& d = devs[e.x];
+  int tx = (d.nby + 31) / 32, ty = (d.nbx + 31) / 32;
+  int which = e.y / (tx * ty), r = e.y % (tx * ty);
+  if (which == 0)
+    transpose_body(d.y1t, d.y1, d.nbx, d.nby, d.st, r % tx, r / tx);
+  else
+    transpose_body(d.y2t, d.y2, d.nbx, d.nby, d.st, r % tx, r / tx);
+}
+__global__ void __launch_bounds__(RP_BLOCK) kb_col_spectral(const RpDev* devs, const int2* tab) {
+  int2 e = tab[blockIdx.x];
+  if (e.x >= 0) col_body(devs[e.x], e.y);
+}
+__global__ void __launch_bounds__(RP_BLOCK) kb_row_inverse(const RpDev* devs, const int2* tab) {
+  int2 e = tab[blockIdx.x];
+  if (e.x >= 0) rowinv_body(devs[e.x], e.y, 0);
+}
+__global__ void __launch_bounds__(RP_BLOCK) kb_wa(const RpDev* devs, const int2* tab) {
+  int2 e = tab[blockIdx.x];
+  if (e.x >= 0) wa_body(devs[e.x], e.y, 0);
+}
+__global__ void __launch_bounds__(RP_BLOCK) kb_grad(const RpDev* devs, const int2* tab) {
+  int2 e = tab[blockIdx.x];
+  if (e.x >= 0) grad_body(devs[e.x], e.y);
+}
+__global__ void __launch_bounds__(RP_CTRL_BLOCK) kb_control_main(const RpDev* devs, const int2* tab) {
+  int2 e = tab[blockIdx.x];
+  if (e.x >= 0) control_main_body(devs[e.x]);
+}
+
 // ---------------------------------------------------------------------------
 // initial placement: CSR SpMV and BiCGSTAB vector kernels (FP64 dots)
 // ---------------------------------------------------------------------------
@@ -1066,12 +1126,9 @@ __global__ void k_bicg_xr(int n, float* x, float* r, const float* p, const float
 // ---------------------------------------------------------------------------
 // host-callable launch wrappers (C++ linkage, used by rp_host.cpp)
 // ---------------------------------------------------------------------------
-static int fft_threads(int n) {
-  int t = n / 2;
-  if (t > RP_BLOCK) t = RP_BLOCK;
-  if (t < 64) t = 64;
-  return t;
-}
+// FFT-based kernels always run RP_BLOCK threads: the per-thread partial
+// sums (overflow) then group identically in single and batched launches.
+static int fft_threads(int) { return RP_BLOCK; }
 
 // One evaluation slot.  ev (optional) holds RP_NUM_EVAL_KERNELS + 1 events
 // recorded around each kernel (per-kernel timing on this stream).
@@ -1097,8 +1154,7 @@ hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_
   size_t shy = sizeof(float2) * 2 * d.nby + sizeof(float) * 4 * d.nby;
   hipLaunchKernelGGL(k_col_spectral, dim3(d.nbx / 2), dim3(fft_threads(d.nby)), shy, s, d);
   RP_MARK();
-  hipLaunchKernelGGL(k_transpose2, dim3((d.nby + 31) / 32, (d.nbx + 31) / 32, 2), dim3(32, 8), 0, s,
-                     (const float*)d.y1t, d.y1, (const float*)d.y2t, d.y2, d.nbx, d.nby, (const RpState*)d.st);
+  hipLaunchKernelGGL(k_transpose2, dim3((d.nby + 31) / 32, (d.nbx + 31) / 32, 2), dim3(32, 8), 0, s, d);
   RP_MARK();
   size_t shr = sizeof(float2) * 2 * d.nbx + sizeof(float) * 2 * d.nbx;
   hipLaunchKernelGGL(k_row_inverse, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shr, s, d, want_phi);
@@ -1132,8 +1188,7 @@ hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int fr
                      (const float*)d.t, d.tt, d.nby, d.nbx, (const RpState*)d.st);
   size_t shy = sizeof(float2) * 2 * d.nby + sizeof(float) * 4 * d.nby;
   hipLaunchKernelGGL(k_col_spectral, dim3(d.nbx / 2), dim3(fft_threads(d.nby)), shy, s, d);
-  hipLaunchKernelGGL(k_transpose2, dim3((d.nby + 31) / 32, (d.nbx + 31) / 32, 2), dim3(32, 8), 0, s,
-                     (const float*)d.y1t, d.y1, (const float*)d.y2t, d.y2, d.nbx, d.nby, (const RpState*)d.st);
+  hipLaunchKernelGGL(k_transpose2, dim3((d.nby + 31) / 32, (d.nbx + 31) / 32, 2), dim3(32, 8), 0, s, d);
   size_t shr = sizeof(float2) * 2 * d.nbx + sizeof(float) * 2 * d.nbx;
   hipLaunchKernelGGL(k_row_inverse, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shr, s, d, want_phi);
   return hipGetLastError();
@@ -1174,5 +1229,43 @@ hipError_t rp_launch_bicg_xr(int n, float* x, float* r, const float* p, const fl
                              double alpha, double w, hipStream_t s) {
   hipLaunchKernelGGL(k_bicg_xr, dim3((n + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, n, x, r, p, sv, t,
                      alpha, w);
+  return hipGetLastError();
+}
+
+__global__ void k_gather_phase(const RpDev* devs, int n, int* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = devs[i].st->phase;
+}
+
+hipError_t rp_launch_gather_phase(const RpDev* devs, int n, int* out, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_gather_phase, dim3((n + 255) / 256), dim3(256), 0, s, devs, n, out);
+  return hipGetLastError();
+}
+
+// One evaluation step of every instance in the tables (same kernels, block
+// sizes and per-instance logical blocks as rp_launch_eval).
+hipError_t rp_launch_eval_batch(const RpBatchLaunch& b, hipStream_t s) {
+  const int* n = b.nblk;
+  if (n[RP_K_STEP_SCATTER])
+    hipLaunchKernelGGL(kb_step_scatter, dim3(n[RP_K_STEP_SCATTER]), dim3(RP_BLOCK), 0, s, b.devs,
+                       b.tab[RP_K_STEP_SCATTER]);
+  if (n[RP_K_BINFIN_ROWDCT])
+    hipLaunchKernelGGL(kb_binfin_rowdct, dim3(n[RP_K_BINFIN_ROWDCT]), dim3(RP_BLOCK), b.lds_binfin, s, b.devs,
+                       b.tab[RP_K_BINFIN_ROWDCT]);
+  if (n[RP_K_TRANSPOSE])
+    hipLaunchKernelGGL(kb_transpose, dim3(n[RP_K_TRANSPOSE]), dim3(32, 8), 0, s, b.devs, b.tab[RP_K_TRANSPOSE]);
+  if (n[RP_K_COL_SPECTRAL])
+    hipLaunchKernelGGL(kb_col_spectral, dim3(n[RP_K_COL_SPECTRAL]), dim3(RP_BLOCK), b.lds_col, s, b.devs,
+                       b.tab[RP_K_COL_SPECTRAL]);
+  if (n[RP_K_TRANSPOSE2])
+    hipLaunchKernelGGL(kb_transpose2, dim3(n[RP_K_TRANSPOSE2]), dim3(32, 8), 0, s, b.devs, b.tab[RP_K_TRANSPOSE2]);
+  if (n[RP_K_ROW_INVERSE])
+    hipLaunchKernelGGL(kb_row_inverse, dim3(n[RP_K_ROW_INVERSE]), dim3(RP_BLOCK), b.lds_rinv, s, b.devs,
+                       b.tab[RP_K_ROW_INVERSE]);
+  if (n[RP_K_WA]) hipLaunchKernelGGL(kb_wa, dim3(n[RP_K_WA]), dim3(RP_BLOCK), 0, s, b.devs, b.tab[RP_K_WA]);
+  if (n[RP_K_GRAD]) hipLaunchKernelGGL(kb_grad, dim3(n[RP_K_GRAD]), dim3(RP_BLOCK), 0, s, b.devs, b.tab[RP_K_GRAD]);
+  if (n[RP_K_CONTROL])
+    hipLaunchKernelGGL(kb_control_main, dim3(n[RP_K_CONTROL]), dim3(RP_CTRL_BLOCK), 0, s, b.devs,
+                       b.tab[RP_K_CONTROL]);
   return hipGetLastError();
 }

@@ -314,3 +314,34 @@ class Netlist:
             for a in (head, self.row_box, self.inst_box, self.inst_fixed.astype(np.int32),
                       self.net_pin_start, self.pin_inst, self.pin_off, pin_box, ips, ipn):
                 f.write(np.ascontiguousarray(a, dtype=np.int32).tobytes())
+
+
+class RpServerOpts(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", ctypes.c_int64),
+        ("max_active", ctypes.c_int32),
+        ("setup_threads", ctypes.c_int32),
+        ("steps_per_poll", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+    ]
+
+
+class RpJobResult(ctypes.Structure):
+    _fields_ = [
+        ("stats", RpNesterovStats),
+        ("code", ctypes.c_int32),
+        ("done", ctypes.c_int32),
+        ("arrival_s", ctypes.c_double),
+        ("admit_s", ctypes.c_double),
+        ("start_s", ctypes.c_double),
+        ("done_s", ctypes.c_double),
+        ("device_bytes", ctypes.c_int64),
+    ]
+
+
+class RpServerStats(ctypes.Structure):
+    _fields_ = [
+        ("steps", ctypes.c_int64),
+        ("rebuilds", ctypes.c_int32),
+        ("wall_s", ctypes.c_double),
+    ]

@@ -1,0 +1,80 @@
+"""Batched / served placement: every job's result (iteration records and
+final positions) is bit-identical to running the same job alone through
+rp_nesterov_place, whatever runs beside it, and jobs are admitted against
+the HBM budget."""
+import numpy as np
+import pytest
+
+from helpers import design
+from replace_amd import netgen
+from replace_amd.capi import Placer, Server
+
+pytestmark = pytest.mark.gpu
+
+
+def _alone(nl, p):
+    g = Placer(nl, p)
+    g.initial_place()
+    g.setup()
+    code, st = g.nesterov_place()
+    return code, st, g.records(), g.positions()
+
+
+def _jobs():
+    jobs = []
+    for name in ("simple01", "ar02", "incremental01"):
+        jobs.append(design(name))
+    for i, nc in enumerate((700, 2500, 9000)):
+        nl = netgen.generate(nc, int(nc * 1.05), seed=500 + i, util=0.65, td=0.7)
+        nb = netgen.next_pow2(int(np.ceil(np.sqrt(nc))))
+        p = netgen.params_default(target_density=0.7, bin_grid_cnt_x=max(32, nb),
+                                  bin_grid_cnt_y=max(32, nb), initial_place_max_iter=0)
+        jobs.append((nl, p))
+    return jobs
+
+
+def test_batch_results_identical_to_alone():
+    jobs = _jobs()
+    alone = [_alone(nl, p) for nl, p in jobs]
+    srv = Server(steps_per_poll=4)
+    ids = [srv.submit(nl, p, 0.0) for nl, p in jobs]
+    srv.run()
+    for jid, (code, st, rec, pos) in zip(ids, alone):
+        r = srv.result(jid)
+        assert r.done == 1 and r.code == code
+        assert srv.records(jid) == rec
+        lx, ly = srv.positions(jid)
+        assert np.array_equal(lx, pos[0]) and np.array_equal(ly, pos[1])
+        assert r.stats.final_hpwl == st.final_hpwl and r.stats.iterations == st.iterations
+
+
+def test_served_arrivals_under_tight_budget():
+    jobs = _jobs()
+    alone = [_alone(nl, p) for nl, p in jobs]
+    # budget below two jobs' footprint: jobs queue and are admitted as others retire
+    srv = Server(hbm_budget_bytes=48 << 20, steps_per_poll=2, setup_threads=2)
+    ids = [srv.submit(nl, p, 0.002 * k) for k, (nl, p) in enumerate(jobs)]
+    srv.run()
+    starts = []
+    for jid, (code, st, rec, pos) in zip(ids, alone):
+        r = srv.result(jid)
+        assert r.done == 1
+        assert r.start_s >= r.arrival_s
+        assert srv.records(jid) == rec
+        lx, ly = srv.positions(jid)
+        assert np.array_equal(lx, pos[0]) and np.array_equal(ly, pos[1])
+        starts.append((r.admit_s, r.done_s))
+    # admission is FIFO in arrival order
+    assert [a for a, _ in starts] == sorted(a for a, _ in starts)
+    assert srv.stats().rebuilds >= 2
+
+
+def test_batch_with_graph_and_without_agree():
+    jobs = _jobs()[3:]
+    out = []
+    for use_graph in (1, 0):
+        srv = Server(use_graph=use_graph, steps_per_poll=3)
+        ids = [srv.submit(nl, p) for nl, p in jobs]
+        srv.run()
+        out.append([srv.records(j) for j in ids])
+    assert out[0] == out[1]
