@@ -38,6 +38,7 @@ struct rp_ctx {
   int64_t place_area = 0, nonplace_area = 0, macro_area = 0, std_area = 0;
   int n_pin = 0, n_net = 0;
   std::vector<int> net_start, pin_inst, pin_offx, pin_offy, pin_cx, pin_cy;
+  std::vector<int> bfs_order;  // rp_movable_order
   std::vector<int> pin_box;  // 4 per pin (BTerms)
   std::vector<int> ipin_start, ipins;
   // ---- initial place stats ----
@@ -139,6 +140,33 @@ static inline hipError_t dupload(DevBuf<T>& b, const std::vector<T>& v, hipStrea
   return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
 }
 
+
+// InitialPlace B2B assembly on the device (rp_ip.hip)
+struct RpIpAsm {
+  int n = 0, n_net = 0, cb = 0;
+  DevBuf<int> net_start, pin_inst, pin_k, pin_cx, pin_cy, icx, icy;
+  DevBuf<int> cnt, off;  // 4 x (n_net + 1) per-net counts and their exclusive offsets
+  DevBuf<unsigned long long> tk[2], tk2[2];
+  DevBuf<float> tv[2], tv2[2];
+  DevBuf<unsigned> rk[2], rk2[2];
+  DevBuf<float> rv[2], rv2[2];
+  DevBuf<int> flag, seg, srow;
+  DevBuf<char> temp;
+  DevBuf<int> rowp[2], col[2];  // per-axis CSR and right-hand side
+  DevBuf<float> val[2], b[2];
+  int* h_tot = nullptr;
+  std::vector<int> h_pin_k, pos;  // pin -> row (locality order), movable index -> row
+  ~RpIpAsm() {
+    if (h_tot) (void)hipHostFree(h_tot);
+  }
+};
+int rp_ip_asm_init(rp_ctx* c, RpIpAsm& a);
+const std::vector<int>& rp_movable_order(rp_ctx* c);  // BFS locality order of movable instances
+int rp_ip_assemble(rp_ctx* c, RpIpAsm& a, const int* unused_x, const int* unused_y);
+// sequential host assembly of the same systems from the device's centres (rp_host.cpp)
+int rp_ip_host_assemble(rp_ctx* c, RpIpAsm& a);
+// the InitialPlace iterations on the device (rp_ip.hip)
+int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve);
 
 // launch wrappers (rp_kernels.hip)
 hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_t* ev);

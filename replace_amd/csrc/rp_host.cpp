@@ -34,16 +34,6 @@ hipError_t rp_launch_ctrl_init(const RpDev& d, hipStream_t s, int stage);
 hipError_t rp_launch_scatter(const RpDev& d, hipStream_t s);
 hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int from_rho);
 hipError_t rp_launch_wa_grad(const RpDev& d, hipStream_t s, int store_nets);
-hipError_t rp_launch_spmv(int n, const int* rowp, const int* col, const float* val, const float* x, float* y,
-                          hipStream_t s);
-hipError_t rp_launch_dot3(int n, const float* a0, const float* b0, const float* a1, const float* b1,
-                          const float* a2, const float* b2, double* parts, int nparts, double* out3,
-                          hipStream_t s);
-hipError_t rp_launch_bicg_p(int n, float* p, const float* r, const float* v, double beta, double w,
-                            hipStream_t s);
-hipError_t rp_launch_bicg_xr(int n, float* x, float* r, const float* p, const float* sv, const float* t,
-                             double alpha, double w, hipStream_t s);
-hipError_t rp_launch_axpy(int n, float* y, const float* a, double alpha, const float* b, hipStream_t s);
 
 #include "rp_ctx.h"
 
@@ -303,30 +293,6 @@ extern "C" void rp_destroy(rp_ctx* c) {
   delete c;
 }
 
-// inst-bbox HPWL over DB boxes and BPin boxes (PlacerBase::hpwl, src/placerBase.cpp:868-876, 480-500)
-static int64_t inst_hpwl(const rp_ctx* c) {
-  int64_t h = 0;
-  for (int n = 0; n < c->n_net; n++) {
-    int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
-    for (int q = c->net_start[n]; q < c->net_start[n + 1]; q++) {
-      int i = c->pin_inst[q];
-      if (i < 0) continue;
-      lx = std::min(c->dblx[i], lx);
-      ly = std::min(c->dbly[i], ly);
-      ux = std::max(c->dblx[i] + c->mw[i], ux);
-      uy = std::max(c->dbly[i] + c->mh[i], uy);
-    }
-    for (int q = c->net_start[n]; q < c->net_start[n + 1]; q++) {
-      if (c->pin_inst[q] >= 0 || c->pin_box[4 * (size_t)q] == INT_MAX) continue;
-      lx = std::min(c->pin_box[4 * (size_t)q], lx);
-      ly = std::min(c->pin_box[4 * (size_t)q + 1], ly);
-      ux = std::max(c->pin_box[4 * (size_t)q + 2], ux);
-      uy = std::max(c->pin_box[4 * (size_t)q + 3], uy);
-    }
-    h += (int64_t)((int)((unsigned)ux - (unsigned)lx) + (int)((unsigned)uy - (unsigned)ly));
-  }
-  return h;
-}
 
 // ===========================================================================
 // InitialPlace: host B2B assembly, device BiCGSTAB
@@ -394,88 +360,153 @@ void build_csr(int n, const std::vector<Trip>& t, Csr& m) {
   m.val.resize(nnz);
 }
 
-struct DevSolver {
-  int n = 0, nparts = 0;
-  DevBuf<int> rowp, col;
-  DevBuf<float> val, b, x, r, r0, v, p, s, t;
-  DevBuf<double> parts, dots;
-  double* h_dots = nullptr;
-};
-
 }  // namespace
 
-static int dev_dots(rp_ctx* c, DevSolver& S, const float* a0, const float* b0, const float* a1, const float* b1,
-                    const float* a2, const float* b2, double out[3]) {
-  HIPCHK(c, rp_launch_dot3(S.n, a0, b0, a1, b1, a2, b2, S.parts.p, S.nparts, S.dots.p, c->stream));
-  HIPCHK(c, hipMemcpyAsync(S.h_dots, S.dots.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  out[0] = S.h_dots[0];
-  out[1] = S.h_dots[1];
-  out[2] = S.h_dots[2];
-  return 0;
+// Locality order of the movable instances: breadth-first over the cell-net
+// graph (cells reached through a shared net get neighbouring indices), roots
+// in index order.  Device arrays of cells use it so that gathers through nets
+// stay within cache lines.  Depends on the netlist only; computed once.
+const std::vector<int>& rp_movable_order(rp_ctx* c) {
+  const int n = (int)c->place.size(), nn = c->n_net, npn = c->n_pin;
+  if ((int)c->bfs_order.size() == n) return c->bfs_order;
+  std::vector<int> cell_net_start(n + 1, 0), cell_nets;
+  for (int g = 0; g < n; g++) {
+    int i = c->place[g];
+    cell_net_start[g + 1] = cell_net_start[g] + (c->ipin_start[i + 1] - c->ipin_start[i]);
+  }
+  std::vector<int> pin_net_ref(npn);
+  for (int e = 0; e < nn; e++)
+    for (int q = c->net_start[e]; q < c->net_start[e + 1]; q++) pin_net_ref[q] = e;
+  cell_nets.resize(cell_net_start[n]);
+  for (int g = 0; g < n; g++) {
+    int i = c->place[g], o = cell_net_start[g];
+    for (int k = c->ipin_start[i]; k < c->ipin_start[i + 1]; k++) cell_nets[o++] = pin_net_ref[c->ipins[k]];
+  }
+  std::vector<uint8_t> cell_seen(n, 0), net_seen(nn, 0);
+  std::vector<int>& order = c->bfs_order;
+  order.clear();
+  order.reserve(n);
+  for (int root = 0; root < n; root++) {
+    if (cell_seen[root]) continue;
+    size_t head = order.size();
+    order.push_back(root);
+    cell_seen[root] = 1;
+    while (head < order.size()) {
+      int g = order[head++];
+      for (int k = cell_net_start[g]; k < cell_net_start[g + 1]; k++) {
+        int e = cell_nets[k];
+        if (net_seen[e]) continue;
+        net_seen[e] = 1;
+        for (int q = c->net_start[e]; q < c->net_start[e + 1]; q++) {
+          int i = c->pin_inst[q];
+          int h = i >= 0 ? c->inst2place[i] : -1;
+          if (h >= 0 && !cell_seen[h]) {
+            cell_seen[h] = 1;
+            order.push_back(h);
+          }
+        }
+      }
+    }
+  }
+  return order;
 }
 
-// BiCGSTAB, identity preconditioner (published algorithm, van der Vorst 1992,
-// in the variant that restarts when rho collapses); reference call site
-// src/initialPlace.cpp:78-86.  x (device) holds the warm start.
-static int dev_bicgstab(rp_ctx* c, DevSolver& S, const Csr& A, const std::vector<float>& b, std::vector<float>& x,
-                        int max_iter, float tol, float* err_out) {
-  int n = A.n;
+// Sequential host assembly of the B2B systems (updatePinInfo,
+// src/initialPlace.cpp:128-182, and createSparseMatrix, :185-378) from the
+// device's current pin and instance centres, in the device's row order
+// (a.h_pin_k); uploads the CSR and right-hand sides into `a`.  The device assembly builds the same bits; this path exists
+// so that tests can check it.
+int rp_ip_host_assemble(rp_ctx* c, RpIpAsm& a) {
+  const rp_params& p = c->prm;
+  const int n = a.n, np = c->n_pin;
   hipStream_t st = c->stream;
-  HIPCHK(c, dupload(S.rowp, A.rowp, st));
-  HIPCHK(c, dupload(S.col, A.col, st));
-  HIPCHK(c, dupload(S.val, A.val, st));
-  HIPCHK(c, hipMemcpyAsync(S.b.p, b.data(), n * sizeof(float), hipMemcpyHostToDevice, st));
-  HIPCHK(c, hipMemcpyAsync(S.x.p, x.data(), n * sizeof(float), hipMemcpyHostToDevice, st));
-  // r = b - A x ; r0 = r
-  HIPCHK(c, rp_launch_spmv(n, S.rowp.p, S.col.p, S.val.p, S.x.p, S.t.p, st));
-  HIPCHK(c, rp_launch_axpy(n, S.r.p, S.b.p, -1.0, S.t.p, st));
-  HIPCHK(c, hipMemcpyAsync(S.r0.p, S.r.p, n * sizeof(float), hipMemcpyDeviceToDevice, st));
-  HIPCHK(c, hipMemsetAsync(S.v.p, 0, n * sizeof(float), st));
-  HIPCHK(c, hipMemsetAsync(S.p.p, 0, n * sizeof(float), st));
-  double d[3];
-  if (dev_dots(c, S, S.r0.p, S.r0.p, S.b.p, S.b.p, nullptr, nullptr, d)) return RP_ERR_HIP;
-  double r0_sq = d[0], rhs_sq = d[1];
-  if (rhs_sq == 0.0) {
-    std::fill(x.begin(), x.end(), 0.0f);
-    *err_out = 0.0f;
-    return 0;
-  }
-  double rho = 1.0, alpha = 1.0, w = 1.0;
-  double tol2 = (double)tol * (double)tol * rhs_sq;
-  const double eps = 1.1920928955078125e-07;
-  double eps2 = eps * eps;
-  int i = 0, restarts = 0;
-  if (dev_dots(c, S, S.r.p, S.r.p, S.r0.p, S.r.p, nullptr, nullptr, d)) return RP_ERR_HIP;
-  double rr = d[0], r0r = d[1];
-  while (rr > tol2 && i < max_iter) {
-    double rho_old = rho;
-    rho = r0r;
-    if (std::fabs(rho) < eps2 * r0_sq) {
-      HIPCHK(c, rp_launch_spmv(n, S.rowp.p, S.col.p, S.val.p, S.x.p, S.t.p, st));
-      HIPCHK(c, rp_launch_axpy(n, S.r.p, S.b.p, -1.0, S.t.p, st));
-      HIPCHK(c, hipMemcpyAsync(S.r0.p, S.r.p, n * sizeof(float), hipMemcpyDeviceToDevice, st));
-      if (dev_dots(c, S, S.r.p, S.r.p, nullptr, nullptr, nullptr, nullptr, d)) return RP_ERR_HIP;
-      rho = r0_sq = d[0];
-      if (restarts++ == 0) i = 0;
+  std::vector<int> pcx(np), pcy(np), icx(n), icy(n);
+  HIPCHK(c, hipMemcpyAsync(pcx.data(), a.pin_cx.p, sizeof(int) * np, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipMemcpyAsync(pcy.data(), a.pin_cy.p, sizeof(int) * np, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipMemcpyAsync(icx.data(), a.icx.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipMemcpyAsync(icy.data(), a.icy.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  // updatePinInfo: the first pin reaching a strict new extreme keeps the flag
+  std::vector<uint8_t> mm(np, 0);
+  for (int e = 0; e < c->n_net; e++) {
+    int pminx = -1, pmaxx = -1, pminy = -1, pmaxy = -1;
+    int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
+    for (int q = c->net_start[e]; q < c->net_start[e + 1]; q++) {
+      if (lx > pcx[q]) {
+        if (pminx >= 0) mm[pminx] &= (uint8_t)~1u;
+        lx = pcx[q];
+        pminx = q;
+        mm[q] |= 1u;
+      }
+      if (ux < pcx[q]) {
+        if (pmaxx >= 0) mm[pmaxx] &= (uint8_t)~2u;
+        ux = pcx[q];
+        pmaxx = q;
+        mm[q] |= 2u;
+      }
+      if (ly > pcy[q]) {
+        if (pminy >= 0) mm[pminy] &= (uint8_t)~4u;
+        ly = pcy[q];
+        pminy = q;
+        mm[q] |= 4u;
+      }
+      if (uy < pcy[q]) {
+        if (pmaxy >= 0) mm[pmaxy] &= (uint8_t)~8u;
+        uy = pcy[q];
+        pmaxy = q;
+        mm[q] |= 8u;
+      }
     }
-    double beta = (rho / rho_old) * (alpha / w);
-    HIPCHK(c, rp_launch_bicg_p(n, S.p.p, S.r.p, S.v.p, beta, w, st));
-    HIPCHK(c, rp_launch_spmv(n, S.rowp.p, S.col.p, S.val.p, S.p.p, S.v.p, st));
-    if (dev_dots(c, S, S.r0.p, S.v.p, nullptr, nullptr, nullptr, nullptr, d)) return RP_ERR_HIP;
-    alpha = rho / d[0];
-    HIPCHK(c, rp_launch_axpy(n, S.s.p, S.r.p, -alpha, S.v.p, st));
-    HIPCHK(c, rp_launch_spmv(n, S.rowp.p, S.col.p, S.val.p, S.s.p, S.t.p, st));
-    if (dev_dots(c, S, S.t.p, S.t.p, S.t.p, S.s.p, nullptr, nullptr, d)) return RP_ERR_HIP;
-    w = d[0] > 0.0 ? d[1] / d[0] : 0.0;
-    HIPCHK(c, rp_launch_bicg_xr(n, S.x.p, S.r.p, S.p.p, S.s.p, S.t.p, alpha, w, st));
-    i++;
-    if (dev_dots(c, S, S.r.p, S.r.p, S.r0.p, S.r.p, nullptr, nullptr, d)) return RP_ERR_HIP;
-    rr = d[0];
-    r0r = d[1];
   }
-  *err_out = (float)std::sqrt(rr / rhs_sq);
-  HIPCHK(c, hipMemcpyAsync(x.data(), S.x.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
+  // createSparseMatrix
+  std::vector<float> bx(n, 0.0f), by(n, 0.0f);
+  std::vector<Trip> tx, ty;
+  const float minDiff = (float)p.initial_place_min_diff_length;
+  auto pair = [&](int p1, int p2, int k1, int k2, const std::vector<int>& ic, float netWeight,
+                  std::vector<Trip>& t, std::vector<float>& b) {
+    int diff = std::abs(p1 - p2);
+    float w = diff > p.initial_place_min_diff_length ? netWeight / (float)diff : netWeight / minDiff;
+    if (k1 >= 0 && k2 >= 0) {
+      t.push_back({k1, k1, w});
+      t.push_back({k2, k2, w});
+      t.push_back({k1, k2, -w});
+      t.push_back({k2, k1, -w});
+      b[k1] += -w * (float)((p1 - ic[k1]) - (p2 - ic[k2]));
+      b[k2] += -w * (float)((p2 - ic[k2]) - (p1 - ic[k1]));
+    } else if (k1 < 0 && k2 >= 0) {
+      t.push_back({k2, k2, w});
+      b[k2] += w * (float)(p1 - (p2 - ic[k2]));
+    } else if (k1 >= 0 && k2 < 0) {
+      t.push_back({k1, k1, w});
+      b[k1] += w * (float)(p2 - (p1 - ic[k1]));
+    }
+  };
+  for (int e = 0; e < c->n_net; e++) {
+    int s0 = c->net_start[e], deg = c->net_start[e + 1] - s0;
+    if (deg <= 1 || deg >= p.initial_place_max_fanout) continue;
+    float netWeight = p.initial_place_net_weight_scale / (float)(deg - 1);
+    for (int a1 = 0; a1 < deg; a1++) {
+      int q1 = s0 + a1, i1 = c->pin_inst[q1];
+      int k1 = a.h_pin_k[q1];
+      for (int a2 = 0; a2 <= a1; a2++) {
+        int q2 = s0 + a2, i2 = c->pin_inst[q2];
+        if (i1 == i2) continue;
+        int k2 = a.h_pin_k[q2];
+        if ((mm[q1] & 3u) || (mm[q2] & 3u)) pair(pcx[q1], pcx[q2], k1, k2, icx, netWeight, tx, bx);
+        if ((mm[q1] & 12u) || (mm[q2] & 12u)) pair(pcy[q1], pcy[q2], k1, k2, icy, netWeight, ty, by);
+      }
+    }
+  }
+  Csr M[2];
+  build_csr(n, tx, M[0]);
+  build_csr(n, ty, M[1]);
+  const std::vector<float>* B[2] = {&bx, &by};
+  for (int ax = 0; ax < 2; ax++) {
+    HIPCHK(c, hipMemcpyAsync(a.rowp[ax].p, M[ax].rowp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, st));
+    HIPCHK(c, dupload(a.col[ax], M[ax].col, st));
+    HIPCHK(c, dupload(a.val[ax], M[ax].val, st));
+    HIPCHK(c, hipMemcpyAsync(a.b[ax].p, B[ax]->data(), sizeof(float) * n, hipMemcpyHostToDevice, st));
+  }
   HIPCHK(c, hipStreamSynchronize(st));
   return 0;
 }
@@ -493,157 +524,15 @@ extern "C" int rp_initial_place(rp_ctx* c, rp_ip_stats* stats) {
     for (int k = 0; k < n; k++) inst_set_center(c, c->place[k], ccx, ccy);
   }
   if (p.initial_place_max_iter > 0 && n > 0) {
-    DevSolver S;
-    S.n = n;
-    S.nparts = std::max(1, std::min(1024, (n + RP_BLOCK - 1) / RP_BLOCK));
-    HIPCHK(c, dalloc(S.b, n));
-    HIPCHK(c, dalloc(S.x, n));
-    HIPCHK(c, dalloc(S.r, n));
-    HIPCHK(c, dalloc(S.r0, n));
-    HIPCHK(c, dalloc(S.v, n));
-    HIPCHK(c, dalloc(S.p, n));
-    HIPCHK(c, dalloc(S.s, n));
-    HIPCHK(c, dalloc(S.t, n));
-    HIPCHK(c, dalloc(S.parts, (size_t)3 * S.nparts));
-    HIPCHK(c, dalloc(S.dots, 3));
-    HIPCHK(c, hipHostMalloc((void**)&S.h_dots, 3 * sizeof(double), hipHostMallocDefault));
-    std::vector<uint8_t> mm(c->n_pin);
-    std::vector<float> bx(n), by(n), xx(n), xy(n);
-    std::vector<Trip> tx, ty;
-    int rc = 0;
-    const bool phase_timing = getenv("RP_IP_TIMING") != nullptr;
-    double t_pins = 0, t_pairs = 0, t_csr = 0, t_solve = 0;
-    using clk = std::chrono::steady_clock;
-    auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
-    for (int it = 1; it <= p.initial_place_max_iter; it++) {
-      auto tp0 = clk::now();
-      // updatePinInfo, src/initialPlace.cpp:128-182 (first strict extreme wins)
-      std::fill(mm.begin(), mm.end(), 0);
-      for (int e = 0; e < c->n_net; e++) {
-        int pminx = -1, pmaxx = -1, pminy = -1, pmaxy = -1;
-        int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
-        for (int q = c->net_start[e]; q < c->net_start[e + 1]; q++) {
-          if (lx > c->pin_cx[q]) {
-            if (pminx >= 0) mm[pminx] &= (uint8_t)~1u;
-            lx = c->pin_cx[q];
-            pminx = q;
-            mm[q] |= 1u;
-          }
-          if (ux < c->pin_cx[q]) {
-            if (pmaxx >= 0) mm[pmaxx] &= (uint8_t)~2u;
-            ux = c->pin_cx[q];
-            pmaxx = q;
-            mm[q] |= 2u;
-          }
-          if (ly > c->pin_cy[q]) {
-            if (pminy >= 0) mm[pminy] &= (uint8_t)~4u;
-            ly = c->pin_cy[q];
-            pminy = q;
-            mm[q] |= 4u;
-          }
-          if (uy < c->pin_cy[q]) {
-            if (pmaxy >= 0) mm[pmaxy] &= (uint8_t)~8u;
-            uy = c->pin_cy[q];
-            pmaxy = q;
-            mm[q] |= 8u;
-          }
-        }
-      }
-      t_pins += since(tp0);
-      auto tp1 = clk::now();
-      // createSparseMatrix, src/initialPlace.cpp:185-378
-      for (int k = 0; k < n; k++) {
-        int i = c->place[k];
-        xx[k] = (float)inst_cx(c, i);
-        xy[k] = (float)inst_cy(c, i);
-        bx[k] = by[k] = 0.0f;
-      }
-      tx.clear();
-      ty.clear();
-      const float minDiff = (float)p.initial_place_min_diff_length;
-      for (int e = 0; e < c->n_net; e++) {
-        int s = c->net_start[e], deg = c->net_start[e + 1] - s;
-        if (deg <= 1 || deg >= p.initial_place_max_fanout) continue;
-        float netWeight = p.initial_place_net_weight_scale / (float)(deg - 1);
-        for (int a1 = 0; a1 < deg; a1++) {
-          int q1 = s + a1;
-          int i1 = c->pin_inst[q1];
-          int k1 = i1 >= 0 ? c->inst2place[i1] : -1;
-          for (int a2 = 0; a2 <= a1; a2++) {
-            int q2 = s + a2;
-            int i2 = c->pin_inst[q2];
-            if (i1 == i2) continue;
-            int k2 = i2 >= 0 ? c->inst2place[i2] : -1;
-            if ((mm[q1] & 3u) || (mm[q2] & 3u)) {
-              int diff = std::abs(c->pin_cx[q1] - c->pin_cx[q2]);
-              float w = diff > p.initial_place_min_diff_length ? netWeight / (float)diff : netWeight / minDiff;
-              if (k1 >= 0 && k2 >= 0) {
-                tx.push_back({k1, k1, w});
-                tx.push_back({k2, k2, w});
-                tx.push_back({k1, k2, -w});
-                tx.push_back({k2, k1, -w});
-                bx[k1] += -w * (float)((c->pin_cx[q1] - inst_cx(c, i1)) - (c->pin_cx[q2] - inst_cx(c, i2)));
-                bx[k2] += -w * (float)((c->pin_cx[q2] - inst_cx(c, i2)) - (c->pin_cx[q1] - inst_cx(c, i1)));
-              } else if (k1 < 0 && k2 >= 0) {
-                tx.push_back({k2, k2, w});
-                bx[k2] += w * (float)(c->pin_cx[q1] - (c->pin_cx[q2] - inst_cx(c, i2)));
-              } else if (k1 >= 0 && k2 < 0) {
-                tx.push_back({k1, k1, w});
-                bx[k1] += w * (float)(c->pin_cx[q2] - (c->pin_cx[q1] - inst_cx(c, i1)));
-              }
-            }
-            if ((mm[q1] & 12u) || (mm[q2] & 12u)) {
-              int diff = std::abs(c->pin_cy[q1] - c->pin_cy[q2]);
-              float w = diff > p.initial_place_min_diff_length ? netWeight / (float)diff : netWeight / minDiff;
-              if (k1 >= 0 && k2 >= 0) {
-                ty.push_back({k1, k1, w});
-                ty.push_back({k2, k2, w});
-                ty.push_back({k1, k2, -w});
-                ty.push_back({k2, k1, -w});
-                by[k1] += -w * (float)((c->pin_cy[q1] - inst_cy(c, i1)) - (c->pin_cy[q2] - inst_cy(c, i2)));
-                by[k2] += -w * (float)((c->pin_cy[q2] - inst_cy(c, i2)) - (c->pin_cy[q1] - inst_cy(c, i1)));
-              } else if (k1 < 0 && k2 >= 0) {
-                ty.push_back({k2, k2, w});
-                by[k2] += w * (float)(c->pin_cy[q1] - (c->pin_cy[q2] - inst_cy(c, i2)));
-              } else if (k1 >= 0 && k2 < 0) {
-                ty.push_back({k1, k1, w});
-                by[k1] += w * (float)(c->pin_cy[q2] - (c->pin_cy[q1] - inst_cy(c, i1)));
-              }
-            }
-          }
-        }
-      }
-      t_pairs += since(tp1);
-      auto tp2 = clk::now();
-      Csr Ax, Ay;
-      build_csr(n, tx, Ax);
-      build_csr(n, ty, Ay);
-      t_csr += since(tp2);
-      auto tp3 = clk::now();
-      const float tol = 1.1920928955078125e-07f;  // NumTraits<float>::epsilon()
-      float ex = 0, ey = 0;
-      rc = dev_bicgstab(c, S, Ax, bx, xx, p.initial_place_max_solver_iter, tol, &ex);
-      if (rc) break;
-      rc = dev_bicgstab(c, S, Ay, by, xy, p.initial_place_max_solver_iter, tol, &ey);
-      if (rc) break;
-      t_solve += since(tp3);
-      float err = std::max(ex, ey);
-      c->ip_err.push_back(err);
-      c->ip_hpwl.push_back(inst_hpwl(c));  // printed before updateCoordi (src/initialPlace.cpp:88-91)
-      // updateCoordi, src/initialPlace.cpp:380-385
-      for (int k = 0; k < n; k++) {
-        int i = c->place[k];
-        inst_set_center(c, i, (int)xx[k], (int)xy[k]);
-        c->dblx[i] = c->ilx[i];
-        c->dbly[i] = c->ily[i];
-      }
-      if (err <= 1e-5 && it >= 5) break;
-    }
-    (void)hipHostFree(S.h_dots);
-    if (phase_timing)
-      fprintf(stderr, "[rp] initial place phases: pin info %.3fs, B2B pairs %.3fs, CSR %.3fs, BiCGSTAB %.3fs\n",
-              t_pins, t_pairs, t_csr, t_solve);
-    if (rc) return rc;
+    // B2B assembly on the device; RP_IP_HOST_ASSEMBLY=1 selects the sequential
+    // host assembly of the same systems
+    const bool host_asm = getenv("RP_IP_HOST_ASSEMBLY") != nullptr;
+    double t_asm = 0, t_solve = 0;
+    int r = rp_ip_device_loop(c, host_asm, &t_asm, &t_solve);
+    if (getenv("RP_IP_TIMING"))
+      fprintf(stderr, "[rp] initial place phases (%s assembly): B2B assembly %.3fs, BiCGSTAB + update %.3fs\n",
+              host_asm ? "host" : "device", t_asm, t_solve);
+    if (r) return r;
   }
   c->nb_ready = false;
   if (stats) {
@@ -839,47 +728,8 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   // reference summation order.
   const int nn = c->n_net, npn = c->n_pin;
   {
-    std::vector<int> cell_net_start(ngi + 1, 0), cell_nets;
-    for (int g = 0; g < ngi; g++) {
-      int i = c->g_inst[g];
-      cell_net_start[g + 1] = cell_net_start[g] + (c->ipin_start[i + 1] - c->ipin_start[i]);
-    }
-    std::vector<int> pin_net_ref(npn);
-    for (int e = 0; e < nn; e++)
-      for (int q = c->net_start[e]; q < c->net_start[e + 1]; q++) pin_net_ref[q] = e;
-    cell_nets.resize(cell_net_start[ngi]);
-    for (int g = 0; g < ngi; g++) {
-      int i = c->g_inst[g], o = cell_net_start[g];
-      for (int k = c->ipin_start[i]; k < c->ipin_start[i + 1]; k++) cell_nets[o++] = pin_net_ref[c->ipins[k]];
-    }
-    std::vector<uint8_t> cell_seen(ngi, 0), net_seen(nn, 0);
-    c->gperm.clear();
-    c->gperm.reserve(ng);
-    std::vector<int> queue;
-    queue.reserve(ngi);
-    for (int root = 0; root < ngi; root++) {
-      if (cell_seen[root]) continue;
-      size_t head = queue.size();
-      queue.push_back(root);
-      cell_seen[root] = 1;
-      while (head < queue.size()) {
-        int g = queue[head++];
-        c->gperm.push_back(g);
-        for (int k = cell_net_start[g]; k < cell_net_start[g + 1]; k++) {
-          int e = cell_nets[k];
-          if (net_seen[e]) continue;
-          net_seen[e] = 1;
-          for (int q = c->net_start[e]; q < c->net_start[e + 1]; q++) {
-            int i = c->pin_inst[q];
-            int h = i >= 0 ? c->inst2place[i] : -1;
-            if (h >= 0 && !cell_seen[h]) {
-              cell_seen[h] = 1;
-              queue.push_back(h);
-            }
-          }
-        }
-      }
-    }
+    const std::vector<int>& order = rp_movable_order(c);
+    c->gperm.assign(order.begin(), order.end());
     for (int g = ngi; g < ng; g++) c->gperm.push_back(g);
     c->ginv.assign(ng, 0);
     for (int k = 0; k < ng; k++) c->ginv[c->gperm[k]] = k;

@@ -1,0 +1,936 @@
+// rp_ip.hip -- InitialPlace's B2B linear systems assembled on the device.
+//
+// Reference: InitialPlace::updatePinInfo (src/initialPlace.cpp:128-182) and
+// createSparseMatrix (src/initialPlace.cpp:185-378), whose triplet lists go
+// through Eigen's SparseMatrix::setFromTriplets (src/initialPlace.cpp:376-377).
+//
+// Per net, one thread
+//   1. finds the pins that end as the x/y min and max of updatePinInfo's
+//      sequential scan (the first pin reaching a strict new extreme keeps the
+//      flag), and
+//   2. walks the reference's pair loop (a1 over pins, a2 <= a1, same-instance
+//      pairs skipped) over the pairs that touch an extreme pin, emitting the
+//      Laplacian triplets and the right-hand-side terms in the reference's
+//      order at offsets from an exclusive scan of per-net counts.
+// The CSR is then what setFromTriplets builds: a stable radix sort of the
+// triplets on (row, col) keeps equal entries in triplet order, and every
+// (row, col) run is summed sequentially from 0.0f.  The right-hand side is
+// summed per row in the same order the reference's `b[k] += ...` adds it.
+// Integer keys and in-order float sums make the result bit-identical to a
+// sequential host assembly, independent of launch shape.
+#include <chrono>
+
+#include <hipcub/hipcub.hpp>
+
+#include "rp_ctx.h"
+
+namespace {
+
+struct IpView {
+  int n_net, n;
+  const int* __restrict__ net_start;
+  const int* __restrict__ pin_inst;
+  const int* __restrict__ pin_k;   // movable index of the pin's instance, -1 for fixed / BTerm
+  const int* __restrict__ pin_cx;
+  const int* __restrict__ pin_cy;
+  const int* __restrict__ icx;     // integer centre of movable instance k
+  const int* __restrict__ icy;
+  int min_diff, max_fanout;
+  float weight_scale;
+};
+
+struct IpOut {
+  const int* off;  // 4 x (n_net + 1) exclusive offsets: x triplets, y triplets, x rhs, y rhs
+  unsigned long long* tk[2];
+  float* tv[2];
+  unsigned* rk[2];
+  float* rv[2];
+  int cb;    // column bits of the packed key
+};
+
+struct Emit {
+  unsigned long long* tk;
+  float* tv;
+  unsigned* rk;
+  float* rv;
+  int nt, nr, cb;
+  __device__ void trip(int r, int c, float v) {
+    tk[nt] = ((unsigned long long)(unsigned)r << cb) | (unsigned)c;
+    tv[nt] = v;
+    nt++;
+  }
+  __device__ void rhs(int r, float v) {
+    rk[nr] = (unsigned)r;
+    rv[nr] = v;
+    nr++;
+  }
+};
+
+// One B2B pair on one axis (src/initialPlace.cpp:242-290 for x, :300-348 for y).
+template <bool EMIT>
+__device__ __forceinline__ void ip_pair(int p1, int p2, int k1, int k2, const int* ic, float netWeight, float minDiff,
+                                        int min_diff, int& nt, int& nr, Emit& em) {
+  int diff = abs(p1 - p2);
+  float w = diff > min_diff ? netWeight / (float)diff : netWeight / minDiff;
+  if (k1 >= 0 && k2 >= 0) {
+    if (EMIT) {
+      em.trip(k1, k1, w);
+      em.trip(k2, k2, w);
+      em.trip(k1, k2, -w);
+      em.trip(k2, k1, -w);
+      em.rhs(k1, -w * (float)((p1 - ic[k1]) - (p2 - ic[k2])));
+      em.rhs(k2, -w * (float)((p2 - ic[k2]) - (p1 - ic[k1])));
+    }
+    nt += 4;
+    nr += 2;
+  } else if (k1 < 0 && k2 >= 0) {
+    if (EMIT) {
+      em.trip(k2, k2, w);
+      em.rhs(k2, w * (float)(p1 - (p2 - ic[k2])));
+    }
+    nt += 1;
+    nr += 1;
+  } else if (k1 >= 0 && k2 < 0) {
+    if (EMIT) {
+      em.trip(k1, k1, w);
+      em.rhs(k1, w * (float)(p2 - (p1 - ic[k1])));
+    }
+    nt += 1;
+    nr += 1;
+  }
+}
+
+// The reference's pair loop over net e, restricted to the pairs that touch
+// an extreme pin of the axis; pairs are visited in (a1, a2) order.
+template <bool EMIT>
+__device__ void ip_net(const IpView& v, int e, int cnt[4], Emit* ex, Emit* ey) {
+  cnt[0] = cnt[1] = cnt[2] = cnt[3] = 0;
+  const int s = v.net_start[e], t = v.net_start[e + 1], deg = t - s;
+  if (deg <= 1 || deg >= v.max_fanout) return;
+  // updatePinInfo: first pin reaching a strict new extreme
+  int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
+  int mnx = -1, mxx = -1, mny = -1, mxy = -1;
+  for (int q = s; q < t; q++) {
+    int x = v.pin_cx[q], y = v.pin_cy[q];
+    if (lx > x) { lx = x; mnx = q; }
+    if (ux < x) { ux = x; mxx = q; }
+    if (ly > y) { ly = y; mny = q; }
+    if (uy < y) { uy = y; mxy = q; }
+  }
+  const float netWeight = v.weight_scale / (float)(deg - 1);
+  const float minDiff = (float)v.min_diff;
+  Emit dummy{};
+  Emit& emx = EMIT ? *ex : dummy;
+  Emit& emy = EMIT ? *ey : dummy;
+  for (int q1 = s; q1 < t; q1++) {
+    const int i1 = v.pin_inst[q1], k1 = v.pin_k[q1];
+    const int x1 = v.pin_cx[q1], y1 = v.pin_cy[q1];
+    const bool xe1 = q1 == mnx || q1 == mxx, ye1 = q1 == mny || q1 == mxy;
+    if (xe1 || ye1) {
+      for (int q2 = s; q2 <= q1; q2++) {
+        const int i2 = v.pin_inst[q2];
+        if (i1 == i2) continue;
+        const int k2 = v.pin_k[q2];
+        if (xe1 || q2 == mnx || q2 == mxx)
+          ip_pair<EMIT>(x1, v.pin_cx[q2], k1, k2, v.icx, netWeight, minDiff, v.min_diff, cnt[0], cnt[2], emx);
+        if (ye1 || q2 == mny || q2 == mxy)
+          ip_pair<EMIT>(y1, v.pin_cy[q2], k1, k2, v.icy, netWeight, minDiff, v.min_diff, cnt[1], cnt[3], emy);
+      }
+    } else {
+      // only extreme partners with q2 <= q1, ascending; x and y lists are independent
+      int ax = min(mnx, mxx), bx = max(mnx, mxx);
+      int ay = min(mny, mxy), by = max(mny, mxy);
+      const int xs[2] = {ax, bx == ax ? -1 : bx};
+      const int ys[2] = {ay, by == ay ? -1 : by};
+      for (int j = 0; j < 2; j++) {
+        int q2 = xs[j];
+        if (q2 < 0 || q2 > q1) continue;
+        const int i2 = v.pin_inst[q2];
+        if (i1 == i2) continue;
+        ip_pair<EMIT>(x1, v.pin_cx[q2], k1, v.pin_k[q2], v.icx, netWeight, minDiff, v.min_diff, cnt[0], cnt[2], emx);
+      }
+      for (int j = 0; j < 2; j++) {
+        int q2 = ys[j];
+        if (q2 < 0 || q2 > q1) continue;
+        const int i2 = v.pin_inst[q2];
+        if (i1 == i2) continue;
+        ip_pair<EMIT>(y1, v.pin_cy[q2], k1, v.pin_k[q2], v.icy, netWeight, minDiff, v.min_diff, cnt[1], cnt[3], emy);
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(RP_BLOCK) k_ip_count(IpView v, int* cnt) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int stride = v.n_net + 1;
+  if (e == v.n_net) {
+    for (int a = 0; a < 4; a++) cnt[a * stride + e] = 0;
+  }
+  if (e >= v.n_net) return;
+  int c[4];
+  ip_net<false>(v, e, c, nullptr, nullptr);
+  for (int a = 0; a < 4; a++) cnt[a * stride + e] = c[a];
+}
+
+__global__ void __launch_bounds__(RP_BLOCK) k_ip_emit(IpView v, IpOut o) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= v.n_net) return;
+  const int stride = v.n_net + 1;
+  Emit ex{o.tk[0], o.tv[0], o.rk[0], o.rv[0], o.off[0 * stride + e], o.off[2 * stride + e], o.cb};
+  Emit ey{o.tk[1], o.tv[1], o.rk[1], o.rv[1], o.off[1 * stride + e], o.off[3 * stride + e], o.cb};
+  int c[4];
+  ip_net<true>(v, e, c, &ex, &ey);
+}
+
+// run heads of a sorted key array
+template <typename K>
+__global__ void k_ip_heads(const K* __restrict__ key, int m, int* __restrict__ flag) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) flag[i] = (i == 0 || key[i] != key[i - 1]) ? 1 : 0;
+}
+
+// one (row, col) run -> one CSR entry, summed in triplet order from 0.0f
+__global__ void k_ip_segsum(const unsigned long long* __restrict__ key, const float* __restrict__ val, int m,
+                            const int* __restrict__ flag, const int* __restrict__ seg, int cb,
+                            int* __restrict__ col, float* __restrict__ out, int* __restrict__ srow) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m || !flag[i]) return;
+  const unsigned long long k = key[i];
+  float s = 0.0f;
+  for (int j = i; j < m && key[j] == k; j++) s += val[j];
+  int at = seg[i];
+  col[at] = (int)(k & ((1ull << cb) - 1));
+  srow[at] = (int)(k >> cb);
+  out[at] = s;
+}
+
+// rowp[r] = first entry whose row >= r
+__global__ void k_ip_rowp(const int* __restrict__ srow, const int* __restrict__ flag, const int* __restrict__ seg,
+                          int m, int n, int* __restrict__ rowp) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > n) return;
+  int nseg = m > 0 ? seg[m - 1] + flag[m - 1] : 0;
+  int lo = 0, hi = nseg;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (srow[mid] < r) lo = mid + 1;
+    else hi = mid;
+  }
+  rowp[r] = lo;
+}
+
+__global__ void k_ip_zero(float* __restrict__ b, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = 0.0f;
+}
+
+// right-hand side: per row, the terms in triplet order, from 0.0f
+__global__ void k_ip_rhs(const unsigned* __restrict__ key, const float* __restrict__ val, int m,
+                         float* __restrict__ b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m || (i > 0 && key[i] == key[i - 1])) return;
+  const unsigned k = key[i];
+  float s = 0.0f;
+  for (int j = i; j < m && key[j] == k; j++) s += val[j];
+  b[k] = s;
+}
+
+inline dim3 grid_of(long n) { return dim3((unsigned)std::max<long>(1, (n + RP_BLOCK - 1) / RP_BLOCK)); }
+
+template <typename T>
+hipError_t ensure(DevBuf<T>& b, size_t n) {
+  if (b.n >= n && b.p) return hipSuccess;
+  return dalloc(b, n + n / 4 + 64);
+}
+
+int bits_for(int n) {
+  int b = 1;
+  while ((1ll << b) < (long long)n) b++;
+  return b;
+}
+
+}  // namespace
+
+int rp_ip_asm_init(rp_ctx* c, RpIpAsm& a) {
+  hipStream_t st = c->stream;
+  const int n = (int)c->place.size();
+  a.n = n;
+  a.n_net = c->n_net;
+  a.cb = bits_for(std::max(2, n));
+  // rows and columns in locality order: row pos[k] holds movable instance k
+  const std::vector<int>& order = rp_movable_order(c);
+  a.pos.assign(n, 0);
+  for (int j = 0; j < n; j++) a.pos[order[j]] = j;
+  std::vector<int>& pk = a.h_pin_k;
+  pk.assign(c->n_pin, -1);
+  for (int q = 0; q < c->n_pin; q++) {
+    int i = c->pin_inst[q];
+    int k = i >= 0 ? c->inst2place[i] : -1;
+    pk[q] = k >= 0 ? a.pos[k] : -1;
+  }
+  HIPCHK(c, dupload(a.net_start, c->net_start, st));
+  HIPCHK(c, dupload(a.pin_inst, c->pin_inst, st));
+  HIPCHK(c, dupload(a.pin_k, pk, st));
+  HIPCHK(c, dalloc(a.pin_cx, c->n_pin));
+  HIPCHK(c, dalloc(a.pin_cy, c->n_pin));
+  HIPCHK(c, dalloc(a.icx, n));
+  HIPCHK(c, dalloc(a.icy, n));
+  HIPCHK(c, dalloc(a.cnt, 4 * ((size_t)c->n_net + 1)));
+  HIPCHK(c, dalloc(a.off, 4 * ((size_t)c->n_net + 1)));
+  for (int ax = 0; ax < 2; ax++) {
+    HIPCHK(c, dalloc(a.rowp[ax], (size_t)n + 1));
+    HIPCHK(c, dalloc(a.b[ax], n));
+  }
+  if (!a.h_tot) HIPCHK(c, hipHostMalloc((void**)&a.h_tot, 4 * sizeof(int), hipHostMallocDefault));
+  HIPCHK(c, hipStreamSynchronize(st));
+  return 0;
+}
+
+// Assemble both axes from the device's current pin and instance centres
+// (a.pin_cx / a.pin_cy / a.icx / a.icy).
+int rp_ip_assemble(rp_ctx* c, RpIpAsm& a, const int*, const int*) {
+  hipStream_t st = c->stream;
+  const int n = a.n, nn = a.n_net;
+  const size_t stride = (size_t)nn + 1;
+  IpView v{nn, n, a.net_start.p, a.pin_inst.p, a.pin_k.p, a.pin_cx.p, a.pin_cy.p, a.icx.p, a.icy.p,
+           c->prm.initial_place_min_diff_length, c->prm.initial_place_max_fanout,
+           c->prm.initial_place_net_weight_scale};
+  hipLaunchKernelGGL(k_ip_count, grid_of((long)nn + 1), dim3(RP_BLOCK), 0, st, v, a.cnt.p);
+  HIPCHK(c, hipGetLastError());
+  // exclusive scans of the four count lists (length nn + 1: the last entry is the total)
+  size_t need = 0;
+  HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, need, a.cnt.p, a.off.p, (int)stride, st));
+  HIPCHK(c, ensure(a.temp, need));
+  for (int q = 0; q < 4; q++) {
+    size_t tb = a.temp.n;
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(a.temp.p, tb, a.cnt.p + q * stride, a.off.p + q * stride,
+                                               (int)stride, st));
+  }
+  for (int q = 0; q < 4; q++)
+    HIPCHK(c, hipMemcpyAsync(a.h_tot + q, a.off.p + q * stride + nn, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  int mt[2] = {a.h_tot[0], a.h_tot[1]}, mr[2] = {a.h_tot[2], a.h_tot[3]};
+  for (int ax = 0; ax < 2; ax++) {
+    if (mt[ax] < 0 || mr[ax] < 0) return fail(c, RP_ERR_NOMEM, "initial place: B2B system exceeds 2^31 entries");
+    HIPCHK(c, ensure(a.tk[ax], mt[ax]));
+    HIPCHK(c, ensure(a.tk2[ax], mt[ax]));
+    HIPCHK(c, ensure(a.tv[ax], mt[ax]));
+    HIPCHK(c, ensure(a.tv2[ax], mt[ax]));
+    HIPCHK(c, ensure(a.rk[ax], mr[ax]));
+    HIPCHK(c, ensure(a.rk2[ax], mr[ax]));
+    HIPCHK(c, ensure(a.rv[ax], mr[ax]));
+    HIPCHK(c, ensure(a.rv2[ax], mr[ax]));
+    HIPCHK(c, ensure(a.col[ax], mt[ax]));
+    HIPCHK(c, ensure(a.val[ax], mt[ax]));
+  }
+  const int mmax = std::max(mt[0], mt[1]);
+  HIPCHK(c, ensure(a.flag, mmax));
+  HIPCHK(c, ensure(a.seg, mmax));
+  HIPCHK(c, ensure(a.srow, mmax));
+  IpOut o{a.off.p, {a.tk[0].p, a.tk[1].p}, {a.tv[0].p, a.tv[1].p}, {a.rk[0].p, a.rk[1].p},
+          {a.rv[0].p, a.rv[1].p}, a.cb};
+  hipLaunchKernelGGL(k_ip_emit, grid_of(nn), dim3(RP_BLOCK), 0, st, v, o);
+  HIPCHK(c, hipGetLastError());
+  for (int ax = 0; ax < 2; ax++) {
+    const int m = mt[ax], r = mr[ax];
+    // stable sorts: equal keys keep triplet order
+    size_t t1 = 0, t2 = 0, t3 = 0;
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, t1, a.tk[ax].p, a.tk2[ax].p, a.tv[ax].p, a.tv2[ax].p,
+                                                 std::max(1, m), 0, 2 * a.cb, st));
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, t2, a.rk[ax].p, a.rk2[ax].p, a.rv[ax].p, a.rv2[ax].p,
+                                                 std::max(1, r), 0, a.cb, st));
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, t3, a.flag.p, a.seg.p, std::max(1, m), st));
+    HIPCHK(c, ensure(a.temp, std::max(t1, std::max(t2, t3))));
+    if (m > 0) {
+      size_t tb = a.temp.n;
+      HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(a.temp.p, tb, a.tk[ax].p, a.tk2[ax].p, a.tv[ax].p,
+                                                   a.tv2[ax].p, m, 0, 2 * a.cb, st));
+      hipLaunchKernelGGL(k_ip_heads<unsigned long long>, grid_of(m), dim3(RP_BLOCK), 0, st,
+                         (const unsigned long long*)a.tk2[ax].p, m, a.flag.p);
+      tb = a.temp.n;
+      HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(a.temp.p, tb, a.flag.p, a.seg.p, m, st));
+      hipLaunchKernelGGL(k_ip_segsum, grid_of(m), dim3(RP_BLOCK), 0, st, (const unsigned long long*)a.tk2[ax].p,
+                         (const float*)a.tv2[ax].p, m, (const int*)a.flag.p, (const int*)a.seg.p, a.cb,
+                         a.col[ax].p, a.val[ax].p, a.srow.p);
+    }
+    hipLaunchKernelGGL(k_ip_rowp, grid_of((long)n + 1), dim3(RP_BLOCK), 0, st, (const int*)a.srow.p,
+                       (const int*)a.flag.p, (const int*)a.seg.p, m, n, a.rowp[ax].p);
+    hipLaunchKernelGGL(k_ip_zero, grid_of(n), dim3(RP_BLOCK), 0, st, a.b[ax].p, n);
+    if (r > 0) {
+      size_t tb = a.temp.n;
+      HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(a.temp.p, tb, a.rk[ax].p, a.rk2[ax].p, a.rv[ax].p,
+                                                   a.rv2[ax].p, r, 0, a.cb, st));
+      hipLaunchKernelGGL(k_ip_rhs, grid_of(r), dim3(RP_BLOCK), 0, st, (const unsigned*)a.rk2[ax].p,
+                         (const float*)a.rv2[ax].p, r, a.b[ax].p);
+    }
+    HIPCHK(c, hipGetLastError());
+  }
+  return 0;
+}
+
+// ===========================================================================
+// BiCGSTAB with the scalar recurrence on the device
+// ===========================================================================
+// Identity preconditioner, the published algorithm (van der Vorst 1992) in
+// the variant that restarts when rho collapses (the reference calls Eigen's
+// BiCGSTAB, src/initialPlace.cpp:78-86).  Both axes run in the same launches
+// (blockIdx.y = axis).  FP64 dot products are reduced in a fixed order:
+// per-block partials of a fixed grid, summed sequentially per thread and
+// across the block by one fixed tree, so every block that needs a scalar
+// computes the same value and results do not depend on timing.
+namespace {
+
+constexpr int BC_MAX_PARTS = 1024;
+constexpr int BC_G = 8;  // lanes per CSR row in the SpMV kernels
+constexpr int BC_SLOTS = 8;  // partial-sum slots per axis
+enum { BC_RR0 = 0, BC_BB, BC_R0V, BC_TT, BC_TS, BC_RR, BC_R0R, BC_RST };
+
+struct BcState {
+  double rho, rho_old, alpha, w, rr, r0r, r0_sq, rhs_sq, tol2, eps2;
+  int i, restarts, done, restart, zero, max_iter, started, pending;  // pending: an iteration awaits its head
+  float err;
+};
+
+struct BcAxis {
+  const int* rowp;
+  const int* col;
+  const float* val;
+  const float* b;
+  float *x, *r, *r0, *v, *p, *s, *t;
+  double* parts;  // BC_SLOTS x nparts
+  BcState* st;
+};
+
+struct BcArgs {
+  BcAxis ax[2];
+  int n, nparts;
+};
+
+__device__ __forceinline__ double wsum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// fixed-order block sum; result valid in every thread
+__device__ double bsum(double v, double* sh) {
+  v = wsum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); k++) r += sh[k];
+  return r;
+}
+
+// sum of one partial slot, identical in every block
+__device__ double psum(const BcAxis& a, int slot, int nparts, double* sh) {
+  double v = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) v += a.parts[slot * nparts + k];
+  return bsum(v, sh);
+}
+
+// y_i = sum_k val_k x_col_k over BC_G lanes (strided entries, FP64), then a
+// fixed xor tree inside the lane group; every lane of the group gets y_i.
+__device__ __forceinline__ float spmv_row(const BcAxis& a, int i, const float* __restrict__ x) {
+  const int lane = threadIdx.x & (BC_G - 1);
+  double s = 0.0;
+  for (int k = a.rowp[i] + lane; k < a.rowp[i + 1]; k += BC_G) s += (double)a.val[k] * (double)x[a.col[k]];
+  for (int o = BC_G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return (float)s;
+}
+
+#define BC_ROWS(i) for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += gridDim.x * blockDim.x)
+// one row per BC_G-lane group; all lanes of a group run the same trip count
+#define BC_GROUP_ROWS(i)                                                                  \
+  for (int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) / BC_G); i < g.n;            \
+       i += (int)(gridDim.x * blockDim.x / BC_G))
+#define BC_LEAD ((threadIdx.x & (BC_G - 1)) == 0)
+
+// r = b - A x ; r0 = r ; v = p = 0 ; partials r.r and b.b
+__global__ void __launch_bounds__(RP_BLOCK) k_bc_init(BcArgs g) {
+  __shared__ double sh[RP_BLOCK / 64];
+  const BcAxis& a = g.ax[blockIdx.y];
+  double d0 = 0.0, d1 = 0.0;
+  BC_GROUP_ROWS(i) {
+    float t = spmv_row(a, i, a.x);
+    if (BC_LEAD) {
+      float r = (float)((double)a.b[i] + -1.0 * (double)t);
+      a.r[i] = r;
+      a.r0[i] = r;
+      a.v[i] = 0.0f;
+      a.p[i] = 0.0f;
+      d0 += (double)r * (double)r;
+      d1 += (double)a.b[i] * (double)a.b[i];
+    }
+  }
+  d0 = bsum(d0, sh);
+  d1 = bsum(d1, sh);
+  if (threadIdx.x == 0) {
+    a.parts[BC_RR0 * g.nparts + blockIdx.x] = d0;
+    a.parts[BC_BB * g.nparts + blockIdx.x] = d1;
+  }
+}
+
+// Scalar head of an iteration: finishes the previous iteration (or the
+// set-up), decides termination, and prepares rho / the restart test.
+__global__ void __launch_bounds__(1024) k_bc_head(BcArgs g) {
+  __shared__ double sh[1024 / 64];
+  const BcAxis& a = g.ax[blockIdx.y];
+  BcState& S = *a.st;
+  if (S.done || (S.started && !S.pending)) return;  // idempotent: nothing to close
+  double rr, r0r;
+  int i;
+  if (!S.started) {
+    double r0_sq = psum(a, BC_RR0, g.nparts, sh);
+    double rhs_sq = psum(a, BC_BB, g.nparts, sh);
+    rr = r0r = r0_sq;
+    i = 0;
+    if (threadIdx.x == 0) {
+      S.r0_sq = r0_sq;
+      S.rhs_sq = rhs_sq;
+      S.zero = rhs_sq == 0.0;
+      S.tol2 = S.tol2 * rhs_sq;  // tol^2 was stored by the host
+      S.rho = S.alpha = S.w = 1.0;
+      S.restarts = 0;
+      S.started = 1;
+    }
+  } else {
+    rr = psum(a, BC_RR, g.nparts, sh);
+    r0r = psum(a, BC_R0R, g.nparts, sh);
+    i = S.i + 1;
+  }
+  if (threadIdx.x != 0) return;
+  const bool zero = S.zero;
+  S.pending = 0;
+  S.rr = rr;
+  S.r0r = r0r;
+  S.i = i;
+  if (zero || !(rr > S.tol2 && i < S.max_iter)) {
+    S.done = 1;
+    S.err = zero ? 0.0f : (float)sqrt(rr / S.rhs_sq);
+    return;
+  }
+  S.rho_old = S.rho;
+  S.rho = r0r;
+  S.restart = fabs(S.rho) < S.eps2 * S.r0_sq;
+}
+
+// restart: r = b - A x ; r0 = r ; partial r.r
+__global__ void __launch_bounds__(RP_BLOCK) k_bc_restart(BcArgs g) {
+  __shared__ double sh[RP_BLOCK / 64];
+  const BcAxis& a = g.ax[blockIdx.y];
+  const BcState& S = *a.st;
+  if (S.done || !S.restart) return;
+  double d0 = 0.0;
+  BC_GROUP_ROWS(i) {
+    float t = spmv_row(a, i, a.x);
+    if (BC_LEAD) {
+      float r = (float)((double)a.b[i] + -1.0 * (double)t);
+      a.r[i] = r;
+      a.r0[i] = r;
+      d0 += (double)r * (double)r;
+    }
+  }
+  d0 = bsum(d0, sh);
+  if (threadIdx.x == 0) a.parts[BC_RST * g.nparts + blockIdx.x] = d0;
+}
+
+// p = r + beta (p - w v)   (after a restart, rho = r0_sq = r.r)
+__global__ void __launch_bounds__(RP_BLOCK) k_bc_p(BcArgs g) {
+  __shared__ double sh[RP_BLOCK / 64];
+  const BcAxis& a = g.ax[blockIdx.y];
+  BcState& S = *a.st;
+  if (S.done) return;
+  double rho = S.rho;
+  if (S.restart) {
+    rho = psum(a, BC_RST, g.nparts, sh);
+    __syncthreads();  // every thread has read S before block 0 writes it
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // no other block reads these here
+      S.rho = S.r0_sq = rho;
+      if (S.restarts++ == 0) S.i = 0;
+    }
+  }
+  const double beta = (rho / S.rho_old) * (S.alpha / S.w), w = S.w;
+  BC_ROWS(i) a.p[i] = (float)((double)a.r[i] + beta * ((double)a.p[i] - w * (double)a.v[i]));
+}
+
+// v = A p ; partial r0.v
+__global__ void __launch_bounds__(RP_BLOCK) k_bc_v(BcArgs g) {
+  __shared__ double sh[RP_BLOCK / 64];
+  const BcAxis& a = g.ax[blockIdx.y];
+  if (a.st->done) return;
+  double d = 0.0;
+  BC_GROUP_ROWS(i) {
+    float v = spmv_row(a, i, a.p);
+    if (BC_LEAD) {
+      a.v[i] = v;
+      d += (double)a.r0[i] * (double)v;
+    }
+  }
+  d = bsum(d, sh);
+  if (threadIdx.x == 0) a.parts[BC_R0V * g.nparts + blockIdx.x] = d;
+}
+
+// alpha = rho / (r0.v) ; s = r - alpha v
+__global__ void __launch_bounds__(RP_BLOCK) k_bc_s(BcArgs g) {
+  __shared__ double sh[RP_BLOCK / 64];
+  const BcAxis& a = g.ax[blockIdx.y];
+  BcState& S = *a.st;
+  if (S.done) return;
+  const double alpha = S.rho / psum(a, BC_R0V, g.nparts, sh);
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) S.alpha = alpha;
+  BC_ROWS(i) a.s[i] = (float)((double)a.r[i] + -alpha * (double)a.v[i]);
+}
+
+// t = A s ; partials t.t and t.s
+__global__ void __launch_bounds__(RP_BLOCK) k_bc_t(BcArgs g) {
+  __shared__ double sh[RP_BLOCK / 64];
+  const BcAxis& a = g.ax[blockIdx.y];
+  if (a.st->done) return;
+  double tt = 0.0, ts = 0.0;
+  BC_GROUP_ROWS(i) {
+    float t = spmv_row(a, i, a.s);
+    if (BC_LEAD) {
+      a.t[i] = t;
+      tt += (double)t * (double)t;
+      ts += (double)t * (double)a.s[i];
+    }
+  }
+  tt = bsum(tt, sh);
+  ts = bsum(ts, sh);
+  if (threadIdx.x == 0) {
+    a.parts[BC_TT * g.nparts + blockIdx.x] = tt;
+    a.parts[BC_TS * g.nparts + blockIdx.x] = ts;
+  }
+}
+
+// w = (t.s)/(t.t) ; x += alpha p + w s ; r = s - w t ; partials r.r and r0.r
+__global__ void __launch_bounds__(RP_BLOCK) k_bc_xr(BcArgs g) {
+  __shared__ double sh[RP_BLOCK / 64];
+  const BcAxis& a = g.ax[blockIdx.y];
+  BcState& S = *a.st;
+  if (S.done) return;
+  const double tt = psum(a, BC_TT, g.nparts, sh);
+  const double ts = psum(a, BC_TS, g.nparts, sh);
+  const double w = tt > 0.0 ? ts / tt : 0.0, alpha = S.alpha;
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S.w = w;
+    S.pending = 1;
+  }
+  double d0 = 0.0, d1 = 0.0;
+  BC_ROWS(i) {
+    a.x[i] = (float)((double)a.x[i] + alpha * (double)a.p[i] + w * (double)a.s[i]);
+    float r = (float)((double)a.s[i] - w * (double)a.t[i]);
+    a.r[i] = r;
+    d0 += (double)r * (double)r;
+    d1 += (double)a.r0[i] * (double)r;
+  }
+  d0 = bsum(d0, sh);
+  d1 = bsum(d1, sh);
+  if (threadIdx.x == 0) {
+    a.parts[BC_RR * g.nparts + blockIdx.x] = d0;
+    a.parts[BC_R0R * g.nparts + blockIdx.x] = d1;
+  }
+}
+
+// a zero right-hand side gives x = 0
+__global__ void __launch_bounds__(RP_BLOCK) k_bc_zero_x(BcArgs g) {
+  const BcAxis& a = g.ax[blockIdx.y];
+  if (!a.st->zero) return;
+  BC_ROWS(i) a.x[i] = 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// InitialPlace iteration kernels: warm start, instance-box HPWL, updateCoordi
+// ---------------------------------------------------------------------------
+struct IpMove {
+  int n, n_net, n_pin;
+  int* icx;
+  int* icy;
+  int* dblx;  // DB lower-left of movable k (what PlacerBase::hpwl reads)
+  int* dbly;
+  const int* hx;  // half density-size used by Instance::setCenterLocation
+  const int* hy;
+  const int* mw;
+  const int* mh;
+  const int* net_start;
+  const int* pin_k;
+  const int* offx;
+  const int* offy;
+  int* pin_cx;
+  int* pin_cy;
+  const int4* sbox;  // per pin: box of a fixed ITerm / placed BTerm, or (MAX, MAX, MIN, MIN)
+  long long* hparts;
+  float* x0;
+  float* x1;
+};
+
+__global__ void __launch_bounds__(RP_BLOCK) k_ip_warm(IpMove m) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m.n) return;
+  m.x0[k] = (float)m.icx[k];
+  m.x1[k] = (float)m.icy[k];
+}
+
+// PlacerBase::hpwl over instance boxes (src/placerBase.cpp:868-876, 480-500)
+__global__ void __launch_bounds__(RP_BLOCK) k_ip_hpwl(IpMove m) {
+  __shared__ long long sh[RP_BLOCK / 64];
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  long long h = 0;
+  if (e < m.n_net) {
+    int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
+    for (int q = m.net_start[e]; q < m.net_start[e + 1]; q++) {
+      int k = m.pin_k[q];
+      int4 b;
+      if (k >= 0) {
+        b.x = m.dblx[k];
+        b.y = m.dbly[k];
+        b.z = b.x + m.mw[k];
+        b.w = b.y + m.mh[k];
+      } else {
+        b = m.sbox[q];
+      }
+      lx = min(lx, b.x);
+      ly = min(ly, b.y);
+      ux = max(ux, b.z);
+      uy = max(uy, b.w);
+    }
+    h = (long long)((int)((unsigned)ux - (unsigned)lx) + (int)((unsigned)uy - (unsigned)ly));
+  }
+  for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = h;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long t = 0;
+    for (int w = 0; w < RP_BLOCK / 64; w++) t += sh[w];
+    m.hparts[blockIdx.x] = t;
+  }
+}
+
+__global__ void k_ip_hpwl_final(const long long* parts, int np, long long* out) {
+  __shared__ long long sh[1024 / 64];
+  long long h = 0;
+  for (int k = threadIdx.x; k < np; k += blockDim.x) h += parts[k];
+  for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = h;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[w];
+    *out = t;
+  }
+}
+
+// updateCoordi (src/initialPlace.cpp:380-385): centres from the solution,
+// Instance::setCenterLocation and the DB location
+__global__ void __launch_bounds__(RP_BLOCK) k_ip_update(IpMove m) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m.n) return;
+  int x = (int)m.x0[k], y = (int)m.x1[k];
+  m.icx[k] = x;
+  m.icy[k] = y;
+  m.dblx[k] = x - m.hx[k];
+  m.dbly[k] = y - m.hy[k];
+}
+
+// Pin::updateLocation for pins of movable instances
+__global__ void __launch_bounds__(RP_BLOCK) k_ip_pins(IpMove m) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m.n_pin) return;
+  int k = m.pin_k[q];
+  if (k < 0) return;
+  m.pin_cx[q] = m.icx[k] + m.offx[q];
+  m.pin_cy[q] = m.icy[k] + m.offy[q];
+}
+
+struct IpRec {
+  BcState st[2];
+  long long hpwl;
+};
+
+}  // namespace
+
+// The InitialPlace loop (src/initialPlace.cpp:56-105) with every iteration on
+// the device; the host reads one record (errors, HPWL) per iteration.
+int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve) {
+  hipStream_t st = c->stream;
+  const rp_params& p = c->prm;
+  const int n = (int)c->place.size(), np = c->n_pin, nn = c->n_net;
+  using clk = std::chrono::steady_clock;
+  auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
+  RpIpAsm A;
+  if (int r = rp_ip_asm_init(c, A)) return r;
+  // movable state and static per-pin data
+  std::vector<int> icx(n), icy(n), dlx(n), dly(n), hx(n), hy(n), mw(n), mh(n);
+  for (int j = 0; j < n; j++) {  // row j = movable instance order[j]
+    const int k = j, i = c->place[rp_movable_order(c)[j]];
+    icx[k] = (c->ilx[i] + c->iux[i]) / 2;
+    icy[k] = (c->ily[i] + c->iuy[i]) / 2;
+    hx[k] = (c->iux[i] - c->ilx[i]) / 2;
+    hy[k] = (c->iuy[i] - c->ily[i]) / 2;
+    dlx[k] = c->dblx[i];
+    dly[k] = c->dbly[i];
+    mw[k] = c->mw[i];
+    mh[k] = c->mh[i];
+  }
+  std::vector<int4> sbox(np);
+  for (int q = 0; q < np; q++) {
+    int i = c->pin_inst[q];
+    if (i >= 0) {
+      sbox[q] = make_int4(c->dblx[i], c->dbly[i], c->dblx[i] + c->mw[i], c->dbly[i] + c->mh[i]);
+    } else if (c->pin_box[4 * (size_t)q] != INT_MAX) {
+      const int* b = &c->pin_box[4 * (size_t)q];
+      sbox[q] = make_int4(b[0], b[1], b[2], b[3]);
+    } else {
+      sbox[q] = make_int4(INT_MAX, INT_MAX, INT_MIN, INT_MIN);
+    }
+  }
+  DevBuf<int> d_dlx, d_dly, d_hx, d_hy, d_mw, d_mh, d_offx, d_offy;
+  DevBuf<int4> d_sbox;
+  DevBuf<long long> d_hparts;
+  HIPCHK(c, dupload(d_dlx, dlx, st));
+  HIPCHK(c, dupload(d_dly, dly, st));
+  HIPCHK(c, dupload(d_hx, hx, st));
+  HIPCHK(c, dupload(d_hy, hy, st));
+  HIPCHK(c, dupload(d_mw, mw, st));
+  HIPCHK(c, dupload(d_mh, mh, st));
+  HIPCHK(c, dupload(d_offx, c->pin_offx, st));
+  HIPCHK(c, dupload(d_offy, c->pin_offy, st));
+  HIPCHK(c, dupload(d_sbox, sbox, st));
+  const int hblocks = std::max(1, (nn + RP_BLOCK - 1) / RP_BLOCK);
+  HIPCHK(c, dalloc(d_hparts, hblocks));
+  HIPCHK(c, hipMemcpyAsync(A.icx.p, icx.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(A.icy.p, icy.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(A.pin_cx.p, c->pin_cx.data(), sizeof(int) * np, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(A.pin_cy.p, c->pin_cy.data(), sizeof(int) * np, hipMemcpyHostToDevice, st));
+  // solver buffers (both axes)
+  const int nparts = (int)std::max<long>(1, std::min<long>(BC_MAX_PARTS, ((long)n * BC_G + RP_BLOCK - 1) / RP_BLOCK));
+  DevBuf<float> vec[2][7];
+  DevBuf<double> parts[2];
+  DevBuf<IpRec> d_rec;
+  for (int ax = 0; ax < 2; ax++) {
+    for (auto& v : vec[ax]) HIPCHK(c, dalloc(v, n));
+    HIPCHK(c, dalloc(parts[ax], (size_t)BC_SLOTS * nparts));
+  }
+  HIPCHK(c, dalloc(d_rec, 1));
+  IpRec* h_rec = nullptr;
+  int* h_done = nullptr;
+  HIPCHK(c, hipHostMalloc((void**)&h_rec, sizeof(IpRec), hipHostMallocDefault));
+  struct Pinned {
+    void* p;
+    ~Pinned() {
+      if (p) (void)hipHostFree(p);
+    }
+  } pin_rec{h_rec};
+  HIPCHK(c, hipHostMalloc((void**)&h_done, 2 * sizeof(int), hipHostMallocDefault));
+  Pinned pin_done{h_done};
+  IpMove mv{n, nn, np, A.icx.p, A.icy.p, d_dlx.p, d_dly.p, d_hx.p, d_hy.p, d_mw.p, d_mh.p, A.net_start.p,
+            A.pin_k.p, d_offx.p, d_offy.p, A.pin_cx.p, A.pin_cy.p, d_sbox.p, d_hparts.p, vec[0][0].p, vec[1][0].p};
+  BcArgs g{};
+  g.n = n;
+  g.nparts = nparts;
+  const float tol = 1.1920928955078125e-07f;  // NumTraits<float>::epsilon()
+  BcState init{};
+  init.tol2 = (double)tol * (double)tol;
+  init.eps2 = 1.1920928955078125e-07 * 1.1920928955078125e-07;
+  init.max_iter = p.initial_place_max_solver_iter;
+  const dim3 gb(nparts, 2), g1(1, 2), blk(RP_BLOCK);
+  const dim3 gn((n + RP_BLOCK - 1) / RP_BLOCK), gp((np + RP_BLOCK - 1) / RP_BLOCK);
+  const int CH = 8;
+  const int cap = 2 * std::max(0, init.max_iter) + 1;  // i restarts from 0 at most once
+  for (int it = 1; it <= p.initial_place_max_iter; it++) {
+    auto ta = clk::now();
+    if (host_asm) {
+      if (int r = rp_ip_host_assemble(c, A)) return r;
+    } else {
+      if (int r = rp_ip_assemble(c, A, nullptr, nullptr)) return r;
+    }
+    *t_asm += since(ta);
+    auto ts = clk::now();
+    for (int ax = 0; ax < 2; ax++) {
+      BcAxis& a = g.ax[ax];
+      a.rowp = A.rowp[ax].p;
+      a.col = A.col[ax].p;
+      a.val = A.val[ax].p;
+      a.b = A.b[ax].p;
+      a.x = vec[ax][0].p;
+      a.r = vec[ax][1].p;
+      a.r0 = vec[ax][2].p;
+      a.v = vec[ax][3].p;
+      a.p = vec[ax][4].p;
+      a.s = vec[ax][5].p;
+      a.t = vec[ax][6].p;
+      a.parts = parts[ax].p;
+      a.st = &d_rec.p->st[ax];
+    }
+    IpRec r0{};
+    r0.st[0] = r0.st[1] = init;
+    *h_rec = r0;
+    HIPCHK(c, hipMemcpyAsync(d_rec.p, h_rec, sizeof(IpRec), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_ip_warm, gn, blk, 0, st, mv);
+    hipLaunchKernelGGL(k_bc_init, gb, blk, 0, st, g);
+    int done_iters = 0;
+    for (;;) {
+      for (int j = 0; j < CH; j++) {
+        hipLaunchKernelGGL(k_bc_head, g1, dim3(1024), 0, st, g);
+        hipLaunchKernelGGL(k_bc_restart, gb, blk, 0, st, g);
+        hipLaunchKernelGGL(k_bc_p, gb, blk, 0, st, g);
+        hipLaunchKernelGGL(k_bc_v, gb, blk, 0, st, g);
+        hipLaunchKernelGGL(k_bc_s, gb, blk, 0, st, g);
+        hipLaunchKernelGGL(k_bc_t, gb, blk, 0, st, g);
+        hipLaunchKernelGGL(k_bc_xr, gb, blk, 0, st, g);
+      }
+      done_iters += CH;
+      hipLaunchKernelGGL(k_bc_head, g1, dim3(1024), 0, st, g);  // closes the chunk's last iteration
+      HIPCHK(c, hipGetLastError());
+      HIPCHK(c, hipMemcpyAsync(&h_done[0], &d_rec.p->st[0].done, sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPCHK(c, hipMemcpyAsync(&h_done[1], &d_rec.p->st[1].done, sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPCHK(c, hipStreamSynchronize(st));
+      if ((h_done[0] && h_done[1]) || done_iters >= cap) break;
+    }
+    hipLaunchKernelGGL(k_bc_zero_x, gb, blk, 0, st, g);
+    // log record: HPWL before updateCoordi (src/initialPlace.cpp:88-91)
+    hipLaunchKernelGGL(k_ip_hpwl, dim3(hblocks), blk, 0, st, mv);
+    hipLaunchKernelGGL(k_ip_hpwl_final, dim3(1), dim3(1024), 0, st, (const long long*)d_hparts.p, hblocks,
+                       &d_rec.p->hpwl);
+    hipLaunchKernelGGL(k_ip_update, gn, blk, 0, st, mv);
+    hipLaunchKernelGGL(k_ip_pins, gp, blk, 0, st, mv);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(h_rec, d_rec.p, sizeof(IpRec), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    *t_solve += since(ts);
+    if (!h_rec->st[0].done || !h_rec->st[1].done)
+      return fail(c, RP_ERR_STATE, "initial place: BiCGSTAB did not terminate");
+    float err = std::max(h_rec->st[0].err, h_rec->st[1].err);
+    c->ip_err.push_back(err);
+    c->ip_hpwl.push_back(h_rec->hpwl);
+    if (err <= 1e-5 && it >= 5) break;
+  }
+  // final positions back to the host model
+  HIPCHK(c, hipMemcpyAsync(icx.data(), A.icx.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipMemcpyAsync(icy.data(), A.icy.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipMemcpyAsync(dlx.data(), d_dlx.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipMemcpyAsync(dly.data(), d_dly.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  const std::vector<int>& order = rp_movable_order(c);
+  for (int k = 0; k < n; k++) {
+    int i = c->place[order[k]];
+    int h0 = hx[k], h1 = hy[k];
+    c->ilx[i] = icx[k] - h0;
+    c->iux[i] = icx[k] + h0;
+    c->ily[i] = icy[k] - h1;
+    c->iuy[i] = icy[k] + h1;
+    c->dblx[i] = dlx[k];
+    c->dbly[i] = dly[k];
+  }
+  for (int q = 0; q < np; q++) {
+    int i = c->pin_inst[q];
+    if (i < 0 || c->inst2place[i] < 0) continue;
+    c->pin_cx[q] = (c->ilx[i] + c->iux[i]) / 2 + c->pin_offx[q];
+    c->pin_cy[q] = (c->ily[i] + c->iuy[i]) / 2 + c->pin_offy[q];
+  }
+  return 0;
+}

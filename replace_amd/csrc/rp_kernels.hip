@@ -1058,72 +1058,6 @@ __global__ void __launch_bounds__(RP_CTRL_BLOCK) kb_control_main(const RpDev* de
 }
 
 // ---------------------------------------------------------------------------
-// initial placement: CSR SpMV and BiCGSTAB vector kernels (FP64 dots)
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(RP_BLOCK) k_spmv(int n, const int* __restrict__ rowp, const int* __restrict__ col,
-                                                   const float* __restrict__ val, const float* __restrict__ x,
-                                                   float* __restrict__ y) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  double s = 0.0;
-  for (int k = rowp[r]; k < rowp[r + 1]; k++) s += (double)val[k] * (double)x[col[k]];
-  y[r] = (float)s;
-}
-
-// partial dot products: out[blk*3 + q] for pairs (a0,b0) (a1,b1) (a2,b2); unused pairs NULL
-__global__ void __launch_bounds__(RP_BLOCK) k_dot3(int n, const float* a0, const float* b0, const float* a1,
-                                                   const float* b1, const float* a2, const float* b2,
-                                                   double* out) {
-  __shared__ double sh[RP_BLOCK / 64];
-  double v0 = 0, v1 = 0, v2 = 0;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    if (a0) v0 += (double)a0[k] * (double)b0[k];
-    if (a1) v1 += (double)a1[k] * (double)b1[k];
-    if (a2) v2 += (double)a2[k] * (double)b2[k];
-  }
-  double r0 = block_sum_d(v0, sh);
-  double r1 = block_sum_d(v1, sh);
-  double r2 = block_sum_d(v2, sh);
-  if (threadIdx.x == 0) {
-    out[blockIdx.x * 3 + 0] = r0;
-    out[blockIdx.x * 3 + 1] = r1;
-    out[blockIdx.x * 3 + 2] = r2;
-  }
-}
-
-__global__ void k_dot_final(int nparts, const double* parts, double* out3) {
-  __shared__ double sh[RP_BLOCK / 64];
-  for (int q = 0; q < 3; q++) {
-    double v = 0;
-    for (int k = threadIdx.x; k < nparts; k += blockDim.x) v += parts[k * 3 + q];
-    double r = block_sum_d(v, sh);
-    if (threadIdx.x == 0) out3[q] = r;
-  }
-}
-
-// y = a + alpha * b  (double coefficients, float vectors)
-__global__ void k_axpy(int n, float* y, const float* a, double alpha, const float* b) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) y[k] = (float)((double)a[k] + alpha * (double)b[k]);
-}
-
-// p = r + beta * (p - w * v)
-__global__ void k_bicg_p(int n, float* p, const float* r, const float* v, double beta, double w) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) p[k] = (float)((double)r[k] + beta * ((double)p[k] - w * (double)v[k]));
-}
-
-// x += alpha * p + w * s ; r = s - w * t
-__global__ void k_bicg_xr(int n, float* x, float* r, const float* p, const float* s, const float* t, double alpha,
-                          double w) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) {
-    x[k] = (float)((double)x[k] + alpha * (double)p[k] + w * (double)s[k]);
-    r[k] = (float)((double)s[k] - w * (double)t[k]);
-  }
-}
-
-// ---------------------------------------------------------------------------
 // host-callable launch wrappers (C++ linkage, used by rp_host.cpp)
 // ---------------------------------------------------------------------------
 // FFT-based kernels always run RP_BLOCK threads: the per-thread partial
@@ -1197,38 +1131,6 @@ hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int fr
 hipError_t rp_launch_wa_grad(const RpDev& d, hipStream_t s, int store_nets) {
   hipLaunchKernelGGL(k_wa, dim3((d.nn + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, d, store_nets);
   hipLaunchKernelGGL(k_grad, dim3((d.nc + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, d);
-  return hipGetLastError();
-}
-
-hipError_t rp_launch_spmv(int n, const int* rowp, const int* col, const float* val, const float* x, float* y,
-                          hipStream_t s) {
-  hipLaunchKernelGGL(k_spmv, dim3((n + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, n, rowp, col, val, x, y);
-  return hipGetLastError();
-}
-
-hipError_t rp_launch_dot3(int n, const float* a0, const float* b0, const float* a1, const float* b1,
-                          const float* a2, const float* b2, double* parts, int nparts, double* out3,
-                          hipStream_t s) {
-  hipLaunchKernelGGL(k_dot3, dim3(nparts), dim3(RP_BLOCK), 0, s, n, a0, b0, a1, b1, a2, b2, parts);
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RP_BLOCK), 0, s, nparts, (const double*)parts, out3);
-  return hipGetLastError();
-}
-
-hipError_t rp_launch_axpy(int n, float* y, const float* a, double alpha, const float* b, hipStream_t s) {
-  hipLaunchKernelGGL(k_axpy, dim3((n + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, n, y, a, alpha, b);
-  return hipGetLastError();
-}
-
-hipError_t rp_launch_bicg_p(int n, float* p, const float* r, const float* v, double beta, double w,
-                            hipStream_t s) {
-  hipLaunchKernelGGL(k_bicg_p, dim3((n + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, n, p, r, v, beta, w);
-  return hipGetLastError();
-}
-
-hipError_t rp_launch_bicg_xr(int n, float* x, float* r, const float* p, const float* sv, const float* t,
-                             double alpha, double w, hipStream_t s) {
-  hipLaunchKernelGGL(k_bicg_xr, dim3((n + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, n, x, r, p, sv, t,
-                     alpha, w);
   return hipGetLastError();
 }
 

@@ -38,7 +38,7 @@ def copy_probe(mb: int, reps: int) -> None:
 
 
 def _short(name: str) -> str:
-    n = name.split("(")[0]
+    n = name.replace("(anonymous namespace)::", "").split("(")[0]
     return n.replace("void ", "")[:70]
 
 

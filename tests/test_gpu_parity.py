@@ -203,20 +203,14 @@ def test_run_is_bit_reproducible():
 def test_initial_place_tracks_reference_log():
     """BiCGSTAB iterates are parity-unpinned (Eigen absent); the IP HPWL lines
     of the reference log pin them to 1e-4, the first line exactly."""
-    import ctypes
-    from replace_amd import capi
     for name in ["simple03", "simple04"]:
         nl, p = design(name)
         g = Placer(nl, p)
-        st = g.initial_place()
+        g.initial_place()
         t = traces()[name]["ip"]
-        n = ctypes.c_int32(0)
-        err = np.zeros(64, np.float32)
-        hp = np.zeros(64, np.int64)
-        capi.lib().rp_get_ip_records.argtypes = [ctypes.c_void_p] * 4 + [ctypes.POINTER(ctypes.c_int32)]
-        capi.lib().rp_get_ip_records(g.h, err.ctypes.data, hp.ctypes.data, 64, ctypes.byref(n))
-        assert n.value == len(t)
+        err, hp = g.ip_records()
+        assert len(hp) == len(t)
         assert hp[0] == t[0][2]
-        for (it, e, h), got in zip(t, hp[: n.value]):
+        for (it, e, h), got in zip(t, hp):
             assert abs(int(got) - h) <= 1e-4 * h
-        assert (err[4:n.value] < 1e-5).all()
+        assert (err[4:] < 1e-5).all()
