@@ -105,19 +105,167 @@ print(json.dumps(dict(init_s=t1 - t0, iter_s=t2 - t1, evals=st.evals, poisson=o.
     }
 
 
+_PHASES = {"create": 0.0, "initial_place": 0.0, "setup": 0.0, "nesterov": 0.0}
+
+
+def _plain_job(nl, p):
+    """One job on the plain path (create, initial place, set-up, Nesterov)."""
+    from replace_amd.capi import Placer
+    t = time.perf_counter()
+    g = Placer(nl, p)
+    t1 = time.perf_counter()
+    g.initial_place()
+    t2 = time.perf_counter()
+    g.setup()
+    t3 = time.perf_counter()
+    code, st = g.nesterov_place()
+    g.close()
+    t4 = time.perf_counter()
+    _PHASES["create"] += t1 - t
+    _PHASES["initial_place"] += t2 - t1
+    _PHASES["setup"] += t3 - t2
+    _PHASES["nesterov"] += t4 - t3
+    return (code, int(st.final_hpwl), float(st.final_overflow), int(st.iterations)), t4 - t
+
+
+def _served_common(jobs, arrivals, steps_per_poll):
+    """Runs `jobs` through the placement service and back to back on the
+    plain path.  Returns (served results, served wall s, plain results, plain
+    service times)."""
+    from replace_amd.capi import Server
+    _plain_job(*jobs[0])  # warm-up: module load, first allocations
+    srv = Server(steps_per_poll=steps_per_poll)
+    ids = [srv.submit(nl, p, a) for (nl, p), a in zip(jobs, arrivals)]
+    t = time.perf_counter()
+    srv.run()
+    wall = time.perf_counter() - t
+    res = [srv.result(j) for j in ids]
+    stats = srv.stats()
+    srv.close()
+    plain, svc = [], []
+    for k in _PHASES:
+        _PHASES[k] = 0.0
+    for nl, p in jobs:
+        r, dt = _plain_job(nl, p)
+        plain.append(r)
+        svc.append(dt)
+    return res, wall, stats, plain, svc
+
+
+def _parity(res, plain, jobs):
+    diffs = [abs(r.stats.final_hpwl - q[1]) / max(1, q[1]) for r, q in zip(res, plain)]
+    unfinished = [{"job": k, "code": int(r.code), "iterations": int(r.stats.iterations),
+                   "overflow": round(float(r.stats.final_overflow), 4), "cells": int(jobs[k][0].num_insts),
+                   "td": round(float(jobs[k][1].target_density), 3)}
+                  for k, r in enumerate(res) if not (r.done == 1 and r.stats.finished == 1)]
+    return {"hpwl_max_rel_diff_vs_plain": max(diffs),
+            "max_final_overflow": max(float(r.stats.final_overflow) for r in res),
+            "all_finished": not unfinished, "unfinished": unfinished[:20]}
+
+
+def _timeline(res):
+    import numpy as np
+    setup = np.array([r.start_s - r.admit_s for r in res])
+    run = np.array([r.done_s - r.start_s for r in res])
+    return {"setup_s": {"p50": round(float(np.percentile(setup, 50)), 4), "max": round(float(setup.max()), 4)},
+            "on_gpu_s": {"p50": round(float(np.percentile(run, 50)), 4), "max": round(float(run.max()), 4)},
+            "last_start_s": round(max(r.start_s for r in res), 4),
+            "last_done_s": round(max(r.done_s for r in res), 4)}
+
+
+def bench_batch(args):
+    """256 uneven instances submitted together: batched service vs back to back."""
+    from replace_amd import netgen
+    n = args.jobs or 256
+    t = time.perf_counter()
+    jobs = [netgen.batch_instance(i) for i in range(n)]
+    gen_s = time.perf_counter() - t
+    res, wall, stats, plain, svc = _served_common(jobs, [0.0] * n, args.steps_per_poll)
+    plain_s = sum(svc)
+    cells = sum(nl.num_insts for nl, _ in jobs)
+    out = {
+        "metric": "placements/s, 256 uneven instances (batched service)",
+        "value": round(n / wall, 3), "unit": "placements/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{n} instances, cells log-uniform 500-50k (sum {cells}), td U[0.6,0.9], "
+                               "bins clamp(nextPow2(sqrt(Nc)),32,256)",
+                   "steps_per_poll": args.steps_per_poll},
+        "wall_s": round(wall, 3),
+        "plain_back_to_back": {"value": round(n / plain_s, 3), "unit": "placements/s", "wall_s": round(plain_s, 3),
+                               "phases_s": {k: round(v, 3) for k, v in _PHASES.items()}},
+        "speedup_vs_plain": round(plain_s / wall, 2),
+        "batched_steps": int(stats.steps), "table_rebuilds": int(stats.rebuilds),
+        "netgen_s": round(gen_s, 2), "timeline": _timeline(res),
+    }
+    out.update(_parity(res, plain, jobs))
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(args):
+    """Poisson arrivals (batched-distribution jobs + 5% large): served latency
+    and throughput vs the same stream back to back on the plain path (FIFO
+    single server, timeline from each job's measured plain service time)."""
+    from replace_amd import netgen
+    n = args.jobs or 200
+    stream = netgen.served_stream(n, args.rate)
+    t = time.perf_counter()
+    jobs = [netgen.batch_instance(i) if kind == "batch" else netgen.large_instance(i) for _, kind, i in stream]
+    gen_s = time.perf_counter() - t
+    arrivals = [a for a, _, _ in stream]
+    res, wall, stats, plain, svc = _served_common(jobs, arrivals, args.steps_per_poll)
+    import numpy as np
+    lat = np.array([r.done_s - r.arrival_s for r in res])
+    done = 0.0
+    plat = []
+    for a, dt in zip(arrivals, svc):
+        done = max(done, a) + dt
+        plat.append(done - a)
+    plat = np.array(plat)
+    makespan = max(r.done_s for r in res)
+    out = {
+        "metric": "served placements/s + p99 latency (Poisson arrivals, 5% large jobs)",
+        "value": round(n / makespan, 3), "unit": "placements/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{n} jobs, Poisson rate {args.rate}/s (seed 7), 95% batched distribution, "
+                               "5% large (200k-1M cells, 512/1024 bins)",
+                   "large_jobs": sum(1 for _, k, _ in stream if k == "large"),
+                   "steps_per_poll": args.steps_per_poll},
+        "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
+                      "max": round(float(lat.max()), 4)},
+        "plain_back_to_back": {"value": round(n / (done), 3), "unit": "placements/s",
+                               "latency_s": {"p50": round(float(np.percentile(plat, 50)), 4),
+                                             "p99": round(float(np.percentile(plat, 99)), 4)},
+                               "service_s_total": round(sum(svc), 3)},
+        "speedup_vs_plain": round(done / makespan, 2),
+        "batched_steps": int(stats.steps), "table_rebuilds": int(stats.rebuilds),
+        "netgen_s": round(gen_s, 2), "timeline": _timeline(res),
+    }
+    out.update(_parity(res, plain, jobs))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1"])
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c3", "c4"])
     ap.add_argument("--cpu-sample-evals", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--jobs", type=int, default=None, help="c3/c4: number of jobs")
+    ap.add_argument("--rate", type=float, default=27.0,
+                    help="c4: arrival rate, jobs/s (default: ~80%% of the measured 34/s batched capacity)")
+    ap.add_argument("--steps-per-poll", type=int, default=8, help="c3/c4: batched steps per graph replay")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build places one instance per GPU and "
               "has no multi-GPU path (see DESIGN.md for how it would shard)", file=sys.stderr)
         sys.exit(2)
+    if args.workload in ("c3", "c4"):
+        import torch
+        torch.cuda.set_device(0)
+        (bench_batch if args.workload == "c3" else bench_serve)(args)
+        return
 
     import torch
     torch.cuda.set_device(0)

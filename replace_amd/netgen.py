@@ -214,3 +214,38 @@ def batch_instance(i: int):
     p = params_default(target_density=td, bin_grid_cnt_x=nb, bin_grid_cnt_y=nb,
                        target_overflow=0.10)
     return nl, p
+
+
+def large_instance(i: int):
+    """A large served job: cells uniform in [200k, 1M], target density
+    U[0.6, 0.9] and utilisation min(0.70, td - 0.01) as in the batched
+    distribution, bins nextPow2(sqrt(Nc)) clamped to [512, 1024] per side,
+    seed 5000 + i."""
+    rng = np.random.default_rng(5000 + i)
+    nc = int(rng.integers(200000, 1000001))
+    td = float(rng.uniform(0.6, 0.9))
+    util = min(0.70, td - 0.01)
+    nb = min(1024, max(512, next_pow2(int(math.ceil(math.sqrt(nc))))))
+    nl = generate(nc, int(round(nc * 1.05)), seed=5000 + i, util=util, td=td, name=f"large_{i}")
+    p = params_default(target_density=td, bin_grid_cnt_x=nb, bin_grid_cnt_y=nb, target_overflow=0.10)
+    return nl, p
+
+
+def served_stream(num_jobs: int, rate: float, large_frac: float = 0.05, seed: int = 7):
+    """Job stream of the served workload: Poisson arrivals at `rate` jobs/s
+    (exponential gaps, seed 7); each job is a large instance with probability
+    `large_frac`, otherwise an instance of the batched distribution.
+    Returns [(arrival_s, kind, index)], kind in {"batch", "large"}."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    nb = nl = 0
+    for _ in range(num_jobs):
+        t += float(rng.exponential(1.0 / rate))
+        if rng.uniform() < large_frac:
+            out.append((t, "large", nl))
+            nl += 1
+        else:
+            out.append((t, "batch", nb))
+            nb += 1
+    return out
