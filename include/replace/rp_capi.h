@@ -120,6 +120,11 @@ typedef struct rp_params {
   float min_preconditioner;               /* 1.0 */
   float initial_prev_coordi_update_coef;  /* 100 */
   float min_wire_length_force_bar;        /* -300 */
+  /* Poisson transform (no reference counterpart; additive):
+   *   0  FP32 FFT-based DCT/DST (default; the reference's fftsg arithmetic class)
+   *   1  dense transform-matrix GEMMs on matrix cores, split-BF16 operands
+   *      (hi + lo) with FP32 accumulation; bin counts must be multiples of 128 */
+  int32_t poisson_transform;              /* 0 */
 } rp_params;
 
 /* Per-iteration record of the Nesterov loop: the values the reference prints

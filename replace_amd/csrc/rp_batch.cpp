@@ -205,6 +205,7 @@ extern "C" int rp_server_create(const rp_server_opts* opt, rp_server** out) {
 extern "C" int rp_server_submit(rp_server* s, const rp_netlist* nl, const rp_params* p, double arrival_s,
                                 int32_t* job) {
   if (!s || !nl || !p) return RP_ERR_ARG;
+  if (p->poisson_transform != 0) return sfail(s, RP_ERR_ARG, "the placement service runs the FFT Poisson transform only");
   std::unique_ptr<Job> j(new Job());
   j->id = (int)s->jobs.size();
   j->arrival = arrival_s;

@@ -23,6 +23,28 @@ struct DevBuf {
   }
 };
 
+// Matrix-core Poisson buffers (rp_dct_mfma.hip); bf16 data held as 16-bit words
+struct RpMfma {
+  bool ready = false;
+  DevBuf<uint16_t> cx_hi, cx_lo, cxs_hi, cxs_lo, cy_hi, cy_lo, cyt_hi, cyt_lo, sy_hi, sy_lo;
+  DevBuf<float> wx2, wy2;
+  DevBuf<uint16_t> rho_hi, rho_lo, t1_hi, t1_lo, p_hi, p_lo, r_hi, r_lo;
+  DevBuf<double> sums;
+  RpMfmaDev view() const {
+    auto b = [](const DevBuf<uint16_t>& x) { return (__bf16*)x.p; };
+    return RpMfmaDev{b(cx_hi), b(cx_lo), b(cxs_hi), b(cxs_lo), b(cy_hi), b(cy_lo), b(cyt_hi), b(cyt_lo),
+                     b(sy_hi), b(sy_lo), wx2.p, wy2.p, b(rho_hi), b(rho_lo), b(t1_hi), b(t1_lo),
+                     b(p_hi), b(p_lo), b(r_hi), b(r_lo), sums.p};
+  }
+  size_t bytes() const {
+    size_t t = 0;
+    for (const DevBuf<uint16_t>* x : {&cx_hi, &cx_lo, &cxs_hi, &cxs_lo, &cy_hi, &cy_lo, &cyt_hi, &cyt_lo, &sy_hi,
+                                      &sy_lo, &rho_hi, &rho_lo, &t1_hi, &t1_lo, &p_hi, &p_lo, &r_hi, &r_lo})
+      t += x->n * 2;
+    return t + (wx2.n + wy2.n) * 4 + sums.n * 8;
+  }
+};
+
 struct rp_ctx {
   rp_params prm;
   std::string err;
@@ -83,6 +105,9 @@ struct rp_ctx {
   std::vector<int> gperm;      // device gcell index -> reference gcell index
   std::vector<int> ginv;       // reference gcell index -> device gcell index
   RpDev dev;
+  RpMfma mfma;
+  RpMfmaDev mfma_view{};
+  const RpMfmaDev* mf() const { return mfma.ready ? &mfma_view : nullptr; }
   // ---- per-kernel timing (optional) ----
   bool timing = false;
   double ktime_ms[RP_NUM_EVAL_KERNELS] = {0};
@@ -102,7 +127,7 @@ struct rp_ctx {
     add(d_t); add(d_tt); add(d_y1t); add(d_y2t); add(d_y1); add(d_y2); add(d_field); add(d_twx); add(d_tw2x);
     add(d_twy); add(d_tw2y); add(d_wx); add(d_wx2); add(d_wy); add(d_wy2); add(d_part_grad); add(d_part_bin);
     add(d_part_net); add(d_state); add(d_rec); add(d_dbg_wl); add(d_dbg_dens);
-    return b;
+    return b + mfma.bytes();
   }
 };
 
@@ -169,10 +194,11 @@ int rp_ip_host_assemble(rp_ctx* c, RpIpAsm& a);
 int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve);
 
 // launch wrappers (rp_kernels.hip)
-hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_t* ev);
+hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_t* ev, const RpMfmaDev* mf);
 hipError_t rp_launch_ctrl_init(const RpDev& d, hipStream_t s, int stage);
 hipError_t rp_launch_scatter(const RpDev& d, hipStream_t s);
-hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int from_rho);
+hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int from_rho, const RpMfmaDev* mf);
+int rp_mfma_setup(rp_ctx* c);  // transform matrices of the matrix-core Poisson solve
 hipError_t rp_launch_wa_grad(const RpDev& d, hipStream_t s, int store_nets);
 hipError_t rp_launch_eval_batch(const RpBatchLaunch& b, hipStream_t s);
 hipError_t rp_launch_gather_phase(const RpDev* devs, int n, int* out, hipStream_t s);

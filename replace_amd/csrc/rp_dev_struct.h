@@ -9,6 +9,7 @@
 // Device pointers and sizes of one placement problem (layout in rp_device.h).
 struct RpDev {
   int nc, nn, np, nbx, nby, bsx, bsy;
+  int mfma;  // Poisson on matrix cores: the bin kernel only produces rho
   int core_lx, core_ly, core_ux, core_uy;
   float td;
   float bar;          // minWireLengthForceBar
@@ -67,3 +68,14 @@ struct RpBatchLaunch {
   int nblk[RP_NUM_EVAL_KERNELS];
   size_t lds_binfin, lds_col, lds_rinv;
 };
+
+// Device view of the matrix-core Poisson solve (rp_dct_mfma.hip): split-BF16
+// transform matrices and intermediates.
+struct RpMfmaDev {
+  const __bf16 *cx_hi, *cx_lo, *cxs_hi, *cxs_lo, *cy_hi, *cy_lo, *cyt_hi, *cyt_lo, *sy_hi, *sy_lo;
+  const float *wx2, *wy2;
+  __bf16 *rho_hi, *rho_lo, *t1_hi, *t1_lo, *p_hi, *p_lo, *r_hi, *r_lo;
+  double* sums;  // mean of rho: partials, then the mean
+};
+
+hipError_t rp_launch_poisson_mfma(const RpDev& d, const RpMfmaDev& m, hipStream_t s, int want_phi);

@@ -29,11 +29,6 @@
 
 #include "rp_dev_struct.h"
 
-hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_t* ev);
-hipError_t rp_launch_ctrl_init(const RpDev& d, hipStream_t s, int stage);
-hipError_t rp_launch_scatter(const RpDev& d, hipStream_t s);
-hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int from_rho);
-hipError_t rp_launch_wa_grad(const RpDev& d, hipStream_t s, int store_nets);
 
 #include "rp_ctx.h"
 
@@ -74,6 +69,7 @@ extern "C" void rp_params_default(rp_params* p) {
   p->min_preconditioner = 1.0f;
   p->initial_prev_coordi_update_coef = 100.0f;
   p->min_wire_length_force_bar = -300.0f;
+  p->poisson_transform = 0;
 }
 
 static int inst_cx(const rp_ctx* c, int i) { return (c->ilx[i] + c->iux[i]) / 2; }
@@ -952,6 +948,16 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   S.ring.prev = 2;
   S.ring.co_cur = 0;
   S.ring.co_next = 1;
+  // Poisson transform: FP32 FFT (default) or split-BF16 GEMMs on matrix cores
+  c->mfma.ready = false;
+  c->dev.mfma = 0;
+  if (p.poisson_transform == 1) {
+    if (int r = rp_mfma_setup(c)) return r;
+    c->mfma_view = c->mfma.view();
+    c->dev.mfma = 1;
+  } else if (p.poisson_transform != 0) {
+    return fail(c, RP_ERR_ARG, "poisson_transform must be 0 (FFT) or 1 (matrix cores)");
+  }
   c->nb_ready = true;
   if (info) {
     memset(info, 0, sizeof *info);
@@ -1033,7 +1039,7 @@ extern "C" int rp_eval_gradient(rp_ctx* c, const float* cx, const float* cy, flo
     HIPCHK(c, hipMemcpyAsync(o->bin_inst, c->d_bin_inst.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, s));
   if (o->bin_filler)
     HIPCHK(c, hipMemcpyAsync(o->bin_filler, c->d_bin_filler.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, s));
-  HIPCHK(c, rp_launch_poisson(d, s, 1, 0));
+  HIPCHK(c, rp_launch_poisson(d, s, 1, 0, c->mf()));
   HIPCHK(c, rp_launch_wa_grad(d, s, 1));
   std::vector<float2> field(nb), sg(ng), wl(ng), dn(ng);
   std::vector<double> pg((size_t)RP_P_COUNT * c->h_state->num_partials_grad), pb(c->h_state->num_partials_bin);
@@ -1125,7 +1131,7 @@ extern "C" int rp_eval_poisson(rp_ctx* c, const float* density, float* phi, floa
   S.phase = 0;
   HIPCHK(c, hipMemcpyAsync(c->d_state.p, &S, sizeof S, hipMemcpyHostToDevice, s));
   HIPCHK(c, hipMemcpyAsync(c->d_rho.p, density, sizeof(float) * nb, hipMemcpyHostToDevice, s));
-  HIPCHK(c, rp_launch_poisson(c->dev, s, 1, 1));
+  HIPCHK(c, rp_launch_poisson(c->dev, s, 1, 1, c->mf()));
   std::vector<float2> field(nb);
   HIPCHK(c, hipMemcpyAsync(field.data(), c->d_field.p, sizeof(float2) * nb, hipMemcpyDeviceToHost, s));
   if (phi) HIPCHK(c, hipMemcpyAsync(phi, c->d_phi.p, sizeof(float) * nb, hipMemcpyDeviceToHost, s));
@@ -1186,13 +1192,13 @@ int rp_nesterov_begin(rp_ctx* c, hipStream_t s) {
   const RpDev& d = c->dev;
   // eval A at curSLP with lambda 0
   HIPCHK(c, rp_launch_scatter(d, s));
-  HIPCHK(c, rp_launch_poisson(d, s, 0, 0));
+  HIPCHK(c, rp_launch_poisson(d, s, 0, 0, c->mf()));
   HIPCHK(c, rp_launch_ctrl_init(d, s, 0));
   HIPCHK(c, rp_launch_wa_grad(d, s, 0));
   HIPCHK(c, rp_launch_ctrl_init(d, s, 1));
   // eval B at prevSLP = clamp(curSLP + 100 * g)
   HIPCHK(c, rp_launch_scatter(d, s));
-  HIPCHK(c, rp_launch_poisson(d, s, 0, 0));
+  HIPCHK(c, rp_launch_poisson(d, s, 0, 0, c->mf()));
   HIPCHK(c, rp_launch_wa_grad(d, s, 0));
   HIPCHK(c, rp_launch_ctrl_init(d, s, 2));
   return RP_OK;
@@ -1257,7 +1263,7 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
   int evals_before = hs->evals;
   for (int b = 0; b < max_batches && hs->phase == 0; b++) {
     for (int k = 0; k < K; k++)
-      HIPCHK(c, rp_launch_eval(d, s, 0, c->timing ? &kev[(size_t)k * (RP_NUM_EVAL_KERNELS + 1)] : nullptr));
+      HIPCHK(c, rp_launch_eval(d, s, 0, c->timing ? &kev[(size_t)k * (RP_NUM_EVAL_KERNELS + 1)] : nullptr, c->mf()));
     HIPCHK(c, hipMemcpyAsync(hs, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipEventRecord(polled, s));
     HIPCHK(c, hipEventSynchronize(polled));

@@ -323,6 +323,7 @@ __device__ void binfin_body(const RpDev& d, int blk, int from_rho) {
   }
   double tot = block_sum_d(ov_sum, sh_red);
   if (threadIdx.x == 0) d.part_bin[blk] = tot;
+  if (d.mfma) return;  // the matrix-core solve takes rho from d.rho
   // block_sum_d synchronised the LDS writes above
   float2* Z = lds_fft(A, B, n, d.twx, -1);
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
@@ -1066,7 +1067,7 @@ static int fft_threads(int) { return RP_BLOCK; }
 
 // One evaluation slot.  ev (optional) holds RP_NUM_EVAL_KERNELS + 1 events
 // recorded around each kernel (per-kernel timing on this stream).
-hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_t* ev) {
+hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_t* ev, const RpMfmaDev* mf) {
   int nb_c = (d.nc + RP_BLOCK - 1) / RP_BLOCK;
   int nb_s = (d.nc + RP_SCATTER_CELLS - 1) / RP_SCATTER_CELLS;
   int nb_n = (d.nn + RP_BLOCK - 1) / RP_BLOCK;
@@ -1082,17 +1083,28 @@ hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_
   size_t shx = sizeof(float2) * 2 * d.nbx;
   hipLaunchKernelGGL(k_binfin_rowdct, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shx, s, d, 0);
   RP_MARK();
-  hipLaunchKernelGGL(k_transpose, dim3((d.nbx + 31) / 32, (d.nby + 31) / 32), dim3(32, 8), 0, s,
-                     (const float*)d.t, d.tt, d.nby, d.nbx, (const RpState*)d.st);
-  RP_MARK();
-  size_t shy = sizeof(float2) * 2 * d.nby + sizeof(float) * 4 * d.nby;
-  hipLaunchKernelGGL(k_col_spectral, dim3(d.nbx / 2), dim3(fft_threads(d.nby)), shy, s, d);
-  RP_MARK();
-  hipLaunchKernelGGL(k_transpose2, dim3((d.nby + 31) / 32, (d.nbx + 31) / 32, 2), dim3(32, 8), 0, s, d);
-  RP_MARK();
-  size_t shr = sizeof(float2) * 2 * d.nbx + sizeof(float) * 2 * d.nbx;
-  hipLaunchKernelGGL(k_row_inverse, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shr, s, d, want_phi);
-  RP_MARK();
+  if (mf) {
+    // matrix-core solve; its time is reported in the transpose slot and the
+    // remaining Poisson slots stay empty
+    hipError_t e = rp_launch_poisson_mfma(d, *mf, s, want_phi);
+    if (e != hipSuccess) return e;
+    RP_MARK();
+    RP_MARK();
+    RP_MARK();
+    RP_MARK();
+  } else {
+    hipLaunchKernelGGL(k_transpose, dim3((d.nbx + 31) / 32, (d.nby + 31) / 32), dim3(32, 8), 0, s,
+                       (const float*)d.t, d.tt, d.nby, d.nbx, (const RpState*)d.st);
+    RP_MARK();
+    size_t shy = sizeof(float2) * 2 * d.nby + sizeof(float) * 4 * d.nby;
+    hipLaunchKernelGGL(k_col_spectral, dim3(d.nbx / 2), dim3(fft_threads(d.nby)), shy, s, d);
+    RP_MARK();
+    hipLaunchKernelGGL(k_transpose2, dim3((d.nby + 31) / 32, (d.nbx + 31) / 32, 2), dim3(32, 8), 0, s, d);
+    RP_MARK();
+    size_t shr = sizeof(float2) * 2 * d.nbx + sizeof(float) * 2 * d.nbx;
+    hipLaunchKernelGGL(k_row_inverse, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shr, s, d, want_phi);
+    RP_MARK();
+  }
   hipLaunchKernelGGL(k_wa, dim3(nb_n), dim3(RP_BLOCK), 0, s, d, 0);
   RP_MARK();
   hipLaunchKernelGGL(k_grad, dim3(nb_c), dim3(RP_BLOCK), 0, s, d);
@@ -1115,9 +1127,10 @@ hipError_t rp_launch_scatter(const RpDev& d, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int from_rho) {
+hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int from_rho, const RpMfmaDev* mf) {
   size_t shx = sizeof(float2) * 2 * d.nbx;
   hipLaunchKernelGGL(k_binfin_rowdct, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shx, s, d, from_rho);
+  if (mf) return rp_launch_poisson_mfma(d, *mf, s, want_phi);
   hipLaunchKernelGGL(k_transpose, dim3((d.nbx + 31) / 32, (d.nby + 31) / 32), dim3(32, 8), 0, s,
                      (const float*)d.t, d.tt, d.nby, d.nbx, (const RpState*)d.st);
   size_t shy = sizeof(float2) * 2 * d.nby + sizeof(float) * 4 * d.nby;

@@ -72,6 +72,7 @@ class RpParams(ctypes.Structure):
         ("min_preconditioner", ctypes.c_float),
         ("initial_prev_coordi_update_coef", ctypes.c_float),
         ("min_wire_length_force_bar", ctypes.c_float),
+        ("poisson_transform", ctypes.c_int32),
     ]
 
 
@@ -181,6 +182,7 @@ def params_default(**overrides) -> RpParams:
         min_preconditioner=1.0,
         initial_prev_coordi_update_coef=100.0,
         min_wire_length_force_bar=-300.0,
+        poisson_transform=0,
     )
     for k, v in overrides.items():
         setattr(p, k, v)

@@ -52,7 +52,7 @@ def test_params_default_matches_library():
 
 def test_struct_sizes_match_header_layout():
     # sizes of the C structs on x86-64 (pointers 8 bytes, natural alignment)
-    assert ctypes.sizeof(RpParams) == 21 * 4
+    assert ctypes.sizeof(RpParams) == 22 * 4
     assert ctypes.sizeof(RpIterRecord) == 24
     assert ctypes.sizeof(RpNesterovStats) == 48
     assert ctypes.sizeof(RpEvalOut) == 14 * 8
