@@ -244,12 +244,62 @@ def bench_serve(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_precision(args):
+    """1M-cell netlist on a 2048 x 2048 grid: time to overflow 0.10 with the
+    Poisson solve as split-BF16 GEMMs on matrix cores vs the FP32 FFT path."""
+    import torch
+    from replace_amd import netgen
+    from replace_amd.capi import Placer
+    nl, p = netgen.workload("c1")
+    p.bin_grid_cnt_x = p.bin_grid_cnt_y = 2048
+    out = {}
+    for mode in (0, 1):
+        p.poisson_transform = mode
+        g = Placer(nl, p)
+        g.initial_place()
+        info = g.setup()
+        for _ in range(args.warmup):
+            g.nesterov_place()
+        g.set_kernel_timing(True)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            code, st = g.nesterov_place()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        kt = g.kernel_timing()
+        out[mode] = {"s": dt, "code": code, "iterations": st.iterations, "evals": st.evals,
+                     "final_overflow": float(st.final_overflow), "final_hpwl": int(st.final_hpwl),
+                     "kernels_us": {k: round(v[0] / max(1, v[1]) * 1e3, 2) for k, v in kt.items()}}
+        g.close()
+    N = 2048
+    flops = 3 * 2 * 12 * N ** 3  # 5 GEMMs without phi: 12 N^3 multiply-adds, 3 split products
+    poisson_us = out[1]["kernels_us"]["transpose"]  # the matrix-core solve's timing slot
+    achieved = flops / (poisson_us * 1e-6) / 1e12
+    res = {
+        "metric": "s to overflow<=0.10 at 1M cells, 2048x2048 bins (matrix-core split-BF16 Poisson)",
+        "value": round(out[1]["s"], 4), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "dtype": "bf16x2 (split) GEMM, f32 accumulate", "data": "synthetic",
+        "config": {"workload": "1M-cell netlist, 2048x2048 bins, td 0.70", "steps": args.steps},
+        "fft_path": {"s": round(out[0]["s"], 4), "iterations": out[0]["iterations"],
+                     "final_hpwl": out[0]["final_hpwl"], "kernels_us": out[0]["kernels_us"]},
+        "mfma_path": {"iterations": out[1]["iterations"], "final_hpwl": out[1]["final_hpwl"],
+                      "final_overflow": round(out[1]["final_overflow"], 5), "kernels_us": out[1]["kernels_us"]},
+        "speedup_vs_fft": round(out[0]["s"] / out[1]["s"], 3),
+        "hpwl_rel_diff_vs_fft": abs(out[1]["final_hpwl"] - out[0]["final_hpwl"]) / out[0]["final_hpwl"],
+        "roofline": {"bound": "mfma", "kernel": "poisson_mfma", "achieved": round(achieved, 1),
+                     "peak": 2500.0, "unit": "TFLOP/s", "frac": round(achieved / 2500.0, 4),
+                     "flops_per_launch": flops, "avg_launch_us": poisson_us},
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c3", "c4"])
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c3", "c4"])
     ap.add_argument("--cpu-sample-evals", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--jobs", type=int, default=None, help="c3/c4: number of jobs")
@@ -261,6 +311,11 @@ def main():
         print("bench.py runs on one GPU only: this build places one instance per GPU and "
               "has no multi-GPU path (see DESIGN.md for how it would shard)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c2":
+        import torch
+        torch.cuda.set_device(0)
+        bench_precision(args)
+        return
     if args.workload in ("c3", "c4"):
         import torch
         torch.cuda.set_device(0)

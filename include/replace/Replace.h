@@ -75,6 +75,8 @@ class Replace {
   // When false, fatal REPL codes are returned by lastCode() instead of
   // calling exit(code) (the reference always exits, src/logger.cpp:31-36).
   void setExitOnError(bool on);
+  // Poisson transform: 0 FP32 FFT (default), 1 split-BF16 GEMMs on matrix cores.
+  void setPoissonTransform(int mode);
   int lastCode() const;
   const rp_nesterov_stats& lastStats() const;
 
@@ -105,6 +107,7 @@ class Replace {
   int verbose_;
 
   bool exitOnError_;
+  int poissonTransform_;
   int lastCode_;
   rp_nesterov_stats lastStats_;
 

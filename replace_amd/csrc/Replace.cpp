@@ -41,6 +41,7 @@ Replace::Replace()
       incrementalPlaceMode_(false),
       verbose_(0),
       exitOnError_(true),
+      poissonTransform_(0),
       lastCode_(0) {
   memset(&lastStats_, 0, sizeof lastStats_);
 }
@@ -79,6 +80,7 @@ void Replace::setDb(odb::dbDatabase* db) { db_ = db; }
 void Replace::setSta(sta::dbSta* sta) { sta_ = sta; }
 void Replace::setNetlist(const rp_netlist* netlist) { netlist_ = netlist; }
 void Replace::setExitOnError(bool on) { exitOnError_ = on; }
+void Replace::setPoissonTransform(int mode) { poissonTransform_ = mode; }
 int Replace::lastCode() const { return lastCode_; }
 const rp_nesterov_stats& Replace::lastStats() const { return lastStats_; }
 
@@ -102,6 +104,7 @@ rp_params Replace::params() const {
   p.reference_hpwl = referenceHpwl_;
   p.incremental_place_mode = incrementalPlaceMode_ ? 1 : 0;
   p.verbose = verbose_;
+  p.poisson_transform = poissonTransform_;
   return p;
 }
 

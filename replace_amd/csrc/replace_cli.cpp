@@ -7,7 +7,7 @@
 //       [-init_wirelength_coef c] [-min_phi_coef m] [-max_phi_coef m]
 //       [-overflow o] [-initial_place_max_iter n] [-initial_place_max_fanout n]
 //       [-bin_grid_count n] [-verbose_level n] [-skip_initial_place]
-//       [-incremental] [-nesterov_max_iter n] [-out POSITIONS.txt]
+//       [-incremental] [-nesterov_max_iter n] [-poisson_mfma] [-out POSITIONS.txt]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -112,6 +112,7 @@ int main(int argc, char** argv) {
       rp.setInitialPlaceMaxIter(0);
       rp.setIncrementalPlaceMode(true);
     } else if (a == "-out") out = next();
+    else if (a == "-poisson_mfma") rp.setPoissonTransform(1);
     else {
       fprintf(stderr, "unknown flag %s\n", a.c_str());
       return 2;
