@@ -36,24 +36,26 @@ def algorithmic_bytes(nc, npin, nn, nb):
       step_scatter   64 B/cell (read cur SLP, gradient, coordinate, half
                      size, scale, kind; write next coordinate, next SLP,
                      integer centre) + 16 B/bin (two int64 accumulators)
+      pin_pos        28 B/pin (cell id, offset, gathered centre; write the
+                     pin centre)
       binfin_rowdct  48 B/bin (read inst, filler, nonPlace int64; write rho,
                      the row transform, and the zeroed accumulators)
       transpose       8 B/bin; col_spectral 12 B/bin; transpose2 16 B/bin;
       row_inverse    16 B/bin (read Y1, Y2; write Ex, Ey)
-      wa             28 B/pin (cell id, offset, gathered centre, per-pin
-                     gradient) + 4 B/net
+      wa             16 B/pin (pin centre, per-pin gradient) + 4 B/net
       grad           64 B/cell + 12 B/pin (pin index, per-pin gradient)
                      + 8 B/bin (field)
       control        5 doubles per gradient block
     """
     return {
         "step_scatter": 64 * nc + 16 * nb,
+        "pin_pos": 28 * npin,
         "binfin_rowdct": 48 * nb,
         "transpose": 8 * nb,
         "col_spectral": 12 * nb,
         "transpose2": 16 * nb,
         "row_inverse": 16 * nb,
-        "wa": 28 * npin + 4 * nn,
+        "wa": 16 * npin + 4 * nn,
         "grad": 64 * nc + 12 * npin + 8 * nb,
         "control": 8 * 5 * ((nc + 255) // 256),
     }

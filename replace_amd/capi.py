@@ -187,7 +187,7 @@ class Placer:
         return [(r.overflow, r.hpwl, r.step_length, r.density_penalty, r.num_back_track)
                 for r in arr[: cnt.value]]
 
-    KERNELS = ("step_scatter", "binfin_rowdct", "transpose", "col_spectral", "transpose2",
+    KERNELS = ("step_scatter", "pin_pos", "binfin_rowdct", "transpose", "col_spectral", "transpose2",
                "row_inverse", "wa", "grad", "control")
 
     def set_kernel_timing(self, on: bool) -> None:

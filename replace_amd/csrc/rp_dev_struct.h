@@ -29,6 +29,7 @@ struct RpDev {
   const int* nstart;
   const int* pcell;
   const int2* ppos;
+  int2* ppin;        // absolute pin centres, rewritten every evaluation (k_pin_pos)
   int4* nbox;
   float4* nsum;
   unsigned long long* bin_inst;

@@ -30,13 +30,16 @@
 
 #define RP_BLOCK 256
 #define RP_CTRL_BLOCK 1024  // controller workgroup (fixed-order reductions)
+#ifndef RP_SCATTER_PER_THREAD
 #define RP_SCATTER_PER_THREAD 4
+#endif
 #define RP_SCATTER_CELLS (RP_BLOCK * RP_SCATTER_PER_THREAD)  // cells per scatter workgroup
 #define RP_MAX_FFT 2048
 
 // Kernels of one evaluation slot, in launch order (per-kernel timing index).
 enum {
   RP_K_STEP_SCATTER = 0,
+  RP_K_PIN_POS,
   RP_K_BINFIN_ROWDCT,
   RP_K_TRANSPOSE,
   RP_K_COL_SPECTRAL,

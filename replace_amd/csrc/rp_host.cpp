@@ -796,7 +796,10 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   for (int k = 0; k < ng; k++) {
     int i = c->g_inst[c->gperm[k]];
     if (i >= 0)
-      for (int j = c->ipin_start[i]; j < c->ipin_start[i + 1]; j++) cp_pin.push_back(pin_dev[c->ipins[j]]);
+      for (int j = c->ipin_start[i]; j < c->ipin_start[i + 1]; j++) {
+        int q = c->ipins[j];
+        cp_pin.push_back(pin_dev[q]);
+      }
     cp_start[k + 1] = (int)cp_pin.size();
   }
   std::vector<float2> twx, tw2x, twy, tw2y;
@@ -829,6 +832,7 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   HIPCHK(c, dupload(c->d_nstart, nstart, s));
   HIPCHK(c, dupload(c->d_pcell, pcell, s));
   HIPCHK(c, dupload(c->d_ppos, ppos, s));
+  HIPCHK(c, dalloc(c->d_ppin, std::max(1, npn)));
   HIPCHK(c, dalloc(c->d_nbox, nn));
   HIPCHK(c, dalloc(c->d_nsum, (size_t)2 * nn));
   HIPCHK(c, dalloc(c->d_bin_inst, nb));
@@ -896,6 +900,7 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   d.nstart = c->d_nstart.p;
   d.pcell = c->d_pcell.p;
   d.ppos = c->d_ppos.p;
+  d.ppin = c->d_ppin.p;
   d.nbox = c->d_nbox.p;
   d.nsum = c->d_nsum.p;
   d.bin_inst = c->d_bin_inst.p;

@@ -192,9 +192,12 @@ __device__ __forceinline__ float2 dct3_pre(float a_k, float a_nk, float b_k, flo
 // AGG_SLOTS bins in LDS and flushes one int64 atomic per touched bin.
 // Colliding bins fall back to direct global atomics.  Integer sums are
 // order independent, so the result is exact and deterministic.
-#define AGG_SLOTS 4096
+#ifndef RP_AGG_BITS
+#define RP_AGG_BITS 12
+#endif
+#define AGG_SLOTS (1 << RP_AGG_BITS)
 
-__device__ __forceinline__ int agg_slot(int b) { return (int)(((unsigned)b * 2654435761u) >> 20); }
+__device__ __forceinline__ int agg_slot(int b) { return (int)(((unsigned)b * 2654435761u) >> (32 - RP_AGG_BITS)); }
 
 __device__ void scatter_body(const RpDev& d, int blk) {
   __shared__ int s_key[AGG_SLOTS];
@@ -277,6 +280,24 @@ __device__ void scatter_body(const RpDev& d, int blk) {
       if (s_fill[k]) atomicAdd(&d.bin_filler[b], s_fill[k]);
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// 1b. pin centres (GPin::updateDensityLocation, src/nesterovBase.cpp:411-415):
+// integer density centre + offset, one thread per pin in the wirelength
+// kernel's pin order (coalesced stores; the centre gather follows the
+// locality order).  Fixed and port pins keep their absolute position.
+__device__ void pinpos_body(const RpDev& d, int blk) {
+  if (d.st->phase != 0) return;
+  int q = blk * RP_BLOCK + threadIdx.x;
+  if (q >= d.np) return;
+  int c = d.pcell[q];
+  int2 o = d.ppos[q];
+  if (c >= 0) {
+    int2 icc = d.ic[c];
+    o = make_int2(icc.x + o.x, icc.y + o.y);
+  }
+  d.ppin[q] = o;
 }
 
 // ---------------------------------------------------------------------------
@@ -501,15 +522,9 @@ __device__ void rowinv_body(const RpDev& d, int blk, int want_phi) {
 // ---------------------------------------------------------------------------
 // 6. WA exponent sums per net (src/nesterovBase.cpp:1304-1360) + HPWL
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int2 pin_pos(const RpDev& d, int q) {
-  int c = d.pcell[q];
-  int2 o = d.ppos[q];
-  if (c >= 0) {
-    int2 b = d.ic[c];
-    return make_int2(b.x + o.x, b.y + o.y);
-  }
-  return o;
-}
+// Pin centre (GPin::updateDensityLocation: integer density centre + offset),
+// written in pin order by the scatter kernel.
+__device__ __forceinline__ int2 pin_pos(const RpDev& d, int q) { return d.ppin[q]; }
 
 // ---------------------------------------------------------------------------
 // getWireLengthGradientPinWA for one pin and one axis (src/nesterovBase.cpp:1383-1440):
@@ -533,6 +548,22 @@ __device__ __forceinline__ float wl_axis(int p, int lo, int hi, float coef, floa
 }
 
 
+// Same as wl_axis with the exponentials fastExp(e0), fastExp(e1) already
+// computed by the sums pass (identical arguments, so identical values).
+__device__ __forceinline__ float wl_axis_x(int p, int lo, int hi, float coef, float ex0, float ex1, float S0,
+                                           float XS0, float S1, float XS1, float bar) {
+  float gmin = 0.0f, gmax = 0.0f;
+  float e0 = (float)(lo - p) * coef;
+  if (e0 > bar)
+    gmin = (float)(((double)S0 * ((double)ex0 * (1.0 - (double)(coef * (float)p))) + (double)(coef * ex0 * XS0)) /
+                   (double)(S0 * S0));
+  float e1 = (float)(p - hi) * coef;
+  if (e1 > bar)
+    gmax = (float)(((double)S1 * ((double)ex1 * (1.0 + (double)(coef * (float)p))) - (double)(coef * ex1 * XS1)) /
+                   (double)(S1 * S1));
+  return gmin - gmax;
+}
+
 #define WA_CHUNK 8
 
 // Pin bbox update and WA exponent accumulation for one pin, in pin order
@@ -541,8 +572,10 @@ struct WaSums {
   float s0, s1, s2, s3, s4, s5, s6, s7;
 };
 
+// ex (optional) receives the four exponentials (min x, max x, min y, max y;
+// unset where the exponent is at or below the force bar).
 __device__ __forceinline__ void wa_pin(int2 p, int lx, int ly, int ux, int uy, float cx, float cy, float bar,
-                                       WaSums& w) {
+                                       WaSums& w, float4* ex = nullptr) {
   float eMinX = (float)(lx - p.x) * cx;
   float eMaxX = (float)(p.x - ux) * cx;
   float eMinY = (float)(ly - p.y) * cy;
@@ -551,21 +584,25 @@ __device__ __forceinline__ void wa_pin(int2 p, int lx, int ly, int ux, int uy, f
     float v = fast_exp(eMinX);
     w.s0 += v;
     w.s1 += (float)p.x * v;
+    if (ex) ex->x = v;
   }
   if (eMaxX > bar) {
     float v = fast_exp(eMaxX);
     w.s2 += v;
     w.s3 += (float)p.x * v;
+    if (ex) ex->y = v;
   }
   if (eMinY > bar) {
     float v = fast_exp(eMinY);
     w.s4 += v;
     w.s5 += (float)p.y * v;
+    if (ex) ex->z = v;
   }
   if (eMaxY > bar) {
     float v = fast_exp(eMaxY);
     w.s6 += v;
     w.s7 += (float)p.y * v;
+    if (ex) ex->w = v;
   }
 }
 
@@ -614,9 +651,12 @@ __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
         }
     }
     WaSums w = {0, 0, 0, 0, 0, 0, 0, 0};
+    float4 xe[WA_CHUNK];  // the first chunk's exponentials, reused by pass 3
 #pragma unroll
-    for (int k = 0; k < WA_CHUNK; k++)
-      if (k < deg) wa_pin(pos[k], lx, ly, ux, uy, cx, cy, d.bar, w);
+    for (int k = 0; k < WA_CHUNK; k++) {
+      xe[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (k < deg) wa_pin(pos[k], lx, ly, ux, uy, cx, cy, d.bar, w, &xe[k]);
+    }
     for (int b = s + WA_CHUNK; b < t; b += WA_CHUNK) {
       int2 pp[WA_CHUNK];
 #pragma unroll
@@ -629,8 +669,9 @@ __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
 #pragma unroll
     for (int k = 0; k < WA_CHUNK; k++)
       if (k < deg)
-        d.pgrad[s + k] = make_float2(wl_axis(pos[k].x, lx, ux, cx, w.s0, w.s1, w.s2, w.s3, d.bar),
-                                     wl_axis(pos[k].y, ly, uy, cy, w.s4, w.s5, w.s6, w.s7, d.bar));
+        d.pgrad[s + k] =
+            make_float2(wl_axis_x(pos[k].x, lx, ux, cx, xe[k].x, xe[k].y, w.s0, w.s1, w.s2, w.s3, d.bar),
+                        wl_axis_x(pos[k].y, ly, uy, cy, xe[k].z, xe[k].w, w.s4, w.s5, w.s6, w.s7, d.bar));
     for (int b = s + WA_CHUNK; b < t; b += WA_CHUNK) {
       int2 pp[WA_CHUNK];
 #pragma unroll
@@ -987,6 +1028,7 @@ __device__ void control_main_body(const RpDev& d) {
 __global__ void __launch_bounds__(RP_BLOCK) k_step_scatter(RpDev d) {
   scatter_body(d, xcd_block(blockIdx.x, gridDim.x));
 }
+__global__ void __launch_bounds__(RP_BLOCK) k_pin_pos(RpDev d) { pinpos_body(d, xcd_block(blockIdx.x, gridDim.x)); }
 __global__ void __launch_bounds__(RP_BLOCK) k_binfin_rowdct(RpDev d, int from_rho) {
   binfin_body(d, blockIdx.x, from_rho);
 }
@@ -1013,6 +1055,10 @@ __global__ void __launch_bounds__(RP_CTRL_BLOCK) k_control_main(RpDev d) { contr
 __global__ void __launch_bounds__(RP_BLOCK) kb_step_scatter(const RpDev* devs, const int2* tab) {
   int2 e = tab[blockIdx.x];
   if (e.x >= 0) scatter_body(devs[e.x], e.y);
+}
+__global__ void __launch_bounds__(RP_BLOCK) kb_pin_pos(const RpDev* devs, const int2* tab) {
+  int2 e = tab[blockIdx.x];
+  if (e.x >= 0) pinpos_body(devs[e.x], e.y);
 }
 __global__ void __launch_bounds__(RP_BLOCK) kb_binfin_rowdct(const RpDev* devs, const int2* tab) {
   int2 e = tab[blockIdx.x];
@@ -1080,6 +1126,8 @@ hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_
   RP_MARK();
   hipLaunchKernelGGL(k_step_scatter, dim3(nb_s), dim3(RP_BLOCK), 0, s, d);
   RP_MARK();
+  hipLaunchKernelGGL(k_pin_pos, dim3(std::max(1, (d.np + RP_BLOCK - 1) / RP_BLOCK)), dim3(RP_BLOCK), 0, s, d);
+  RP_MARK();
   size_t shx = sizeof(float2) * 2 * d.nbx;
   hipLaunchKernelGGL(k_binfin_rowdct, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shx, s, d, 0);
   RP_MARK();
@@ -1124,6 +1172,7 @@ hipError_t rp_launch_ctrl_init(const RpDev& d, hipStream_t s, int stage) {
 hipError_t rp_launch_scatter(const RpDev& d, hipStream_t s) {
   hipLaunchKernelGGL(k_step_scatter, dim3((d.nc + RP_SCATTER_CELLS - 1) / RP_SCATTER_CELLS), dim3(RP_BLOCK), 0, s,
                      d);
+  hipLaunchKernelGGL(k_pin_pos, dim3(std::max(1, (d.np + RP_BLOCK - 1) / RP_BLOCK)), dim3(RP_BLOCK), 0, s, d);
   return hipGetLastError();
 }
 
@@ -1164,6 +1213,8 @@ hipError_t rp_launch_eval_batch(const RpBatchLaunch& b, hipStream_t s) {
   if (n[RP_K_STEP_SCATTER])
     hipLaunchKernelGGL(kb_step_scatter, dim3(n[RP_K_STEP_SCATTER]), dim3(RP_BLOCK), 0, s, b.devs,
                        b.tab[RP_K_STEP_SCATTER]);
+  if (n[RP_K_PIN_POS])
+    hipLaunchKernelGGL(kb_pin_pos, dim3(n[RP_K_PIN_POS]), dim3(RP_BLOCK), 0, s, b.devs, b.tab[RP_K_PIN_POS]);
   if (n[RP_K_BINFIN_ROWDCT])
     hipLaunchKernelGGL(kb_binfin_rowdct, dim3(n[RP_K_BINFIN_ROWDCT]), dim3(RP_BLOCK), b.lds_binfin, s, b.devs,
                        b.tab[RP_K_BINFIN_ROWDCT]);
