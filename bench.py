@@ -362,9 +362,10 @@ def main():
     avg_ms = kt[dom][0] / max(1, kt[dom][1])
     achieved = abytes[dom] / (avg_ms * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"{dom}_{args.workload}_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        ent = json.load(open(tpath))["kernels"].get("k_" + dom)
+        traffic = ent["hbm_bytes_per_launch"] if ent else None
     code, iters, evals, ovf, hpwl, dev_s, fin = runs[-1]
     result = {
         "metric": METRIC,

@@ -7,6 +7,10 @@
       markdown summary of a rocprofv3 --kernel-trace --stats run
   python -m replace_amd.profiling pmc DIR COUNTER [--regex R]
       mean of one counter per dispatch and kernel from a --pmc run
+  python -m replace_amd.profiling traffic FETCH_DIR WRITE_DIR COPYF_DIR COPYW_DIR --copy-mb 512 --out F
+      HBM bytes per launch and kernel: FETCH_SIZE and WRITE_SIZE (KiB per
+      dispatch) each scaled by what the same counter reads for a device copy
+      of known size, written as JSON {kernel: bytes}
 """
 from __future__ import annotations
 
@@ -97,9 +101,38 @@ def pmc_means(d: str, counter: str, regex: str = ""):
     return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
 
 
+def calibrated_traffic(fetch_dir, write_dir, copyf_dir, copyw_dir, copy_mb):
+    """{kernel: (bytes per launch, fetch scale, write scale)} from two --pmc
+    runs of the workload and two of the calibration copy."""
+    nbytes = copy_mb * (1 << 20)
+
+    def copy_kib(d, counter):
+        m = pmc_means(d, counter, "copyBuffer|elementwise|copy")
+        best = max(m.values(), key=lambda v: v[0])
+        return best[0]
+
+    fs = nbytes / (copy_kib(copyf_dir, "FETCH_SIZE") * 1024.0)
+    ws = nbytes / (copy_kib(copyw_dir, "WRITE_SIZE") * 1024.0)
+    f = pmc_means(fetch_dir, "FETCH_SIZE")
+    w = pmc_means(write_dir, "WRITE_SIZE")
+    out = {}
+    for k in sorted(set(f) & set(w)):
+        out[k] = {"hbm_bytes_per_launch": int(f[k][0] * 1024 * fs + w[k][0] * 1024 * ws),
+                  "fetch_kib": round(f[k][0], 1), "write_kib": round(w[k][0], 1)}
+    return out, fs, ws
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_dir")
+    t.add_argument("write_dir")
+    t.add_argument("copyf_dir")
+    t.add_argument("copyw_dir")
+    t.add_argument("--copy-mb", type=int, default=512)
+    t.add_argument("--workload", default="")
+    t.add_argument("--out")
     c = sub.add_parser("copy")
     c.add_argument("--mb", type=int, default=512)
     c.add_argument("--reps", type=int, default=20)
@@ -111,7 +144,17 @@ def main(argv=None):
     p.add_argument("counter")
     p.add_argument("--regex", default="")
     a = ap.parse_args(argv)
-    if a.cmd == "copy":
+    if a.cmd == "traffic":
+        import json
+        k, fs, ws = calibrated_traffic(a.fetch_dir, a.write_dir, a.copyf_dir, a.copyw_dir, a.copy_mb)
+        doc = {"workload": a.workload, "fetch_scale": round(fs, 4), "write_scale": round(ws, 4),
+               "source": "rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE, KiB per dispatch, scaled by "
+                         f"the same counters on a {a.copy_mb} MiB device copy", "kernels": k}
+        txt = json.dumps(doc, indent=1)
+        if a.out:
+            open(a.out, "w").write(txt + "\n")
+        print(txt)
+    elif a.cmd == "copy":
         copy_probe(a.mb, a.reps)
     elif a.cmd == "stats":
         txt = stats_summary(a.dir)
