@@ -191,12 +191,14 @@ const char* rp_last_error(rp_ctx* ctx);
 /* Per-kernel timing of the evaluation slots (no reference counterpart: the
  * reference has no timers).  When enabled, rp_nesterov_place brackets every
  * kernel of every executed slot with HIP events on the context's stream and
- * accumulates the elapsed time.  rp_get_kernel_timing returns, for the 10
- * kernels of a slot in launch order (step+scatter, pin centres, bin
- * finalize + row DCT, transpose, column spectral pass, transpose, row
- * inverse, WA, gradient, controller), the summed milliseconds and the number
- * of executed slots.  With the matrix-core Poisson transform its whole solve
- * is reported in the first transpose slot and the next three stay empty. */
+ * accumulates the elapsed time.  rp_get_kernel_timing fills 12 entries:
+ * for the 11 kernels of a slot in launch order (step, density scatter, pin
+ * centres, bin finalize + row DCT, transpose, column spectral pass,
+ * transpose, row inverse, WA, gradient, controller) the summed milliseconds,
+ * then the summed milliseconds of the spatial-order refresh that runs once
+ * per batch of queued slots; *launches is the number of executed slots.
+ * With the matrix-core Poisson transform its whole solve is reported in the
+ * first transpose slot and the next three stay empty. */
 int rp_set_kernel_timing(rp_ctx* ctx, int enable);
 int rp_get_kernel_timing(rp_ctx* ctx, double* total_ms_per_kernel, int64_t* launches);
 

@@ -248,7 +248,8 @@ static int rebuild(rp_server* s) {
     const RpDev& d = s->jobs[s->running[i]]->ctx->dev;
     devs[i] = d;
     int tx = (d.nbx + 31) / 32, ty = (d.nby + 31) / 32;
-    cnt[RP_K_STEP_SCATTER].push_back((d.nc + RP_SCATTER_CELLS - 1) / RP_SCATTER_CELLS);
+    cnt[RP_K_STEP].push_back(0);  // the batched scatter kernel steps its own cells
+    cnt[RP_K_SCATTER].push_back((d.nc + RP_SCATTER_CELLS - 1) / RP_SCATTER_CELLS);
     cnt[RP_K_PIN_POS].push_back((d.np + RP_BLOCK - 1) / RP_BLOCK);
     cnt[RP_K_BINFIN_ROWDCT].push_back(d.nby / 2);
     cnt[RP_K_TRANSPOSE].push_back(tx * ty);

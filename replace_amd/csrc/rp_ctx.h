@@ -83,6 +83,8 @@ struct rp_ctx {
   DevBuf<uint8_t> d_kind;
   DevBuf<float2> d_slp, d_sumg, d_co;
   DevBuf<int2> d_ic;
+  DevBuf<int> d_sord, d_srank, d_scnt, d_soff;  // spatial cell order of the scatter and its counting sort
+  RpSortBufs sortb{};
   DevBuf<int> d_cp_start, d_cp_pin, d_nstart, d_pcell;
   DevBuf<int2> d_ppos, d_ppin;
   DevBuf<float2> d_pgrad;
@@ -110,7 +112,7 @@ struct rp_ctx {
   const RpMfmaDev* mf() const { return mfma.ready ? &mfma_view : nullptr; }
   // ---- per-kernel timing (optional) ----
   bool timing = false;
-  double ktime_ms[RP_NUM_EVAL_KERNELS] = {0};
+  double ktime_ms[RP_NUM_EVAL_KERNELS + 1] = {0};  // slot kernels, then the per-batch sort refresh
   long long kcount = 0;
   // ---- results ----
   std::vector<RpIterRec> records;
@@ -194,7 +196,8 @@ int rp_ip_host_assemble(rp_ctx* c, RpIpAsm& a);
 int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve);
 
 // launch wrappers (rp_kernels.hip)
-hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_t* ev, const RpMfmaDev* mf);
+hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_t* ev, const RpMfmaDev* mf,
+                          const RpSortBufs* sb = nullptr);
 hipError_t rp_launch_ctrl_init(const RpDev& d, hipStream_t s, int stage);
 hipError_t rp_launch_scatter(const RpDev& d, hipStream_t s);
 hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int from_rho, const RpMfmaDev* mf);

@@ -23,6 +23,7 @@ struct RpDev {
   float2* sumg;
   float2* co;
   int2* ic;
+  const int* sord;     // cells in spatial order (density scatter of the single-instance path)
   const int* cp_start;
   const int* cp_pin;   // cell's pins (reference ITerm order) -> device pin index
   float2* pgrad;       // per-pin WA gradient gMin - gMax, device pin order
@@ -80,3 +81,20 @@ struct RpMfmaDev {
 };
 
 hipError_t rp_launch_poisson_mfma(const RpDev& d, const RpMfmaDev& m, hipStream_t s, int want_phi);
+
+// Work arrays of the spatial cell order (single-instance scatter): a
+// counting sort of the cells by the tile holding their integer centre.
+#define RP_SORT_TILES 16384  // 128 x 128 tiles at most
+struct RpSortBufs {
+  int* cnt;        // RP_SORT_TILES tile counts (zero between refreshes)
+  int* off;        // RP_SORT_TILES exclusive offsets
+  int* rank;       // Nc rank of each cell within its tile
+  int* sord;       // Nc cells in tile order (RpDev::sord)
+  int tile_shift;  // log2 of the tile side in bins
+};
+
+// Tile side (log2, in bins) so that an nbx x nby grid has <= 128 tiles per side.
+int rp_sort_tile_shift(int nbx, int nby);
+// Finishes the counting sort started by the last counting step launch:
+// offsets, then d.sord.
+hipError_t rp_launch_sort_refresh(const RpDev& d, const RpSortBufs& b, hipStream_t s);

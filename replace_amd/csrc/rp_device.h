@@ -33,12 +33,17 @@
 #ifndef RP_SCATTER_PER_THREAD
 #define RP_SCATTER_PER_THREAD 4
 #endif
-#define RP_SCATTER_CELLS (RP_BLOCK * RP_SCATTER_PER_THREAD)  // cells per scatter workgroup
+#define RP_SCATTER_CELLS (RP_BLOCK * RP_SCATTER_PER_THREAD)  // cells per fused step+scatter workgroup
+#ifndef RP_SS_PER_THREAD
+#define RP_SS_PER_THREAD 4
+#endif
+#define RP_SS_CELLS (RP_BLOCK * RP_SS_PER_THREAD)  // cells per spatial-order scatter workgroup
 #define RP_MAX_FFT 2048
 
 // Kernels of one evaluation slot, in launch order (per-kernel timing index).
 enum {
-  RP_K_STEP_SCATTER = 0,
+  RP_K_STEP = 0,  // single instance: step + integer centres (batched: fused into RP_K_SCATTER)
+  RP_K_SCATTER,   // density scatter (single instance: spatial cell order)
   RP_K_PIN_POS,
   RP_K_BINFIN_ROWDCT,
   RP_K_TRANSPOSE,

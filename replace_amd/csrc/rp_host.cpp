@@ -825,6 +825,21 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   HIPCHK(c, dalloc(c->d_sumg, (size_t)3 * ng));
   HIPCHK(c, dalloc(c->d_co, (size_t)2 * ng));
   HIPCHK(c, dalloc(c->d_ic, ng));
+  {
+    std::vector<int> iota(ng);
+    for (int k = 0; k < ng; k++) iota[k] = k;
+    HIPCHK(c, dupload(c->d_sord, iota, s));  // any permutation is valid until the first refresh
+    HIPCHK(c, dalloc(c->d_srank, std::max(1, ng)));
+    HIPCHK(c, dalloc(c->d_scnt, RP_SORT_TILES));
+    HIPCHK(c, dalloc(c->d_soff, RP_SORT_TILES));
+    HIPCHK(c, hipMemsetAsync(c->d_scnt.p, 0, sizeof(int) * RP_SORT_TILES, s));
+    RpSortBufs& sb = c->sortb;
+    sb.cnt = c->d_scnt.p;
+    sb.off = c->d_soff.p;
+    sb.rank = c->d_srank.p;
+    sb.sord = c->d_sord.p;
+    sb.tile_shift = rp_sort_tile_shift(c->nbx, c->nby);
+  }
   HIPCHK(c, hipMemsetAsync(c->d_sumg.p, 0, sizeof(float2) * 3 * ng, s));
   HIPCHK(c, dupload(c->d_cp_start, cp_start, s));
   HIPCHK(c, dupload(c->d_cp_pin, cp_pin, s));
@@ -894,6 +909,7 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   d.sumg = c->d_sumg.p;
   d.co = c->d_co.p;
   d.ic = c->d_ic.p;
+  d.sord = c->d_sord.p;
   d.cp_start = c->d_cp_start.p;
   d.cp_pin = c->d_cp_pin.p;
   d.pgrad = c->d_pgrad.p;
@@ -1260,15 +1276,24 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
   int max_batches = (c->prm.nesterov_place_max_iter * std::max(1, c->prm.max_back_track)) / K + 4;
   std::vector<hipEvent_t> kev;
   if (c->timing) {
-    kev.resize((size_t)K * (RP_NUM_EVAL_KERNELS + 1));
+    kev.resize((size_t)K * (RP_NUM_EVAL_KERNELS + 1) + 2);  // + the sort refresh bracket
     for (auto& e : kev) HIPCHK(c, hipEventCreate(&e));
   }
+  hipEvent_t* sev = c->timing ? &kev[(size_t)K * (RP_NUM_EVAL_KERNELS + 1)] : nullptr;
   HIPCHK(c, hipMemcpyAsync(hs, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
   int evals_before = hs->evals;
+  // tile counts left by a run that stopped after its last counting slot
+  HIPCHK(c, hipMemsetAsync(c->sortb.cnt, 0, sizeof(int) * RP_SORT_TILES, s));
   for (int b = 0; b < max_batches && hs->phase == 0; b++) {
+    // Spatial order of the scatter from the centres counted by the last slot
+    // of the previous batch (it ran: the state polled after it still runs).
+    if (sev) HIPCHK(c, hipEventRecord(sev[0], s));
+    if (b > 0) HIPCHK(c, rp_launch_sort_refresh(d, c->sortb, s));
+    if (sev) HIPCHK(c, hipEventRecord(sev[1], s));
     for (int k = 0; k < K; k++)
-      HIPCHK(c, rp_launch_eval(d, s, 0, c->timing ? &kev[(size_t)k * (RP_NUM_EVAL_KERNELS + 1)] : nullptr, c->mf()));
+      HIPCHK(c, rp_launch_eval(d, s, 0, c->timing ? &kev[(size_t)k * (RP_NUM_EVAL_KERNELS + 1)] : nullptr, c->mf(),
+                               k == K - 1 ? &c->sortb : nullptr));
     HIPCHK(c, hipMemcpyAsync(hs, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipEventRecord(polled, s));
     HIPCHK(c, hipEventSynchronize(polled));
@@ -1284,6 +1309,9 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
         }
       }
       c->kcount += std::max(0, ran);
+      float sms = 0;
+      HIPCHK(c, hipEventElapsedTime(&sms, sev[0], sev[1]));
+      c->ktime_ms[RP_NUM_EVAL_KERNELS] += sms;
     }
     evals_before = hs->evals;
   }
@@ -1349,7 +1377,7 @@ extern "C" int rp_set_kernel_timing(rp_ctx* c, int enable) {
 
 extern "C" int rp_get_kernel_timing(rp_ctx* c, double* total_ms, int64_t* launches) {
   if (!c || !total_ms || !launches) return RP_ERR_ARG;
-  for (int q = 0; q < RP_NUM_EVAL_KERNELS; q++) total_ms[q] = c->ktime_ms[q];
+  for (int q = 0; q <= RP_NUM_EVAL_KERNELS; q++) total_ms[q] = c->ktime_ms[q];
   *launches = c->kcount;
   return RP_OK;
 }

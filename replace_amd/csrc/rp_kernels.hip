@@ -3,7 +3,9 @@
 //
 // One evaluation (reference src/nesterovPlace.cpp:338-377) is the launch
 // sequence
-//   k_step_scatter      step + clamp + int centre + density scatter (A14, A5, A6)
+//   k_step              step + clamp + int centre (A14, A5)
+//   k_scatter           density scatter in spatial cell order (A6)
+//   k_pin_pos           pin centres (A5)
 //   k_binfin_rowdct     bin density, overflow partials, DCT-II along x (A6 tail, A8)
 //   k_transpose         (Ny x Nx) -> (Nx x Ny)
 //   k_col_spectral      DCT-II along y, spectral divide, DCT-III / DST-III along y (A8)
@@ -187,98 +189,230 @@ __device__ __forceinline__ float2 dct3_pre(float a_k, float a_nk, float b_k, flo
 // ---------------------------------------------------------------------------
 // 1. step + clamp + integer centre + density scatter
 // ---------------------------------------------------------------------------
-// LDS aggregation: each workgroup scatters RP_SCATTER_CELLS consecutive
-// cells (locality order, so mostly neighbours) into a hashed table of
-// AGG_SLOTS bins in LDS and flushes one int64 atomic per touched bin.
-// Colliding bins fall back to direct global atomics.  Integer sums are
-// order independent, so the result is exact and deterministic.
+// LDS aggregation: each workgroup scatters a run of cells into a hashed
+// table of bins in LDS (linear probing, RP_AGG_PROBES slots) and flushes one
+// int64 atomic per touched bin; a bin that finds no free slot goes straight
+// to global memory.  Integer sums are order independent, so the result is
+// exact and deterministic whatever order the cells come in.
 #ifndef RP_AGG_BITS
-#define RP_AGG_BITS 12
+#define RP_AGG_BITS 12  // fused step + scatter (batched instances)
 #endif
-#define AGG_SLOTS (1 << RP_AGG_BITS)
+#ifndef RP_SS_BITS
+#define RP_SS_BITS 11  // scatter in spatial order (single instance)
+#endif
+#ifndef RP_AGG_PROBES
+#define RP_AGG_PROBES 4
+#endif
 
-__device__ __forceinline__ int agg_slot(int b) { return (int)(((unsigned)b * 2654435761u) >> (32 - RP_AGG_BITS)); }
+template <int BITS>
+struct AggTable {
+  static constexpr int SLOTS = 1 << BITS;
+  int key[SLOTS];
+  unsigned long long inst[SLOTS];
+  unsigned long long fill[SLOTS];
+};
 
+template <int BITS>
+__device__ __forceinline__ void agg_clear(AggTable<BITS>& t) {
+  for (int k = threadIdx.x; k < AggTable<BITS>::SLOTS; k += blockDim.x) {
+    t.key[k] = -1;
+    t.inst[k] = 0;
+    t.fill[k] = 0;
+  }
+}
+
+template <int BITS>
+__device__ __forceinline__ void agg_add(const RpDev& d, AggTable<BITS>& t, int b, unsigned long long a, bool filler) {
+  int slot = (int)(((unsigned)b * 2654435761u) >> (32 - BITS));
+#pragma unroll
+  for (int q = 0; q < RP_AGG_PROBES; q++) {
+    int sl = (slot + q) & (AggTable<BITS>::SLOTS - 1);
+    int old = atomicCAS(&t.key[sl], -1, b);
+    if (old == -1 || old == b) {
+      atomicAdd(filler ? &t.fill[sl] : &t.inst[sl], a);
+      return;
+    }
+  }
+  atomicAdd(filler ? &d.bin_filler[b] : &d.bin_inst[b], a);
+}
+
+template <int BITS>
+__device__ __forceinline__ void agg_flush(const RpDev& d, const AggTable<BITS>& t) {
+  for (int k = threadIdx.x; k < AggTable<BITS>::SLOTS; k += blockDim.x) {
+    int b = t.key[k];
+    if (b >= 0) {
+      if (t.inst[k]) atomicAdd(&d.bin_inst[b], t.inst[k]);
+      if (t.fill[k]) atomicAdd(&d.bin_filler[b], t.fill[k]);
+    }
+  }
+}
+
+// Nesterov step of one cell (src/nesterovPlace.cpp:339-361, no FMA: built
+// with -ffp-contract=off) with the layout clamp of both results
+// (getDensityCoordiLayoutInsideX/Y).  Writes the next coordinate and the
+// next SLP; returns the SLP, the iterate that is evaluated.
+__device__ __forceinline__ float2 step_cell(const RpDev& d, const RpState* st, int c, int2 h) {
+  int cur = st->ring.cur;
+  float2 x = d.slp[(size_t)cur * d.nc + c];
+  float2 g = d.sumg[(size_t)cur * d.nc + c];
+  float2 o = d.co[(size_t)st->ring.co_cur * d.nc + c];
+  float sl = st->step_length, cf = st->coeff;
+  float nx = x.x + sl * g.x;
+  float ny = x.y + sl * g.y;
+  float sx = nx + cf * (nx - o.x);
+  float sy = ny + cf * (ny - o.y);
+  float2 nco = make_float2(clamp_in(nx, h.x, d.core_lx, d.core_ux), clamp_in(ny, h.y, d.core_ly, d.core_uy));
+  float2 p = make_float2(clamp_in(sx, h.x, d.core_lx, d.core_ux), clamp_in(sy, h.y, d.core_ly, d.core_uy));
+  d.co[(size_t)st->ring.co_next * d.nc + c] = nco;
+  d.slp[(size_t)st->dst * d.nc + c] = p;
+  return p;
+}
+
+// The iterate one evaluation scatters: the stepped SLP (mode 0) or slot dst
+// as it is (mode 1).  setDensityCenterLocation(int, int) truncates the float
+// centre toward zero (src/nesterovBase.cpp:1203); the integer centre is kept
+// for the pin, wirelength and field kernels.
+__device__ __forceinline__ int2 eval_centre(const RpDev& d, const RpState* st, int c, int2 h) {
+  float2 p = st->mode == 0 ? step_cell(d, st, c, h) : d.slp[(size_t)st->dst * d.nc + c];
+  int2 icc = make_int2((int)p.x, (int)p.y);
+  d.ic[c] = icc;
+  return icc;
+}
+
+// getOverlapDensityArea * densityScale (* td for macros), truncated to int64,
+// for every bin of the cell's density box (updateBinsGCellDensityArea,
+// src/nesterovBase.cpp:745-799).
+template <int BITS>
+__device__ __forceinline__ void box_scatter(const RpDev& d, AggTable<BITS>& t, int2 icc, int2 h, float sc,
+                                            uint8_t kd) {
+  int dlx = icc.x - h.x, dux = icc.x + h.x, dly = icc.y - h.y, duy = icc.y + h.y;
+  int x0, x1, y0, y1;
+  idx_range(dlx, dux, d.core_lx, d.bsx, d.nbx, x0, x1);
+  idx_range(dly, duy, d.core_ly, d.bsy, d.nby, y0, y1);
+  for (int i = x0; i < x1; i++) {
+    int blx, bux;
+    bin_x(d, i, blx, bux);
+    int w = min(bux, dux) - max(blx, dlx);
+    if (w <= 0) continue;
+    for (int j = y0; j < y1; j++) {
+      int bly, buy;
+      bin_y(d, j, bly, buy);
+      int hh = min(buy, duy) - max(bly, dly);
+      if (hh <= 0) continue;
+      float v = (float)w * (float)hh;
+      v = v * sc;
+      if (kd == 1) v = v * d.td;
+      unsigned long long a = (unsigned long long)(long long)v;
+      if (a == 0) continue;
+      agg_add(d, t, j * d.nbx + i, a, kd == 2);
+    }
+  }
+}
+
+// Fused step + scatter over RP_SCATTER_CELLS consecutive cells (batched
+// instances: small grids, where the locality order keeps a workgroup's bins
+// together).
 __device__ void scatter_body(const RpDev& d, int blk) {
-  __shared__ int s_key[AGG_SLOTS];
-  __shared__ unsigned long long s_inst[AGG_SLOTS];
-  __shared__ unsigned long long s_fill[AGG_SLOTS];
+  __shared__ AggTable<RP_AGG_BITS> t;
   const RpState* st = d.st;
   if (st->phase != 0) return;
-  for (int k = threadIdx.x; k < AGG_SLOTS; k += blockDim.x) {
-    s_key[k] = -1;
-    s_inst[k] = 0;
-    s_fill[k] = 0;
-  }
+  agg_clear(t);
   __syncthreads();
-  const int dst = st->dst;
-  const int mode = st->mode;
   for (int r = 0; r < RP_SCATTER_PER_THREAD; r++) {
     int c = blk * RP_SCATTER_CELLS + r * RP_BLOCK + threadIdx.x;
     if (c >= d.nc) break;
     int2 h = d.dhalf[c];
-    float2 p;
-    if (mode == 0) {
-      int cur = st->ring.cur;
-      float2 x = d.slp[(size_t)cur * d.nc + c];
-      float2 g = d.sumg[(size_t)cur * d.nc + c];
-      float2 o = d.co[(size_t)st->ring.co_cur * d.nc + c];
-      float sl = st->step_length, cf = st->coeff;
-      // src/nesterovPlace.cpp:339-361 (no FMA: built with -ffp-contract=off)
-      float nx = x.x + sl * g.x;
-      float ny = x.y + sl * g.y;
-      float sx = nx + cf * (nx - o.x);
-      float sy = ny + cf * (ny - o.y);
-      float2 nco = make_float2(clamp_in(nx, h.x, d.core_lx, d.core_ux), clamp_in(ny, h.y, d.core_ly, d.core_uy));
-      p = make_float2(clamp_in(sx, h.x, d.core_lx, d.core_ux), clamp_in(sy, h.y, d.core_ly, d.core_uy));
-      d.co[(size_t)st->ring.co_next * d.nc + c] = nco;
-      d.slp[(size_t)dst * d.nc + c] = p;
-    } else {
-      p = d.slp[(size_t)dst * d.nc + c];
-    }
-    // setDensityCenterLocation(int, int): float -> int truncation (src/nesterovBase.cpp:1203)
-    int2 icc = make_int2((int)p.x, (int)p.y);
-    d.ic[c] = icc;
-    int dlx = icc.x - h.x, dux = icc.x + h.x, dly = icc.y - h.y, duy = icc.y + h.y;
-    int x0, x1, y0, y1;
-    idx_range(dlx, dux, d.core_lx, d.bsx, d.nbx, x0, x1);
-    idx_range(dly, duy, d.core_ly, d.bsy, d.nby, y0, y1);
-    float sc = d.scale[c];
-    uint8_t kd = d.kind[c];
-    for (int i = x0; i < x1; i++) {
-      int blx, bux;
-      bin_x(d, i, blx, bux);
-      int w = min(bux, dux) - max(blx, dlx);
-      if (w <= 0) continue;
-      for (int j = y0; j < y1; j++) {
-        int bly, buy;
-        bin_y(d, j, bly, buy);
-        int hh = min(buy, duy) - max(bly, dly);
-        if (hh <= 0) continue;
-        // getOverlapDensityArea * densityScale (* td for macros), truncated to int64
-        float v = (float)w * (float)hh;
-        v = v * sc;
-        if (kd == 1) v = v * d.td;
-        unsigned long long a = (unsigned long long)(long long)v;
-        if (a == 0) continue;
-        int b = j * d.nbx + i;
-        int slot = agg_slot(b);
-        int old = atomicCAS(&s_key[slot], -1, b);
-        if (old == -1 || old == b) {
-          atomicAdd(kd == 2 ? &s_fill[slot] : &s_inst[slot], a);
-        } else {
-          atomicAdd(kd == 2 ? &d.bin_filler[b] : &d.bin_inst[b], a);
+    int2 icc = eval_centre(d, st, c, h);
+    box_scatter(d, t, icc, h, d.scale[c], d.kind[c]);
+  }
+  __syncthreads();
+  agg_flush(d, t);
+}
+
+// Tile key of an integer centre: Morton code of the (2^tile_shift)^2-bin
+// tile holding it (at most 128 tiles per side, so keys < RP_SORT_TILES).
+__device__ __forceinline__ unsigned spread7(unsigned v) {
+  v &= 0x7fu;
+  v = (v | (v << 4)) & 0x0f0fu;
+  v = (v | (v << 2)) & 0x3333u;
+  v = (v | (v << 1)) & 0x5555u;
+  return v;
+}
+__device__ __forceinline__ int tile_key(const RpDev& d, int2 p, int tile_shift) {
+  int bx = min(max((p.x - d.core_lx) / d.bsx, 0), d.nbx - 1) >> tile_shift;
+  int by = min(max((p.y - d.core_ly) / d.bsy, 0), d.nby - 1) >> tile_shift;
+  return (int)(spread7((unsigned)bx) | (spread7((unsigned)by) << 1));
+}
+
+// Step only, one thread per cell in storage order (coalesced): next
+// coordinate, next SLP and the integer centre.
+__device__ void step_body(const RpDev& d, int blk) {
+  const RpState* st = d.st;
+  if (st->phase != 0) return;
+  int c = blk * RP_BLOCK + threadIdx.x;
+  if (c >= d.nc) return;
+  (void)eval_centre(d, st, c, d.dhalf[c]);
+}
+
+// Scatter only, over RP_SS_CELLS cells of the spatial order d.sord
+// (cells sorted by the Morton code of their bin tile), so a workgroup's
+// cells share bins and nearly every update lands in the LDS table.  Any
+// permutation gives the same integer sums; the order only decides how many
+// updates go to global memory.
+//
+// With `count` set the kernel also starts the next refresh of the order (a
+// counting sort by tile): each cell is counted into its tile, first in a
+// small LDS table (the workgroup's cells cover few tiles), then with one
+// global atomic per tile and workgroup, and its rank within the tile is
+// kept in sb.rank.  rp_launch_sort_refresh turns counts and ranks into the
+// new order.
+#define RP_TILE_SLOTS 256
+__device__ void sorted_scatter_body(const RpDev& d, int blk, const RpSortBufs& sb, int count) {
+  __shared__ AggTable<RP_SS_BITS> t;
+  __shared__ int t_key[RP_TILE_SLOTS], t_cnt[RP_TILE_SLOTS], t_base[RP_TILE_SLOTS];
+  const RpState* st = d.st;
+  if (st->phase != 0) return;
+  agg_clear(t);
+  for (int k = threadIdx.x; k < RP_TILE_SLOTS; k += blockDim.x) {
+    t_key[k] = -1;
+    t_cnt[k] = 0;
+  }
+  __syncthreads();
+  int cell[RP_SS_PER_THREAD], slot[RP_SS_PER_THREAD], lrank[RP_SS_PER_THREAD];
+#pragma unroll
+  for (int r = 0; r < RP_SS_PER_THREAD; r++) {
+    cell[r] = -1;
+    int k = blk * RP_SS_CELLS + r * RP_BLOCK + threadIdx.x;
+    if (k >= d.nc) continue;
+    int c = d.sord[k];
+    int2 icc = d.ic[c];
+    box_scatter(d, t, icc, d.dhalf[c], d.scale[c], d.kind[c]);
+    if (count) {
+      cell[r] = c;
+      slot[r] = -1;
+      int key = tile_key(d, icc, sb.tile_shift);
+      int h = (int)(((unsigned)key * 2654435761u) >> 24);  // 8-bit hash
+      for (int q = 0; q < 8; q++) {
+        int sl = (h + q) & (RP_TILE_SLOTS - 1);
+        int old = atomicCAS(&t_key[sl], -1, key);
+        if (old == -1 || old == key) {
+          slot[r] = sl;
+          lrank[r] = atomicAdd(&t_cnt[sl], 1);
+          break;
         }
       }
+      if (slot[r] < 0) sb.rank[c] = atomicAdd(&sb.cnt[key], 1);  // table full: count directly
     }
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < AGG_SLOTS; k += blockDim.x) {
-    int b = s_key[k];
-    if (b >= 0) {
-      if (s_inst[k]) atomicAdd(&d.bin_inst[b], s_inst[k]);
-      if (s_fill[k]) atomicAdd(&d.bin_filler[b], s_fill[k]);
-    }
+  agg_flush(d, t);
+  if (count) {
+    for (int k = threadIdx.x; k < RP_TILE_SLOTS; k += blockDim.x)
+      if (t_key[k] >= 0) t_base[k] = atomicAdd(&sb.cnt[t_key[k]], t_cnt[k]);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RP_SS_PER_THREAD; r++)
+      if (cell[r] >= 0 && slot[r] >= 0) sb.rank[cell[r]] = t_base[slot[r]] + lrank[r];
   }
 }
 
@@ -1025,8 +1159,9 @@ __device__ void control_main_body(const RpDev& d) {
 // blocks, block size and reduction order of its single-instance launch, so
 // its result is bit-identical alone or in a batch.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(RP_BLOCK) k_step_scatter(RpDev d) {
-  scatter_body(d, xcd_block(blockIdx.x, gridDim.x));
+__global__ void __launch_bounds__(RP_BLOCK) k_step(RpDev d) { step_body(d, xcd_block(blockIdx.x, gridDim.x)); }
+__global__ void __launch_bounds__(RP_BLOCK) k_scatter(RpDev d, RpSortBufs sb, int count) {
+  sorted_scatter_body(d, xcd_block(blockIdx.x, gridDim.x), sb, count);
 }
 __global__ void __launch_bounds__(RP_BLOCK) k_pin_pos(RpDev d) { pinpos_body(d, xcd_block(blockIdx.x, gridDim.x)); }
 __global__ void __launch_bounds__(RP_BLOCK) k_binfin_rowdct(RpDev d, int from_rho) {
@@ -1113,9 +1248,9 @@ static int fft_threads(int) { return RP_BLOCK; }
 
 // One evaluation slot.  ev (optional) holds RP_NUM_EVAL_KERNELS + 1 events
 // recorded around each kernel (per-kernel timing on this stream).
-hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_t* ev, const RpMfmaDev* mf) {
+hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_t* ev, const RpMfmaDev* mf,
+                          const RpSortBufs* sb) {
   int nb_c = (d.nc + RP_BLOCK - 1) / RP_BLOCK;
-  int nb_s = (d.nc + RP_SCATTER_CELLS - 1) / RP_SCATTER_CELLS;
   int nb_n = (d.nn + RP_BLOCK - 1) / RP_BLOCK;
   int k = 0;
 #define RP_MARK()                            \
@@ -1124,7 +1259,10 @@ hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_
     k++;                                     \
   } while (0)
   RP_MARK();
-  hipLaunchKernelGGL(k_step_scatter, dim3(nb_s), dim3(RP_BLOCK), 0, s, d);
+  hipLaunchKernelGGL(k_step, dim3(nb_c), dim3(RP_BLOCK), 0, s, d);
+  RP_MARK();
+  hipLaunchKernelGGL(k_scatter, dim3((d.nc + RP_SS_CELLS - 1) / RP_SS_CELLS), dim3(RP_BLOCK), 0, s, d,
+                     sb ? *sb : RpSortBufs{}, sb ? 1 : 0);
   RP_MARK();
   hipLaunchKernelGGL(k_pin_pos, dim3(std::max(1, (d.np + RP_BLOCK - 1) / RP_BLOCK)), dim3(RP_BLOCK), 0, s, d);
   RP_MARK();
@@ -1170,8 +1308,9 @@ hipError_t rp_launch_ctrl_init(const RpDev& d, hipStream_t s, int stage) {
 
 // Split stages for the stage-level test entry points.
 hipError_t rp_launch_scatter(const RpDev& d, hipStream_t s) {
-  hipLaunchKernelGGL(k_step_scatter, dim3((d.nc + RP_SCATTER_CELLS - 1) / RP_SCATTER_CELLS), dim3(RP_BLOCK), 0, s,
-                     d);
+  hipLaunchKernelGGL(k_step, dim3((d.nc + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, d);
+  hipLaunchKernelGGL(k_scatter, dim3((d.nc + RP_SS_CELLS - 1) / RP_SS_CELLS), dim3(RP_BLOCK), 0, s, d,
+                     RpSortBufs{}, 0);
   hipLaunchKernelGGL(k_pin_pos, dim3(std::max(1, (d.np + RP_BLOCK - 1) / RP_BLOCK)), dim3(RP_BLOCK), 0, s, d);
   return hipGetLastError();
 }
@@ -1196,6 +1335,62 @@ hipError_t rp_launch_wa_grad(const RpDev& d, hipStream_t s, int store_nets) {
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// spatial cell order for the density scatter (counting sort by tile)
+// ---------------------------------------------------------------------------
+// Exclusive scan of the RP_SORT_TILES tile counts into offsets (one
+// workgroup of RP_CTRL_BLOCK threads, each owning a contiguous run of
+// tiles); the counts are cleared for the next refresh.
+__global__ void __launch_bounds__(RP_CTRL_BLOCK) k_sort_scan(RpSortBufs sb) {
+  constexpr int PER = RP_SORT_TILES / RP_CTRL_BLOCK;
+  __shared__ int sh[RP_CTRL_BLOCK];
+  int base = threadIdx.x * PER;
+  int v[PER];
+  int run = 0;
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    v[k] = sb.cnt[base + k];
+    sb.cnt[base + k] = 0;
+    run += v[k];
+  }
+  sh[threadIdx.x] = run;
+  __syncthreads();
+  for (int o = 1; o < RP_CTRL_BLOCK; o <<= 1) {  // Hillis-Steele inclusive scan of the run totals
+    int t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int acc = sh[threadIdx.x] - run;
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    sb.off[base + k] = acc;
+    acc += v[k];
+  }
+}
+
+// Second pass: each cell goes to its tile's offset + its rank.  The counts
+// came from the same integer centres (no step ran in between), so this
+// writes a permutation of 0 .. Nc-1.
+__global__ void __launch_bounds__(RP_BLOCK) k_sort_place(RpDev d, RpSortBufs sb) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.nc) return;
+  sb.sord[sb.off[tile_key(d, d.ic[c], sb.tile_shift)] + sb.rank[c]] = c;
+}
+
+int rp_sort_tile_shift(int nbx, int nby) {
+  int m = std::max(nbx, nby), lg = 0;
+  while ((1 << lg) < m) lg++;
+  return std::max(0, lg - 7);
+}
+
+hipError_t rp_launch_sort_refresh(const RpDev& d, const RpSortBufs& b, hipStream_t s) {
+  if (d.nc <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(RP_CTRL_BLOCK), 0, s, b);
+  hipLaunchKernelGGL(k_sort_place, dim3((d.nc + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, d, b);
+  return hipGetLastError();
+}
+
 __global__ void k_gather_phase(const RpDev* devs, int n, int* out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = devs[i].st->phase;
@@ -1210,9 +1405,8 @@ hipError_t rp_launch_gather_phase(const RpDev* devs, int n, int* out, hipStream_
 // sizes and per-instance logical blocks as rp_launch_eval).
 hipError_t rp_launch_eval_batch(const RpBatchLaunch& b, hipStream_t s) {
   const int* n = b.nblk;
-  if (n[RP_K_STEP_SCATTER])
-    hipLaunchKernelGGL(kb_step_scatter, dim3(n[RP_K_STEP_SCATTER]), dim3(RP_BLOCK), 0, s, b.devs,
-                       b.tab[RP_K_STEP_SCATTER]);
+  if (n[RP_K_SCATTER])
+    hipLaunchKernelGGL(kb_step_scatter, dim3(n[RP_K_SCATTER]), dim3(RP_BLOCK), 0, s, b.devs, b.tab[RP_K_SCATTER]);
   if (n[RP_K_PIN_POS])
     hipLaunchKernelGGL(kb_pin_pos, dim3(n[RP_K_PIN_POS]), dim3(RP_BLOCK), 0, s, b.devs, b.tab[RP_K_PIN_POS]);
   if (n[RP_K_BINFIN_ROWDCT])
