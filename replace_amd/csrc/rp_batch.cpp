@@ -259,9 +259,9 @@ static int rebuild(rp_server* s) {
     cnt[RP_K_WA].push_back((d.nn + RP_BLOCK - 1) / RP_BLOCK);
     cnt[RP_K_GRAD].push_back((d.nc + RP_BLOCK - 1) / RP_BLOCK);
     cnt[RP_K_CONTROL].push_back(1);
-    lb = std::max(lb, sizeof(float2) * 2 * d.nbx);
-    lc = std::max(lc, sizeof(float2) * 2 * d.nby + sizeof(float) * 4 * d.nby);
-    lr = std::max(lr, sizeof(float2) * 2 * d.nbx + sizeof(float) * 2 * d.nbx);
+    lb = std::max(lb, rp_lds_binfin(d.nbx));
+    lc = std::max(lc, rp_lds_col(d.nby));
+    lr = std::max(lr, rp_lds_rinv(d.nbx));
   }
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   if (s->graph) (void)hipGraphDestroy(s->graph);

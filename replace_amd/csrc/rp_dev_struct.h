@@ -62,6 +62,13 @@ struct RpDev {
   float2* dbg_dens;
 };
 
+// Dynamic LDS bytes of the FFT kernels for an n-point row / column: two
+// ping-pong buffers and the twiddle circle (n float2 each), plus the staged
+// rows of the column and inverse passes.
+inline size_t rp_lds_binfin(int n) { return sizeof(float2) * 3 * (size_t)n; }
+inline size_t rp_lds_col(int n) { return sizeof(float2) * 3 * (size_t)n + sizeof(float) * 4 * (size_t)n; }
+inline size_t rp_lds_rinv(int n) { return sizeof(float2) * 3 * (size_t)n + sizeof(float) * 4 * (size_t)n; }
+
 // One batched evaluation step over many instances (kb_* kernels): per kernel
 // a block table of (instance, logical block) entries (instance -1 = idle).
 struct RpBatchLaunch {

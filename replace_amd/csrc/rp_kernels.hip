@@ -121,31 +121,75 @@ __device__ __forceinline__ int xcd_block(int b, int nblocks) {
 }
 
 // ---------------------------------------------------------------------------
-// radix-2 Stockham FFT in LDS (natural order in, natural order out).
-// dir = -1 forward (e^{-i}), +1 inverse (e^{+i}), unnormalised.
-// Returns the buffer holding the result.
+// Stockham FFT in LDS (natural order in, natural order out): radix-4
+// stages, preceded by one radix-2 stage when log2(n) is odd, so a 1024-point
+// transform takes 5 barriers.  T holds the full circle of twiddles
+// e^{-2 pi i m / n}, m < n (load_twiddles).  dir = -1 forward (e^{-i}),
+// +1 inverse (e^{+i}), unnormalised.  Returns the buffer holding the result.
 // ---------------------------------------------------------------------------
-__device__ float2* lds_fft(float2* a, float2* b, int n, const float2* __restrict__ tw, int dir) {
+__device__ __forceinline__ void load_twiddles(float2* T, const float2* __restrict__ tw, int n) {
+  int half = n >> 1;
+  for (int m = threadIdx.x; m < half; m += blockDim.x) {
+    float2 w = tw[m];
+    T[m] = w;
+    T[m + half] = make_float2(-w.x, -w.y);
+  }
+}
+
+// z * (dir * i)
+__device__ __forceinline__ float2 mul_dir_i(float2 z, int dir) {
+  return dir > 0 ? make_float2(-z.y, z.x) : make_float2(z.y, -z.x);
+}
+
+__device__ float2* lds_fft(float2* a, float2* b, int n, const float2* T, int dir) {
   float2* in = a;
   float2* out = b;
-  int half = n >> 1;
-  for (int ns = 1; ns < n; ns <<= 1) {
-    int tstride = n / (2 * ns);
+  int ns = 1;
+  if ((__ffs(n) - 1) & 1) {  // odd log2(n): radix-2 first stage (no twiddles)
+    int half = n >> 1;
     for (int j = threadIdx.x; j < half; j += blockDim.x) {
-      int k = j & (ns - 1);
-      float2 u = in[j];
-      float2 v = in[j + half];
-      float2 w = tw[k * tstride];
-      if (dir > 0) w.y = -w.y;
-      float2 tv = cmul(v, w);
-      int dst = ((j - k) << 1) + k;
-      out[dst] = make_float2(u.x + tv.x, u.y + tv.y);
-      out[dst + ns] = make_float2(u.x - tv.x, u.y - tv.y);
+      float2 u = in[j], v = in[j + half];
+      out[2 * j] = make_float2(u.x + v.x, u.y + v.y);
+      out[2 * j + 1] = make_float2(u.x - v.x, u.y - v.y);
     }
     __syncthreads();
-    float2* s = in;
+    float2* t = in;
     in = out;
-    out = s;
+    out = t;
+    ns = 2;
+  }
+  const int q = n >> 2;
+  for (; ns < n; ns <<= 2) {
+    int tstride = n / (4 * ns);
+    for (int j = threadIdx.x; j < q; j += blockDim.x) {
+      int k = j & (ns - 1);
+      float2 a0 = in[j], a1 = in[j + q], a2 = in[j + 2 * q], a3 = in[j + 3 * q];
+      if (k) {
+        int m = k * tstride;
+        float2 w1 = T[m], w2 = T[2 * m], w3 = T[3 * m];
+        if (dir > 0) {
+          w1.y = -w1.y;
+          w2.y = -w2.y;
+          w3.y = -w3.y;
+        }
+        a1 = cmul(a1, w1);
+        a2 = cmul(a2, w2);
+        a3 = cmul(a3, w3);
+      }
+      float2 s02 = make_float2(a0.x + a2.x, a0.y + a2.y);
+      float2 d02 = make_float2(a0.x - a2.x, a0.y - a2.y);
+      float2 s13 = make_float2(a1.x + a3.x, a1.y + a3.y);
+      float2 d13 = mul_dir_i(make_float2(a1.x - a3.x, a1.y - a3.y), dir);
+      int dst = (j - k) * 4 + k;
+      out[dst] = make_float2(s02.x + s13.x, s02.y + s13.y);
+      out[dst + ns] = make_float2(d02.x + d13.x, d02.y + d13.y);
+      out[dst + 2 * ns] = make_float2(s02.x - s13.x, s02.y - s13.y);
+      out[dst + 3 * ns] = make_float2(d02.x - d13.x, d02.y - d13.y);
+    }
+    __syncthreads();
+    float2* t = in;
+    in = out;
+    out = t;
   }
   return in;
 }
@@ -444,8 +488,10 @@ __device__ void binfin_body(const RpDev& d, int blk, int from_rho) {
   const int n = d.nbx;
   float2* A = smem;
   float2* B = smem + n;
+  float2* T = smem + 2 * n;
   int ja = 2 * blk, jb = ja + 1;
   double ov_sum = 0.0;
+  if (!d.mfma) load_twiddles(T, d.twx, n);
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     int blx, bux;
     bin_x(d, i, blx, bux);
@@ -480,7 +526,7 @@ __device__ void binfin_body(const RpDev& d, int blk, int from_rho) {
   if (threadIdx.x == 0) d.part_bin[blk] = tot;
   if (d.mfma) return;  // the matrix-core solve takes rho from d.rho
   // block_sum_d synchronised the LDS writes above
-  float2* Z = lds_fft(A, B, n, d.twx, -1);
+  float2* Z = lds_fft(A, B, n, T, -1);
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     float xa, xb;
     dct2_post(Z, n, k, d.tw2x, xa, xb);
@@ -521,15 +567,17 @@ __device__ void col_body(const RpDev& d, int blk) {
   const int n = d.nby;
   float2* A = smem;
   float2* B = smem + n;
-  float* pa = (float*)(smem + 2 * n);  // phi hat rows a, b; Ey hat rows a, b
+  float2* T = smem + 2 * n;
+  float* pa = (float*)(smem + 3 * n);  // phi hat rows a, b; Ey hat rows a, b
   float* pb = pa + n;
   float* ea = pb + n;
   float* eb = ea + n;
   int ua = 2 * blk, ub = ua + 1;
+  load_twiddles(T, d.twy, n);
   for (int v = threadIdx.x; v < n; v += blockDim.x)
     A[mk_pos(v, n)] = make_float2(d.tt[(size_t)ua * n + v], d.tt[(size_t)ub * n + v]);
   __syncthreads();
-  float2* Z = lds_fft(A, B, n, d.twy, -1);
+  float2* Z = lds_fft(A, B, n, T, -1);
   const float scale4 = (float)(4.0 / (double)d.nbx / (double)n);
   for (int v = threadIdx.x; v < n; v += blockDim.x) {
     float xa, xb;
@@ -560,7 +608,7 @@ __device__ void col_body(const RpDev& d, int blk) {
     A[k] = dct3_pre(pa[k], ank, pb[k], bnk, k, d.tw2y);
   }
   __syncthreads();
-  Z = lds_fft(A, B, n, d.twy, +1);
+  Z = lds_fft(A, B, n, T, +1);
   for (int m = threadIdx.x; m < n; m += blockDim.x) {
     // y[2q] = w[q], y[2q+1] = w[N-1-q]
     float2 z = Z[(m & 1) ? (n - 1 - (m >> 1)) : (m >> 1)];
@@ -578,7 +626,7 @@ __device__ void col_body(const RpDev& d, int blk) {
     A[k] = dct3_pre(ak, ank, bk, bnk, k, d.tw2y);
   }
   __syncthreads();
-  Z = lds_fft(A, B, n, d.twy, +1);
+  Z = lds_fft(A, B, n, T, +1);
   for (int m = threadIdx.x; m < n; m += blockDim.x) {
     float2 z = Z[(m & 1) ? (n - 1 - (m >> 1)) : (m >> 1)];
     float sg = (m & 1) ? -1.0f : 1.0f;
@@ -596,13 +644,20 @@ __device__ void rowinv_body(const RpDev& d, int blk, int want_phi) {
   const int n = d.nbx;
   float2* A = smem;
   float2* B = smem + n;
-  float* ra = (float*)(smem + 2 * n);
+  float2* T = smem + 2 * n;
+  float* ra = (float*)(smem + 3 * n);  // input rows a, b, then Ex rows a, b
   float* rb = ra + n;
+  float* qa = rb + n;  // second input rows a, b (Y2, then Y1 for phi)
+  float* qb = qa + n;
   int ja = 2 * blk, jb = ja + 1;
-  // Ex: DST-III of (wx[u] * Y1[j][u])
+  load_twiddles(T, d.twx, n);
+  // Ex: DST-III of (wx[u] * Y1[j][u]); Ey: DCT-III of Y2 rows
   for (int u = threadIdx.x; u < n; u += blockDim.x) {
-    ra[u] = d.wx[u] * d.y1[(size_t)ja * n + u];
-    rb[u] = d.wx[u] * d.y1[(size_t)jb * n + u];
+    float w = d.wx[u];
+    ra[u] = w * d.y1[(size_t)ja * n + u];
+    rb[u] = w * d.y1[(size_t)jb * n + u];
+    qa[u] = d.y2[(size_t)ja * n + u];
+    qb[u] = d.y2[(size_t)jb * n + u];
   }
   __syncthreads();
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
@@ -614,7 +669,7 @@ __device__ void rowinv_body(const RpDev& d, int blk, int want_phi) {
     A[k] = dct3_pre(ak, ank, bk, bnk, k, d.tw2x);
   }
   __syncthreads();
-  float2* Z = lds_fft(A, B, n, d.twx, +1);
+  float2* Z = lds_fft(A, B, n, T, +1);
   for (int m = threadIdx.x; m < n; m += blockDim.x) {
     float2 z = Z[(m & 1) ? (n - 1 - (m >> 1)) : (m >> 1)];
     float sg = (m & 1) ? -1.0f : 1.0f;
@@ -622,15 +677,12 @@ __device__ void rowinv_body(const RpDev& d, int blk, int want_phi) {
     rb[m] = sg * z.y;
   }
   __syncthreads();
-  // Ey: DCT-III of Y2 rows
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     int nk = (n - k) & (n - 1);
-    const float* qa = d.y2 + (size_t)ja * n;
-    const float* qb = d.y2 + (size_t)jb * n;
     A[k] = dct3_pre(qa[k], k ? qa[nk] : 0.0f, qb[k], k ? qb[nk] : 0.0f, k, d.tw2x);
   }
   __syncthreads();
-  Z = lds_fft(A, B, n, d.twx, +1);
+  Z = lds_fft(A, B, n, T, +1);
   for (int m = threadIdx.x; m < n; m += blockDim.x) {
     float2 z = Z[(m & 1) ? (n - 1 - (m >> 1)) : (m >> 1)];
     d.field[(size_t)ja * n + m] = make_float2(ra[m], z.x);
@@ -638,14 +690,17 @@ __device__ void rowinv_body(const RpDev& d, int blk, int want_phi) {
   }
   if (!want_phi) return;
   __syncthreads();
+  for (int u = threadIdx.x; u < n; u += blockDim.x) {
+    qa[u] = d.y1[(size_t)ja * n + u];
+    qb[u] = d.y1[(size_t)jb * n + u];
+  }
+  __syncthreads();
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     int nk = (n - k) & (n - 1);
-    const float* qa = d.y1 + (size_t)ja * n;
-    const float* qb = d.y1 + (size_t)jb * n;
     A[k] = dct3_pre(qa[k], k ? qa[nk] : 0.0f, qb[k], k ? qb[nk] : 0.0f, k, d.tw2x);
   }
   __syncthreads();
-  Z = lds_fft(A, B, n, d.twx, +1);
+  Z = lds_fft(A, B, n, T, +1);
   for (int m = threadIdx.x; m < n; m += blockDim.x) {
     float2 z = Z[(m & 1) ? (n - 1 - (m >> 1)) : (m >> 1)];
     d.phi[(size_t)ja * n + m] = z.x;
@@ -1266,7 +1321,7 @@ hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_
   RP_MARK();
   hipLaunchKernelGGL(k_pin_pos, dim3(std::max(1, (d.np + RP_BLOCK - 1) / RP_BLOCK)), dim3(RP_BLOCK), 0, s, d);
   RP_MARK();
-  size_t shx = sizeof(float2) * 2 * d.nbx;
+  size_t shx = rp_lds_binfin(d.nbx);
   hipLaunchKernelGGL(k_binfin_rowdct, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shx, s, d, 0);
   RP_MARK();
   if (mf) {
@@ -1282,12 +1337,12 @@ hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_
     hipLaunchKernelGGL(k_transpose, dim3((d.nbx + 31) / 32, (d.nby + 31) / 32), dim3(32, 8), 0, s,
                        (const float*)d.t, d.tt, d.nby, d.nbx, (const RpState*)d.st);
     RP_MARK();
-    size_t shy = sizeof(float2) * 2 * d.nby + sizeof(float) * 4 * d.nby;
+    size_t shy = rp_lds_col(d.nby);
     hipLaunchKernelGGL(k_col_spectral, dim3(d.nbx / 2), dim3(fft_threads(d.nby)), shy, s, d);
     RP_MARK();
     hipLaunchKernelGGL(k_transpose2, dim3((d.nby + 31) / 32, (d.nbx + 31) / 32, 2), dim3(32, 8), 0, s, d);
     RP_MARK();
-    size_t shr = sizeof(float2) * 2 * d.nbx + sizeof(float) * 2 * d.nbx;
+    size_t shr = rp_lds_rinv(d.nbx);
     hipLaunchKernelGGL(k_row_inverse, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shr, s, d, want_phi);
     RP_MARK();
   }
@@ -1316,15 +1371,15 @@ hipError_t rp_launch_scatter(const RpDev& d, hipStream_t s) {
 }
 
 hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int from_rho, const RpMfmaDev* mf) {
-  size_t shx = sizeof(float2) * 2 * d.nbx;
+  size_t shx = rp_lds_binfin(d.nbx);
   hipLaunchKernelGGL(k_binfin_rowdct, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shx, s, d, from_rho);
   if (mf) return rp_launch_poisson_mfma(d, *mf, s, want_phi);
   hipLaunchKernelGGL(k_transpose, dim3((d.nbx + 31) / 32, (d.nby + 31) / 32), dim3(32, 8), 0, s,
                      (const float*)d.t, d.tt, d.nby, d.nbx, (const RpState*)d.st);
-  size_t shy = sizeof(float2) * 2 * d.nby + sizeof(float) * 4 * d.nby;
+  size_t shy = rp_lds_col(d.nby);
   hipLaunchKernelGGL(k_col_spectral, dim3(d.nbx / 2), dim3(fft_threads(d.nby)), shy, s, d);
   hipLaunchKernelGGL(k_transpose2, dim3((d.nby + 31) / 32, (d.nbx + 31) / 32, 2), dim3(32, 8), 0, s, d);
-  size_t shr = sizeof(float2) * 2 * d.nbx + sizeof(float) * 2 * d.nbx;
+  size_t shr = rp_lds_rinv(d.nbx);
   hipLaunchKernelGGL(k_row_inverse, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shr, s, d, want_phi);
   return hipGetLastError();
 }
