@@ -268,13 +268,15 @@ def bench_precision(args):
         info = g.setup()
         for _ in range(args.warmup):
             g.nesterov_place()
-        g.set_kernel_timing(True)
         torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(args.steps):
             code, st = g.nesterov_place()
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / args.steps
+        g.set_kernel_timing(True)  # per-kernel times from one more, separately timed placement
+        g.nesterov_place()
+        g.set_kernel_timing(False)
         kt = g.kernel_timing()
         out[mode] = {"s": dt, "code": code, "iterations": st.iterations, "evals": st.evals,
                      "final_overflow": float(st.final_overflow), "final_hpwl": int(st.final_hpwl),
@@ -345,7 +347,6 @@ def main():
     info = placer.setup()
     for _ in range(args.warmup):
         placer.nesterov_place()
-    placer.set_kernel_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     runs = []
@@ -356,6 +357,11 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) / args.steps * 1e3
+    # Per-kernel times come from one more placement, untimed above: the HIP
+    # events bracketing every kernel add their own cost to the run.
+    placer.set_kernel_timing(True)
+    placer.nesterov_place()
+    placer.set_kernel_timing(False)
     kt = placer.kernel_timing()
     nb = info.bin_cnt_x * info.bin_cnt_y
     abytes = algorithmic_bytes(info.num_gcells, info.num_gpins, info.num_gnets, nb)

@@ -191,7 +191,9 @@ const char* rp_last_error(rp_ctx* ctx);
 /* Per-kernel timing of the evaluation slots (no reference counterpart: the
  * reference has no timers).  When enabled, rp_nesterov_place brackets every
  * kernel of every executed slot with HIP events on the context's stream and
- * accumulates the elapsed time.  rp_get_kernel_timing fills 12 entries:
+ * accumulates the elapsed time (enabling clears the totals, disabling keeps
+ * them; the events themselves slow the run, so time whole placements with
+ * timing off).  rp_get_kernel_timing fills 12 entries:
  * for the 11 kernels of a slot in launch order (step, density scatter, pin
  * centres, bin finalize + row DCT, transpose, column spectral pass,
  * transpose, row inverse, WA, gradient, controller) the summed milliseconds,

@@ -256,7 +256,7 @@ static int rebuild(rp_server* s) {
     cnt[RP_K_COL_SPECTRAL].push_back(d.nbx / 2);
     cnt[RP_K_TRANSPOSE2].push_back(2 * tx * ty);
     cnt[RP_K_ROW_INVERSE].push_back(d.nby / 2);
-    cnt[RP_K_WA].push_back((d.nn + RP_BLOCK - 1) / RP_BLOCK);
+    cnt[RP_K_WA].push_back(d.nwa);
     cnt[RP_K_GRAD].push_back((d.nc + RP_BLOCK - 1) / RP_BLOCK);
     cnt[RP_K_CONTROL].push_back(1);
     lb = std::max(lb, rp_lds_binfin(d.nbx));

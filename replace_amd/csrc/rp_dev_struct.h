@@ -28,6 +28,8 @@ struct RpDev {
   const int* cp_pin;   // cell's pins (reference ITerm order) -> device pin index
   float2* pgrad;       // per-pin WA gradient gMin - gMax, device pin order
   const int* nstart;
+  int nwa;             // wirelength workgroups
+  const int2* wa_rng;  // their net ranges [x, y)
   const int* pcell;
   const int2* ppos;
   int2* ppin;        // absolute pin centres, rewritten every evaluation (k_pin_pos)

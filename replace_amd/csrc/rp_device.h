@@ -39,6 +39,10 @@
 #endif
 #define RP_SS_CELLS (RP_BLOCK * RP_SS_PER_THREAD)  // cells per spatial-order scatter workgroup
 #define RP_MAX_FFT 2048
+#ifndef RP_WA_CAP
+#define RP_WA_CAP 512         // pins per wirelength workgroup (a larger net gets one of its own)
+#endif
+#define RP_WA_NETS RP_BLOCK   // nets per wirelength workgroup
 
 // Kernels of one evaluation slot, in launch order (per-kernel timing index).
 enum {

@@ -845,6 +845,22 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   HIPCHK(c, dupload(c->d_cp_pin, cp_pin, s));
   HIPCHK(c, dalloc(c->d_pgrad, std::max(1, npn)));
   HIPCHK(c, dupload(c->d_nstart, nstart, s));
+  // wirelength workgroups: consecutive nets, at most RP_WA_NETS nets and
+  // RP_WA_CAP pins each; a net with more pins gets a workgroup of its own
+  std::vector<int2> wa_rng;
+  for (int e = 0; e < nn;) {
+    const int e0 = e;
+    if (nstart[e + 1] - nstart[e] > RP_WA_CAP) {
+      wa_rng.push_back(make_int2(e, e + 1));
+      e++;
+      continue;
+    }
+    while (e < nn && e - e0 < RP_WA_NETS && nstart[e + 1] - nstart[e0] <= RP_WA_CAP) e++;
+    wa_rng.push_back(make_int2(e0, e));
+  }
+  if (wa_rng.empty()) wa_rng.push_back(make_int2(0, 0));
+  const int nwa = (int)wa_rng.size();
+  HIPCHK(c, dupload(c->d_wa_rng, wa_rng, s));
   HIPCHK(c, dupload(c->d_pcell, pcell, s));
   HIPCHK(c, dupload(c->d_ppos, ppos, s));
   HIPCHK(c, dalloc(c->d_ppin, std::max(1, npn)));
@@ -873,7 +889,7 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   HIPCHK(c, dupload(c->d_wx2, wx2, s));
   HIPCHK(c, dupload(c->d_wy, wy, s));
   HIPCHK(c, dupload(c->d_wy2, wy2, s));
-  int npg = (ng + RP_BLOCK - 1) / RP_BLOCK, npb = c->nby / 2, npn2 = (nn + RP_BLOCK - 1) / RP_BLOCK;
+  int npg = (ng + RP_BLOCK - 1) / RP_BLOCK, npb = c->nby / 2, npn2 = nwa;
   HIPCHK(c, dalloc(c->d_part_grad, (size_t)RP_P_COUNT * npg));
   HIPCHK(c, dalloc(c->d_part_bin, npb));
   HIPCHK(c, dalloc(c->d_part_net, std::max(1, npn2)));
@@ -914,6 +930,8 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   d.cp_pin = c->d_cp_pin.p;
   d.pgrad = c->d_pgrad.p;
   d.nstart = c->d_nstart.p;
+  d.nwa = nwa;
+  d.wa_rng = c->d_wa_rng.p;
   d.pcell = c->d_pcell.p;
   d.ppos = c->d_ppos.p;
   d.ppin = c->d_ppin.p;
@@ -1370,8 +1388,10 @@ extern "C" int rp_set_positions(rp_ctx* c, const int32_t* lx, const int32_t* ly)
 extern "C" int rp_set_kernel_timing(rp_ctx* c, int enable) {
   if (!c) return RP_ERR_ARG;
   c->timing = enable != 0;
-  for (double& v : c->ktime_ms) v = 0.0;
-  c->kcount = 0;
+  if (c->timing) {  // a new measurement; disabling keeps the totals for rp_get_kernel_timing
+    for (double& v : c->ktime_ms) v = 0.0;
+    c->kcount = 0;
+  }
   return RP_OK;
 }
 

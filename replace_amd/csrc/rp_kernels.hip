@@ -711,173 +711,192 @@ __device__ void rowinv_body(const RpDev& d, int blk, int want_phi) {
 // ---------------------------------------------------------------------------
 // 6. WA exponent sums per net (src/nesterovBase.cpp:1304-1360) + HPWL
 // ---------------------------------------------------------------------------
-// Pin centre (GPin::updateDensityLocation: integer density centre + offset),
-// written in pin order by the scatter kernel.
-__device__ __forceinline__ int2 pin_pos(const RpDev& d, int q) { return d.ppin[q]; }
 
-// ---------------------------------------------------------------------------
-// getWireLengthGradientPinWA for one pin and one axis (src/nesterovBase.cpp:1383-1440):
-// returns gMin - gMax with the exponentials recomputed as the WA pass stored them.
-__device__ __forceinline__ float wl_axis(int p, int lo, int hi, float coef, float S0, float XS0, float S1,
-                                         float XS1, float bar) {
-  float gmin = 0.0f, gmax = 0.0f;
-  float e0 = (float)(lo - p) * coef;
-  if (e0 > bar) {
-    float ex = fast_exp(e0);
-    gmin = (float)(((double)S0 * ((double)ex * (1.0 - (double)(coef * (float)p))) + (double)(coef * ex * XS0)) /
-                   (double)(S0 * S0));
-  }
-  float e1 = (float)(p - hi) * coef;
-  if (e1 > bar) {
-    float ex = fast_exp(e1);
-    gmax = (float)(((double)S1 * ((double)ex * (1.0 + (double)(coef * (float)p))) - (double)(coef * ex * XS1)) /
-                   (double)(S1 * S1));
-  }
-  return gmin - gmax;
+// The four WA exponentials of a pin (src/nesterovBase.cpp:1316-1356): the
+// min-x, max-x, min-y and max-y terms against the net's pin bbox, each
+// fastExp(exponent) where the exponent exceeds the force bar and -1 (never a
+// fastExp value, which is >= 0) where it does not.
+__device__ __forceinline__ float4 wa_exps(int2 p, int4 box, float cx, float cy, float bar) {
+  float e0 = (float)(box.x - p.x) * cx;
+  float e1 = (float)(p.x - box.z) * cx;
+  float e2 = (float)(box.y - p.y) * cy;
+  float e3 = (float)(p.y - box.w) * cy;
+  return make_float4(e0 > bar ? fast_exp(e0) : -1.0f, e1 > bar ? fast_exp(e1) : -1.0f,
+                     e2 > bar ? fast_exp(e2) : -1.0f, e3 > bar ? fast_exp(e3) : -1.0f);
 }
 
+// The eight per-net sums, accumulated in pin order.
+struct WaSums {
+  float s0, s1, s2, s3, s4, s5, s6, s7;
+};
 
-// Same as wl_axis with the exponentials fastExp(e0), fastExp(e1) already
-// computed by the sums pass (identical arguments, so identical values).
-__device__ __forceinline__ float wl_axis_x(int p, int lo, int hi, float coef, float ex0, float ex1, float S0,
-                                           float XS0, float S1, float XS1, float bar) {
+__device__ __forceinline__ void wa_accumulate(WaSums& w, int2 p, float4 v) {
+  if (v.x >= 0.0f) {
+    w.s0 += v.x;
+    w.s1 += (float)p.x * v.x;
+  }
+  if (v.y >= 0.0f) {
+    w.s2 += v.y;
+    w.s3 += (float)p.x * v.y;
+  }
+  if (v.z >= 0.0f) {
+    w.s4 += v.z;
+    w.s5 += (float)p.y * v.z;
+  }
+  if (v.w >= 0.0f) {
+    w.s6 += v.w;
+    w.s7 += (float)p.y * v.w;
+  }
+}
+
+// getWireLengthGradientPinWA for one pin and one axis (src/nesterovBase.cpp:1383-1440):
+// gMin - gMax from the pin's exponentials (negative = inactive) and the net sums.
+__device__ __forceinline__ float wl_axis(int p, float coef, float ex0, float ex1, float S0, float XS0, float S1,
+                                         float XS1) {
   float gmin = 0.0f, gmax = 0.0f;
-  float e0 = (float)(lo - p) * coef;
-  if (e0 > bar)
+  if (ex0 >= 0.0f)
     gmin = (float)(((double)S0 * ((double)ex0 * (1.0 - (double)(coef * (float)p))) + (double)(coef * ex0 * XS0)) /
                    (double)(S0 * S0));
-  float e1 = (float)(p - hi) * coef;
-  if (e1 > bar)
+  if (ex1 >= 0.0f)
     gmax = (float)(((double)S1 * ((double)ex1 * (1.0 + (double)(coef * (float)p))) - (double)(coef * ex1 * XS1)) /
                    (double)(S1 * S1));
   return gmin - gmax;
 }
 
-#define WA_CHUNK 8
-
-// Pin bbox update and WA exponent accumulation for one pin, in pin order
-// (src/nesterovBase.cpp:1316-1356).
-struct WaSums {
-  float s0, s1, s2, s3, s4, s5, s6, s7;
-};
-
-// ex (optional) receives the four exponentials (min x, max x, min y, max y;
-// unset where the exponent is at or below the force bar).
-__device__ __forceinline__ void wa_pin(int2 p, int lx, int ly, int ux, int uy, float cx, float cy, float bar,
-                                       WaSums& w, float4* ex = nullptr) {
-  float eMinX = (float)(lx - p.x) * cx;
-  float eMaxX = (float)(p.x - ux) * cx;
-  float eMinY = (float)(ly - p.y) * cy;
-  float eMaxY = (float)(p.y - uy) * cy;
-  if (eMinX > bar) {
-    float v = fast_exp(eMinX);
-    w.s0 += v;
-    w.s1 += (float)p.x * v;
-    if (ex) ex->x = v;
-  }
-  if (eMaxX > bar) {
-    float v = fast_exp(eMaxX);
-    w.s2 += v;
-    w.s3 += (float)p.x * v;
-    if (ex) ex->y = v;
-  }
-  if (eMinY > bar) {
-    float v = fast_exp(eMinY);
-    w.s4 += v;
-    w.s5 += (float)p.y * v;
-    if (ex) ex->z = v;
-  }
-  if (eMaxY > bar) {
-    float v = fast_exp(eMaxY);
-    w.s6 += v;
-    w.s7 += (float)p.y * v;
-    if (ex) ex->w = v;
-  }
+__device__ __forceinline__ float2 wa_pin_grad(int2 p, float4 v, const float4& sa, const float4& sb, float cx,
+                                              float cy) {
+  return make_float2(wl_axis(p.x, cx, v.x, v.y, sa.x, sa.y, sa.z, sa.w),
+                     wl_axis(p.y, cy, v.z, v.w, sb.x, sb.y, sb.z, sb.w));
 }
 
-// One thread per net.  Pin positions are gathered WA_CHUNK at a time with
-// independent loads (the first chunk stays in registers for the later
-// passes).  Pass 1: pin bbox; pass 2: the eight exponent sums in pin order;
-// pass 3: each pin's WA gradient gMin - gMax (getWireLengthGradientPinWA,
-// src/nesterovBase.cpp:1383-1440) from the net's final sums, stored per pin
-// so the cell gradient kernel gathers 8 bytes per pin.  The net bbox and
-// sums are stored only when store_nets is set (stage-level checks).
+// Weighted-average wirelength per workgroup of nets (updateWireLengthForceWA
+// and getWireLengthGradientPinWA, src/nesterovBase.cpp:1304-1440), pin
+// parallel.  The workgroup takes the nets d.wa_rng[blk] (consecutive, at
+// most RP_WA_NETS nets and RP_WA_CAP pins, or one larger net alone):
+//   1. pin centres into LDS (coalesced);
+//   2. one lane per net: pin bbox, the pin -> net map, HPWL term;
+//   3. one lane per pin: the four exponentials;
+//   4. one lane per net: the eight sums in pin order (the reference's float
+//      summation order, so the sums are bit-identical);
+//   5. one lane per pin: the per-pin gradient gMin - gMax, stored in pin
+//      order (coalesced; the cell gradient kernel gathers it).
+// The net bbox and sums are stored only when store_nets is set (stage-level
+// checks).
 __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
+  __shared__ int2 s_pos[RP_WA_CAP];
+  __shared__ float4 s_ex[RP_WA_CAP];
+  __shared__ uint8_t s_net[RP_WA_CAP];  // net index within the workgroup (< RP_WA_NETS = 256)
+  __shared__ int4 s_box[RP_WA_NETS];
+  __shared__ float4 s_sum[2 * RP_WA_NETS];
   __shared__ long long sh_red[RP_BLOCK / 64];
   const RpState* st = d.st;
   if (st->phase != 0) return;
-  int e = blk * blockDim.x + threadIdx.x;
+  const float cx = st->wl_coef_x, cy = st->wl_coef_y, bar = d.bar;
+  const int2 rng = d.wa_rng[blk];
+  const int p0 = d.nstart[rng.x], p1 = d.nstart[rng.y];
+  const int npw = p1 - p0;
+  const int tid = threadIdx.x;
   long long h = 0;
-  if (e < d.nn) {
-    float cx = st->wl_coef_x, cy = st->wl_coef_y;
-    int s = d.nstart[e], t = d.nstart[e + 1];
-    int deg = t - s;
-    int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
-    int2 pos[WA_CHUNK];
-#pragma unroll
-    for (int k = 0; k < WA_CHUNK; k++)
-      if (k < deg) pos[k] = pin_pos(d, s + k);
-#pragma unroll
-    for (int k = 0; k < WA_CHUNK; k++)
-      if (k < deg) {
-        lx = min(pos[k].x, lx);
-        ly = min(pos[k].y, ly);
-        ux = max(pos[k].x, ux);
-        uy = max(pos[k].y, uy);
+  if (npw <= RP_WA_CAP) {
+    for (int j = tid; j < npw; j += blockDim.x) s_pos[j] = d.ppin[p0 + j];
+    __syncthreads();
+    const int e = rng.x + tid;
+    const bool owner = e < rng.y;
+    int s = 0, t = 0;
+    int4 box = make_int4(INT_MAX, INT_MAX, INT_MIN, INT_MIN);
+    if (owner) {
+      s = d.nstart[e] - p0;
+      t = d.nstart[e + 1] - p0;
+      for (int q = s; q < t; q++) {
+        int2 p = s_pos[q];
+        box.x = min(p.x, box.x);
+        box.y = min(p.y, box.y);
+        box.z = max(p.x, box.z);
+        box.w = max(p.y, box.w);
+        s_net[q] = (uint8_t)tid;
       }
-    for (int b = s + WA_CHUNK; b < t; b += WA_CHUNK) {
-      int2 pp[WA_CHUNK];
-#pragma unroll
-      for (int k = 0; k < WA_CHUNK; k++)
-        if (b + k < t) pp[k] = pin_pos(d, b + k);
-#pragma unroll
-      for (int k = 0; k < WA_CHUNK; k++)
-        if (b + k < t) {
-          lx = min(pp[k].x, lx);
-          ly = min(pp[k].y, ly);
-          ux = max(pp[k].x, ux);
-          uy = max(pp[k].y, uy);
-        }
+      s_box[tid] = box;
+      if (t > s) h = (long long)((box.z - box.x) + (box.w - box.y));
+    }
+    __syncthreads();
+    for (int j = tid; j < npw; j += blockDim.x) s_ex[j] = wa_exps(s_pos[j], s_box[s_net[j]], cx, cy, bar);
+    __syncthreads();
+    if (owner) {
+      WaSums w = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int q = s; q < t; q++) wa_accumulate(w, s_pos[q], s_ex[q]);
+      float4 sa = make_float4(w.s0, w.s1, w.s2, w.s3), sb = make_float4(w.s4, w.s5, w.s6, w.s7);
+      s_sum[2 * tid] = sa;
+      s_sum[2 * tid + 1] = sb;
+      if (store_nets) {
+        d.nbox[e] = box;
+        d.nsum[2 * (size_t)e] = sa;
+        d.nsum[2 * (size_t)e + 1] = sb;
+      }
+    }
+    __syncthreads();
+    for (int j = tid; j < npw; j += blockDim.x) {
+      int L = s_net[j];
+      d.pgrad[p0 + j] = wa_pin_grad(s_pos[j], s_ex[j], s_sum[2 * L], s_sum[2 * L + 1], cx, cy);
+    }
+  } else {
+    // one net with more than RP_WA_CAP pins: bbox by a block min/max, the
+    // sums chunk by chunk (exponentials in parallel, summed in pin order by
+    // lane 0), then the per-pin gradients with the exponentials recomputed
+    // (same arguments, same values)
+    const int e = rng.x;
+    int4 box = make_int4(INT_MAX, INT_MAX, INT_MIN, INT_MIN);
+    for (int q = tid; q < npw; q += blockDim.x) {
+      int2 p = d.ppin[p0 + q];
+      box.x = min(p.x, box.x);
+      box.y = min(p.y, box.y);
+      box.z = max(p.x, box.z);
+      box.w = max(p.y, box.w);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      box.x = min(box.x, __shfl_xor(box.x, o, 64));
+      box.y = min(box.y, __shfl_xor(box.y, o, 64));
+      box.z = max(box.z, __shfl_xor(box.z, o, 64));
+      box.w = max(box.w, __shfl_xor(box.w, o, 64));
+    }
+    if ((tid & 63) == 0) s_box[tid >> 6] = box;
+    __syncthreads();
+    box = s_box[0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
+      int4 b = s_box[k];
+      box.x = min(b.x, box.x);
+      box.y = min(b.y, box.y);
+      box.z = max(b.z, box.z);
+      box.w = max(b.w, box.w);
     }
     WaSums w = {0, 0, 0, 0, 0, 0, 0, 0};
-    float4 xe[WA_CHUNK];  // the first chunk's exponentials, reused by pass 3
-#pragma unroll
-    for (int k = 0; k < WA_CHUNK; k++) {
-      xe[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (k < deg) wa_pin(pos[k], lx, ly, ux, uy, cx, cy, d.bar, w, &xe[k]);
+    for (int c0 = 0; c0 < npw; c0 += RP_WA_CAP) {
+      const int cnt = min(RP_WA_CAP, npw - c0);
+      __syncthreads();  // previous chunk consumed (and s_box read by everyone)
+      for (int j = tid; j < cnt; j += blockDim.x) {
+        int2 p = d.ppin[p0 + c0 + j];
+        s_pos[j] = p;
+        s_ex[j] = wa_exps(p, box, cx, cy, bar);
+      }
+      __syncthreads();
+      if (tid == 0)
+        for (int j = 0; j < cnt; j++) wa_accumulate(w, s_pos[j], s_ex[j]);
     }
-    for (int b = s + WA_CHUNK; b < t; b += WA_CHUNK) {
-      int2 pp[WA_CHUNK];
-#pragma unroll
-      for (int k = 0; k < WA_CHUNK; k++)
-        if (b + k < t) pp[k] = pin_pos(d, b + k);
-#pragma unroll
-      for (int k = 0; k < WA_CHUNK; k++)
-        if (b + k < t) wa_pin(pp[k], lx, ly, ux, uy, cx, cy, d.bar, w);
+    if (tid == 0) {
+      s_sum[0] = make_float4(w.s0, w.s1, w.s2, w.s3);
+      s_sum[1] = make_float4(w.s4, w.s5, w.s6, w.s7);
+      h = (long long)((box.z - box.x) + (box.w - box.y));
+      if (store_nets) {
+        d.nbox[e] = box;
+        d.nsum[2 * (size_t)e] = s_sum[0];
+        d.nsum[2 * (size_t)e + 1] = s_sum[1];
+      }
     }
-#pragma unroll
-    for (int k = 0; k < WA_CHUNK; k++)
-      if (k < deg)
-        d.pgrad[s + k] =
-            make_float2(wl_axis_x(pos[k].x, lx, ux, cx, xe[k].x, xe[k].y, w.s0, w.s1, w.s2, w.s3, d.bar),
-                        wl_axis_x(pos[k].y, ly, uy, cy, xe[k].z, xe[k].w, w.s4, w.s5, w.s6, w.s7, d.bar));
-    for (int b = s + WA_CHUNK; b < t; b += WA_CHUNK) {
-      int2 pp[WA_CHUNK];
-#pragma unroll
-      for (int k = 0; k < WA_CHUNK; k++)
-        if (b + k < t) pp[k] = pin_pos(d, b + k);
-#pragma unroll
-      for (int k = 0; k < WA_CHUNK; k++)
-        if (b + k < t)
-          d.pgrad[b + k] = make_float2(wl_axis(pp[k].x, lx, ux, cx, w.s0, w.s1, w.s2, w.s3, d.bar),
-                                       wl_axis(pp[k].y, ly, uy, cy, w.s4, w.s5, w.s6, w.s7, d.bar));
+    __syncthreads();
+    const float4 sa = s_sum[0], sb = s_sum[1];
+    for (int q = tid; q < npw; q += blockDim.x) {
+      int2 p = d.ppin[p0 + q];
+      d.pgrad[p0 + q] = wa_pin_grad(p, wa_exps(p, box, cx, cy, bar), sa, sb, cx, cy);
     }
-    if (store_nets) {
-      d.nbox[e] = make_int4(lx, ly, ux, uy);
-      d.nsum[2 * (size_t)e] = make_float4(w.s0, w.s1, w.s2, w.s3);
-      d.nsum[2 * (size_t)e + 1] = make_float4(w.s4, w.s5, w.s6, w.s7);
-    }
-    if (t > s) h = (long long)((ux - lx) + (uy - ly));
   }
   long long tot = block_sum_ll(h, sh_red);
   if (threadIdx.x == 0) d.part_net[blk] = tot;
@@ -1306,7 +1325,6 @@ static int fft_threads(int) { return RP_BLOCK; }
 hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_t* ev, const RpMfmaDev* mf,
                           const RpSortBufs* sb) {
   int nb_c = (d.nc + RP_BLOCK - 1) / RP_BLOCK;
-  int nb_n = (d.nn + RP_BLOCK - 1) / RP_BLOCK;
   int k = 0;
 #define RP_MARK()                            \
   do {                                       \
@@ -1346,7 +1364,7 @@ hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_
     hipLaunchKernelGGL(k_row_inverse, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shr, s, d, want_phi);
     RP_MARK();
   }
-  hipLaunchKernelGGL(k_wa, dim3(nb_n), dim3(RP_BLOCK), 0, s, d, 0);
+  hipLaunchKernelGGL(k_wa, dim3(std::max(1, d.nwa)), dim3(RP_BLOCK), 0, s, d, 0);
   RP_MARK();
   hipLaunchKernelGGL(k_grad, dim3(nb_c), dim3(RP_BLOCK), 0, s, d);
   RP_MARK();
@@ -1385,7 +1403,7 @@ hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int fr
 }
 
 hipError_t rp_launch_wa_grad(const RpDev& d, hipStream_t s, int store_nets) {
-  hipLaunchKernelGGL(k_wa, dim3((d.nn + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, d, store_nets);
+  hipLaunchKernelGGL(k_wa, dim3(std::max(1, d.nwa)), dim3(RP_BLOCK), 0, s, d, store_nets);
   hipLaunchKernelGGL(k_grad, dim3((d.nc + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, d);
   return hipGetLastError();
 }
