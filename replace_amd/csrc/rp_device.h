@@ -35,7 +35,7 @@
 #endif
 #define RP_SCATTER_CELLS (RP_BLOCK * RP_SCATTER_PER_THREAD)  // cells per fused step+scatter workgroup
 #ifndef RP_SS_PER_THREAD
-#define RP_SS_PER_THREAD 4
+#define RP_SS_PER_THREAD 1
 #endif
 #define RP_SS_CELLS (RP_BLOCK * RP_SS_PER_THREAD)  // cells per spatial-order scatter workgroup
 #define RP_MAX_FFT 2048

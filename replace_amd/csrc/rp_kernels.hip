@@ -241,9 +241,6 @@ __device__ __forceinline__ float2 dct3_pre(float a_k, float a_nk, float b_k, flo
 #ifndef RP_AGG_BITS
 #define RP_AGG_BITS 12  // fused step + scatter (batched instances)
 #endif
-#ifndef RP_SS_BITS
-#define RP_SS_BITS 11  // scatter in spatial order (single instance)
-#endif
 #ifndef RP_AGG_PROBES
 #define RP_AGG_PROBES 4
 #endif
@@ -283,6 +280,59 @@ __device__ __forceinline__ void agg_add(const RpDev& d, AggTable<BITS>& t, int b
 template <int BITS>
 __device__ __forceinline__ void agg_flush(const RpDev& d, const AggTable<BITS>& t) {
   for (int k = threadIdx.x; k < AggTable<BITS>::SLOTS; k += blockDim.x) {
+    int b = t.key[k];
+    if (b >= 0) {
+      if (t.inst[k]) atomicAdd(&d.bin_inst[b], t.inst[k]);
+      if (t.fill[k]) atomicAdd(&d.bin_filler[b], t.fill[k]);
+    }
+  }
+}
+
+// Window table for the spatially ordered scatter: bin (i, j) maps to slot
+// (j mod RP_WIN_Y) * RP_WIN_X + (i mod RP_WIN_X), a tiling of the plane, so
+// the compact region a workgroup's cells cover lands on distinct slots and
+// the flush, walking the slots in order, issues atomics to consecutive bins
+// of a row (coalesced).  A slot held by another bin is probed past in
+// steps of 7; a bin that finds no slot goes straight to global memory.
+#ifndef RP_WIN_X
+#define RP_WIN_X 32
+#endif
+#ifndef RP_WIN_Y
+#define RP_WIN_Y 32
+#endif
+struct WinTable {
+  static constexpr int SLOTS = RP_WIN_X * RP_WIN_Y;
+  int key[SLOTS];
+  unsigned long long inst[SLOTS];
+  unsigned long long fill[SLOTS];
+};
+
+__device__ __forceinline__ void win_clear(WinTable& t) {
+  for (int k = threadIdx.x; k < WinTable::SLOTS; k += blockDim.x) {
+    t.key[k] = -1;
+    t.inst[k] = 0;
+    t.fill[k] = 0;
+  }
+}
+
+__device__ __forceinline__ void win_add(const RpDev& d, WinTable& t, int i, int j, unsigned long long a, bool filler) {
+  const int b = j * d.nbx + i;
+  const int s0 = (j & (RP_WIN_Y - 1)) * RP_WIN_X + (i & (RP_WIN_X - 1));
+#pragma unroll
+  for (int q = 0; q < RP_AGG_PROBES; q++) {
+    int sl = (s0 + 7 * q) & (WinTable::SLOTS - 1);
+    int k = t.key[sl];
+    if (k == -1) k = atomicCAS(&t.key[sl], -1, b);  // claim (returns the holder if another lane won)
+    if (k == -1 || k == b) {
+      atomicAdd(filler ? &t.fill[sl] : &t.inst[sl], a);
+      return;
+    }
+  }
+  atomicAdd(filler ? &d.bin_filler[b] : &d.bin_inst[b], a);
+}
+
+__device__ __forceinline__ void win_flush(const RpDev& d, const WinTable& t) {
+  for (int k = threadIdx.x; k < WinTable::SLOTS; k += blockDim.x) {
     int b = t.key[k];
     if (b >= 0) {
       if (t.inst[k]) atomicAdd(&d.bin_inst[b], t.inst[k]);
@@ -353,6 +403,32 @@ __device__ __forceinline__ void box_scatter(const RpDev& d, AggTable<BITS>& t, i
   }
 }
 
+__device__ __forceinline__ void box_scatter_win(const RpDev& d, WinTable& t, int2 icc, int2 h, float sc,
+                                                uint8_t kd) {
+  int dlx = icc.x - h.x, dux = icc.x + h.x, dly = icc.y - h.y, duy = icc.y + h.y;
+  int x0, x1, y0, y1;
+  idx_range(dlx, dux, d.core_lx, d.bsx, d.nbx, x0, x1);
+  idx_range(dly, duy, d.core_ly, d.bsy, d.nby, y0, y1);
+  for (int i = x0; i < x1; i++) {
+    int blx, bux;
+    bin_x(d, i, blx, bux);
+    int w = min(bux, dux) - max(blx, dlx);
+    if (w <= 0) continue;
+    for (int j = y0; j < y1; j++) {
+      int bly, buy;
+      bin_y(d, j, bly, buy);
+      int hh = min(buy, duy) - max(bly, dly);
+      if (hh <= 0) continue;
+      float v = (float)w * (float)hh;
+      v = v * sc;
+      if (kd == 1) v = v * d.td;
+      unsigned long long a = (unsigned long long)(long long)v;
+      if (a == 0) continue;
+      win_add(d, t, i, j, a, kd == 2);
+    }
+  }
+}
+
 // Fused step + scatter over RP_SCATTER_CELLS consecutive cells (batched
 // instances: small grids, where the locality order keeps a workgroup's bins
 // together).
@@ -412,25 +488,41 @@ __device__ void step_body(const RpDev& d, int blk) {
 // new order.
 #define RP_TILE_SLOTS 256
 __device__ void sorted_scatter_body(const RpDev& d, int blk, const RpSortBufs& sb, int count) {
-  __shared__ AggTable<RP_SS_BITS> t;
+  __shared__ WinTable t;
   __shared__ int t_key[RP_TILE_SLOTS], t_cnt[RP_TILE_SLOTS], t_base[RP_TILE_SLOTS];
   const RpState* st = d.st;
   if (st->phase != 0) return;
-  agg_clear(t);
+  win_clear(t);
   for (int k = threadIdx.x; k < RP_TILE_SLOTS; k += blockDim.x) {
     t_key[k] = -1;
     t_cnt[k] = 0;
   }
   __syncthreads();
   int cell[RP_SS_PER_THREAD], slot[RP_SS_PER_THREAD], lrank[RP_SS_PER_THREAD];
+  // all loads before the first atomic (which could alias them)
+  int2 v_ic[RP_SS_PER_THREAD], v_h[RP_SS_PER_THREAD];
+  float v_sc[RP_SS_PER_THREAD];
+  uint8_t v_kd[RP_SS_PER_THREAD];
 #pragma unroll
   for (int r = 0; r < RP_SS_PER_THREAD; r++) {
-    cell[r] = -1;
     int k = blk * RP_SS_CELLS + r * RP_BLOCK + threadIdx.x;
-    if (k >= d.nc) continue;
-    int c = d.sord[k];
-    int2 icc = d.ic[c];
-    box_scatter(d, t, icc, d.dhalf[c], d.scale[c], d.kind[c]);
+    cell[r] = k < d.nc ? d.sord[k] : -1;
+  }
+#pragma unroll
+  for (int r = 0; r < RP_SS_PER_THREAD; r++) {
+    int c = cell[r];
+    if (c < 0) continue;
+    v_ic[r] = d.ic[c];
+    v_h[r] = d.dhalf[c];
+    v_sc[r] = d.scale[c];
+    v_kd[r] = d.kind[c];
+  }
+#pragma unroll
+  for (int r = 0; r < RP_SS_PER_THREAD; r++) {
+    int c = cell[r];
+    if (c < 0) continue;
+    int2 icc = v_ic[r];
+    box_scatter_win(d, t, icc, v_h[r], v_sc[r], v_kd[r]);
     if (count) {
       cell[r] = c;
       slot[r] = -1;
@@ -449,7 +541,7 @@ __device__ void sorted_scatter_body(const RpDev& d, int blk, const RpSortBufs& s
     }
   }
   __syncthreads();
-  agg_flush(d, t);
+  win_flush(d, t);
   if (count) {
     for (int k = threadIdx.x; k < RP_TILE_SLOTS; k += blockDim.x)
       if (t_key[k] >= 0) t_base[k] = atomicAdd(&sb.cnt[t_key[k]], t_cnt[k]);
@@ -492,35 +584,52 @@ __device__ void binfin_body(const RpDev& d, int blk, int from_rho) {
   int ja = 2 * blk, jb = ja + 1;
   double ov_sum = 0.0;
   if (!d.mfma) load_twiddles(T, d.twx, n);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    int blx, bux;
-    bin_x(d, i, blx, bux);
-    float r2[2];
-    for (int r = 0; r < 2; r++) {
-      int j = ja + r;
-      int bly, buy;
-      bin_y(d, j, bly, buy);
-      size_t b = (size_t)j * n + i;
-      if (from_rho) {  // Poisson-only entry: density given
-        r2[r] = d.rho[b];
-        continue;
-      }
-      long long inst = (long long)d.bin_inst[b];
-      long long fil = (long long)d.bin_filler[b];
-      long long np = d.bin_nonplace[b];
-      d.bin_inst[b] = 0ull;
-      d.bin_filler[b] = 0ull;
-      long long bin_area = (long long)(bux - blx) * (long long)(buy - bly);
-      float at = (float)bin_area * d.td;
-      // src/nesterovBase.cpp:806-816
-      float rho = ((float)inst + (float)fil + (float)np) / at;
-      float ov = (float)inst + (float)np - at;
-      ov = (0.0f < ov) ? ov : 0.0f;
-      ov_sum += (double)ov;
-      d.rho[b] = rho;
-      r2[r] = rho;
+  if (from_rho) {  // Poisson-only entry: density given
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+      A[mk_pos(i, n)] = make_float2(d.rho[(size_t)ja * n + i], d.rho[(size_t)jb * n + i]);
+  } else {
+    // every load of the thread's bins is issued before any store (the zeroed
+    // accumulators could otherwise alias later loads and serialise them)
+    constexpr int MAXC = RP_MAX_FFT / RP_BLOCK;
+    long long v_in[2][MAXC], v_fi[2][MAXC], v_np[2][MAXC];
+#pragma unroll
+    for (int t = 0; t < MAXC; t++) {
+      int i = threadIdx.x + t * blockDim.x;
+      if (i < n)
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+          size_t b = (size_t)(ja + r) * n + i;
+          v_in[r][t] = (long long)d.bin_inst[b];
+          v_fi[r][t] = (long long)d.bin_filler[b];
+          v_np[r][t] = d.bin_nonplace[b];
+        }
     }
-    A[mk_pos(i, n)] = make_float2(r2[0], r2[1]);
+#pragma unroll
+    for (int t = 0; t < MAXC; t++) {
+      int i = threadIdx.x + t * blockDim.x;
+      if (i >= n) continue;
+      int blx, bux;
+      bin_x(d, i, blx, bux);
+      float r2[2];
+#pragma unroll
+      for (int r = 0; r < 2; r++) {
+        int bly, buy;
+        bin_y(d, ja + r, bly, buy);
+        size_t b = (size_t)(ja + r) * n + i;
+        long long bin_area = (long long)(bux - blx) * (long long)(buy - bly);
+        float at = (float)bin_area * d.td;
+        // src/nesterovBase.cpp:806-816
+        float rho = ((float)v_in[r][t] + (float)v_fi[r][t] + (float)v_np[r][t]) / at;
+        float ov = (float)v_in[r][t] + (float)v_np[r][t] - at;
+        ov = (0.0f < ov) ? ov : 0.0f;
+        ov_sum += (double)ov;
+        d.rho[b] = rho;
+        d.bin_inst[b] = 0ull;
+        d.bin_filler[b] = 0ull;
+        r2[r] = rho;
+      }
+      A[mk_pos(i, n)] = make_float2(r2[0], r2[1]);
+    }
   }
   double tot = block_sum_d(ov_sum, sh_red);
   if (threadIdx.x == 0) d.part_bin[blk] = tot;
