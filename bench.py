@@ -36,9 +36,10 @@ def algorithmic_bytes(nc, npin, nn, nb, sort_bits=16):
       step           56 B/cell (read cur SLP, gradient, coordinate, half
                      size; write next coordinate, next SLP, integer centre)
       scatter        25 B/cell (order index, integer centre, half size,
-                     scale, kind) + 16 B/bin (two int64 accumulators)
-      pin_pos        28 B/pin (cell id, offset, gathered centre; write the
-                     pin centre)
+                     scale, kind) + 16 B/bin (two int64 accumulators), and in
+                     the same launch the pin centres: 28 B/pin (cell id,
+                     offset, gathered centre; write the pin centre)
+      pin_pos        the pin centres alone (batched launches; 0 here)
       binfin_rowdct  48 B/bin (read inst, filler, nonPlace int64; write rho,
                      the row transform, and the zeroed accumulators)
       transpose       8 B/bin; col_spectral 12 B/bin; transpose2 16 B/bin;
@@ -53,7 +54,7 @@ def algorithmic_bytes(nc, npin, nn, nb, sort_bits=16):
     """
     return {
         "step": 56 * nc,
-        "scatter": 25 * nc + 16 * nb,
+        "scatter": 25 * nc + 16 * nb + 28 * npin,
         "pin_pos": 28 * npin,
         "binfin_rowdct": 48 * nb,
         "transpose": 8 * nb,

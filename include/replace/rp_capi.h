@@ -194,13 +194,14 @@ const char* rp_last_error(rp_ctx* ctx);
  * accumulates the elapsed time (enabling clears the totals, disabling keeps
  * them; the events themselves slow the run, so time whole placements with
  * timing off).  rp_get_kernel_timing fills 12 entries:
- * for the 11 kernels of a slot in launch order (step, density scatter, pin
- * centres, bin finalize + row DCT, transpose, column spectral pass,
- * transpose, row inverse, WA, gradient, controller) the summed milliseconds,
- * then the summed milliseconds of the spatial-order refresh that runs once
- * per batch of queued slots; *launches is the number of executed slots.
- * With the matrix-core Poisson transform its whole solve is reported in the
- * first transpose slot and the next three stay empty. */
+ * for the 11 kernel slots in launch order (step, density scatter together
+ * with the pin centres, pin centres alone (batched launches only: 0 here),
+ * bin finalize + row DCT, transpose, column spectral pass, transpose, row
+ * inverse, WA, gradient, controller) the summed milliseconds, then the
+ * summed milliseconds of the spatial-order refresh that runs once per batch
+ * of queued slots; *launches is the number of executed slots.  With the
+ * matrix-core Poisson transform its whole solve is reported in the first
+ * transpose slot and the next three stay empty. */
 int rp_set_kernel_timing(rp_ctx* ctx, int enable);
 int rp_get_kernel_timing(rp_ctx* ctx, double* total_ms_per_kernel, int64_t* launches);
 

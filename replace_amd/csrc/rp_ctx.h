@@ -198,7 +198,7 @@ int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve);
 
 // launch wrappers (rp_kernels.hip)
 hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_t* ev, const RpMfmaDev* mf,
-                          const RpSortBufs* sb = nullptr);
+                          const RpSortBufs* sb = nullptr, unsigned* launched = nullptr);
 hipError_t rp_launch_ctrl_init(const RpDev& d, hipStream_t s, int stage);
 hipError_t rp_launch_scatter(const RpDev& d, hipStream_t s);
 hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int from_rho, const RpMfmaDev* mf);

@@ -1293,11 +1293,13 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
   HIPCHK(c, hipEventCreateWithFlags(&polled, hipEventDisableTiming));
   int max_batches = (c->prm.nesterov_place_max_iter * std::max(1, c->prm.max_back_track)) / K + 4;
   std::vector<hipEvent_t> kev;
+  constexpr int EV = 2 * RP_NUM_EVAL_KERNELS;  // events per slot
   if (c->timing) {
-    kev.resize((size_t)K * (RP_NUM_EVAL_KERNELS + 1) + 2);  // + the sort refresh bracket
+    kev.resize((size_t)K * EV + 2);  // + the sort refresh bracket
     for (auto& e : kev) HIPCHK(c, hipEventCreate(&e));
   }
-  hipEvent_t* sev = c->timing ? &kev[(size_t)K * (RP_NUM_EVAL_KERNELS + 1)] : nullptr;
+  hipEvent_t* sev = c->timing ? &kev[(size_t)K * EV] : nullptr;
+  unsigned launched = 0;
   HIPCHK(c, hipMemcpyAsync(hs, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
   int evals_before = hs->evals;
@@ -1310,8 +1312,8 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
     if (b > 0) HIPCHK(c, rp_launch_sort_refresh(d, c->sortb, s));
     if (sev) HIPCHK(c, hipEventRecord(sev[1], s));
     for (int k = 0; k < K; k++)
-      HIPCHK(c, rp_launch_eval(d, s, 0, c->timing ? &kev[(size_t)k * (RP_NUM_EVAL_KERNELS + 1)] : nullptr, c->mf(),
-                               k == K - 1 ? &c->sortb : nullptr));
+      HIPCHK(c, rp_launch_eval(d, s, 0, c->timing ? &kev[(size_t)k * EV] : nullptr, c->mf(),
+                               k == K - 1 ? &c->sortb : nullptr, &launched));
     HIPCHK(c, hipMemcpyAsync(hs, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipEventRecord(polled, s));
     HIPCHK(c, hipEventSynchronize(polled));
@@ -1320,9 +1322,10 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
       int ran = std::min(K, hs->evals - evals_before);
       for (int k = 0; k < ran; k++) {
         for (int q = 0; q < RP_NUM_EVAL_KERNELS; q++) {
+          if (!(launched >> q & 1)) continue;
           float ms = 0;
-          hipEvent_t* ev = &kev[(size_t)k * (RP_NUM_EVAL_KERNELS + 1)];
-          HIPCHK(c, hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+          hipEvent_t* ev = &kev[(size_t)k * EV];
+          HIPCHK(c, hipEventElapsedTime(&ms, ev[2 * q], ev[2 * q + 1]));
           c->ktime_ms[q] += ms;
         }
       }

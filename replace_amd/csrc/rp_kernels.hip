@@ -4,8 +4,7 @@
 // One evaluation (reference src/nesterovPlace.cpp:338-377) is the launch
 // sequence
 //   k_step              step + clamp + int centre (A14, A5)
-//   k_scatter           density scatter in spatial cell order (A6)
-//   k_pin_pos           pin centres (A5)
+//   k_scatter_pins      density scatter in spatial cell order (A6) and pin centres (A5)
 //   k_binfin_rowdct     bin density, overflow partials, DCT-II along x (A6 tail, A8)
 //   k_transpose         (Ny x Nx) -> (Nx x Ny)
 //   k_col_spectral      DCT-II along y, spectral divide, DCT-III / DST-III along y (A8)
@@ -1113,15 +1112,38 @@ __device__ void grad_body(const RpDev& d, int blk) {
 // ---------------------------------------------------------------------------
 // Fixed-order reduction of `n` partials with stride `stride` (1 workgroup).
 // Fixed-order reductions of the per-block partials (one workgroup).
+// Each thread sums a fixed strided subset in index order; the loads of
+// RP_RED_BATCH iterations are issued before their additions (the sum order
+// is unchanged) so the loop does not wait on one load at a time.
+#define RP_RED_BATCH 8
 __device__ double reduce_parts_d(const double* p, int n, int stride, int off, double* sh) {
   double v = 0.0;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) v += p[(size_t)k * stride + off];
+  for (int k0 = threadIdx.x; k0 < n; k0 += RP_RED_BATCH * blockDim.x) {
+    double x[RP_RED_BATCH];
+#pragma unroll
+    for (int u = 0; u < RP_RED_BATCH; u++) {
+      int k = k0 + u * blockDim.x;
+      x[u] = k < n ? p[(size_t)k * stride + off] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < RP_RED_BATCH; u++)
+      if (k0 + u * (int)blockDim.x < n) v += x[u];
+  }
   return block_sum_d(v, sh);
 }
 
 __device__ long long reduce_parts_ll(const long long* p, int n, long long* sh) {
   long long v = 0;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) v += p[k];
+  for (int k0 = threadIdx.x; k0 < n; k0 += RP_RED_BATCH * blockDim.x) {
+    long long x[RP_RED_BATCH];
+#pragma unroll
+    for (int u = 0; u < RP_RED_BATCH; u++) {
+      int k = k0 + u * blockDim.x;
+      x[u] = k < n ? p[k] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < RP_RED_BATCH; u++) v += x[u];
+  }
   return block_sum_ll(v, sh);
 }
 
@@ -1137,10 +1159,21 @@ __device__ void reduce_all(const RpDev& d, Totals& t) {
   // one pass over the gradient partials: each thread sums a fixed subset
   double v[RP_P_COUNT] = {0, 0, 0, 0, 0};
   const int ng = st->num_partials_grad;
-  for (int k = threadIdx.x; k < ng; k += blockDim.x) {
-    const double* q = d.part_grad + (size_t)k * RP_P_COUNT;
+  constexpr int GB = 4;  // iterations whose loads are issued together
+  for (int k0 = threadIdx.x; k0 < ng; k0 += GB * blockDim.x) {
+    double x[GB][RP_P_COUNT];
 #pragma unroll
-    for (int j = 0; j < RP_P_COUNT; j++) v[j] += q[j];
+    for (int u = 0; u < GB; u++) {
+      int k = k0 + u * blockDim.x;
+      const double* q = d.part_grad + (size_t)k * RP_P_COUNT;
+#pragma unroll
+      for (int j = 0; j < RP_P_COUNT; j++) x[u][j] = k < ng ? q[j] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < GB; u++)
+      if (k0 + u * (int)blockDim.x < ng)
+#pragma unroll
+        for (int j = 0; j < RP_P_COUNT; j++) v[j] += x[u][j];
   }
   t.wl = block_sum_d(v[RP_P_WL], shd);
   t.dens = block_sum_d(v[RP_P_DENS], shd);
@@ -1343,9 +1376,21 @@ __device__ void control_main_body(const RpDev& d) {
 // its result is bit-identical alone or in a batch.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(RP_BLOCK) k_step(RpDev d) { step_body(d, xcd_block(blockIdx.x, gridDim.x)); }
-__global__ void __launch_bounds__(RP_BLOCK) k_scatter(RpDev d, RpSortBufs sb, int count) {
-  sorted_scatter_body(d, xcd_block(blockIdx.x, gridDim.x), sb, count);
+// Density scatter (blocks [0, nss)) and pin centres (the rest) in one
+// launch: both read only the integer centres the step wrote.  nss is a
+// multiple of the XCD count, so both ranges keep the XCD-contiguous order.
+__global__ void __launch_bounds__(RP_BLOCK) k_scatter_pins(RpDev d, RpSortBufs sb, int count, int nss) {
+  const int b = blockIdx.x;
+  if (b < nss)
+    sorted_scatter_body(d, xcd_block(b, nss), sb, count);
+  else
+    pinpos_body(d, xcd_block(b - nss, gridDim.x - nss));
 }
+static int scatter_pins_blocks(const RpDev& d) {
+  int n = (d.nc + RP_SS_CELLS - 1) / RP_SS_CELLS;
+  return (n + RP_NUM_XCD - 1) / RP_NUM_XCD * RP_NUM_XCD;
+}
+static int pin_blocks(const RpDev& d) { return std::max(1, (d.np + RP_BLOCK - 1) / RP_BLOCK); }
 __global__ void __launch_bounds__(RP_BLOCK) k_pin_pos(RpDev d) { pinpos_body(d, xcd_block(blockIdx.x, gridDim.x)); }
 __global__ void __launch_bounds__(RP_BLOCK) k_binfin_rowdct(RpDev d, int from_rho) {
   binfin_body(d, blockIdx.x, from_rho);
@@ -1429,57 +1474,63 @@ __global__ void __launch_bounds__(RP_CTRL_BLOCK) kb_control_main(const RpDev* de
 // sums (overflow) then group identically in single and batched launches.
 static int fft_threads(int) { return RP_BLOCK; }
 
-// One evaluation slot.  ev (optional) holds RP_NUM_EVAL_KERNELS + 1 events
-// recorded around each kernel (per-kernel timing on this stream).
+// One evaluation slot.  ev (optional) holds 2 * RP_NUM_EVAL_KERNELS events:
+// ev[2q] and ev[2q + 1] bracket kernel q when it is launched on this path;
+// *launched (optional) receives the mask of the kernels that were.
 hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_t* ev, const RpMfmaDev* mf,
-                          const RpSortBufs* sb) {
-  int nb_c = (d.nc + RP_BLOCK - 1) / RP_BLOCK;
-  int k = 0;
-#define RP_MARK()                            \
-  do {                                       \
-    if (ev) (void)hipEventRecord(ev[k], s);  \
-    k++;                                     \
-  } while (0)
-  RP_MARK();
+                          const RpSortBufs* sb, unsigned* launched) {
+  const int nb_c = (d.nc + RP_BLOCK - 1) / RP_BLOCK;
+  unsigned mask = 0;
+  auto beg = [&](int q) {
+    if (ev) (void)hipEventRecord(ev[2 * q], s);
+    mask |= 1u << q;
+  };
+  auto end = [&](int q) {
+    if (ev) (void)hipEventRecord(ev[2 * q + 1], s);
+  };
+  beg(RP_K_STEP);
   hipLaunchKernelGGL(k_step, dim3(nb_c), dim3(RP_BLOCK), 0, s, d);
-  RP_MARK();
-  hipLaunchKernelGGL(k_scatter, dim3((d.nc + RP_SS_CELLS - 1) / RP_SS_CELLS), dim3(RP_BLOCK), 0, s, d,
-                     sb ? *sb : RpSortBufs{}, sb ? 1 : 0);
-  RP_MARK();
-  hipLaunchKernelGGL(k_pin_pos, dim3(std::max(1, (d.np + RP_BLOCK - 1) / RP_BLOCK)), dim3(RP_BLOCK), 0, s, d);
-  RP_MARK();
-  size_t shx = rp_lds_binfin(d.nbx);
-  hipLaunchKernelGGL(k_binfin_rowdct, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shx, s, d, 0);
-  RP_MARK();
+  end(RP_K_STEP);
+  beg(RP_K_SCATTER);
+  const int nss = scatter_pins_blocks(d);
+  hipLaunchKernelGGL(k_scatter_pins, dim3(nss + pin_blocks(d)), dim3(RP_BLOCK), 0, s, d, sb ? *sb : RpSortBufs{},
+                     sb ? 1 : 0, nss);
+  end(RP_K_SCATTER);
+  beg(RP_K_BINFIN_ROWDCT);
+  hipLaunchKernelGGL(k_binfin_rowdct, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), rp_lds_binfin(d.nbx), s, d, 0);
+  end(RP_K_BINFIN_ROWDCT);
   if (mf) {
-    // matrix-core solve; its time is reported in the transpose slot and the
-    // remaining Poisson slots stay empty
+    // matrix-core solve, reported in the transpose slot
+    beg(RP_K_TRANSPOSE);
     hipError_t e = rp_launch_poisson_mfma(d, *mf, s, want_phi);
     if (e != hipSuccess) return e;
-    RP_MARK();
-    RP_MARK();
-    RP_MARK();
-    RP_MARK();
+    end(RP_K_TRANSPOSE);
   } else {
+    beg(RP_K_TRANSPOSE);
     hipLaunchKernelGGL(k_transpose, dim3((d.nbx + 31) / 32, (d.nby + 31) / 32), dim3(32, 8), 0, s,
                        (const float*)d.t, d.tt, d.nby, d.nbx, (const RpState*)d.st);
-    RP_MARK();
-    size_t shy = rp_lds_col(d.nby);
-    hipLaunchKernelGGL(k_col_spectral, dim3(d.nbx / 2), dim3(fft_threads(d.nby)), shy, s, d);
-    RP_MARK();
+    end(RP_K_TRANSPOSE);
+    beg(RP_K_COL_SPECTRAL);
+    hipLaunchKernelGGL(k_col_spectral, dim3(d.nbx / 2), dim3(fft_threads(d.nby)), rp_lds_col(d.nby), s, d);
+    end(RP_K_COL_SPECTRAL);
+    beg(RP_K_TRANSPOSE2);
     hipLaunchKernelGGL(k_transpose2, dim3((d.nby + 31) / 32, (d.nbx + 31) / 32, 2), dim3(32, 8), 0, s, d);
-    RP_MARK();
-    size_t shr = rp_lds_rinv(d.nbx);
-    hipLaunchKernelGGL(k_row_inverse, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shr, s, d, want_phi);
-    RP_MARK();
+    end(RP_K_TRANSPOSE2);
+    beg(RP_K_ROW_INVERSE);
+    hipLaunchKernelGGL(k_row_inverse, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), rp_lds_rinv(d.nbx), s, d,
+                       want_phi);
+    end(RP_K_ROW_INVERSE);
   }
+  beg(RP_K_WA);
   hipLaunchKernelGGL(k_wa, dim3(std::max(1, d.nwa)), dim3(RP_BLOCK), 0, s, d, 0);
-  RP_MARK();
+  end(RP_K_WA);
+  beg(RP_K_GRAD);
   hipLaunchKernelGGL(k_grad, dim3(nb_c), dim3(RP_BLOCK), 0, s, d);
-  RP_MARK();
+  end(RP_K_GRAD);
+  beg(RP_K_CONTROL);
   hipLaunchKernelGGL(k_control_main, dim3(1), dim3(RP_CTRL_BLOCK), 0, s, d);
-  RP_MARK();
-#undef RP_MARK
+  end(RP_K_CONTROL);
+  if (launched) *launched = mask;
   return hipGetLastError();
 }
 
@@ -1491,9 +1542,8 @@ hipError_t rp_launch_ctrl_init(const RpDev& d, hipStream_t s, int stage) {
 // Split stages for the stage-level test entry points.
 hipError_t rp_launch_scatter(const RpDev& d, hipStream_t s) {
   hipLaunchKernelGGL(k_step, dim3((d.nc + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, d);
-  hipLaunchKernelGGL(k_scatter, dim3((d.nc + RP_SS_CELLS - 1) / RP_SS_CELLS), dim3(RP_BLOCK), 0, s, d,
-                     RpSortBufs{}, 0);
-  hipLaunchKernelGGL(k_pin_pos, dim3(std::max(1, (d.np + RP_BLOCK - 1) / RP_BLOCK)), dim3(RP_BLOCK), 0, s, d);
+  const int nss = scatter_pins_blocks(d);
+  hipLaunchKernelGGL(k_scatter_pins, dim3(nss + pin_blocks(d)), dim3(RP_BLOCK), 0, s, d, RpSortBufs{}, 0, nss);
   return hipGetLastError();
 }
 
