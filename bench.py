@@ -42,8 +42,9 @@ def algorithmic_bytes(nc, npin, nn, nb, sort_bits=16):
       pin_pos        the pin centres alone (batched launches; 0 here)
       binfin_rowdct  48 B/bin (read inst, filler, nonPlace int64; write rho,
                      the row transform, and the zeroed accumulators)
-      transpose       8 B/bin; col_spectral 12 B/bin; transpose2 16 B/bin;
+      col_spectral   12 B/bin (read the row transform, write Y1, Y2)
       row_inverse    16 B/bin (read Y1, Y2; write Ex, Ey)
+      mfma_poisson   the matrix-core solve (flops, see the precision leg)
       wa             16 B/pin (pin centre, per-pin gradient) + 4 B/net
       grad           64 B/cell + 12 B/pin (pin index, per-pin gradient)
                      + 8 B/bin (field)
@@ -57,9 +58,8 @@ def algorithmic_bytes(nc, npin, nn, nb, sort_bits=16):
         "scatter": 25 * nc + 16 * nb + 28 * npin,
         "pin_pos": 28 * npin,
         "binfin_rowdct": 48 * nb,
-        "transpose": 8 * nb,
+        "mfma_poisson": 0,
         "col_spectral": 12 * nb,
-        "transpose2": 16 * nb,
         "row_inverse": 16 * nb,
         "wa": 16 * npin + 4 * nn,
         "grad": 64 * nc + 12 * npin + 8 * nb,
@@ -285,7 +285,7 @@ def bench_precision(args):
         g.close()
     N = 2048
     flops = 3 * 2 * 12 * N ** 3  # 5 GEMMs without phi: 12 N^3 multiply-adds, 3 split products
-    poisson_us = out[1]["kernels_us"]["transpose"]  # the matrix-core solve's timing slot
+    poisson_us = out[1]["kernels_us"]["mfma_poisson"]
     achieved = flops / (poisson_us * 1e-6) / 1e12
     res = {
         "metric": "s to overflow<=0.10 at 1M cells, 2048x2048 bins (matrix-core split-BF16 Poisson)",
@@ -373,7 +373,7 @@ def main():
             kernels[k] = {"avg_us_per_slot": round(avg * 1e3, 2), "total_ms": round(tot, 2)}
             continue
         kernels[k] = {"avg_us": round(avg * 1e3, 2), "launches": n,
-                      "GBs": round(abytes[k] / (avg * 1e-3) / 1e9, 1) if avg > 0 else None}
+                      "GBs": round(abytes[k] / (avg * 1e-3) / 1e9, 1) if avg > 0 and abytes[k] else None}
     dom = max((k for k in kt if k != "sort_refresh"), key=lambda k: kt[k][0])
     avg_ms = kt[dom][0] / max(1, kt[dom][1])
     achieved = abytes[dom] / (avg_ms * 1e-3) / 1e9

@@ -193,15 +193,15 @@ const char* rp_last_error(rp_ctx* ctx);
  * kernel of every executed slot with HIP events on the context's stream and
  * accumulates the elapsed time (enabling clears the totals, disabling keeps
  * them; the events themselves slow the run, so time whole placements with
- * timing off).  rp_get_kernel_timing fills 12 entries:
- * for the 11 kernel slots in launch order (step, density scatter together
+ * timing off).  rp_get_kernel_timing fills 11 entries:
+ * for the 10 kernel slots in launch order (step, density scatter together
  * with the pin centres, pin centres alone (batched launches only: 0 here),
- * bin finalize + row DCT, transpose, column spectral pass, transpose, row
- * inverse, WA, gradient, controller) the summed milliseconds, then the
- * summed milliseconds of the spatial-order refresh that runs once per batch
- * of queued slots; *launches is the number of executed slots.  With the
- * matrix-core Poisson transform its whole solve is reported in the first
- * transpose slot and the next three stay empty. */
+ * bin finalize + row DCT, matrix-core Poisson solve (0 with the FFT
+ * transform), column spectral pass, row inverse (both 0 with the
+ * matrix-core transform), WA, gradient, controller) the summed
+ * milliseconds, then the summed milliseconds of the spatial-order refresh
+ * that runs once per batch of queued slots; *launches is the number of
+ * executed slots. */
 int rp_set_kernel_timing(rp_ctx* ctx, int enable);
 int rp_get_kernel_timing(rp_ctx* ctx, double* total_ms_per_kernel, int64_t* launches);
 

@@ -187,8 +187,8 @@ class Placer:
         return [(r.overflow, r.hpwl, r.step_length, r.density_penalty, r.num_back_track)
                 for r in arr[: cnt.value]]
 
-    KERNELS = ("step", "scatter", "pin_pos", "binfin_rowdct", "transpose", "col_spectral", "transpose2",
-               "row_inverse", "wa", "grad", "control", "sort_refresh")
+    KERNELS = ("step", "scatter", "pin_pos", "binfin_rowdct", "mfma_poisson", "col_spectral", "row_inverse",
+               "wa", "grad", "control", "sort_refresh")
 
     def set_kernel_timing(self, on: bool) -> None:
         self._check(lib().rp_set_kernel_timing(self.h, 1 if on else 0))

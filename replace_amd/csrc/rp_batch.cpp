@@ -247,14 +247,12 @@ static int rebuild(rp_server* s) {
   for (int i = 0; i < n; i++) {
     const RpDev& d = s->jobs[s->running[i]]->ctx->dev;
     devs[i] = d;
-    int tx = (d.nbx + 31) / 32, ty = (d.nby + 31) / 32;
     cnt[RP_K_STEP].push_back(0);  // the batched scatter kernel steps its own cells
     cnt[RP_K_SCATTER].push_back((d.nc + RP_SCATTER_CELLS - 1) / RP_SCATTER_CELLS);
     cnt[RP_K_PIN_POS].push_back((d.np + RP_BLOCK - 1) / RP_BLOCK);
     cnt[RP_K_BINFIN_ROWDCT].push_back(d.nby / 2);
-    cnt[RP_K_TRANSPOSE].push_back(tx * ty);
+    cnt[RP_K_MFMA_POISSON].push_back(0);  // batched instances use the FFT solve
     cnt[RP_K_COL_SPECTRAL].push_back(d.nbx / 2);
-    cnt[RP_K_TRANSPOSE2].push_back(2 * tx * ty);
     cnt[RP_K_ROW_INVERSE].push_back(d.nby / 2);
     cnt[RP_K_WA].push_back(d.nwa);
     cnt[RP_K_GRAD].push_back((d.nc + RP_BLOCK - 1) / RP_BLOCK);

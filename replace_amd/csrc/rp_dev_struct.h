@@ -42,9 +42,6 @@ struct RpDev {
   float2* field;
   float* phi;
   float* t;
-  float* tt;
-  float* y1t;
-  float* y2t;
   float* y1;
   float* y2;
   const float2* twx;   // e^{-2 pi i k / Nx}, k < Nx/2

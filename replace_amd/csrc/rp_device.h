@@ -23,7 +23,7 @@
 //   bins     (B = Nx*Ny, y-major j*Nx+i as BinGrid::bins_)
 //     inst, filler, nonplace  int64 (scatter targets, unit DBU^2)
 //     rho    float, field float2 (Ex, Ey), phi float (tests only)
-//     FFT work arrays: t (Ny x Nx), tt (Nx x Ny), y1t/y2t (Nx x Ny), y1/y2 (Ny x Nx)
+//     FFT work arrays (Ny x Nx, row-major): t (row DCT), y1 / y2 (column pass)
 #pragma once
 
 #include <stdint.h>
@@ -50,9 +50,8 @@ enum {
   RP_K_SCATTER,   // density scatter (single instance: spatial cell order)
   RP_K_PIN_POS,
   RP_K_BINFIN_ROWDCT,
-  RP_K_TRANSPOSE,
+  RP_K_MFMA_POISSON,  // matrix-core Poisson solve (replaces the next two)
   RP_K_COL_SPECTRAL,
-  RP_K_TRANSPOSE2,
   RP_K_ROW_INVERSE,
   RP_K_WA,
   RP_K_GRAD,

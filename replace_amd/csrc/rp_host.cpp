@@ -876,9 +876,6 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   HIPCHK(c, dalloc(c->d_phi, nb));
   HIPCHK(c, dalloc(c->d_field, nb));
   HIPCHK(c, dalloc(c->d_t, nb));
-  HIPCHK(c, dalloc(c->d_tt, nb));
-  HIPCHK(c, dalloc(c->d_y1t, nb));
-  HIPCHK(c, dalloc(c->d_y2t, nb));
   HIPCHK(c, dalloc(c->d_y1, nb));
   HIPCHK(c, dalloc(c->d_y2, nb));
   HIPCHK(c, dupload(c->d_twx, twx, s));
@@ -944,9 +941,6 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   d.field = c->d_field.p;
   d.phi = c->d_phi.p;
   d.t = c->d_t.p;
-  d.tt = c->d_tt.p;
-  d.y1t = c->d_y1t.p;
-  d.y2t = c->d_y2t.p;
   d.y1 = c->d_y1.p;
   d.y2 = c->d_y2.p;
   d.twx = c->d_twx.p;

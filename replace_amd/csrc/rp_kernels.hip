@@ -6,9 +6,7 @@
 //   k_step              step + clamp + int centre (A14, A5)
 //   k_scatter_pins      density scatter in spatial cell order (A6) and pin centres (A5)
 //   k_binfin_rowdct     bin density, overflow partials, DCT-II along x (A6 tail, A8)
-//   k_transpose         (Ny x Nx) -> (Nx x Ny)
 //   k_col_spectral      DCT-II along y, spectral divide, DCT-III / DST-III along y (A8)
-//   k_transpose2        both y-inverse results back to (Ny x Nx)
 //   k_row_inverse       DST-III / DCT-III along x -> field (Ex, Ey) (A8/A9)
 //   k_wa                net bbox, WA exponent sums, HPWL partials (A10, A16)
 //   k_grad              WL gradient, field gather, preconditioner, partial sums (A11-A13, A15)
@@ -644,30 +642,9 @@ __device__ void binfin_body(const RpDev& d, int blk, int from_rho) {
 }
 
 // ---------------------------------------------------------------------------
-// 3. tiled transposes
-// ---------------------------------------------------------------------------
-// 32x32 tile (tx0, ty0) of a rows x cols -> cols x rows transpose
-__device__ void transpose_body(const float* __restrict__ src, float* __restrict__ dst, int rows, int cols,
-                               const RpState* st, int tile_x, int tile_y) {
-  __shared__ float tile[32][33];
-  if (st->phase != 0) return;
-  int bx = tile_x * 32, by = tile_y * 32;
-  int tx = threadIdx.x, ty = threadIdx.y;
-  for (int k = 0; k < 32; k += 8) {
-    int r = by + ty + k, c = bx + tx;
-    if (r < rows && c < cols) tile[ty + k][tx] = src[(size_t)r * cols + c];
-  }
-  __syncthreads();
-  for (int k = 0; k < 32; k += 8) {
-    int r = bx + ty + k, c = by + tx;
-    if (r < cols && c < rows) dst[(size_t)r * rows + c] = tile[tx][ty + k];
-  }
-}
-
-// ---------------------------------------------------------------------------
-// 4. column pass on the transposed spectrum rows (u fixed, v along the row):
-// DCT-II along y, FFT::doFFT's scaling and spectral divide
-// (src/fft.cpp:117-166), then DCT-III (phi) and DST-III (Ey) along y.
+// 4. column pass (u fixed, v along the column), read from and written to
+// the row-major grids: DCT-II along y, FFT::doFFT's scaling and spectral
+// divide (src/fft.cpp:117-166), then DCT-III (phi) and DST-III (Ey) along y.
 // ---------------------------------------------------------------------------
 __device__ void col_body(const RpDev& d, int blk) {
   extern __shared__ float2 smem[];
@@ -681,9 +658,12 @@ __device__ void col_body(const RpDev& d, int blk) {
   float* ea = pb + n;
   float* eb = ea + n;
   int ua = 2 * blk, ub = ua + 1;
+  const size_t nx = (size_t)d.nbx;
   load_twiddles(T, d.twy, n);
+  // columns ua, ub of the row-transformed grid t (row-major): one 8-byte load
+  // per row (the workgroups sharing a cache line run on one XCD)
   for (int v = threadIdx.x; v < n; v += blockDim.x)
-    A[mk_pos(v, n)] = make_float2(d.tt[(size_t)ua * n + v], d.tt[(size_t)ub * n + v]);
+    A[mk_pos(v, n)] = *(const float2*)(d.t + v * nx + ua);
   __syncthreads();
   float2* Z = lds_fft(A, B, n, T, -1);
   const float scale4 = (float)(4.0 / (double)d.nbx / (double)n);
@@ -709,7 +689,7 @@ __device__ void col_body(const RpDev& d, int blk) {
     }
   }
   __syncthreads();
-  // DCT-III of phi hat (rows a, b) -> y1t
+  // DCT-III of phi hat (columns a, b) -> y1
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     int nk = (n - k) & (n - 1);
     float ank = k ? pa[nk] : 0.0f, bnk = k ? pb[nk] : 0.0f;
@@ -720,8 +700,7 @@ __device__ void col_body(const RpDev& d, int blk) {
   for (int m = threadIdx.x; m < n; m += blockDim.x) {
     // y[2q] = w[q], y[2q+1] = w[N-1-q]
     float2 z = Z[(m & 1) ? (n - 1 - (m >> 1)) : (m >> 1)];
-    d.y1t[(size_t)ua * n + m] = z.x;
-    d.y1t[(size_t)ub * n + m] = z.y;
+    *(float2*)(d.y1 + m * nx + ua) = z;  // row-major, as the row pass reads it
   }
   __syncthreads();
   // DST-III of Ey hat: b'[0] = a[0] (the A[N] slot), b'[j] = a[N-j]; sign (-1)^k
@@ -738,8 +717,7 @@ __device__ void col_body(const RpDev& d, int blk) {
   for (int m = threadIdx.x; m < n; m += blockDim.x) {
     float2 z = Z[(m & 1) ? (n - 1 - (m >> 1)) : (m >> 1)];
     float sg = (m & 1) ? -1.0f : 1.0f;
-    d.y2t[(size_t)ua * n + m] = sg * z.x;
-    d.y2t[(size_t)ub * n + m] = sg * z.y;
+    *(float2*)(d.y2 + m * nx + ua) = make_float2(sg * z.x, sg * z.y);
   }
 }
 
@@ -1395,17 +1373,7 @@ __global__ void __launch_bounds__(RP_BLOCK) k_pin_pos(RpDev d) { pinpos_body(d, 
 __global__ void __launch_bounds__(RP_BLOCK) k_binfin_rowdct(RpDev d, int from_rho) {
   binfin_body(d, blockIdx.x, from_rho);
 }
-__global__ void __launch_bounds__(RP_BLOCK) k_transpose(const float* __restrict__ src, float* __restrict__ dst,
-                                                        int rows, int cols, const RpState* st) {
-  transpose_body(src, dst, rows, cols, st, blockIdx.x, blockIdx.y);
-}
-__global__ void __launch_bounds__(RP_BLOCK) k_transpose2(RpDev d) {
-  if (blockIdx.z == 0)
-    transpose_body(d.y1t, d.y1, d.nbx, d.nby, d.st, blockIdx.x, blockIdx.y);
-  else
-    transpose_body(d.y2t, d.y2, d.nbx, d.nby, d.st, blockIdx.x, blockIdx.y);
-}
-__global__ void __launch_bounds__(RP_BLOCK) k_col_spectral(RpDev d) { col_body(d, blockIdx.x); }
+__global__ void __launch_bounds__(RP_BLOCK) k_col_spectral(RpDev d) { col_body(d, xcd_block(blockIdx.x, gridDim.x)); }
 __global__ void __launch_bounds__(RP_BLOCK) k_row_inverse(RpDev d, int want_phi) {
   rowinv_body(d, blockIdx.x, want_phi);
 }
@@ -1426,25 +1394,6 @@ __global__ void __launch_bounds__(RP_BLOCK) kb_pin_pos(const RpDev* devs, const 
 __global__ void __launch_bounds__(RP_BLOCK) kb_binfin_rowdct(const RpDev* devs, const int2* tab) {
   int2 e = tab[blockIdx.x];
   if (e.x >= 0) binfin_body(devs[e.x], e.y, 0);
-}
-// transposes: local index -> (tile_x, tile_y[, which])
-__global__ void __launch_bounds__(RP_BLOCK) kb_transpose(const RpDev* devs, const int2* tab) {
-  int2 e = tab[blockIdx.x];
-  if (e.x < 0) return;
-  const RpDev& d = devs[e.x];
-  int tx = (d.nbx + 31) / 32;
-  transpose_body(d.t, d.tt, d.nby, d.nbx, d.st, e.y % tx, e.y / tx);
-}
-__global__ void __launch_bounds__(RP_BLOCK) kb_transpose2(const RpDev* devs, const int2* tab) {
-  int2 e = tab[blockIdx.x];
-  if (e.x < 0) return;
-  const RpDev& d = devs[e.x];
-  int tx = (d.nby + 31) / 32, ty = (d.nbx + 31) / 32;
-  int which = e.y / (tx * ty), r = e.y % (tx * ty);
-  if (which == 0)
-    transpose_body(d.y1t, d.y1, d.nbx, d.nby, d.st, r % tx, r / tx);
-  else
-    transpose_body(d.y2t, d.y2, d.nbx, d.nby, d.st, r % tx, r / tx);
 }
 __global__ void __launch_bounds__(RP_BLOCK) kb_col_spectral(const RpDev* devs, const int2* tab) {
   int2 e = tab[blockIdx.x];
@@ -1500,22 +1449,15 @@ hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_
   hipLaunchKernelGGL(k_binfin_rowdct, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), rp_lds_binfin(d.nbx), s, d, 0);
   end(RP_K_BINFIN_ROWDCT);
   if (mf) {
-    // matrix-core solve, reported in the transpose slot
-    beg(RP_K_TRANSPOSE);
+    // matrix-core solve
+    beg(RP_K_MFMA_POISSON);
     hipError_t e = rp_launch_poisson_mfma(d, *mf, s, want_phi);
     if (e != hipSuccess) return e;
-    end(RP_K_TRANSPOSE);
+    end(RP_K_MFMA_POISSON);
   } else {
-    beg(RP_K_TRANSPOSE);
-    hipLaunchKernelGGL(k_transpose, dim3((d.nbx + 31) / 32, (d.nby + 31) / 32), dim3(32, 8), 0, s,
-                       (const float*)d.t, d.tt, d.nby, d.nbx, (const RpState*)d.st);
-    end(RP_K_TRANSPOSE);
     beg(RP_K_COL_SPECTRAL);
     hipLaunchKernelGGL(k_col_spectral, dim3(d.nbx / 2), dim3(fft_threads(d.nby)), rp_lds_col(d.nby), s, d);
     end(RP_K_COL_SPECTRAL);
-    beg(RP_K_TRANSPOSE2);
-    hipLaunchKernelGGL(k_transpose2, dim3((d.nby + 31) / 32, (d.nbx + 31) / 32, 2), dim3(32, 8), 0, s, d);
-    end(RP_K_TRANSPOSE2);
     beg(RP_K_ROW_INVERSE);
     hipLaunchKernelGGL(k_row_inverse, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), rp_lds_rinv(d.nbx), s, d,
                        want_phi);
@@ -1551,11 +1493,8 @@ hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int fr
   size_t shx = rp_lds_binfin(d.nbx);
   hipLaunchKernelGGL(k_binfin_rowdct, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shx, s, d, from_rho);
   if (mf) return rp_launch_poisson_mfma(d, *mf, s, want_phi);
-  hipLaunchKernelGGL(k_transpose, dim3((d.nbx + 31) / 32, (d.nby + 31) / 32), dim3(32, 8), 0, s,
-                     (const float*)d.t, d.tt, d.nby, d.nbx, (const RpState*)d.st);
   size_t shy = rp_lds_col(d.nby);
   hipLaunchKernelGGL(k_col_spectral, dim3(d.nbx / 2), dim3(fft_threads(d.nby)), shy, s, d);
-  hipLaunchKernelGGL(k_transpose2, dim3((d.nby + 31) / 32, (d.nbx + 31) / 32, 2), dim3(32, 8), 0, s, d);
   size_t shr = rp_lds_rinv(d.nbx);
   hipLaunchKernelGGL(k_row_inverse, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shr, s, d, want_phi);
   return hipGetLastError();
@@ -1644,13 +1583,9 @@ hipError_t rp_launch_eval_batch(const RpBatchLaunch& b, hipStream_t s) {
   if (n[RP_K_BINFIN_ROWDCT])
     hipLaunchKernelGGL(kb_binfin_rowdct, dim3(n[RP_K_BINFIN_ROWDCT]), dim3(RP_BLOCK), b.lds_binfin, s, b.devs,
                        b.tab[RP_K_BINFIN_ROWDCT]);
-  if (n[RP_K_TRANSPOSE])
-    hipLaunchKernelGGL(kb_transpose, dim3(n[RP_K_TRANSPOSE]), dim3(32, 8), 0, s, b.devs, b.tab[RP_K_TRANSPOSE]);
   if (n[RP_K_COL_SPECTRAL])
     hipLaunchKernelGGL(kb_col_spectral, dim3(n[RP_K_COL_SPECTRAL]), dim3(RP_BLOCK), b.lds_col, s, b.devs,
                        b.tab[RP_K_COL_SPECTRAL]);
-  if (n[RP_K_TRANSPOSE2])
-    hipLaunchKernelGGL(kb_transpose2, dim3(n[RP_K_TRANSPOSE2]), dim3(32, 8), 0, s, b.devs, b.tab[RP_K_TRANSPOSE2]);
   if (n[RP_K_ROW_INVERSE])
     hipLaunchKernelGGL(kb_row_inverse, dim3(n[RP_K_ROW_INVERSE]), dim3(RP_BLOCK), b.lds_rinv, s, b.devs,
                        b.tab[RP_K_ROW_INVERSE]);
