@@ -28,6 +28,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "s to overflow≤0.10 at 1M cells; served placements/s + p99 at ≤0.5% HPWL"
 HBM_PEAK_GBS = 8000.0
+# timing slot -> kernel name in rocprofv3 output (profiles/traffic_*.json)
+ROCPROF_NAME = {"scatter": "k_scatter_pins", "control": "k_control_main"}
 
 
 def algorithmic_bytes(nc, npin, nn, nb, sort_bits=16):
@@ -380,7 +382,7 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tpath):
-        ent = json.load(open(tpath))["kernels"].get("k_" + dom)
+        ent = json.load(open(tpath))["kernels"].get(ROCPROF_NAME.get(dom, "k_" + dom))
         traffic = ent["hbm_bytes_per_launch"] if ent else None
     code, iters, evals, ovf, hpwl, dev_s, fin = runs[-1]
     result = {
