@@ -32,9 +32,12 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// fastExp, reference src/nesterovBase.cpp:1568-1582
+// fastExp, reference src/nesterovBase.cpp:1568-1582.  Its first step,
+// (float)(1.0 + (double)a / 1024.0), is computed in float: a / 1024 is exact,
+// and the double sum is exact whenever it can differ from 1.0f, so one float
+// rounding gives the same value (checked for every finite float).
 __device__ __forceinline__ float fast_exp(float a) {
-  a = (float)(1.0 + (double)a / 1024.0);
+  a = 1.0f + a * 0.0009765625f;
   a = a * a; a = a * a; a = a * a; a = a * a; a = a * a;
   a = a * a; a = a * a; a = a * a; a = a * a; a = a * a;
   return a;
@@ -1130,8 +1133,8 @@ struct Totals {
   long long hpwl;
 };
 
+// Totals of the gradient, bin and net partials; valid in thread 0.
 __device__ void reduce_all(const RpDev& d, Totals& t) {
-  __shared__ double shd[RP_CTRL_BLOCK / 64];
   __shared__ long long shl[RP_CTRL_BLOCK / 64];
   const RpState* st = d.st;
   // one pass over the gradient partials: each thread sums a fixed subset
@@ -1153,13 +1156,63 @@ __device__ void reduce_all(const RpDev& d, Totals& t) {
 #pragma unroll
         for (int j = 0; j < RP_P_COUNT; j++) v[j] += x[u][j];
   }
-  t.wl = block_sum_d(v[RP_P_WL], shd);
-  t.dens = block_sum_d(v[RP_P_DENS], shd);
-  t.grad = block_sum_d(v[RP_P_GRAD], shd);
-  t.dc = block_sum_d(v[RP_P_DCOORD], shd);
-  t.dg = block_sum_d(v[RP_P_DGRAD], shd);
-  t.ovf = reduce_parts_d(d.part_bin, st->num_partials_bin, 1, 0, shd);
-  t.hpwl = reduce_parts_ll(d.part_net, st->num_partials_net, shl);
+  // bin (overflow) and net (HPWL) partials, each thread a fixed strided subset
+  double vb = 0.0;
+  long long vh = 0;
+  const int nbp = st->num_partials_bin, nnp = st->num_partials_net;
+  for (int k0 = threadIdx.x; k0 < nbp; k0 += RP_RED_BATCH * blockDim.x) {
+    double x[RP_RED_BATCH];
+#pragma unroll
+    for (int u = 0; u < RP_RED_BATCH; u++) {
+      int k = k0 + u * blockDim.x;
+      x[u] = k < nbp ? d.part_bin[k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < RP_RED_BATCH; u++)
+      if (k0 + u * (int)blockDim.x < nbp) vb += x[u];
+  }
+  for (int k0 = threadIdx.x; k0 < nnp; k0 += RP_RED_BATCH * blockDim.x) {
+    long long x[RP_RED_BATCH];
+#pragma unroll
+    for (int u = 0; u < RP_RED_BATCH; u++) {
+      int k = k0 + u * blockDim.x;
+      x[u] = k < nnp ? d.part_net[k] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < RP_RED_BATCH; u++) vh += x[u];
+  }
+  // all seven totals in one pass: wavefront trees, then the wavefronts in
+  // order (the same arithmetic as one block_sum per value)
+  double w[RP_P_COUNT + 1];
+#pragma unroll
+  for (int j = 0; j < RP_P_COUNT; j++) w[j] = wave_sum_d(v[j]);
+  w[RP_P_COUNT] = wave_sum_d(vb);
+  vh = wave_sum_ll(vh);
+  __shared__ double shw[RP_CTRL_BLOCK / 64][RP_P_COUNT + 1];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j <= RP_P_COUNT; j++) shw[wv][j] = w[j];
+    shl[wv] = vh;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r[RP_P_COUNT + 1] = {0, 0, 0, 0, 0, 0};
+    long long h = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
+#pragma unroll
+      for (int j = 0; j <= RP_P_COUNT; j++) r[j] += shw[k][j];
+      h += shl[k];
+    }
+    t.wl = r[RP_P_WL];
+    t.dens = r[RP_P_DENS];
+    t.grad = r[RP_P_GRAD];
+    t.dc = r[RP_P_DCOORD];
+    t.dg = r[RP_P_DGRAD];
+    t.ovf = r[RP_P_COUNT];
+    t.hpwl = h;
+  }
+  __syncthreads();
 }
 
 // updateWireLengthCoef, src/nesterovPlace.cpp:474-490
