@@ -182,6 +182,13 @@ int rp_get_iter_records(rp_ctx* ctx, rp_iter_record* out, int32_t capacity, int3
 
 /* Lower-left corners of all num_insts instances (as the DB would hold them). */
 int rp_get_positions(rp_ctx* ctx, int32_t* lx, int32_t* ly);
+
+/* Density centres (GCell dCx, dCy: the integer centre of the last evaluated
+ * iterate, src/nesterovBase.cpp:164-179) of every gcell, in the reference's
+ * gcell order (placed instances, then fillers; rp_nb_info.num_gcells
+ * entries each).  No reference counterpart as an API: it is the state the
+ * reference's PlotEnv draws (src/plot.cpp:284-426), for debug dumps. */
+int rp_get_gcell_centres(rp_ctx* ctx, float* cx, float* cy);
 /* Move instances (incremental mode / resume); lx, ly for all num_insts. */
 int rp_set_positions(rp_ctx* ctx, const int32_t* lx, const int32_t* ly);
 

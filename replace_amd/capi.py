@@ -42,6 +42,7 @@ def lib() -> ctypes.CDLL:
         L.rp_get_iter_records.argtypes = [vp, ctypes.POINTER(RpIterRecord), ctypes.c_int32,
                                           ctypes.POINTER(ctypes.c_int32)]
         L.rp_get_positions.argtypes = [vp, vp, vp]
+        L.rp_get_gcell_centres.argtypes = [vp, vp, vp]
         L.rp_set_positions.argtypes = [vp, vp, vp]
         L.rp_destroy.argtypes = [vp]
         L.rp_last_error.restype = ctypes.c_char_p
@@ -207,6 +208,15 @@ class Placer:
         hp = np.zeros(max(1, n.value), np.int64)
         self._check(lib().rp_get_ip_records(self.h, _p(err), _p(hp), n.value, ctypes.byref(n)))
         return err[: n.value], hp[: n.value]
+
+    def gcell_centres(self):
+        """Density centres of every gcell (reference gcell order) of the last
+        evaluated iterate."""
+        n = self.info.num_gcells
+        cx = np.zeros(n, np.float32)
+        cy = np.zeros(n, np.float32)
+        self._check(lib().rp_get_gcell_centres(self.h, _p(cx), _p(cy)))
+        return cx, cy
 
     def positions(self):
         n = self._nl.num_insts

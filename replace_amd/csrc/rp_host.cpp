@@ -1357,6 +1357,20 @@ extern "C" int rp_get_iter_records(rp_ctx* c, rp_iter_record* out, int32_t cap, 
   return RP_OK;
 }
 
+extern "C" int rp_get_gcell_centres(rp_ctx* c, float* cx, float* cy) {
+  if (!c || !cx || !cy) return RP_ERR_ARG;
+  if (!c->nb_ready) return fail(c, RP_ERR_ARG, "rp_get_gcell_centres before rp_nesterov_setup");
+  const int ng = c->n_g;
+  std::vector<int2> ic(ng);
+  HIPCHK(c, hipMemcpy(ic.data(), c->d_ic.p, sizeof(int2) * ng, hipMemcpyDeviceToHost));
+  for (int g = 0; g < ng; g++) {
+    int k = c->ginv[g];
+    cx[g] = (float)ic[k].x;
+    cy[g] = (float)ic[k].y;
+  }
+  return RP_OK;
+}
+
 extern "C" int rp_get_positions(rp_ctx* c, int32_t* lx, int32_t* ly) {
   if (!c || !lx || !ly) return RP_ERR_ARG;
   for (int i = 0; i < c->n_real; i++) {

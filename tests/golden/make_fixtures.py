@@ -13,8 +13,12 @@ container, where the reference tree exists; the fixtures are data only).
   poisson_ref.npz     inputs and outputs of the reference's own FFT Poisson
                       solve (src/fft.cpp, Ooura fftsg), built into
                       oracle/_ref by oracle/Makefile
+  defok.npz           final lower-left corners of every component in the
+                      reference's golden result DEFs (test/<design>.defok,
+                      written by write_def after global_placement), in the
+                      component order of test/<design>.def
 
-Usage: python tests/golden/make_fixtures.py [/root/reference]
+Usage: python tests/golden/make_fixtures.py [/root/reference] [--defok-only]
 """
 import json
 import os
@@ -82,10 +86,28 @@ int main() { std::mt19937 r(0); for (int i = 0; i < 2000; i++) printf("%lu\n", (
 """
 
 
+def write_defok(test, lef):
+    out = {}
+    for name in DESIGNS:
+        path = os.path.join(test, name + ".defok")
+        if not os.path.exists(path):
+            continue
+        src = read_def(os.path.join(test, name + ".def"), lef, name=name)
+        ok = read_def(path, lef, name=name)
+        assert ok.num_insts == src.num_insts, name
+        assert np.array_equal(ok.inst_box[:, 2] - ok.inst_box[:, 0], src.inst_box[:, 2] - src.inst_box[:, 0]), name
+        out[name] = ok.inst_box[:, 0:2].astype(np.int32)
+    np.savez_compressed(os.path.join(HERE, "defok.npz"), **out)
+
+
 def main():
-    ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    ref = args[0] if args else "/root/reference"
     test = os.path.join(ref, "test")
     lef = read_lef(os.path.join(test, "nangate45.lef"))
+    write_defok(test, lef)
+    if "--defok-only" in sys.argv:
+        return
     traces = {}
     for name, flags in DESIGNS.items():
         nl = read_def(os.path.join(test, name + ".def"), lef, name=name)

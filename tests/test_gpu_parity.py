@@ -12,6 +12,7 @@ Tolerances:
     overflow area 1e-5 relative, gradient sums 1e-4 relative.
   * end to end: final HPWL within 0.5 % at overflow <= 0.10.
 """
+import dataclasses
 import os
 
 import numpy as np
@@ -214,3 +215,35 @@ def test_initial_place_tracks_reference_log():
         for (it, e, h), got in zip(t, hp):
             assert abs(int(got) - h) <= 1e-4 * h
         assert (err[4:] < 1e-5).all()
+
+
+def _placement_hpwl(nl, lx, ly):
+    """PlacerBase::hpwl (pin bboxes over instance boxes, src/placerBase.cpp:
+    868-876) of a netlist with its instances moved to lower-left (lx, ly),
+    computed by the oracle's PlacerBase restatement."""
+    box = nl.inst_box.copy()
+    w, h = box[:, 2] - box[:, 0], box[:, 3] - box[:, 1]
+    box[:, 0], box[:, 1] = lx, ly
+    box[:, 2], box[:, 3] = lx + w, ly + h
+    moved = dataclasses.replace(nl, inst_box=box)
+    return oracle.Oracle(moved, design("simple01")[1]).inst_hpwl()
+
+
+@pytest.mark.parametrize("name", ["simple01", "simple02", "ar01", "ar02", "core01", "incremental01"])
+def test_final_placement_tracks_reference_defok(name):
+    """The GPU's final placement against the reference's own result
+    (test/<design>.defok): instance-box HPWL within 0.5 %, fixed instances
+    untouched, every movable instance inside the die."""
+    nl, p = design(name)
+    g = Placer(nl, p)
+    g.initial_place()
+    g.setup()
+    code, st = g.nesterov_place()
+    assert code == 0 and st.finished
+    lx, ly = g.positions()
+    ok = np.load(os.path.join(GOLDEN, "defok.npz"))[name]
+    fixed = nl.inst_fixed.astype(bool)
+    assert np.array_equal(lx[fixed], ok[fixed, 0]) and np.array_equal(ly[fixed], ok[fixed, 1])
+    h_gpu = _placement_hpwl(nl, lx, ly)
+    h_ref = _placement_hpwl(nl, ok[:, 0], ok[:, 1])
+    assert abs(h_gpu - h_ref) <= 0.005 * h_ref, (h_gpu, h_ref)
