@@ -291,7 +291,24 @@ typedef struct rp_server_opts {
   int32_t setup_threads;     /* host workers for the per-job setup */
   int32_t steps_per_poll;    /* batched steps per graph replay */
   int32_t use_graph;         /* 1: replay a captured HIP graph */
+  int32_t autotune;          /* 1: time the tile shapes of the batched scatter and
+                                wirelength kernels per job size bucket on the
+                                bucket's first jobs and keep the fastest */
 } rp_server_opts;
+
+/* Tile shapes the service uses for one job size bucket (rp_server_tuning).
+ * Buckets by movable + filler cell count: < 2k, < 8k, < 32k, < 128k, larger.
+ * Every shape gives bit-identical results; only the time differs. */
+#define RP_NUM_SIZE_BUCKETS 5
+typedef struct rp_tile_choice {
+  int32_t bucket;
+  int32_t tuned;             /* 1: measured on this bucket's jobs, 0: defaults */
+  int32_t jobs_sampled;
+  int32_t scatter_cells;     /* cells per fused step+scatter workgroup (256, 512, 1024) */
+  int32_t wa_pins;           /* pins per wirelength workgroup (256, 512, 1024) */
+  float scatter_us[3];       /* measured mean launch time of each candidate */
+  float wa_us[3];
+} rp_tile_choice;
 
 typedef struct rp_job_result {
   rp_nesterov_stats stats;
@@ -319,6 +336,11 @@ int rp_server_job(rp_server* s, int32_t job, rp_job_result* out);
 int rp_server_job_positions(rp_server* s, int32_t job, int32_t* lx, int32_t* ly);
 int rp_server_job_records(rp_server* s, int32_t job, rp_iter_record* out, int32_t capacity, int32_t* count);
 int rp_server_get_stats(rp_server* s, rp_server_stats* out);
+/* The tile shape of every size bucket (RP_NUM_SIZE_BUCKETS entries). */
+int rp_server_tuning(rp_server* s, rp_tile_choice* out, int32_t capacity, int32_t* count);
+/* Pins a bucket's tile shape (256, 512 or 1024 each) before rp_server_run,
+ * e.g. to reuse an earlier tuning; the tuner then leaves the bucket alone. */
+int rp_server_set_tiles(rp_server* s, int32_t bucket, int32_t scatter_cells, int32_t wa_pins);
 const char* rp_server_last_error(rp_server* s);
 void rp_server_destroy(rp_server* s);
 

@@ -13,7 +13,7 @@ from typing import Optional
 import numpy as np
 
 from .netlist import (Netlist, RpEvalOut, RpIpStats, RpIterRecord, RpJobResult, RpNbInfo,
-                      RpNesterovStats, RpParams, RpServerOpts, RpServerStats, params_default)
+                      RpNesterovStats, RpParams, RpServerOpts, RpServerStats, RpTileChoice, params_default)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libreplace_amd.so")
@@ -57,6 +57,8 @@ def lib() -> ctypes.CDLL:
         L.rp_get_kernel_timing.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int64)]
         L.rp_get_ip_records.argtypes = [vp, vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
         L.rp_server_opts_default.argtypes = [ctypes.POINTER(RpServerOpts)]
+        L.rp_server_tuning.argtypes = [vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+        L.rp_server_set_tiles.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
         L.rp_server_create.argtypes = [ctypes.POINTER(RpServerOpts), ctypes.POINTER(vp)]
         L.rp_server_submit.argtypes = [vp, vp, ctypes.POINTER(RpParams), ctypes.c_double,
                                        ctypes.POINTER(ctypes.c_int32)]
@@ -230,12 +232,12 @@ class Server:
     """Placement service: many jobs in continuous batches on one GPU."""
 
     def __init__(self, hbm_budget_bytes=None, max_active=None, setup_threads=None,
-                 steps_per_poll=None, use_graph=None):
+                 steps_per_poll=None, use_graph=None, autotune=None):
         o = RpServerOpts()
         lib().rp_server_opts_default(ctypes.byref(o))
         for k, v in dict(hbm_budget_bytes=hbm_budget_bytes, max_active=max_active,
                          setup_threads=setup_threads, steps_per_poll=steps_per_poll,
-                         use_graph=use_graph).items():
+                         use_graph=use_graph, autotune=autotune).items():
             if v is not None:
                 setattr(o, k, int(v))
         self.opts = o
@@ -286,6 +288,23 @@ class Server:
         lib().rp_server_job_records(self.h, job, arr, cnt.value, ctypes.byref(cnt))
         return [(r.overflow, r.hpwl, r.step_length, r.density_penalty, r.num_back_track)
                 for r in arr[: cnt.value]]
+
+    def set_tiles(self, bucket: int, scatter_cells: int, wa_pins: int) -> None:
+        r = lib().rp_server_set_tiles(self.h, bucket, scatter_cells, wa_pins)
+        if r < 0:
+            raise ReplaceError(r, self.last_error())
+
+    def tuning(self):
+        """Tile shape per job size bucket: list of dicts (rp_tile_choice)."""
+        arr = (RpTileChoice * 8)()
+        n = ctypes.c_int32(0)
+        r = lib().rp_server_tuning(self.h, arr, 8, ctypes.byref(n))
+        if r < 0:
+            raise ReplaceError(r, self.last_error())
+        return [dict(bucket=t.bucket, tuned=t.tuned, jobs_sampled=t.jobs_sampled,
+                     scatter_cells=t.scatter_cells, wa_pins=t.wa_pins,
+                     scatter_us=[round(v, 2) for v in t.scatter_us], wa_us=[round(v, 2) for v in t.wa_us])
+                for t in arr[: n.value]]
 
     def stats(self) -> RpServerStats:
         st = RpServerStats()

@@ -19,6 +19,13 @@
 //   4. the phase word of every running job is gathered; finished jobs are
 //      collected (stats, records, positions), their memory is released and
 //      the budget is returned; the tables are rebuilt.
+// Tile shapes: the fused step+scatter and the wirelength kernel come in
+// RP_SC_VARIANTS / RP_WA_VARIANTS shapes (cells / pins per workgroup).  With
+// opts.autotune the first table rebuild that sees a job size bucket times
+// each shape on up to kTuneJobs of the bucket's running jobs (launches of
+// that kernel alone: both are idempotent on a job's state apart from the
+// bin accumulators, which are cleared again afterwards) and keeps the
+// fastest for the bucket; the tables then hold one launch per shape.
 // A job's arithmetic does not depend on which other jobs run beside it, so
 // its result is bit-identical alone and in any mix.
 #include <algorithm>
@@ -99,10 +106,23 @@ void build_table(const std::vector<int>& counts, std::vector<int2>& tab) {
     for (size_t j = 0; j < lists[x].size(); j++) tab[j * X + x] = lists[x][j];
 }
 
+constexpr int kTuneJobs = 16;  // jobs per bucket the tuner times
+constexpr int kTuneReps = 3;   // timed launches per shape
+
+int size_bucket(int ncells) {
+  if (ncells < 2000) return 0;
+  if (ncells < 8000) return 1;
+  if (ncells < 32000) return 2;
+  if (ncells < 128000) return 3;
+  return 4;
+}
+
 }  // namespace
 
 struct rp_server {
   rp_server_opts opt{};
+  rp_tile_choice tiles[RP_NUM_SIZE_BUCKETS]{};
+  DevBuf<int2> d_tune_tab;
   std::string err;
   int device = 0;
   hipStream_t stream = nullptr;
@@ -110,6 +130,7 @@ struct rp_server {
   // tables of the running set
   DevBuf<RpDev> d_devs;
   DevBuf<int2> d_tab[RP_NUM_EVAL_KERNELS];
+  DevBuf<int2> d_tab_sc[RP_SC_VARIANTS], d_tab_wa[RP_WA_VARIANTS];
   DevBuf<int> d_phase;
   int* h_phase = nullptr;
   size_t h_phase_cap = 0;
@@ -148,6 +169,88 @@ extern "C" void rp_server_opts_default(rp_server_opts* o) {
   o->setup_threads = 8;
   o->steps_per_poll = 8;
   o->use_graph = 1;
+  o->autotune = 1;
+}
+
+static void tiles_default(rp_server* s) {
+  for (int b = 0; b < RP_NUM_SIZE_BUCKETS; b++) {
+    rp_tile_choice& t = s->tiles[b];
+    memset(&t, 0, sizeof t);
+    t.bucket = b;
+    t.scatter_cells = 1024;
+    t.wa_pins = 256 << RP_WA_DEFAULT_VARIANT;
+  }
+}
+
+static int variant_of_cells(int cells) { return cells <= 256 ? 0 : cells <= 512 ? 1 : 2; }
+static int variant_of_pins(int pins) { return pins <= 256 ? 0 : pins <= 512 ? 1 : 2; }
+
+// Times every tile shape of the scatter and wirelength kernels on up to
+// kTuneJobs running jobs of bucket b (their device views in devs) and keeps
+// the fastest.  Runs on the service stream between batched steps.
+static int tune_bucket(rp_server* s, int b, const std::vector<int>& idx, const std::vector<RpDev>& devs) {
+  rp_tile_choice& t = s->tiles[b];
+  const int n = std::min<int>(kTuneJobs, (int)idx.size());
+  std::vector<RpDev> sample(n);
+  for (int i = 0; i < n; i++) sample[i] = devs[idx[i]];
+  DevBuf<RpDev> d_sample;
+  SCHK(s, dupload(d_sample, sample, s->stream));
+  hipEvent_t e0, e1;
+  SCHK(s, hipEventCreate(&e0));
+  SCHK(s, hipEventCreate(&e1));
+  auto time_variant = [&](int kernel, int v, float& us) -> int {
+    std::vector<int> cnt(n);
+    for (int i = 0; i < n; i++)
+      cnt[i] = kernel == RP_K_SCATTER ? (sample[i].nc + (256 << v) - 1) / (256 << v) : sample[i].nwa_v[v];
+    std::vector<int2> tab;
+    build_table(cnt, tab);
+    if (tab.empty()) {
+      us = 0.0f;
+      return RP_OK;
+    }
+    std::vector<RpDev> dv = sample;
+    for (auto& d : dv) {  // the wirelength shape needs its own packing
+      d.nwa = d.nwa_v[kernel == RP_K_WA ? v : RP_WA_DEFAULT_VARIANT];
+      d.wa_rng = d.wa_rng_v[kernel == RP_K_WA ? v : RP_WA_DEFAULT_VARIANT];
+    }
+    SCHK(s, hipMemcpyAsync(d_sample.p, dv.data(), sizeof(RpDev) * n, hipMemcpyHostToDevice, s->stream));
+    if (s->d_tune_tab.n < tab.size()) SCHK(s, dalloc(s->d_tune_tab, 2 * tab.size()));
+    SCHK(s, hipMemcpyAsync(s->d_tune_tab.p, tab.data(), sizeof(int2) * tab.size(), hipMemcpyHostToDevice, s->stream));
+    SCHK(s, rp_launch_batch_variant(kernel, v, d_sample.p, s->d_tune_tab.p, (int)tab.size(), s->stream));  // warm
+    SCHK(s, hipEventRecord(e0, s->stream));
+    for (int r = 0; r < kTuneReps; r++)
+      SCHK(s, rp_launch_batch_variant(kernel, v, d_sample.p, s->d_tune_tab.p, (int)tab.size(), s->stream));
+    SCHK(s, hipEventRecord(e1, s->stream));
+    SCHK(s, hipEventSynchronize(e1));
+    float ms = 0;
+    SCHK(s, hipEventElapsedTime(&ms, e0, e1));
+    us = ms * 1e3f / kTuneReps;
+    return RP_OK;
+  };
+  int best_sc = 0, best_wa = 0;
+  for (int v = 0; v < RP_SC_VARIANTS; v++) {
+    if (int r = time_variant(RP_K_SCATTER, v, t.scatter_us[v])) return r;
+    if (t.scatter_us[v] < t.scatter_us[best_sc]) best_sc = v;
+  }
+  for (int v = 0; v < RP_WA_VARIANTS; v++) {
+    if (int r = time_variant(RP_K_WA, v, t.wa_us[v])) return r;
+    if (t.wa_us[v] < t.wa_us[best_wa]) best_wa = v;
+  }
+  // the timed scatters added into the bin accumulators: clear them again
+  for (int i = 0; i < n; i++) {
+    const RpDev& d = sample[i];
+    size_t nb = (size_t)d.nbx * d.nby;
+    SCHK(s, hipMemsetAsync(d.bin_inst, 0, sizeof(unsigned long long) * nb, s->stream));
+    SCHK(s, hipMemsetAsync(d.bin_filler, 0, sizeof(unsigned long long) * nb, s->stream));
+  }
+  SCHK(s, hipStreamSynchronize(s->stream));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  t.tuned = 1;
+  t.jobs_sampled = n;
+  t.scatter_cells = 256 << best_sc;
+  t.wa_pins = 256 << best_wa;
+  return RP_OK;
 }
 
 static void setup_worker(rp_server* s) {
@@ -195,6 +298,7 @@ extern "C" int rp_server_create(const rp_server_opts* opt, rp_server** out) {
     *out = s.release();
     return sfail(*out, RP_ERR_HIP, "hipStreamCreate failed");
   }
+  tiles_default(s.get());
   int nt = std::max(1, s->opt.setup_threads);
   rp_server* raw = s.get();
   for (int k = 0; k < nt; k++) s->workers.emplace_back(setup_worker, raw);
@@ -243,18 +347,36 @@ static int rebuild(rp_server* s) {
   const int n = (int)s->running.size();
   std::vector<RpDev> devs(n);
   std::vector<int> cnt[RP_NUM_EVAL_KERNELS];
+  std::vector<int> cnt_sc[RP_SC_VARIANTS], cnt_wa[RP_WA_VARIANTS];
   size_t lb = 0, lc = 0, lr = 0;
+  SCHK(s, hipStreamSynchronize(s->stream));
+  if (s->opt.autotune) {
+    std::vector<int> in_bucket[RP_NUM_SIZE_BUCKETS];
+    for (int i = 0; i < n; i++) {
+      devs[i] = s->jobs[s->running[i]]->ctx->dev;
+      in_bucket[size_bucket(devs[i].nc)].push_back(i);
+    }
+    for (int b = 0; b < RP_NUM_SIZE_BUCKETS; b++)
+      if (!s->tiles[b].tuned && !in_bucket[b].empty())
+        if (int r = tune_bucket(s, b, in_bucket[b], devs)) return r;
+  }
   for (int i = 0; i < n; i++) {
-    const RpDev& d = s->jobs[s->running[i]]->ctx->dev;
+    RpDev d = s->jobs[s->running[i]]->ctx->dev;
+    const rp_tile_choice& t = s->tiles[size_bucket(d.nc)];
+    const int vs = variant_of_cells(t.scatter_cells), vw = variant_of_pins(t.wa_pins);
+    d.nwa = d.nwa_v[vw];
+    d.wa_rng = d.wa_rng_v[vw];
     devs[i] = d;
-    cnt[RP_K_STEP].push_back(0);  // the batched scatter kernel steps its own cells
-    cnt[RP_K_SCATTER].push_back((d.nc + RP_SCATTER_CELLS - 1) / RP_SCATTER_CELLS);
+    cnt[RP_K_STEP].push_back(0);     // the batched scatter kernel steps its own cells
+    cnt[RP_K_SCATTER].push_back(0);  // launched per tile shape below
+    for (int v = 0; v < RP_SC_VARIANTS; v++) cnt_sc[v].push_back(v == vs ? (d.nc + (256 << v) - 1) / (256 << v) : 0);
+    for (int v = 0; v < RP_WA_VARIANTS; v++) cnt_wa[v].push_back(v == vw ? d.nwa : 0);
     cnt[RP_K_PIN_POS].push_back((d.np + RP_BLOCK - 1) / RP_BLOCK);
     cnt[RP_K_BINFIN_ROWDCT].push_back(d.nby / 2);
     cnt[RP_K_MFMA_POISSON].push_back(0);  // batched instances use the FFT solve
     cnt[RP_K_COL_SPECTRAL].push_back(d.nbx / 2);
     cnt[RP_K_ROW_INVERSE].push_back(d.nby / 2);
-    cnt[RP_K_WA].push_back(d.nwa);
+    cnt[RP_K_WA].push_back(0);  // launched per tile shape
     cnt[RP_K_GRAD].push_back((d.nc + RP_BLOCK - 1) / RP_BLOCK);
     cnt[RP_K_CONTROL].push_back(1);
     lb = std::max(lb, rp_lds_binfin(d.nbx));
@@ -284,6 +406,21 @@ static int rebuild(rp_server* s) {
     L.tab[k] = s->d_tab[k].p;
     L.nblk[k] = (int)tab.size();
   }
+  std::vector<int2> tabs_sc[RP_SC_VARIANTS], tabs_wa[RP_WA_VARIANTS];
+  auto upload_variant = [&](const std::vector<int>& c, std::vector<int2>& tab, DevBuf<int2>& buf, const int2*& p,
+                            int& nb) -> int {
+    build_table(c, tab);
+    if (buf.n < std::max<size_t>(1, tab.size())) SCHK(s, dalloc(buf, std::max<size_t>(1, 2 * tab.size())));
+    if (!tab.empty())
+      SCHK(s, hipMemcpyAsync(buf.p, tab.data(), sizeof(int2) * tab.size(), hipMemcpyHostToDevice, s->stream));
+    p = buf.p;
+    nb = (int)tab.size();
+    return RP_OK;
+  };
+  for (int v = 0; v < RP_SC_VARIANTS; v++)
+    if (int r = upload_variant(cnt_sc[v], tabs_sc[v], s->d_tab_sc[v], L.tab_sc[v], L.nblk_sc[v])) return r;
+  for (int v = 0; v < RP_WA_VARIANTS; v++)
+    if (int r = upload_variant(cnt_wa[v], tabs_wa[v], s->d_tab_wa[v], L.tab_wa[v], L.nblk_wa[v])) return r;
   L.lds_binfin = lb;
   L.lds_col = lc;
   L.lds_rinv = lr;
@@ -471,6 +608,25 @@ extern "C" int rp_server_job_records(rp_server* s, int32_t id, rp_iter_record* o
 extern "C" int rp_server_get_stats(rp_server* s, rp_server_stats* out) {
   if (!s || !out) return RP_ERR_ARG;
   *out = s->stats;
+  return RP_OK;
+}
+
+extern "C" int rp_server_set_tiles(rp_server* s, int32_t bucket, int32_t scatter_cells, int32_t wa_pins) {
+  if (!s || bucket < 0 || bucket >= RP_NUM_SIZE_BUCKETS) return RP_ERR_ARG;
+  auto ok = [](int v) { return v == 256 || v == 512 || v == 1024; };
+  if (!ok(scatter_cells) || !ok(wa_pins)) return sfail(s, RP_ERR_ARG, "tile sizes are 256, 512 or 1024");
+  rp_tile_choice& t = s->tiles[bucket];
+  t.scatter_cells = scatter_cells;
+  t.wa_pins = wa_pins;
+  t.tuned = 1;  // pinned: the tuner leaves this bucket alone
+  t.jobs_sampled = 0;
+  return RP_OK;
+}
+
+extern "C" int rp_server_tuning(rp_server* s, rp_tile_choice* out, int32_t cap, int32_t* count) {
+  if (!s || !count) return RP_ERR_ARG;
+  *count = RP_NUM_SIZE_BUCKETS;
+  for (int b = 0; b < std::min<int>(cap, RP_NUM_SIZE_BUCKETS) && out; b++) out[b] = s->tiles[b];
   return RP_OK;
 }
 

@@ -86,7 +86,7 @@ struct rp_ctx {
   DevBuf<int> d_sord, d_srank, d_scnt, d_soff;  // spatial cell order of the scatter and its counting sort
   RpSortBufs sortb{};
   DevBuf<int> d_cp_start, d_cp_pin, d_nstart, d_pcell;
-  DevBuf<int2> d_wa_rng;
+  DevBuf<int2> d_wa_rng[RP_WA_VARIANTS];
   DevBuf<int2> d_ppos, d_ppin;
   DevBuf<float2> d_pgrad;
   DevBuf<int4> d_nbox;
@@ -130,6 +130,8 @@ struct rp_ctx {
     add(d_t); add(d_y1); add(d_y2); add(d_field); add(d_twx); add(d_tw2x);
     add(d_twy); add(d_tw2y); add(d_wx); add(d_wx2); add(d_wy); add(d_wy2); add(d_part_grad); add(d_part_bin);
     add(d_part_net); add(d_state); add(d_rec); add(d_dbg_wl); add(d_dbg_dens);
+    add(d_sord); add(d_srank); add(d_scnt); add(d_soff);
+    for (const auto& w : d_wa_rng) add(w);
     return b + mfma.bytes();
   }
 };

@@ -28,8 +28,10 @@ struct RpDev {
   const int* cp_pin;   // cell's pins (reference ITerm order) -> device pin index
   float2* pgrad;       // per-pin WA gradient gMin - gMax, device pin order
   const int* nstart;
-  int nwa;             // wirelength workgroups
+  int nwa;             // wirelength workgroups of the packing in use
   const int2* wa_rng;  // their net ranges [x, y)
+  int nwa_v[RP_WA_VARIANTS];              // the packings for every tile-shape variant
+  const int2* wa_rng_v[RP_WA_VARIANTS];
   const int* pcell;
   const int2* ppos;
   int2* ppin;        // absolute pin centres, rewritten every evaluation (k_pin_pos)
@@ -74,6 +76,10 @@ struct RpBatchLaunch {
   const RpDev* devs;
   const int2* tab[RP_NUM_EVAL_KERNELS];
   int nblk[RP_NUM_EVAL_KERNELS];
+  const int2* tab_sc[RP_SC_VARIANTS];  // scatter and wirelength: one table per tile shape
+  int nblk_sc[RP_SC_VARIANTS];
+  const int2* tab_wa[RP_WA_VARIANTS];
+  int nblk_wa[RP_WA_VARIANTS];
   size_t lds_binfin, lds_col, lds_rinv;
 };
 
@@ -87,6 +93,10 @@ struct RpMfmaDev {
 };
 
 hipError_t rp_launch_poisson_mfma(const RpDev& d, const RpMfmaDev& m, hipStream_t s, int want_phi);
+// One batched launch of tile-shape variant `variant` of kernel RP_K_SCATTER or
+// RP_K_WA over block table tab (nblk entries).
+hipError_t rp_launch_batch_variant(int kernel, int variant, const RpDev* devs, const int2* tab, int nblk,
+                                   hipStream_t s);
 
 // Work arrays of the spatial cell order (single-instance scatter): a
 // counting sort of the cells by the tile holding their integer centre.

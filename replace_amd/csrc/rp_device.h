@@ -30,10 +30,6 @@
 
 #define RP_BLOCK 256
 #define RP_CTRL_BLOCK 1024  // controller workgroup (fixed-order reductions)
-#ifndef RP_SCATTER_PER_THREAD
-#define RP_SCATTER_PER_THREAD 4
-#endif
-#define RP_SCATTER_CELLS (RP_BLOCK * RP_SCATTER_PER_THREAD)  // cells per fused step+scatter workgroup
 #ifndef RP_SS_PER_THREAD
 #define RP_SS_PER_THREAD 1
 #endif
@@ -43,6 +39,12 @@
 #define RP_WA_CAP 512         // pins per wirelength workgroup (a larger net gets one of its own)
 #endif
 #define RP_WA_NETS RP_BLOCK   // nets per wirelength workgroup
+// Tile-shape variants of the batched launches (picked per size bucket by the
+// placement service's tuner): cells per fused step+scatter workgroup
+// 256 / 512 / 1024, pins per wirelength workgroup 256 / 512 / 1024.
+#define RP_SC_VARIANTS 3
+#define RP_WA_VARIANTS 3
+#define RP_WA_DEFAULT_VARIANT 1  // 512 pins (RP_WA_CAP), the single-instance launch
 
 // Kernels of one evaluation slot, in launch order (per-kernel timing index).
 enum {

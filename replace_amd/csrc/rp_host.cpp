@@ -846,21 +846,28 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   HIPCHK(c, dalloc(c->d_pgrad, std::max(1, npn)));
   HIPCHK(c, dupload(c->d_nstart, nstart, s));
   // wirelength workgroups: consecutive nets, at most RP_WA_NETS nets and
-  // RP_WA_CAP pins each; a net with more pins gets a workgroup of its own
-  std::vector<int2> wa_rng;
-  for (int e = 0; e < nn;) {
-    const int e0 = e;
-    if (nstart[e + 1] - nstart[e] > RP_WA_CAP) {
-      wa_rng.push_back(make_int2(e, e + 1));
-      e++;
-      continue;
+  // `cap` pins each; a net with more pins gets a workgroup of its own.  One
+  // packing per tile-shape variant (256 / 512 / 1024 pins).
+  int nwa_v[RP_WA_VARIANTS];
+  for (int v = 0; v < RP_WA_VARIANTS; v++) {
+    const int cap = 256 << v;
+    std::vector<int2> wa_rng;
+    for (int e = 0; e < nn;) {
+      const int e0 = e;
+      if (nstart[e + 1] - nstart[e] > cap) {
+        wa_rng.push_back(make_int2(e, e + 1));
+        e++;
+        continue;
+      }
+      while (e < nn && e - e0 < RP_WA_NETS && nstart[e + 1] - nstart[e0] <= cap) e++;
+      wa_rng.push_back(make_int2(e0, e));
     }
-    while (e < nn && e - e0 < RP_WA_NETS && nstart[e + 1] - nstart[e0] <= RP_WA_CAP) e++;
-    wa_rng.push_back(make_int2(e0, e));
+    if (wa_rng.empty()) wa_rng.push_back(make_int2(0, 0));
+    nwa_v[v] = (int)wa_rng.size();
+    HIPCHK(c, dupload(c->d_wa_rng[v], wa_rng, s));
   }
-  if (wa_rng.empty()) wa_rng.push_back(make_int2(0, 0));
-  const int nwa = (int)wa_rng.size();
-  HIPCHK(c, dupload(c->d_wa_rng, wa_rng, s));
+  const int nwa = nwa_v[RP_WA_DEFAULT_VARIANT];
+  const int nwa_max = *std::max_element(nwa_v, nwa_v + RP_WA_VARIANTS);
   HIPCHK(c, dupload(c->d_pcell, pcell, s));
   HIPCHK(c, dupload(c->d_ppos, ppos, s));
   HIPCHK(c, dalloc(c->d_ppin, std::max(1, npn)));
@@ -886,7 +893,7 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   HIPCHK(c, dupload(c->d_wx2, wx2, s));
   HIPCHK(c, dupload(c->d_wy, wy, s));
   HIPCHK(c, dupload(c->d_wy2, wy2, s));
-  int npg = (ng + RP_BLOCK - 1) / RP_BLOCK, npb = c->nby / 2, npn2 = nwa;
+  int npg = (ng + RP_BLOCK - 1) / RP_BLOCK, npb = c->nby / 2, npn2 = nwa_max;
   HIPCHK(c, dalloc(c->d_part_grad, (size_t)RP_P_COUNT * npg));
   HIPCHK(c, dalloc(c->d_part_bin, npb));
   HIPCHK(c, dalloc(c->d_part_net, std::max(1, npn2)));
@@ -927,8 +934,12 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   d.cp_pin = c->d_cp_pin.p;
   d.pgrad = c->d_pgrad.p;
   d.nstart = c->d_nstart.p;
+  for (int v = 0; v < RP_WA_VARIANTS; v++) {
+    d.nwa_v[v] = nwa_v[v];
+    d.wa_rng_v[v] = c->d_wa_rng[v].p;
+  }
   d.nwa = nwa;
-  d.wa_rng = c->d_wa_rng.p;
+  d.wa_rng = c->d_wa_rng[RP_WA_DEFAULT_VARIANT].p;
   d.pcell = c->d_pcell.p;
   d.ppos = c->d_ppos.p;
   d.ppin = c->d_ppin.p;
@@ -966,7 +977,7 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   S.max_back_track = p.max_back_track;
   S.num_partials_grad = npg;
   S.num_partials_bin = npb;
-  S.num_partials_net = npn2;
+  S.num_partials_net = nwa;
   S.target_overflow = p.target_overflow;
   S.min_phi_coef = p.min_phi_coef;
   S.reference_hpwl = p.reference_hpwl;

@@ -238,9 +238,6 @@ __device__ __forceinline__ float2 dct3_pre(float a_k, float a_nk, float b_k, flo
 // int64 atomic per touched bin; a bin that finds no free slot goes straight
 // to global memory.  Integer sums are order independent, so the result is
 // exact and deterministic whatever order the cells come in.
-#ifndef RP_AGG_BITS
-#define RP_AGG_BITS 12  // fused step + scatter (batched instances)
-#endif
 #ifndef RP_AGG_PROBES
 #define RP_AGG_PROBES 4
 #endif
@@ -429,17 +426,20 @@ __device__ __forceinline__ void box_scatter_win(const RpDev& d, WinTable& t, int
   }
 }
 
-// Fused step + scatter over RP_SCATTER_CELLS consecutive cells (batched
-// instances: small grids, where the locality order keeps a workgroup's bins
-// together).
+// Fused step + scatter over PER * RP_BLOCK consecutive cells into a
+// 2^BITS-slot table (batched instances: small grids, where the locality
+// order keeps a workgroup's bins together).  The batched step launches one
+// of the RP_SC_VARIANTS tile shapes per size bucket (chosen by the service's
+// tuner); the integer sums do not depend on the shape.
+template <int PER, int BITS>
 __device__ void scatter_body(const RpDev& d, int blk) {
-  __shared__ AggTable<RP_AGG_BITS> t;
+  __shared__ AggTable<BITS> t;
   const RpState* st = d.st;
   if (st->phase != 0) return;
   agg_clear(t);
   __syncthreads();
-  for (int r = 0; r < RP_SCATTER_PER_THREAD; r++) {
-    int c = blk * RP_SCATTER_CELLS + r * RP_BLOCK + threadIdx.x;
+  for (int r = 0; r < PER; r++) {
+    int c = blk * PER * RP_BLOCK + r * RP_BLOCK + threadIdx.x;
     if (c >= d.nc) break;
     int2 h = d.dhalf[c];
     int2 icc = eval_centre(d, st, c, h);
@@ -861,7 +861,8 @@ __device__ __forceinline__ float2 wa_pin_grad(int2 p, float4 v, const float4& sa
 // Weighted-average wirelength per workgroup of nets (updateWireLengthForceWA
 // and getWireLengthGradientPinWA, src/nesterovBase.cpp:1304-1440), pin
 // parallel.  The workgroup takes the nets d.wa_rng[blk] (consecutive, at
-// most RP_WA_NETS nets and RP_WA_CAP pins, or one larger net alone):
+// most RP_WA_NETS nets and CAP pins, or one larger net alone; the packing
+// for each of the RP_WA_VARIANTS capacities is built at set-up):
 //   1. pin centres into LDS (coalesced);
 //   2. one lane per net: pin bbox, the pin -> net map, HPWL term;
 //   3. one lane per pin: the four exponentials;
@@ -871,10 +872,11 @@ __device__ __forceinline__ float2 wa_pin_grad(int2 p, float4 v, const float4& sa
 //      order (coalesced; the cell gradient kernel gathers it).
 // The net bbox and sums are stored only when store_nets is set (stage-level
 // checks).
+template <int CAP>
 __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
-  __shared__ int2 s_pos[RP_WA_CAP];
-  __shared__ float4 s_ex[RP_WA_CAP];
-  __shared__ uint8_t s_net[RP_WA_CAP];  // net index within the workgroup (< RP_WA_NETS = 256)
+  __shared__ int2 s_pos[CAP];
+  __shared__ float4 s_ex[CAP];
+  __shared__ uint8_t s_net[CAP];  // net index within the workgroup (< RP_WA_NETS = 256)
   __shared__ int4 s_box[RP_WA_NETS];
   __shared__ float4 s_sum[2 * RP_WA_NETS];
   __shared__ long long sh_red[RP_BLOCK / 64];
@@ -886,7 +888,7 @@ __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
   const int npw = p1 - p0;
   const int tid = threadIdx.x;
   long long h = 0;
-  if (npw <= RP_WA_CAP) {
+  if (npw <= CAP) {
     for (int j = tid; j < npw; j += blockDim.x) s_pos[j] = d.ppin[p0 + j];
     __syncthreads();
     const int e = rng.x + tid;
@@ -928,7 +930,7 @@ __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
       d.pgrad[p0 + j] = wa_pin_grad(s_pos[j], s_ex[j], s_sum[2 * L], s_sum[2 * L + 1], cx, cy);
     }
   } else {
-    // one net with more than RP_WA_CAP pins: bbox by a block min/max, the
+    // one net with more than CAP pins: bbox by a block min/max, the
     // sums chunk by chunk (exponentials in parallel, summed in pin order by
     // lane 0), then the per-pin gradients with the exponentials recomputed
     // (same arguments, same values)
@@ -958,8 +960,8 @@ __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
       box.w = max(b.w, box.w);
     }
     WaSums w = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int c0 = 0; c0 < npw; c0 += RP_WA_CAP) {
-      const int cnt = min(RP_WA_CAP, npw - c0);
+    for (int c0 = 0; c0 < npw; c0 += CAP) {
+      const int cnt = min(CAP, npw - c0);
       __syncthreads();  // previous chunk consumed (and s_box read by everyone)
       for (int j = tid; j < cnt; j += blockDim.x) {
         int2 p = d.ppin[p0 + c0 + j];
@@ -1159,7 +1161,7 @@ __device__ void reduce_all(const RpDev& d, Totals& t) {
   // bin (overflow) and net (HPWL) partials, each thread a fixed strided subset
   double vb = 0.0;
   long long vh = 0;
-  const int nbp = st->num_partials_bin, nnp = st->num_partials_net;
+  const int nbp = st->num_partials_bin, nnp = d.nwa;  // one HPWL partial per wirelength workgroup
   for (int k0 = threadIdx.x; k0 < nbp; k0 += RP_RED_BATCH * blockDim.x) {
     double x[RP_RED_BATCH];
 #pragma unroll
@@ -1431,14 +1433,15 @@ __global__ void __launch_bounds__(RP_BLOCK) k_row_inverse(RpDev d, int want_phi)
   rowinv_body(d, blockIdx.x, want_phi);
 }
 __global__ void __launch_bounds__(RP_BLOCK) k_wa(RpDev d, int store_nets) {
-  wa_body(d, xcd_block(blockIdx.x, gridDim.x), store_nets);
+  wa_body<RP_WA_CAP>(d, xcd_block(blockIdx.x, gridDim.x), store_nets);
 }
 __global__ void __launch_bounds__(RP_BLOCK) k_grad(RpDev d) { grad_body(d, xcd_block(blockIdx.x, gridDim.x)); }
 __global__ void __launch_bounds__(RP_CTRL_BLOCK) k_control_main(RpDev d) { control_main_body(d); }
 
+template <int PER, int BITS>
 __global__ void __launch_bounds__(RP_BLOCK) kb_step_scatter(const RpDev* devs, const int2* tab) {
   int2 e = tab[blockIdx.x];
-  if (e.x >= 0) scatter_body(devs[e.x], e.y);
+  if (e.x >= 0) scatter_body<PER, BITS>(devs[e.x], e.y);
 }
 __global__ void __launch_bounds__(RP_BLOCK) kb_pin_pos(const RpDev* devs, const int2* tab) {
   int2 e = tab[blockIdx.x];
@@ -1456,9 +1459,10 @@ __global__ void __launch_bounds__(RP_BLOCK) kb_row_inverse(const RpDev* devs, co
   int2 e = tab[blockIdx.x];
   if (e.x >= 0) rowinv_body(devs[e.x], e.y, 0);
 }
+template <int CAP>
 __global__ void __launch_bounds__(RP_BLOCK) kb_wa(const RpDev* devs, const int2* tab) {
   int2 e = tab[blockIdx.x];
-  if (e.x >= 0) wa_body(devs[e.x], e.y, 0);
+  if (e.x >= 0) wa_body<CAP>(devs[e.x], e.y, 0);
 }
 __global__ void __launch_bounds__(RP_BLOCK) kb_grad(const RpDev* devs, const int2* tab) {
   int2 e = tab[blockIdx.x];
@@ -1627,10 +1631,34 @@ hipError_t rp_launch_gather_phase(const RpDev* devs, int n, int* out, hipStream_
 
 // One evaluation step of every instance in the tables (same kernels, block
 // sizes and per-instance logical blocks as rp_launch_eval).
+// Tile-shape variants of the batched scatter and wirelength kernels.
+hipError_t rp_launch_batch_variant(int kernel, int variant, const RpDev* devs, const int2* tab, int nblk,
+                                   hipStream_t s) {
+  if (nblk <= 0) return hipSuccess;
+  if (kernel == RP_K_SCATTER) {
+    if (variant == 0)
+      hipLaunchKernelGGL((kb_step_scatter<1, 10>), dim3(nblk), dim3(RP_BLOCK), 0, s, devs, tab);
+    else if (variant == 1)
+      hipLaunchKernelGGL((kb_step_scatter<2, 11>), dim3(nblk), dim3(RP_BLOCK), 0, s, devs, tab);
+    else
+      hipLaunchKernelGGL((kb_step_scatter<4, 12>), dim3(nblk), dim3(RP_BLOCK), 0, s, devs, tab);
+  } else if (kernel == RP_K_WA) {
+    if (variant == 0)
+      hipLaunchKernelGGL((kb_wa<256>), dim3(nblk), dim3(RP_BLOCK), 0, s, devs, tab);
+    else if (variant == 1)
+      hipLaunchKernelGGL((kb_wa<512>), dim3(nblk), dim3(RP_BLOCK), 0, s, devs, tab);
+    else
+      hipLaunchKernelGGL((kb_wa<1024>), dim3(nblk), dim3(RP_BLOCK), 0, s, devs, tab);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t rp_launch_eval_batch(const RpBatchLaunch& b, hipStream_t s) {
   const int* n = b.nblk;
-  if (n[RP_K_SCATTER])
-    hipLaunchKernelGGL(kb_step_scatter, dim3(n[RP_K_SCATTER]), dim3(RP_BLOCK), 0, s, b.devs, b.tab[RP_K_SCATTER]);
+  for (int v = 0; v < RP_SC_VARIANTS; v++)
+    if (b.nblk_sc[v]) (void)rp_launch_batch_variant(RP_K_SCATTER, v, b.devs, b.tab_sc[v], b.nblk_sc[v], s);
   if (n[RP_K_PIN_POS])
     hipLaunchKernelGGL(kb_pin_pos, dim3(n[RP_K_PIN_POS]), dim3(RP_BLOCK), 0, s, b.devs, b.tab[RP_K_PIN_POS]);
   if (n[RP_K_BINFIN_ROWDCT])
@@ -1642,7 +1670,8 @@ hipError_t rp_launch_eval_batch(const RpBatchLaunch& b, hipStream_t s) {
   if (n[RP_K_ROW_INVERSE])
     hipLaunchKernelGGL(kb_row_inverse, dim3(n[RP_K_ROW_INVERSE]), dim3(RP_BLOCK), b.lds_rinv, s, b.devs,
                        b.tab[RP_K_ROW_INVERSE]);
-  if (n[RP_K_WA]) hipLaunchKernelGGL(kb_wa, dim3(n[RP_K_WA]), dim3(RP_BLOCK), 0, s, b.devs, b.tab[RP_K_WA]);
+  for (int v = 0; v < RP_WA_VARIANTS; v++)
+    if (b.nblk_wa[v]) (void)rp_launch_batch_variant(RP_K_WA, v, b.devs, b.tab_wa[v], b.nblk_wa[v], s);
   if (n[RP_K_GRAD]) hipLaunchKernelGGL(kb_grad, dim3(n[RP_K_GRAD]), dim3(RP_BLOCK), 0, s, b.devs, b.tab[RP_K_GRAD]);
   if (n[RP_K_CONTROL])
     hipLaunchKernelGGL(kb_control_main, dim3(n[RP_K_CONTROL]), dim3(RP_CTRL_BLOCK), 0, s, b.devs,

@@ -325,6 +325,21 @@ class RpServerOpts(ctypes.Structure):
         ("setup_threads", ctypes.c_int32),
         ("steps_per_poll", ctypes.c_int32),
         ("use_graph", ctypes.c_int32),
+        ("autotune", ctypes.c_int32),
+    ]
+
+
+class RpTileChoice(ctypes.Structure):
+    """ctypes mirror of ``rp_tile_choice``."""
+
+    _fields_ = [
+        ("bucket", ctypes.c_int32),
+        ("tuned", ctypes.c_int32),
+        ("jobs_sampled", ctypes.c_int32),
+        ("scatter_cells", ctypes.c_int32),
+        ("wa_pins", ctypes.c_int32),
+        ("scatter_us", ctypes.c_float * 3),
+        ("wa_us", ctypes.c_float * 3),
     ]
 
 

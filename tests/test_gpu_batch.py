@@ -78,3 +78,35 @@ def test_batch_with_graph_and_without_agree():
         srv.run()
         out.append([srv.records(j) for j in ids])
     assert out[0] == out[1]
+
+
+@pytest.mark.parametrize("cells,pins", [(256, 256), (512, 1024), (1024, 512)])
+def test_every_tile_shape_identical_to_alone(cells, pins):
+    """The tile shapes the tuner chooses between give bit-identical jobs."""
+    jobs = _jobs()
+    alone = [_alone(nl, p) for nl, p in jobs]
+    srv = Server(steps_per_poll=4)
+    for b in range(5):
+        srv.set_tiles(b, cells, pins)
+    ids = [srv.submit(nl, p, 0.0) for nl, p in jobs]
+    srv.run()
+    for jid, (code, st, rec, pos) in zip(ids, alone):
+        assert srv.records(jid) == rec
+        lx, ly = srv.positions(jid)
+        assert np.array_equal(lx, pos[0]) and np.array_equal(ly, pos[1])
+    assert all(t["scatter_cells"] == cells and t["wa_pins"] == pins for t in srv.tuning())
+
+
+def test_tuner_times_every_shape_of_the_buckets_present():
+    jobs = _jobs()
+    srv = Server(steps_per_poll=4, autotune=1)
+    ids = [srv.submit(nl, p, 0.0) for nl, p in jobs]
+    srv.run()
+    assert all(srv.result(j).done == 1 for j in ids)
+    tiles = {t["bucket"]: t for t in srv.tuning()}
+    for b in (0, 1, 2):  # 294-cell designs and the 700 / 2.5k / 9k-cell jobs
+        t = tiles[b]
+        assert t["tuned"] == 1 and t["jobs_sampled"] >= 1
+        assert min(t["scatter_us"]) > 0 and min(t["wa_us"]) > 0
+        assert t["scatter_cells"] == 256 << int(np.argmin(t["scatter_us"]))
+        assert t["wa_pins"] == 256 << int(np.argmin(t["wa_us"]))
