@@ -117,6 +117,8 @@ print(json.dumps(dict(init_s=t1 - t0, iter_s=t2 - t1, evals=st.evals, poisson=o.
 
 
 _PHASES = {"create": 0.0, "initial_place": 0.0, "setup": 0.0, "nesterov": 0.0}
+SETUP_THREADS = [16]  # host workers of the placement service (--setup-threads)
+TILES = [None]  # tile shapes the service's tuner chose
 
 
 def _plain_job(nl, p):
@@ -145,13 +147,14 @@ def _served_common(jobs, arrivals, steps_per_poll):
     service times)."""
     from replace_amd.capi import Server
     _plain_job(*jobs[0])  # warm-up: module load, first allocations
-    srv = Server(steps_per_poll=steps_per_poll)
+    srv = Server(steps_per_poll=steps_per_poll, setup_threads=SETUP_THREADS[0])
     ids = [srv.submit(nl, p, a) for (nl, p), a in zip(jobs, arrivals)]
     t = time.perf_counter()
     srv.run()
     wall = time.perf_counter() - t
     res = [srv.result(j) for j in ids]
     stats = srv.stats()
+    TILES[0] = [t for t in srv.tuning() if t["tuned"]]
     srv.close()
     plain, svc = [], []
     for k in _PHASES:
@@ -206,7 +209,8 @@ def bench_batch(args):
                                "phases_s": {k: round(v, 3) for k, v in _PHASES.items()}},
         "speedup_vs_plain": round(plain_s / wall, 2),
         "batched_steps": int(stats.steps), "table_rebuilds": int(stats.rebuilds),
-        "netgen_s": round(gen_s, 2), "timeline": _timeline(res),
+        "netgen_s": round(gen_s, 2), "timeline": _timeline(res), "setup_threads": args.setup_threads,
+        "tiles": TILES[0],
     }
     out.update(_parity(res, plain, jobs))
     print(json.dumps(out), flush=True)
@@ -319,7 +323,9 @@ def main():
     ap.add_argument("--rate", type=float, default=27.0,
                     help="c4: arrival rate, jobs/s (default: ~80%% of the measured 34/s batched capacity)")
     ap.add_argument("--steps-per-poll", type=int, default=8, help="c3/c4: batched steps per graph replay")
+    ap.add_argument("--setup-threads", type=int, default=16, help="c3/c4: host set-up workers of the service")
     args = ap.parse_args()
+    SETUP_THREADS[0] = args.setup_threads
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build places one instance per GPU and "
               "has no multi-GPU path (see DESIGN.md for how it would shard)", file=sys.stderr)

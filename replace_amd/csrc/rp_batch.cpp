@@ -45,6 +45,7 @@ using clk = std::chrono::steady_clock;
 
 struct Job {
   int id = 0;
+  int est_cells = 0;  // movable instances (the size bucket before set-up counts fillers too)
   double arrival = 0;
   std::vector<int32_t> rows, insts, nstart, pinst, poff, pbox, ips, ipn;
   std::vector<uint8_t> fixed;
@@ -106,8 +107,10 @@ void build_table(const std::vector<int>& counts, std::vector<int2>& tab) {
     for (size_t j = 0; j < lists[x].size(); j++) tab[j * X + x] = lists[x][j];
 }
 
-constexpr int kTuneJobs = 16;  // jobs per bucket the tuner times
-constexpr int kTuneReps = 3;   // timed launches per shape
+constexpr int kTuneJobs = 16;   // jobs per bucket the tuner times
+constexpr int kTuneMinJobs = 4;  // running jobs a bucket needs before it is tuned
+constexpr int kTuneReps = 5;     // timed launches per shape (the fastest counts: other
+                                 // streams' set-up kernels can share the GPU meanwhile)
 
 int size_bucket(int ncells) {
   if (ncells < 2000) return 0;
@@ -166,7 +169,7 @@ static int sfail(rp_server* s, int code, const char* msg) {
 extern "C" void rp_server_opts_default(rp_server_opts* o) {
   o->hbm_budget_bytes = (int64_t)200 << 30;
   o->max_active = 512;
-  o->setup_threads = 8;
+  o->setup_threads = 16;
   o->steps_per_poll = 8;
   o->use_graph = 1;
   o->autotune = 1;
@@ -217,14 +220,16 @@ static int tune_bucket(rp_server* s, int b, const std::vector<int>& idx, const s
     if (s->d_tune_tab.n < tab.size()) SCHK(s, dalloc(s->d_tune_tab, 2 * tab.size()));
     SCHK(s, hipMemcpyAsync(s->d_tune_tab.p, tab.data(), sizeof(int2) * tab.size(), hipMemcpyHostToDevice, s->stream));
     SCHK(s, rp_launch_batch_variant(kernel, v, d_sample.p, s->d_tune_tab.p, (int)tab.size(), s->stream));  // warm
-    SCHK(s, hipEventRecord(e0, s->stream));
-    for (int r = 0; r < kTuneReps; r++)
+    us = 1e30f;
+    for (int r = 0; r < kTuneReps; r++) {
+      SCHK(s, hipEventRecord(e0, s->stream));
       SCHK(s, rp_launch_batch_variant(kernel, v, d_sample.p, s->d_tune_tab.p, (int)tab.size(), s->stream));
-    SCHK(s, hipEventRecord(e1, s->stream));
-    SCHK(s, hipEventSynchronize(e1));
-    float ms = 0;
-    SCHK(s, hipEventElapsedTime(&ms, e0, e1));
-    us = ms * 1e3f / kTuneReps;
+      SCHK(s, hipEventRecord(e1, s->stream));
+      SCHK(s, hipEventSynchronize(e1));
+      float ms = 0;
+      SCHK(s, hipEventElapsedTime(&ms, e0, e1));
+      us = std::min(us, ms * 1e3f);
+    }
     return RP_OK;
   };
   int best_sc = 0, best_wa = 0;
@@ -338,6 +343,7 @@ extern "C" int rp_server_submit(rp_server* s, const rp_netlist* nl, const rp_par
   j->nl.pin_box = nl->pin_box ? j->pbox.data() : nullptr;
   j->nl.inst_pin_start = j->ips.empty() ? nullptr : j->ips.data();
   j->nl.inst_pins = j->ips.empty() ? nullptr : j->ipn.data();
+  for (uint8_t f : j->fixed) j->est_cells += f ? 0 : 1;
   if (job) *job = j->id;
   s->jobs.push_back(std::move(j));
   return RP_OK;
@@ -356,8 +362,18 @@ static int rebuild(rp_server* s) {
       devs[i] = s->jobs[s->running[i]]->ctx->dev;
       in_bucket[size_bucket(devs[i].nc)].push_back(i);
     }
+    // a bucket is tuned once it has kTuneMinJobs running jobs, or when no
+    // job of its size is still queued or being set up
+    int waiting[RP_NUM_SIZE_BUCKETS] = {0};
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      for (auto& jp : s->jobs)
+        if (jp->status == Job::QUEUED || jp->status == Job::SETUP || jp->status == Job::READY)
+          waiting[size_bucket(jp->est_cells)]++;
+    }
     for (int b = 0; b < RP_NUM_SIZE_BUCKETS; b++)
-      if (!s->tiles[b].tuned && !in_bucket[b].empty())
+      if (!s->tiles[b].tuned && !in_bucket[b].empty() &&
+          ((int)in_bucket[b].size() >= kTuneMinJobs || waiting[b] == 0))
         if (int r = tune_bucket(s, b, in_bucket[b], devs)) return r;
   }
   for (int i = 0; i < n; i++) {
