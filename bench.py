@@ -290,7 +290,9 @@ def bench_precision(args):
                      "kernels_us": {k: round(v[0] / max(1, v[1]) * 1e3, 2) for k, v in kt.items()}}
         g.close()
     N = 2048
-    flops = 3 * 2 * 12 * N ** 3  # 5 GEMMs without phi: 12 N^3 multiply-adds, 3 split products
+    # 5 products without phi: N^3 + N^3 + 2 N^3 + N^3 + N^3 = 6 N^3 multiply-adds, each for
+    # 3 split components (hi.hi, hi.lo, lo.hi), 2 flops per multiply-add
+    flops = 3 * 2 * 6 * N ** 3
     poisson_us = out[1]["kernels_us"]["mfma_poisson"]
     achieved = flops / (poisson_us * 1e-6) / 1e12
     res = {

@@ -123,7 +123,7 @@ typedef struct rp_params {
   /* Poisson transform (no reference counterpart; additive):
    *   0  FP32 FFT-based DCT/DST (default; the reference's fftsg arithmetic class)
    *   1  dense transform-matrix GEMMs on matrix cores, split-BF16 operands
-   *      (hi + lo) with FP32 accumulation; bin counts must be multiples of 128 */
+   *      (hi + lo) with FP32 accumulation, run by hipBLASLt */
   int32_t poisson_transform;              /* 0 */
 } rp_params;
 

@@ -49,7 +49,7 @@ def build_lib() -> str:
         lang = ["-x", "hip"]
         _run([HIPCC, *COMMON, *lang, "-c", path, "-o", obj])
         objs.append(obj)
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", LIB, *objs])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", LIB, *objs, "-lhipblaslt"])
     for o in objs:
         os.remove(o)
     return LIB

@@ -23,26 +23,18 @@ struct DevBuf {
   }
 };
 
-// Matrix-core Poisson buffers (rp_dct_mfma.hip); bf16 data held as 16-bit words
+// Matrix-core Poisson solve (rp_dct_mfma.hip): an opaque plan (GEMM
+// descriptors, split-BF16 matrices, intermediates), released with the context.
 struct RpMfma {
   bool ready = false;
-  DevBuf<uint16_t> cx_hi, cx_lo, cxs_hi, cxs_lo, cy_hi, cy_lo, cyt_hi, cyt_lo, sy_hi, sy_lo;
-  DevBuf<float> wx2, wy2;
-  DevBuf<uint16_t> rho_hi, rho_lo, t1_hi, t1_lo, p_hi, p_lo, r_hi, r_lo;
-  DevBuf<double> sums;
-  RpMfmaDev view() const {
-    auto b = [](const DevBuf<uint16_t>& x) { return (__bf16*)x.p; };
-    return RpMfmaDev{b(cx_hi), b(cx_lo), b(cxs_hi), b(cxs_lo), b(cy_hi), b(cy_lo), b(cyt_hi), b(cyt_lo),
-                     b(sy_hi), b(sy_lo), wx2.p, wy2.p, b(rho_hi), b(rho_lo), b(t1_hi), b(t1_lo),
-                     b(p_hi), b(p_lo), b(r_hi), b(r_lo), sums.p};
-  }
-  size_t bytes() const {
-    size_t t = 0;
-    for (const DevBuf<uint16_t>* x : {&cx_hi, &cx_lo, &cxs_hi, &cxs_lo, &cy_hi, &cy_lo, &cyt_hi, &cyt_lo, &sy_hi,
-                                      &sy_lo, &rho_hi, &rho_lo, &t1_hi, &t1_lo, &p_hi, &p_lo, &r_hi, &r_lo})
-      t += x->n * 2;
-    return t + (wx2.n + wy2.n) * 4 + sums.n * 8;
-  }
+  void* plan = nullptr;
+  size_t nbytes = 0;
+  RpMfmaDev view() const { return RpMfmaDev{plan}; }
+  size_t bytes() const { return nbytes; }
+  RpMfma() = default;
+  RpMfma(const RpMfma&) = delete;
+  RpMfma& operator=(const RpMfma&) = delete;
+  ~RpMfma();
 };
 
 struct rp_ctx {

@@ -14,50 +14,65 @@
 // constant transform matrix and whose B is the data, so both operands are
 // read along k and no transpose pass is needed:
 //   G1  T1t[u][j]          = Cx[u][i]            . rho[j][i]
-//   G2  P[v][u]            = Cy[v][j]            . T1t[u][j]     (spectral scaling in the epilogue)
+//   G2  P[v][u]            = Cy[v][j]            . T1t[u][j]     (spectral scaling after it)
 //   G3  [R1t; R2t][i][v]   = [Cxt; wx Sxt][i][u] . P[v][u]
 //   G4  [phi | Ex][j][.]   = Cyt[j][v]           . [R1t; R2t][.][v]
 //   G5  Ey[j][i]           = (wy Syt)[j][v]      . R1t[i][v]
-// 14 N^3 multiply-adds per evaluation at nx = ny = N (12 N^3 without phi).
-// Each product of split operands is hi.hi + hi.lo + lo.hi (the lo.lo term is
-// below FP32 rounding of the accumulation), i.e. three bf16 MFMAs.
+//
+// Split products: with A = Ahi + Alo and B = Bhi + Blo (bf16 each),
+// A.B ~ Ahi.Bhi + Ahi.Blo + Alo.Bhi (the lo.lo term is below the FP32
+// rounding of the accumulation).  That is ONE plain bf16 GEMM with FP32
+// accumulation over concatenated operands, A' = [Ahi | Ahi | Alo] and
+// B' = [Bhi | Blo | Bhi] (K' = 3K), run by hipBLASLt on the matrix cores.  The
+// constant A' are built once per set-up; the data operands B' are written by
+// this file's kernels between the GEMMs (split, spectral scaling, the scatter
+// of the last products into phi and the field).  Per evaluation at nx = ny = N
+// the products are N^3 + N^3 + 2 N^3 + N^3 + N^3 = 6 N^3 multiply-adds (7 N^3
+// with phi), each over the 3 split components.  For each GEMM
+// the set-up times every algorithm hipBLASLt's heuristic offers and keeps the
+// fastest.
+#include <hipblaslt/hipblaslt.h>
+
 #include <cmath>
 
 #include "rp_ctx.h"
 
 typedef __bf16 rp_bf16;
-typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
-typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
 
 namespace {
 
-constexpr int GT = 128;      // block tile (M and N)
-constexpr int GK = 32;       // k step staged in LDS
-constexpr int GKP = GK + 8;  // padded LDS row (bank spread for 16-byte fragment reads)
+constexpr int MEAN_PARTS = 256;
+constexpr size_t kWorkspace = (size_t)64 << 20;
+constexpr int kAlgos = 8;  // heuristic candidates timed per GEMM
 
-enum { EPI_SPLIT = 0, EPI_SPECTRAL = 1, EPI_PHI_EX = 2, EPI_EY = 3 };
+enum { G1 = 0, G2, G3, G4_PHI, G4, G5, NGEMM };
 
-struct SplitMat {
-  const rp_bf16* hi;
-  const rp_bf16* lo;
-  int ld;
+struct Gemm {
+  hipblasLtMatmulDesc_t desc = nullptr;
+  hipblasLtMatrixLayout_t lb = nullptr, la = nullptr, lc = nullptr;
+  hipblasLtMatmulAlgo_t algo{};
+  int M = 0, N = 0, K = 0;  // C[M][N] = sum over 3K of A'[M][3K] B'[N][3K]
+  float us = 0;             // measured time of the chosen algorithm
 };
 
-struct GemmArgs {
-  SplitMat a, b;
-  int M, N, K;
-  // EPI_SPLIT / EPI_SPECTRAL output
-  rp_bf16* ohi;
-  rp_bf16* olo;
-  int ldc;
-  // EPI_SPECTRAL
-  const float* wx2;  // (pi u / nx)^2
-  const float* wy2;  // (pi v / ny * asp)^2
-  float wscale;      // 4 / (nx ny)
-  // EPI_PHI_EX / EPI_EY
-  float* phi;
-  float2* field;
-  int nx, col0;  // col0: first output column's index in [phi | Ex]
+struct Plan {
+  hipblasLtHandle_t h = nullptr;
+  Gemm g[NGEMM];
+  DevBuf<uint16_t> a[5];                  // A' of G1, G2, G3, G4, G5 (row-major M x 3K)
+  DevBuf<uint16_t> rho3, t13, p3, r3;     // B' of G1, G2, G3, G4/G5 (row-major N x 3K)
+  DevBuf<float> c;                        // FP32 product (largest: 2N x N)
+  DevBuf<float> wx2, wy2;
+  DevBuf<double> sums;                    // mean of rho: partials, then the mean
+  DevBuf<char> ws;
+  ~Plan() {
+    for (Gemm& x : g) {
+      if (x.desc) hipblasLtMatmulDescDestroy(x.desc);
+      if (x.la) hipblasLtMatrixLayoutDestroy(x.la);
+      if (x.lb) hipblasLtMatrixLayoutDestroy(x.lb);
+      if (x.lc) hipblasLtMatrixLayoutDestroy(x.lc);
+    }
+    if (h) hipblasLtDestroy(h);
+  }
 };
 
 __device__ __forceinline__ void split_bf16(float v, rp_bf16& h, rp_bf16& l) {
@@ -65,118 +80,13 @@ __device__ __forceinline__ void split_bf16(float v, rp_bf16& h, rp_bf16& l) {
   l = (rp_bf16)(v - (float)h);
 }
 
-template <int EPI>
-__device__ __forceinline__ void epilogue(const GemmArgs& g, int m, int n, float v) {
-  if (EPI == EPI_SPLIT) {
-    rp_bf16 h, l;
-    split_bf16(v, h, l);
-    g.ohi[(size_t)m * g.ldc + n] = h;
-    g.olo[(size_t)m * g.ldc + n] = l;
-  } else if (EPI == EPI_SPECTRAL) {  // m = v, n = u
-    float s = 0.0f;
-    if (m != 0 || n != 0) {
-      float w = g.wscale;
-      if (n == 0) w *= 0.5f;
-      if (m == 0) w *= 0.5f;
-      s = w / (g.wx2[n] + g.wy2[m]);
-    }
-    rp_bf16 h, l;
-    split_bf16(v * s, h, l);
-    g.ohi[(size_t)m * g.ldc + n] = h;
-    g.olo[(size_t)m * g.ldc + n] = l;
-  } else if (EPI == EPI_PHI_EX) {  // m = j, column col0 + n of [phi | Ex]
-    int c = g.col0 + n;
-    if (c < g.nx)
-      g.phi[(size_t)m * g.nx + c] = v;
-    else
-      g.field[(size_t)m * g.nx + (c - g.nx)].x = v;
-  } else {  // EPI_EY
-    g.field[(size_t)m * g.nx + n].y = v;
-  }
-}
-
-// One 128 x 128 output tile per 256-thread workgroup; each of the four waves
-// owns a 64 x 64 quarter as 2 x 2 tiles of v_mfma_f32_32x32x16_bf16.
-// Fragment layout of the 32x32x16 instruction: lane l supplies row (l % 32)
-// of A and column (l % 32) of B with k = 8 (l / 32) .. +7; accumulator
-// element e of lane l is row 8 (e / 4) + 4 (l / 32) + e % 4, column l % 32.
-template <int EPI>
-__global__ void __launch_bounds__(256) k_gemm_split(GemmArgs g) {
-  __shared__ __attribute__((aligned(16))) rp_bf16 As[2][GT][GKP];
-  __shared__ __attribute__((aligned(16))) rp_bf16 Bs[2][GT][GKP];
-  const int tiles_n = g.N / GT;
-  const int nt = tiles_n * (g.M / GT);
-  // XCD-aware order: hardware block b runs on XCD b % 8; give each XCD a
-  // contiguous run of tiles (neighbouring tiles share A rows in L2)
-  int b = blockIdx.x;
-  if (nt % 8 == 0) b = (b % 8) * (nt / 8) + b / 8;
-  const int tm = b / tiles_n, tn = b % tiles_n;
-  const int bm = tm * GT, bn = tn * GT;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-
-  floatx16 acc[2][2];
-  for (int i = 0; i < 2; i++)
-    for (int j = 0; j < 2; j++)
-      for (int e = 0; e < 16; e++) acc[i][j][e] = 0.0f;
-
-  // staging: 128 rows x 32 k = 512 chunks of 8 bf16 per matrix and part
-  uint4 ra[2][2], rb[2][2];  // [part][chunk]
-  auto gload = [&](int k0) {
-    for (int c = 0; c < 2; c++) {
-      int ch = tid + c * 256, row = ch >> 2, kc = (ch & 3) * 8;
-      size_t oa = (size_t)(bm + row) * g.a.ld + k0 + kc;
-      size_t ob = (size_t)(bn + row) * g.b.ld + k0 + kc;
-      ra[0][c] = *(const uint4*)(g.a.hi + oa);
-      ra[1][c] = *(const uint4*)(g.a.lo + oa);
-      rb[0][c] = *(const uint4*)(g.b.hi + ob);
-      rb[1][c] = *(const uint4*)(g.b.lo + ob);
-    }
-  };
-  auto lstore = [&]() {
-    for (int c = 0; c < 2; c++) {
-      int ch = tid + c * 256, row = ch >> 2, kc = (ch & 3) * 8;
-      for (int p = 0; p < 2; p++) {
-        *(uint4*)&As[p][row][kc] = ra[p][c];
-        *(uint4*)&Bs[p][row][kc] = rb[p][c];
-      }
-    }
-  };
-  gload(0);
-  lstore();
-  __syncthreads();
-  const int fr = lane & 31, fk = (lane >> 5) * 8;
-  for (int k0 = 0; k0 < g.K; k0 += GK) {
-    const bool more = k0 + GK < g.K;
-    if (more) gload(k0 + GK);
-    for (int kk = 0; kk < GK; kk += 16) {
-      bf16x8 ah[2], al[2], bh[2], bl[2];
-      for (int i = 0; i < 2; i++) {
-        ah[i] = *(const bf16x8*)&As[0][wm + i * 32 + fr][kk + fk];
-        al[i] = *(const bf16x8*)&As[1][wm + i * 32 + fr][kk + fk];
-        bh[i] = *(const bf16x8*)&Bs[0][wn + i * 32 + fr][kk + fk];
-        bl[i] = *(const bf16x8*)&Bs[1][wn + i * 32 + fr][kk + fk];
-      }
-      for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-        }
-    }
-    __syncthreads();
-    if (more) {
-      lstore();
-      __syncthreads();
-    }
-  }
-  for (int i = 0; i < 2; i++)
-    for (int j = 0; j < 2; j++)
-      for (int e = 0; e < 16; e++) {
-        int m = bm + wm + i * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-        int n = bn + wn + j * 32 + (lane & 31);
-        epilogue<EPI>(g, m, n, acc[i][j][e]);
-      }
+// Row n of a B' operand: [hi | lo | hi] of n values x[0..k).
+__device__ __forceinline__ void put_split3(rp_bf16* row, int k, int i, float v) {
+  rp_bf16 h, l;
+  split_bf16(v, h, l);
+  row[i] = h;
+  row[k + i] = l;
+  row[2 * k + i] = h;
 }
 
 // Mean of rho (fixed-order FP64 partials, then one block).  Subtracting a
@@ -184,8 +94,6 @@ __global__ void __launch_bounds__(256) k_gemm_split(GemmArgs g) {
 // spectral step discards, so the transform of (rho - mean) is exact; it keeps
 // the 16-bit rounding of the large mean out of the low modes that the
 // spectral division amplifies.
-constexpr int MEAN_PARTS = 256;
-
 __device__ double block_sum(double v, double* sh) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
@@ -210,23 +118,52 @@ __global__ void __launch_bounds__(256) k_rho_mean(const double* parts, size_t n,
   if (threadIdx.x == 0) mean[0] = s / (double)n;
 }
 
-// rho (FP32, [ny][nx]) - mean -> split pair
-__global__ void k_split(const float* __restrict__ x, const double* __restrict__ mean, rp_bf16* __restrict__ hi,
-                        rp_bf16* __restrict__ lo, size_t n) {
+// rho (FP32, [ny][nx]) - mean -> B' of G1 (rows j, k = i)
+__global__ void __launch_bounds__(256) k_split_rho(const float* __restrict__ x, const double* __restrict__ mean,
+                                                   rp_bf16* __restrict__ out, int nx, int ny) {
   size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) {
-    rp_bf16 h, l;
-    split_bf16((float)((double)x[k] - mean[0]), h, l);
-    hi[k] = h;
-    lo[k] = l;
-  }
+  if (k >= (size_t)nx * ny) return;
+  int j = (int)(k / nx), i = (int)(k % nx);
+  put_split3(out + (size_t)j * 3 * nx, nx, i, (float)((double)x[k] - mean[0]));
 }
 
-template <int EPI>
-hipError_t gemm(const GemmArgs& g, hipStream_t s) {
-  int nt = (g.M / GT) * (g.N / GT);
-  hipLaunchKernelGGL(k_gemm_split<EPI>, dim3(nt), dim3(256), 0, s, g);
-  return hipGetLastError();
+// FP32 product C[m][n] -> B' rows m (k = n); with `spectral` the scaling of
+// FFT::doFFT first: w(u,v) / (wx_u^2 + wy_v^2), m = v, n = u, 0 at u = v = 0.
+__global__ void __launch_bounds__(256) k_split_c(const float* __restrict__ c, rp_bf16* __restrict__ out, int M,
+                                                 int N, int spectral, const float* __restrict__ wx2,
+                                                 const float* __restrict__ wy2, float wscale) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)M * N) return;
+  int m = (int)(k / N), n = (int)(k % N);
+  float v = c[k];
+  if (spectral) {
+    float s = 0.0f;
+    if (m != 0 || n != 0) {
+      float w = wscale;
+      if (n == 0) w *= 0.5f;
+      if (m == 0) w *= 0.5f;
+      s = w / (wx2[n] + wy2[m]);
+    }
+    v = v * s;
+  }
+  put_split3(out + (size_t)m * 3 * N, N, n, v);
+}
+
+// G4's [phi | Ex][j][c] (columns from col0) and G5's Ey[j][i] into phi and the field
+__global__ void __launch_bounds__(256) k_phi_ex(const float* __restrict__ c, float* __restrict__ phi,
+                                                float2* __restrict__ field, int nx, int ny, int ncol, int col0) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)ny * ncol) return;
+  int j = (int)(k / ncol), cc = col0 + (int)(k % ncol);
+  if (cc < nx)
+    phi[(size_t)j * nx + cc] = c[k];
+  else
+    field[(size_t)j * nx + (cc - nx)].x = c[k];
+}
+
+__global__ void __launch_bounds__(256) k_ey(const float* __restrict__ c, float2* __restrict__ field, size_t n) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) field[k].y = c[k];
 }
 
 // bf16 round-to-nearest-even of a float (host)
@@ -244,14 +181,19 @@ float bf16_float(uint16_t b) {
   return f;
 }
 
-void split_host(const std::vector<double>& v, std::vector<uint16_t>& hi, std::vector<uint16_t>& lo) {
-  hi.resize(v.size());
-  lo.resize(v.size());
-  for (size_t k = 0; k < v.size(); k++) {
-    float f = (float)v[k];
-    hi[k] = bf16_bits(f);
-    lo[k] = bf16_bits(f - bf16_float(hi[k]));
-  }
+// A' = [hi | hi | lo] of a row-major M x K matrix
+std::vector<uint16_t> split3_const(const std::vector<double>& v, int M, int K) {
+  std::vector<uint16_t> out((size_t)M * 3 * K);
+  for (int m = 0; m < M; m++)
+    for (int k = 0; k < K; k++) {
+      float f = (float)v[(size_t)m * K + k];
+      uint16_t h = bf16_bits(f), l = bf16_bits(f - bf16_float(h));
+      uint16_t* row = &out[(size_t)m * 3 * K];
+      row[k] = h;
+      row[K + k] = h;
+      row[2 * K + k] = l;
+    }
+  return out;
 }
 
 // cos / sin of pi * num / den with the angle reduced exactly in integers
@@ -264,121 +206,205 @@ double sinpi_frac(long long num, long long den) {
   return std::sin(M_PI * (double)r / (double)den);
 }
 
+#define BLT(c, x)                                                                        \
+  do {                                                                                   \
+    hipblasStatus_t e_ = (x);                                                            \
+    if (e_ != HIPBLAS_STATUS_SUCCESS) return fail(c, RP_ERR_HIP, "%s: status %d", #x, (int)e_); \
+  } while (0)
+
+// Row-major C[M][N] = A'[M][3K] . B'[N][3K]^T in hipBLASLt's column-major
+// terms: D (N x M, ld N) = op(X) op(Y) with X = B' seen as 3K x N (transposed)
+// and Y = A' seen as 3K x M.
+int make_gemm(rp_ctx* c, Plan& P, Gemm& g, int M, int N, int K) {
+  g.M = M, g.N = N, g.K = K;
+  const int K3 = 3 * K;
+  BLT(c, hipblasLtMatmulDescCreate(&g.desc, HIPBLAS_COMPUTE_32F, HIP_R_32F));
+  hipblasOperation_t ta = HIPBLAS_OP_T, tb = HIPBLAS_OP_N;
+  BLT(c, hipblasLtMatmulDescSetAttribute(g.desc, HIPBLASLT_MATMUL_DESC_TRANSA, &ta, sizeof ta));
+  BLT(c, hipblasLtMatmulDescSetAttribute(g.desc, HIPBLASLT_MATMUL_DESC_TRANSB, &tb, sizeof tb));
+  BLT(c, hipblasLtMatrixLayoutCreate(&g.lb, HIP_R_16BF, K3, N, K3));
+  BLT(c, hipblasLtMatrixLayoutCreate(&g.la, HIP_R_16BF, K3, M, K3));
+  BLT(c, hipblasLtMatrixLayoutCreate(&g.lc, HIP_R_32F, N, M, N));
+  hipblasLtMatmulPreference_t pref;
+  BLT(c, hipblasLtMatmulPreferenceCreate(&pref));
+  size_t wsz = kWorkspace;
+  hipblasStatus_t st =
+      hipblasLtMatmulPreferenceSetAttribute(pref, HIPBLASLT_MATMUL_PREF_MAX_WORKSPACE_BYTES, &wsz, sizeof wsz);
+  hipblasLtMatmulHeuristicResult_t res[kAlgos];
+  int n = 0;
+  if (st == HIPBLAS_STATUS_SUCCESS)
+    st = hipblasLtMatmulAlgoGetHeuristic(P.h, g.desc, g.lb, g.la, g.lc, g.lc, pref, kAlgos, res, &n);
+  hipblasLtMatmulPreferenceDestroy(pref);
+  BLT(c, st);
+  if (n <= 0) return fail(c, RP_ERR_HIP, "hipBLASLt has no bf16 GEMM for %d x %d x %d", M, N, K3);
+  g.algo = res[0].algo;
+  return 0;
+}
+
+hipblasStatus_t run_gemm(const Plan& P, const Gemm& g, const uint16_t* a, const uint16_t* b, float* out,
+                         hipStream_t s, const hipblasLtMatmulAlgo_t* algo = nullptr) {
+  const float alpha = 1.0f, beta = 0.0f;
+  return hipblasLtMatmul(P.h, g.desc, &alpha, b, g.lb, a, g.la, &beta, out, g.lc, out, g.lc,
+                         algo ? algo : &g.algo, P.ws.p, kWorkspace, s);
+}
+
+// Times every heuristic candidate of g on its real operands, keeps the fastest.
+int tune_gemm(rp_ctx* c, Plan& P, Gemm& g, const uint16_t* a, const uint16_t* b, hipStream_t s) {
+  hipblasLtMatmulPreference_t pref;
+  BLT(c, hipblasLtMatmulPreferenceCreate(&pref));
+  size_t wsz = kWorkspace;
+  hipblasLtMatmulPreferenceSetAttribute(pref, HIPBLASLT_MATMUL_PREF_MAX_WORKSPACE_BYTES, &wsz, sizeof wsz);
+  hipblasLtMatmulHeuristicResult_t res[kAlgos];
+  int n = 0;
+  hipblasStatus_t st = hipblasLtMatmulAlgoGetHeuristic(P.h, g.desc, g.lb, g.la, g.lc, g.lc, pref, kAlgos, res, &n);
+  hipblasLtMatmulPreferenceDestroy(pref);
+  BLT(c, st);
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  float best = 1e30f;
+  for (int k = 0; k < n; k++) {
+    if (run_gemm(P, g, a, b, P.c.p, s, &res[k].algo) != HIPBLAS_STATUS_SUCCESS) continue;  // warm
+    HIPCHK(c, hipEventRecord(e0, s));
+    for (int r = 0; r < 3; r++) BLT(c, run_gemm(P, g, a, b, P.c.p, s, &res[k].algo));
+    HIPCHK(c, hipEventRecord(e1, s));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    if (ms / 3 < best) {
+      best = ms / 3;
+      g.algo = res[k].algo;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  g.us = best * 1e3f;
+  return 0;
+}
+
+inline dim3 grid_of(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
 }  // namespace
 
-// Transform matrices of the dense split-BF16 Poisson solve (once per set-up).
+RpMfma::~RpMfma() { delete static_cast<Plan*>(plan); }
+
+// GEMM plans and constant split-BF16 transform matrices (once per set-up).
 int rp_mfma_setup(rp_ctx* c) {
   const int nx = c->nbx, ny = c->nby;
-  if (nx % GT || ny % GT)
-    return fail(c, RP_ERR_ARG, "matrix-core Poisson needs bin counts that are multiples of %d (got %dx%d)", GT,
-                nx, ny);
   hipStream_t st = c->stream;
   RpMfma& m = c->mfma;
+  delete static_cast<Plan*>(m.plan);
+  m.plan = nullptr;
+  m.ready = false;
+  Plan* P = new Plan();
+  m.plan = P;
+  BLT(c, hipblasLtCreate(&P->h));
   const double asp = (double)c->bsy / (double)c->bsx;
-  auto put = [&](const std::vector<double>& v, DevBuf<uint16_t>& hi, DevBuf<uint16_t>& lo) -> int {
-    std::vector<uint16_t> h, l;
-    split_host(v, h, l);
-    HIPCHK(c, dupload(hi, h, st));
-    HIPCHK(c, dupload(lo, l, st));
+  auto put = [&](const std::vector<double>& v, int M, int K, DevBuf<uint16_t>& dst) -> int {
+    HIPCHK(c, dupload(dst, split3_const(v, M, K), st));
     HIPCHK(c, hipStreamSynchronize(st));
     return 0;
   };
-  std::vector<double> M((size_t)2 * nx * nx);
-  // Cx[u][i]
-  M.resize((size_t)nx * nx);
+  std::vector<double> M;
+  // G1: Cx[u][i]
+  M.assign((size_t)nx * nx, 0.0);
   for (int u = 0; u < nx; u++)
     for (int i = 0; i < nx; i++) M[(size_t)u * nx + i] = cospi_frac((long long)u * (2 * i + 1), 2LL * nx);
-  if (int r = put(M, m.cx_hi, m.cx_lo)) return r;
-  // [Cxt; wx Sxt][i][u]
-  M.resize((size_t)2 * nx * nx);
+  if (int r = put(M, nx, nx, P->a[0])) return r;
+  // G2: Cy[v][j]
+  M.assign((size_t)ny * ny, 0.0);
+  for (int v = 0; v < ny; v++)
+    for (int j = 0; j < ny; j++) M[(size_t)v * ny + j] = cospi_frac((long long)v * (2 * j + 1), 2LL * ny);
+  if (int r = put(M, ny, ny, P->a[1])) return r;
+  // G3: [Cxt; wx Sxt][i][u]
+  M.assign((size_t)2 * nx * nx, 0.0);
   for (int i = 0; i < nx; i++)
     for (int u = 0; u < nx; u++) {
       M[(size_t)i * nx + u] = cospi_frac((long long)u * (2 * i + 1), 2LL * nx);
       M[(size_t)(nx + i) * nx + u] = (M_PI * u / nx) * sinpi_frac((long long)u * (2 * i + 1), 2LL * nx);
     }
-  if (int r = put(M, m.cxs_hi, m.cxs_lo)) return r;
-  // Cy[v][j], Cyt[j][v], (wy Syt)[j][v]
-  M.resize((size_t)ny * ny);
-  for (int v = 0; v < ny; v++)
-    for (int j = 0; j < ny; j++) M[(size_t)v * ny + j] = cospi_frac((long long)v * (2 * j + 1), 2LL * ny);
-  if (int r = put(M, m.cy_hi, m.cy_lo)) return r;
+  if (int r = put(M, 2 * nx, nx, P->a[2])) return r;
+  // G4: Cyt[j][v]
+  M.assign((size_t)ny * ny, 0.0);
   for (int j = 0; j < ny; j++)
     for (int v = 0; v < ny; v++) M[(size_t)j * ny + v] = cospi_frac((long long)v * (2 * j + 1), 2LL * ny);
-  if (int r = put(M, m.cyt_hi, m.cyt_lo)) return r;
+  if (int r = put(M, ny, ny, P->a[3])) return r;
+  // G5: (wy Syt)[j][v]
   for (int j = 0; j < ny; j++)
     for (int v = 0; v < ny; v++)
       M[(size_t)j * ny + v] = (M_PI * v / ny * asp) * sinpi_frac((long long)v * (2 * j + 1), 2LL * ny);
-  if (int r = put(M, m.sy_hi, m.sy_lo)) return r;
+  if (int r = put(M, ny, ny, P->a[4])) return r;
   std::vector<float> wx2(nx), wy2(ny);
   for (int u = 0; u < nx; u++) wx2[u] = (float)((M_PI * u / nx) * (M_PI * u / nx));
   for (int v = 0; v < ny; v++) wy2[v] = (float)((M_PI * v / ny * asp) * (M_PI * v / ny * asp));
-  HIPCHK(c, dupload(m.wx2, wx2, st));
-  HIPCHK(c, dupload(m.wy2, wy2, st));
+  HIPCHK(c, dupload(P->wx2, wx2, st));
+  HIPCHK(c, dupload(P->wy2, wy2, st));
   const size_t B = (size_t)nx * ny;
-  HIPCHK(c, dalloc(m.rho_hi, B));
-  HIPCHK(c, dalloc(m.rho_lo, B));
-  HIPCHK(c, dalloc(m.t1_hi, B));
-  HIPCHK(c, dalloc(m.t1_lo, B));
-  HIPCHK(c, dalloc(m.p_hi, B));
-  HIPCHK(c, dalloc(m.p_lo, B));
-  HIPCHK(c, dalloc(m.r_hi, 2 * B));
-  HIPCHK(c, dalloc(m.r_lo, 2 * B));
-  HIPCHK(c, dalloc(m.sums, MEAN_PARTS + 1));
+  HIPCHK(c, dalloc(P->rho3, 3 * B));
+  HIPCHK(c, dalloc(P->t13, 3 * B));
+  HIPCHK(c, dalloc(P->p3, 3 * B));
+  HIPCHK(c, dalloc(P->r3, 6 * B));
+  HIPCHK(c, dalloc(P->c, 2 * B));
+  HIPCHK(c, dalloc(P->sums, MEAN_PARTS + 1));
+  HIPCHK(c, dalloc(P->ws, kWorkspace));
+  for (auto* buf : {&P->rho3, &P->t13, &P->p3, &P->r3})
+    HIPCHK(c, hipMemsetAsync(buf->p, 0, buf->n * sizeof(uint16_t), st));
+  if (int r = make_gemm(c, *P, P->g[G1], nx, ny, nx)) return r;
+  if (int r = make_gemm(c, *P, P->g[G2], ny, nx, ny)) return r;
+  if (int r = make_gemm(c, *P, P->g[G3], 2 * nx, ny, nx)) return r;
+  if (int r = make_gemm(c, *P, P->g[G4_PHI], ny, 2 * nx, ny)) return r;
+  if (int r = make_gemm(c, *P, P->g[G4], ny, nx, ny)) return r;
+  if (int r = make_gemm(c, *P, P->g[G5], ny, nx, ny)) return r;
+  // keep each product's fastest algorithm (timed on zero data of the real shapes)
+  if (int r = tune_gemm(c, *P, P->g[G1], P->a[0].p, P->rho3.p, st)) return r;
+  if (int r = tune_gemm(c, *P, P->g[G2], P->a[1].p, P->t13.p, st)) return r;
+  if (int r = tune_gemm(c, *P, P->g[G3], P->a[2].p, P->p3.p, st)) return r;
+  if (int r = tune_gemm(c, *P, P->g[G4_PHI], P->a[3].p, P->r3.p, st)) return r;
+  if (int r = tune_gemm(c, *P, P->g[G4], P->a[3].p, P->r3.p + 3 * B, st)) return r;
+  if (int r = tune_gemm(c, *P, P->g[G5], P->a[4].p, P->r3.p, st)) return r;
   HIPCHK(c, hipStreamSynchronize(st));
+  size_t bytes = 0;
+  for (auto& x : P->a) bytes += x.n * 2;
+  bytes += (P->rho3.n + P->t13.n + P->p3.n + P->r3.n) * 2 + P->c.n * 4 + kWorkspace;
+  m.nbytes = bytes;
   m.ready = true;
   return 0;
 }
 
 // rho (d.rho) -> d.phi (if want_phi) and d.field through G1..G5.
-hipError_t rp_launch_poisson_mfma(const RpDev& d, const RpMfmaDev& m, hipStream_t s, int want_phi) {
+hipError_t rp_launch_poisson_mfma(const RpDev& d, const RpMfmaDev& mv, hipStream_t s, int want_phi) {
+  const Plan& P = *static_cast<const Plan*>(mv.plan);
   const int nx = d.nbx, ny = d.nby;
   const size_t B = (size_t)nx * ny;
-  hipLaunchKernelGGL(k_rho_parts, dim3(MEAN_PARTS), dim3(256), 0, s, (const float*)d.rho, B, m.sums);
-  hipLaunchKernelGGL(k_rho_mean, dim3(1), dim3(MEAN_PARTS), 0, s, (const double*)m.sums, B, m.sums + MEAN_PARTS);
-  hipLaunchKernelGGL(k_split, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, (const float*)d.rho,
-                     (const double*)(m.sums + MEAN_PARTS), m.rho_hi, m.rho_lo, B);
-  GemmArgs g{};
-  g.wx2 = m.wx2;
-  g.wy2 = m.wy2;
-  g.wscale = (float)(4.0 / nx / ny);
-  g.phi = d.phi;
-  g.field = d.field;
-  g.nx = nx;
+  auto blt = [](hipblasStatus_t st) { return st == HIPBLAS_STATUS_SUCCESS ? hipSuccess : hipErrorUnknown; };
+  hipLaunchKernelGGL(k_rho_parts, dim3(MEAN_PARTS), dim3(256), 0, s, (const float*)d.rho, B, P.sums.p);
+  hipLaunchKernelGGL(k_rho_mean, dim3(1), dim3(MEAN_PARTS), 0, s, (const double*)P.sums.p, B, P.sums.p + MEAN_PARTS);
+  hipLaunchKernelGGL(k_split_rho, grid_of(B), dim3(256), 0, s, (const float*)d.rho,
+                     (const double*)(P.sums.p + MEAN_PARTS), (rp_bf16*)P.rho3.p, nx, ny);
   // G1: T1t[u][j] = Cx[u][i] . rho[j][i]
-  g.a = {m.cx_hi, m.cx_lo, nx};
-  g.b = {m.rho_hi, m.rho_lo, nx};
-  g.M = nx, g.N = ny, g.K = nx;
-  g.ohi = m.t1_hi, g.olo = m.t1_lo, g.ldc = ny;
-  hipError_t e = gemm<EPI_SPLIT>(g, s);
+  hipError_t e = blt(run_gemm(P, P.g[G1], P.a[0].p, P.rho3.p, P.c.p, s));
   if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_split_c, grid_of(B), dim3(256), 0, s, (const float*)P.c.p, (rp_bf16*)P.t13.p, nx, ny, 0,
+                     (const float*)nullptr, (const float*)nullptr, 0.0f);
   // G2: P[v][u] = Cy[v][j] . T1t[u][j], scaled
-  g.a = {m.cy_hi, m.cy_lo, ny};
-  g.b = {m.t1_hi, m.t1_lo, ny};
-  g.M = ny, g.N = nx, g.K = ny;
-  g.ohi = m.p_hi, g.olo = m.p_lo, g.ldc = nx;
-  if ((e = gemm<EPI_SPECTRAL>(g, s)) != hipSuccess) return e;
+  if ((e = blt(run_gemm(P, P.g[G2], P.a[1].p, P.t13.p, P.c.p, s))) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_split_c, grid_of(B), dim3(256), 0, s, (const float*)P.c.p, (rp_bf16*)P.p3.p, ny, nx, 1,
+                     (const float*)P.wx2.p, (const float*)P.wy2.p, (float)(4.0 / nx / ny));
   // G3: [R1t; R2t][i][v] = [Cxt; wx Sxt][i][u] . P[v][u]
-  g.a = {m.cxs_hi, m.cxs_lo, nx};
-  g.b = {m.p_hi, m.p_lo, nx};
-  g.M = 2 * nx, g.N = ny, g.K = nx;
-  g.ohi = m.r_hi, g.olo = m.r_lo, g.ldc = ny;
-  if ((e = gemm<EPI_SPLIT>(g, s)) != hipSuccess) return e;
+  if ((e = blt(run_gemm(P, P.g[G3], P.a[2].p, P.p3.p, P.c.p, s))) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_split_c, grid_of(2 * B), dim3(256), 0, s, (const float*)P.c.p, (rp_bf16*)P.r3.p, 2 * nx, ny,
+                     0, (const float*)nullptr, (const float*)nullptr, 0.0f);
   // G4: [phi | Ex][j][.] = Cyt[j][v] . [R1t; R2t][.][v]  (Ex only without phi)
-  g.a = {m.cyt_hi, m.cyt_lo, ny};
   if (want_phi) {
-    g.b = {m.r_hi, m.r_lo, ny};
-    g.N = 2 * nx;
-    g.col0 = 0;
+    if ((e = blt(run_gemm(P, P.g[G4_PHI], P.a[3].p, P.r3.p, P.c.p, s))) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_phi_ex, grid_of(2 * B), dim3(256), 0, s, (const float*)P.c.p, d.phi, d.field, nx, ny,
+                       2 * nx, 0);
   } else {
-    g.b = {m.r_hi + B, m.r_lo + B, ny};
-    g.N = nx;
-    g.col0 = nx;
+    if ((e = blt(run_gemm(P, P.g[G4], P.a[3].p, P.r3.p + 3 * B, P.c.p, s))) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_phi_ex, grid_of(B), dim3(256), 0, s, (const float*)P.c.p, d.phi, d.field, nx, ny, nx, nx);
   }
-  g.M = ny, g.K = ny;
-  if ((e = gemm<EPI_PHI_EX>(g, s)) != hipSuccess) return e;
   // G5: Ey[j][i] = (wy Syt)[j][v] . R1t[i][v]
-  g.a = {m.sy_hi, m.sy_lo, ny};
-  g.b = {m.r_hi, m.r_lo, ny};
-  g.M = ny, g.N = nx, g.K = ny;
-  return gemm<EPI_EY>(g, s);
+  if ((e = blt(run_gemm(P, P.g[G5], P.a[4].p, P.r3.p, P.c.p, s))) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_ey, grid_of(B), dim3(256), 0, s, (const float*)P.c.p, d.field, B);
+  return hipGetLastError();
 }

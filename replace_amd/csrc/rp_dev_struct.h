@@ -83,13 +83,10 @@ struct RpBatchLaunch {
   size_t lds_binfin, lds_col, lds_rinv;
 };
 
-// Device view of the matrix-core Poisson solve (rp_dct_mfma.hip): split-BF16
-// transform matrices and intermediates.
+// Handle of the matrix-core Poisson solve (rp_dct_mfma.hip): its GEMM plans,
+// split-BF16 transform matrices and intermediates.
 struct RpMfmaDev {
-  const __bf16 *cx_hi, *cx_lo, *cxs_hi, *cxs_lo, *cy_hi, *cy_lo, *cyt_hi, *cyt_lo, *sy_hi, *sy_lo;
-  const float *wx2, *wy2;
-  __bf16 *rho_hi, *rho_lo, *t1_hi, *t1_lo, *p_hi, *p_lo, *r_hi, *r_lo;
-  double* sums;  // mean of rho: partials, then the mean
+  void* plan;
 };
 
 hipError_t rp_launch_poisson_mfma(const RpDev& d, const RpMfmaDev& m, hipStream_t s, int want_phi);
