@@ -19,6 +19,8 @@
 // Integer keys and in-order float sums make the result bit-identical to a
 // sequential host assembly, independent of launch shape.
 #include <chrono>
+#include <cstdlib>
+#include <cstring>
 
 #include <hipcub/hipcub.hpp>
 
@@ -644,6 +646,357 @@ __global__ void __launch_bounds__(RP_BLOCK) k_bc_zero_x(BcArgs g) {
 }
 
 // ---------------------------------------------------------------------------
+// Small systems: the whole solve in one workgroup per axis
+// ---------------------------------------------------------------------------
+// For n <= BCS_THREADS * BCS_MAX_R rows the recurrence above (same formulas,
+// same restart rule, same termination test) runs in one 1024-thread
+// workgroup per axis, with no launch or host poll per iteration.
+//
+// The system is solved in a row order sorted by length (a stable sort,
+// longest first; the permuted system P A P^T (P x) = P b has the same
+// iterates).  Thread tid owns sorted rows q = tid + 1024 k (k < R): r, v and
+// t of its rows live in registers; p and s (gathered by the products), r0
+// and x live in LDS.  The matrix is held as sliced ELL (SELL-C-sigma,
+// Kreutzer et al., SIAM J. Sci. Comput. 36(5), 2014) with C = 64: each
+// 64-row slice, one per wave, stores its rows slot-major at the slice's
+// width, so every load instruction of a wave reads 64 consecutive entries,
+// and the sort keeps rows of one slice at nearly the same length; within a
+// slice, each lane's slots are grouped in fours (16-byte loads).  The
+// entries of a row keep their CSR order and are summed in FP64 in that order;
+// dot products are per-thread FP64 sums over its rows in k order, then one
+// fixed tree (wave xor, then the waves in order), so every thread holds the
+// same scalars and takes the same branches.
+constexpr int BCS_THREADS = 1024;
+constexpr int BCS_MAX_R = 8;
+constexpr int SELL_C = 64;
+constexpr int SELL_PAD = 16;  // slice widths are multiples of this many slots
+
+struct SellAxis {
+  const int* rowp;
+  const int* col;
+  const float* val;
+  const float* b;
+  float* x;
+  const int* perm;    // sorted position -> row
+  const int* slen;    // sorted position -> row length
+  const int* width;   // per slice: entries, SELL_C x its slots (its first row's length, rounded up)
+  const int* soff;    // per slice: first entry
+  const float* sval;  // SELL entries, slot-major per slice
+  const int* scol;    // sorted column positions (pads: the row itself)
+  BcState* st;
+};
+
+struct SellArgs {
+  SellAxis ax[2];
+  int n;
+};
+
+// prep: descending-length sort keys
+__global__ void __launch_bounds__(RP_BLOCK) k_sell_keys(const int* __restrict__ rowp, int n,
+                                                        unsigned* __restrict__ key, int* __restrict__ id) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  key[i] = 0xFFFFu - (unsigned)min(rowp[i + 1] - rowp[i], 0xFFFF);
+  id[i] = i;
+}
+
+// prep: inverse order, sorted lengths, slice widths (rows past n are empty;
+// widths rounded up to a multiple of SELL_PAD slots)
+__global__ void __launch_bounds__(RP_BLOCK) k_sell_meta(const int* __restrict__ rowp, const int* __restrict__ perm,
+                                                        int n, int nslices, int* __restrict__ iperm,
+                                                        int* __restrict__ slen, int* __restrict__ wslots) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) {
+    const int i = perm[q];
+    iperm[i] = q;
+    slen[q] = rowp[i + 1] - rowp[i];
+  }
+  if (q < nslices) {
+    const int q0 = q * SELL_C;
+    const int w = q0 < n ? rowp[perm[q0] + 1] - rowp[perm[q0]] : 0;
+    wslots[q] = SELL_C * ((w + SELL_PAD - 1) / SELL_PAD * SELL_PAD);
+  }
+}
+
+// prep: SELL entries; a slice's slots past a row's length hold val 0 and the
+// row's own column (never summed)
+__global__ void __launch_bounds__(RP_BLOCK) k_sell_fill(const int* __restrict__ rowp, const int* __restrict__ col,
+                                                        const float* __restrict__ val, const int* __restrict__ perm,
+                                                        const int* __restrict__ iperm, const int* __restrict__ soff,
+                                                        int n, int nrows_pad, float* __restrict__ sval,
+                                                        int* __restrict__ scol) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nrows_pad) return;
+  const int s = q / SELL_C, lane = q % SELL_C;
+  const int q0 = s * SELL_C;
+  const int w0 = q0 < n ? rowp[perm[q0] + 1] - rowp[perm[q0]] : 0;
+  const int w = (w0 + SELL_PAD - 1) / SELL_PAD * SELL_PAD;
+  const int rb = q < n ? rowp[perm[q]] : 0;
+  const int len = q < n ? rowp[perm[q] + 1] - rb : 0;
+  const int own = q < n ? q : 0;
+  for (int j = 0; j < w; j++) {
+    const int e = soff[s] + (j / 4) * (4 * SELL_C) + lane * 4 + (j % 4);
+    const bool in = j < len;
+    sval[e] = in ? val[rb + j] : 0.0f;
+    scol[e] = in ? iperm[col[rb + j]] : own;
+  }
+}
+
+// DPP move of a double (both halves), CTRL a dpp_ctrl encoding
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Wave sum on the DPP network: lane pairs (xor 1, xor 2), then the mirrored
+// half-row and row, which gives every lane of a 16-lane row the same row sum
+// (each step adds two values that are the same pair in every lane, and a
+// floating-point add is commutative); the four row sums are then added in
+// row order.  The result is identical in every lane.
+__device__ __forceinline__ double wsum_dpp(double v) {
+  v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f64<0x141>(v);  // row_half_mirror
+  v += dpp_f64<0x140>(v);  // row_mirror
+  return ((readlane_f64(v, 0) + readlane_f64(v, 16)) + readlane_f64(v, 32)) + readlane_f64(v, 48);
+}
+
+// sums of a and b over the workgroup, identical in every thread
+__device__ __forceinline__ void bcs_sum2(double& a, double& b, double* red) {
+  a = wsum_dpp(a);
+  b = wsum_dpp(b);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    red[2 * w] = a;
+    red[2 * w + 1] = b;
+  }
+  __syncthreads();
+  double x = 0.0, y = 0.0;
+  for (int k = 0; k < BCS_THREADS / 64; k++) {
+    x += red[2 * k];
+    y += red[2 * k + 1];
+  }
+  a = x;
+  b = y;
+}
+
+// y = A X over the thread's rows (X in LDS, sorted order).  A round loads
+// SB slots (SB divides SELL_PAD, so slice widths are whole rounds) and issues
+// its loads and gathers unconditionally; an entry past the row's length is
+// dropped by a select, so each row's sum is exactly the CSR-order sum of its
+// entries.  Wider rounds keep more loads in flight where registers allow.
+template <int R, int SB = (R <= 4 ? 16 : 8)>
+__device__ __forceinline__ void bcs_spmv(const SellAxis& a, const int (&len)[R], const float* X, float (&y)[R]) {
+  static_assert(SELL_PAD % SB == 0 && SB % 4 == 0, "rounds must tile the slice widths in quads");
+  const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const unsigned lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    const unsigned s = wave + (BCS_THREADS / SELL_C) * k;
+    const int w = __builtin_amdgcn_readfirstlane(a.width[s]) / SELL_C;
+    const unsigned base = __builtin_amdgcn_readfirstlane(a.soff[s]);
+    const float* __restrict__ pv = a.sval + base;
+    const int* __restrict__ pc = a.scol + base;
+    double acc = 0.0;
+    for (int j = 0; j < w; j += SB) {
+      float vv[SB], xx[SB];
+      int cc[SB];
+#pragma unroll
+      for (int u = 0; u < SB; u += 4) {
+        const unsigned e = (unsigned)(j + u) * SELL_C + lane * 4;
+        const float4 v4 = *reinterpret_cast<const float4*>(pv + e);
+        const int4 c4 = *reinterpret_cast<const int4*>(pc + e);
+        vv[u] = v4.x;
+        vv[u + 1] = v4.y;
+        vv[u + 2] = v4.z;
+        vv[u + 3] = v4.w;
+        cc[u] = c4.x;
+        cc[u + 1] = c4.y;
+        cc[u + 2] = c4.z;
+        cc[u + 3] = c4.w;
+      }
+#pragma unroll
+      for (int u = 0; u < SB; u++) xx[u] = X[cc[u]];
+#pragma unroll
+      for (int u = 0; u < SB; u++) {
+        const double t = acc + (double)vv[u] * (double)xx[u];
+        acc = j + u < len[k] ? t : acc;
+      }
+    }
+    y[k] = (float)acc;
+  }
+}
+
+template <int R>
+__global__ void __launch_bounds__(BCS_THREADS) k_bc_small(SellArgs g) {
+  extern __shared__ float bcs_lds[];
+  __shared__ double red[2 * (BCS_THREADS / 64)];
+  const SellAxis& a = g.ax[blockIdx.x];
+  BcState& S = *a.st;
+  const int n = g.n, tid = threadIdx.x;
+  float* P = bcs_lds;           // p
+  float* Q = bcs_lds + n;       // x for r = b - A x, otherwise s
+  float* R0 = bcs_lds + 2 * n;  // r0
+  float* X = bcs_lds + 3 * n;   // x
+  int len[R];
+  float r[R], v[R], t[R];
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    const int q = tid + BCS_THREADS * k;
+    len[k] = q < n ? a.slen[q] : 0;
+    if (q < n) Q[q] = X[q] = a.x[a.perm[q]];
+  }
+  __syncthreads();
+  // r = b - A x ; r0 = r ; v = p = 0
+  double d0 = 0.0, d1 = 0.0;
+  bcs_spmv<R>(a, len, Q, t);
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    const int q = tid + BCS_THREADS * k;
+    r[k] = v[k] = 0.0f;
+    if (q < n) {
+      const float bi = a.b[a.perm[q]];
+      const float ri = (float)((double)bi + -1.0 * (double)t[k]);
+      r[k] = R0[q] = ri;
+      P[q] = 0.0f;
+      d0 += (double)ri * (double)ri;
+      d1 += (double)bi * (double)bi;
+    }
+  }
+  bcs_sum2(d0, d1, red);
+  const double rhs_sq = d1, tol2 = S.tol2 * rhs_sq, eps2 = S.eps2;  // S.tol2 holds tol^2
+  const int max_iter = S.max_iter;
+  const bool zero = rhs_sq == 0.0;
+  double r0_sq = d0, rr = d0, r0r = d0, rho = 1.0, alpha = 1.0, w = 1.0;
+  int it = 0, restarts = 0, done = 0;
+  // i restarts from 0 at most once, so 2 max_iter + 1 iterations bound the loop
+  const int guard = 2 * max(0, max_iter) + 2;
+  int pass = 0;
+  for (; pass < guard; pass++) {
+    if (zero || !(rr > tol2 && it < max_iter)) {
+      done = 1;
+      break;
+    }
+    const double rho_old = rho;
+    rho = r0r;
+    if (fabs(rho) < eps2 * r0_sq) {  // restart: r = b - A x ; r0 = r ; rho = r.r
+#pragma unroll
+      for (int k = 0; k < R; k++) {
+        const int q = tid + BCS_THREADS * k;
+        if (q < n) Q[q] = X[q];
+      }
+      __syncthreads();
+      bcs_spmv<R>(a, len, Q, t);
+      d0 = d1 = 0.0;
+#pragma unroll
+      for (int k = 0; k < R; k++) {
+        const int q = tid + BCS_THREADS * k;
+        if (q < n) {
+          const float ri = (float)((double)a.b[a.perm[q]] + -1.0 * (double)t[k]);
+          r[k] = R0[q] = ri;
+          d0 += (double)ri * (double)ri;
+        }
+      }
+      bcs_sum2(d0, d1, red);
+      rho = r0_sq = d0;
+      if (restarts++ == 0) it = 0;
+    }
+    const double beta = (rho / rho_old) * (alpha / w);
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      const int q = tid + BCS_THREADS * k;
+      if (q < n) P[q] = (float)((double)r[k] + beta * ((double)P[q] - w * (double)v[k]));
+    }
+    __syncthreads();
+    // v = A p ; alpha = rho / r0.v
+    bcs_spmv<R>(a, len, P, v);
+    d0 = d1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      const int q = tid + BCS_THREADS * k;
+      if (q < n) d0 += (double)R0[q] * (double)v[k];
+    }
+    bcs_sum2(d0, d1, red);
+    alpha = rho / d0;
+    // s = r - alpha v ; t = A s ; w = t.s / t.t
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      const int q = tid + BCS_THREADS * k;
+      if (q < n) Q[q] = (float)((double)r[k] + -alpha * (double)v[k]);
+    }
+    __syncthreads();
+    bcs_spmv<R>(a, len, Q, t);
+    double tt = 0.0, ts = 0.0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      const int q = tid + BCS_THREADS * k;
+      if (q < n) {
+        tt += (double)t[k] * (double)t[k];
+        ts += (double)t[k] * (double)Q[q];
+      }
+    }
+    bcs_sum2(tt, ts, red);
+    w = tt > 0.0 ? ts / tt : 0.0;
+    // x += alpha p + w s ; r = s - w t
+    d0 = d1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      const int q = tid + BCS_THREADS * k;
+      if (q < n) {
+        X[q] = (float)((double)X[q] + alpha * (double)P[q] + w * (double)Q[q]);
+        const float ri = (float)((double)Q[q] - w * (double)t[k]);
+        r[k] = ri;
+        d0 += (double)ri * (double)ri;
+        d1 += (double)R0[q] * (double)ri;
+      }
+    }
+    bcs_sum2(d0, d1, red);
+    rr = d0;
+    r0r = d1;
+    it++;
+  }
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    const int q = tid + BCS_THREADS * k;
+    if (q < n) a.x[a.perm[q]] = zero ? 0.0f : X[q];
+  }
+  if (tid == 0) {
+    S.started = 1;
+    S.done = done;
+    S.zero = zero;
+    S.i = it;
+    S.restarts = restarts;
+    S.pending = pass;  // iterations run (diagnostics)
+    S.rr = rr;
+    S.rhs_sq = rhs_sq;
+    S.err = zero ? 0.0f : (float)sqrt(rr / rhs_sq);
+  }
+}
+
+// rows per thread of the one-workgroup solve, 0 when n is too large for it
+inline int bcs_rows_per_thread(int n) {
+  for (int R = 1; R <= BCS_MAX_R; R *= 2)
+    if (n <= BCS_THREADS * R) return R;
+  return 0;
+}
+
+// Work arrays of the sorted SELL copy of one axis.
+struct SellBufs {
+  DevBuf<unsigned> key, key2;
+  DevBuf<int> id, perm, iperm, slen, wslots, soff, scol;
+  DevBuf<float> sval;
+  DevBuf<char> temp;
+};
+
+// ---------------------------------------------------------------------------
 // InitialPlace iteration kernels: warm start, instance-box HPWL, updateCoordi
 // ---------------------------------------------------------------------------
 struct IpMove {
@@ -841,6 +1194,31 @@ int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve) 
   const dim3 gn((n + RP_BLOCK - 1) / RP_BLOCK), gp((np + RP_BLOCK - 1) / RP_BLOCK);
   const int CH = 8;
   const int cap = 2 * std::max(0, init.max_iter) + 1;  // i restarts from 0 at most once
+  // one workgroup per axis for small systems; RP_IP_SOLVER=blocks keeps the
+  // grid-wide kernels at every size
+  const char* solver = getenv("RP_IP_SOLVER");
+  const int small_r = (solver && strcmp(solver, "blocks") == 0) ? 0 : bcs_rows_per_thread(n);
+  const int sell_rows = BCS_THREADS * small_r, sell_slices = sell_rows / SELL_C;
+  SellBufs sb[2];
+  SellArgs gs{};
+  gs.n = n;
+  if (small_r) {
+    for (int ax = 0; ax < 2; ax++) {
+      HIPCHK(c, dalloc(sb[ax].key, n));
+      HIPCHK(c, dalloc(sb[ax].key2, n));
+      HIPCHK(c, dalloc(sb[ax].id, n));
+      HIPCHK(c, dalloc(sb[ax].perm, n));
+      HIPCHK(c, dalloc(sb[ax].iperm, n));
+      HIPCHK(c, dalloc(sb[ax].slen, n));
+      HIPCHK(c, dalloc(sb[ax].wslots, sell_slices));
+      HIPCHK(c, dalloc(sb[ax].soff, sell_slices));
+      size_t t1 = 0, t2 = 0;
+      HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, t1, sb[ax].key.p, sb[ax].key2.p, sb[ax].id.p,
+                                                   sb[ax].perm.p, n, 0, 16, st));
+      HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, t2, sb[ax].wslots.p, sb[ax].soff.p, sell_slices, st));
+      HIPCHK(c, dalloc(sb[ax].temp, std::max(t1, t2)));
+    }
+  }
   for (int it = 1; it <= p.initial_place_max_iter; it++) {
     auto ta = clk::now();
     if (host_asm) {
@@ -871,6 +1249,51 @@ int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve) 
     *h_rec = r0;
     HIPCHK(c, hipMemcpyAsync(d_rec.p, h_rec, sizeof(IpRec), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_ip_warm, gn, blk, 0, st, mv);
+    if (small_r) {
+      // sorted SELL copy of each axis' matrix
+      for (int ax = 0; ax < 2; ax++) {
+        SellBufs& B = sb[ax];
+        // nnz + one slice of the longest row + the rounding of every slice
+        const size_t cap = A.col[ax].n + (size_t)SELL_C * n + (size_t)SELL_C * (SELL_PAD - 1) * sell_slices;
+        if (B.sval.n < cap) {
+          HIPCHK(c, dalloc(B.sval, cap));
+          HIPCHK(c, dalloc(B.scol, cap));
+        }
+        hipLaunchKernelGGL(k_sell_keys, gn, blk, 0, st, (const int*)A.rowp[ax].p, n, B.key.p, B.id.p);
+        size_t tb = B.temp.n;
+        HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(B.temp.p, tb, B.key.p, B.key2.p, B.id.p, B.perm.p, n, 0, 16,
+                                                     st));
+        hipLaunchKernelGGL(k_sell_meta, grid_of(std::max(n, sell_slices)), blk, 0, st, (const int*)A.rowp[ax].p,
+                           (const int*)B.perm.p, n, sell_slices, B.iperm.p, B.slen.p, B.wslots.p);
+        tb = B.temp.n;
+        HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(B.temp.p, tb, B.wslots.p, B.soff.p, sell_slices, st));
+        hipLaunchKernelGGL(k_sell_fill, grid_of(sell_rows), blk, 0, st, (const int*)A.rowp[ax].p,
+                           (const int*)A.col[ax].p, (const float*)A.val[ax].p, (const int*)B.perm.p,
+                           (const int*)B.iperm.p, (const int*)B.soff.p, n, sell_rows, B.sval.p, B.scol.p);
+        HIPCHK(c, hipGetLastError());
+        SellAxis& a = gs.ax[ax];
+        a.rowp = g.ax[ax].rowp;
+        a.col = g.ax[ax].col;
+        a.val = g.ax[ax].val;
+        a.b = g.ax[ax].b;
+        a.x = g.ax[ax].x;
+        a.perm = B.perm.p;
+        a.slen = B.slen.p;
+        a.width = B.wslots.p;
+        a.soff = B.soff.p;
+        a.sval = B.sval.p;
+        a.scol = B.scol.p;
+        a.st = g.ax[ax].st;
+      }
+      const size_t lds = 4 * sizeof(float) * (size_t)n;
+      switch (small_r) {
+        case 1: hipLaunchKernelGGL(k_bc_small<1>, dim3(2), dim3(BCS_THREADS), lds, st, gs); break;
+        case 2: hipLaunchKernelGGL(k_bc_small<2>, dim3(2), dim3(BCS_THREADS), lds, st, gs); break;
+        case 4: hipLaunchKernelGGL(k_bc_small<4>, dim3(2), dim3(BCS_THREADS), lds, st, gs); break;
+        default: hipLaunchKernelGGL(k_bc_small<8>, dim3(2), dim3(BCS_THREADS), lds, st, gs); break;
+      }
+      HIPCHK(c, hipGetLastError());
+    } else {
     hipLaunchKernelGGL(k_bc_init, gb, blk, 0, st, g);
     int done_iters = 0;
     for (;;) {
@@ -892,6 +1315,7 @@ int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve) 
       if ((h_done[0] && h_done[1]) || done_iters >= cap) break;
     }
     hipLaunchKernelGGL(k_bc_zero_x, gb, blk, 0, st, g);
+    }
     // log record: HPWL before updateCoordi (src/initialPlace.cpp:88-91)
     hipLaunchKernelGGL(k_ip_hpwl, dim3(hblocks), blk, 0, st, mv);
     hipLaunchKernelGGL(k_ip_hpwl_final, dim3(1), dim3(1024), 0, st, (const long long*)d_hparts.p, hblocks,
@@ -905,6 +1329,9 @@ int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve) 
     if (!h_rec->st[0].done || !h_rec->st[1].done)
       return fail(c, RP_ERR_STATE, "initial place: BiCGSTAB did not terminate");
     float err = std::max(h_rec->st[0].err, h_rec->st[1].err);
+    if (small_r && getenv("RP_IP_TIMING"))
+      fprintf(stderr, "[rp] initial place %d: one-workgroup solves of %d rows, %d / %d iterations\n", it, n,
+              h_rec->st[0].pending, h_rec->st[1].pending);
     c->ip_err.push_back(err);
     c->ip_hpwl.push_back(h_rec->hpwl);
     if (err <= 1e-5 && it >= 5) break;
