@@ -166,8 +166,9 @@ static inline hipError_t dupload(DevBuf<T>& b, const std::vector<T>& v, hipStrea
 // InitialPlace B2B assembly on the device (rp_ip.hip)
 struct RpIpAsm {
   int n = 0, n_net = 0, cb = 0;
-  DevBuf<int> net_start, pin_inst, pin_k, pin_cx, pin_cy, icx, icy;
-  DevBuf<int> cnt, off;  // 4 x (n_net + 1) per-net counts and their exclusive offsets
+  DevBuf<int> net_start, pin_inst, pin_k, pin_cx, pin_cy, icx, icy, pin_net;
+  DevBuf<int4> ext;      // per net: the pins of the x/y extremes
+  DevBuf<int> cnt, off;  // 4 x (n_pin + 1) per-pin counts and their exclusive offsets
   DevBuf<unsigned long long> tk[2], tk2[2];
   DevBuf<float> tv[2], tv2[2];
   DevBuf<unsigned> rk[2], rk2[2];

@@ -4,14 +4,14 @@
 // createSparseMatrix (src/initialPlace.cpp:185-378), whose triplet lists go
 // through Eigen's SparseMatrix::setFromTriplets (src/initialPlace.cpp:376-377).
 //
-// Per net, one thread
-//   1. finds the pins that end as the x/y min and max of updatePinInfo's
-//      sequential scan (the first pin reaching a strict new extreme keeps the
-//      flag), and
-//   2. walks the reference's pair loop (a1 over pins, a2 <= a1, same-instance
-//      pairs skipped) over the pairs that touch an extreme pin, emitting the
-//      Laplacian triplets and the right-hand-side terms in the reference's
-//      order at offsets from an exclusive scan of per-net counts.
+//   1. Per net, one thread finds the pins that end as the x/y min and max of
+//      updatePinInfo's sequential scan (the first pin reaching a strict new
+//      extreme keeps the flag).
+//   2. Per pin a1, one thread walks the reference's pair loop (a2 <= a1,
+//      same-instance pairs skipped) over the pairs that touch an extreme pin,
+//      emitting the Laplacian triplets and the right-hand-side terms at
+//      offsets from an exclusive scan of per-pin counts; pins are net-major,
+//      so this is the reference's order.
 // The CSR is then what setFromTriplets builds: a stable radix sort of the
 // triplets on (row, col) keeps equal entries in triplet order, and every
 // (row, col) run is summed sequentially from 0.0f.  The right-hand side is
@@ -29,7 +29,7 @@
 namespace {
 
 struct IpView {
-  int n_net, n;
+  int n_net, n, n_pin;
   const int* __restrict__ net_start;
   const int* __restrict__ pin_inst;
   const int* __restrict__ pin_k;   // movable index of the pin's instance, -1 for fixed / BTerm
@@ -42,7 +42,7 @@ struct IpView {
 };
 
 struct IpOut {
-  const int* off;  // 4 x (n_net + 1) exclusive offsets: x triplets, y triplets, x rhs, y rhs
+  const int* off;  // 4 x (n_pin + 1) exclusive offsets: x triplets, y triplets, x rhs, y rhs
   unsigned long long* tk[2];
   float* tv[2];
   unsigned* rk[2];
@@ -102,86 +102,104 @@ __device__ __forceinline__ void ip_pair(int p1, int p2, int k1, int k2, const in
   }
 }
 
-// The reference's pair loop over net e, restricted to the pairs that touch
-// an extreme pin of the axis; pairs are visited in (a1, a2) order.
-template <bool EMIT>
-__device__ void ip_net(const IpView& v, int e, int cnt[4], Emit* ex, Emit* ey) {
-  cnt[0] = cnt[1] = cnt[2] = cnt[3] = 0;
+// updatePinInfo for net e: the pins that end as the x/y min and max of the
+// reference's sequential scan (the first pin reaching a strict new extreme
+// keeps the flag); all -1 for nets the B2B model skips.  The loads of several
+// pins are independent, so the scan is latency-light even for long nets.
+__global__ void __launch_bounds__(RP_BLOCK) k_ip_ext(IpView v, int4* __restrict__ ext) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= v.n_net) return;
   const int s = v.net_start[e], t = v.net_start[e + 1], deg = t - s;
-  if (deg <= 1 || deg >= v.max_fanout) return;
-  // updatePinInfo: first pin reaching a strict new extreme
+  if (deg <= 1 || deg >= v.max_fanout) {
+    ext[e] = make_int4(-1, -1, -1, -1);
+    return;
+  }
   int lx = INT_MAX, ly = INT_MAX, ux = INT_MIN, uy = INT_MIN;
   int mnx = -1, mxx = -1, mny = -1, mxy = -1;
+#pragma unroll 4
   for (int q = s; q < t; q++) {
-    int x = v.pin_cx[q], y = v.pin_cy[q];
+    const int x = v.pin_cx[q], y = v.pin_cy[q];
     if (lx > x) { lx = x; mnx = q; }
     if (ux < x) { ux = x; mxx = q; }
     if (ly > y) { ly = y; mny = q; }
     if (uy < y) { uy = y; mxy = q; }
   }
-  const float netWeight = v.weight_scale / (float)(deg - 1);
+  ext[e] = make_int4(mnx, mxx, mny, mxy);
+}
+
+// The reference's pair loop (a1 over the net's pins, a2 <= a1, same-instance
+// pairs skipped) for one a1 = pin q1 of net [s, t), restricted to the pairs
+// that touch an extreme pin of the axis, partners visited in ascending order.
+// Visiting the pins in order gives the reference's triplet order per axis.
+template <bool EMIT>
+__device__ void ip_pin(const IpView& v, int q1, int s, int t, int4 ext, int cnt[4], Emit* ex, Emit* ey) {
+  cnt[0] = cnt[1] = cnt[2] = cnt[3] = 0;
+  const int mnx = ext.x, mxx = ext.y, mny = ext.z, mxy = ext.w;
+  if (mnx < 0) return;  // net skipped
+  const float netWeight = v.weight_scale / (float)(t - s - 1);
   const float minDiff = (float)v.min_diff;
   Emit dummy{};
   Emit& emx = EMIT ? *ex : dummy;
   Emit& emy = EMIT ? *ey : dummy;
-  for (int q1 = s; q1 < t; q1++) {
-    const int i1 = v.pin_inst[q1], k1 = v.pin_k[q1];
-    const int x1 = v.pin_cx[q1], y1 = v.pin_cy[q1];
-    const bool xe1 = q1 == mnx || q1 == mxx, ye1 = q1 == mny || q1 == mxy;
-    if (xe1 || ye1) {
-      for (int q2 = s; q2 <= q1; q2++) {
-        const int i2 = v.pin_inst[q2];
-        if (i1 == i2) continue;
-        const int k2 = v.pin_k[q2];
-        if (xe1 || q2 == mnx || q2 == mxx)
-          ip_pair<EMIT>(x1, v.pin_cx[q2], k1, k2, v.icx, netWeight, minDiff, v.min_diff, cnt[0], cnt[2], emx);
-        if (ye1 || q2 == mny || q2 == mxy)
-          ip_pair<EMIT>(y1, v.pin_cy[q2], k1, k2, v.icy, netWeight, minDiff, v.min_diff, cnt[1], cnt[3], emy);
-      }
-    } else {
-      // only extreme partners with q2 <= q1, ascending; x and y lists are independent
-      int ax = min(mnx, mxx), bx = max(mnx, mxx);
-      int ay = min(mny, mxy), by = max(mny, mxy);
-      const int xs[2] = {ax, bx == ax ? -1 : bx};
-      const int ys[2] = {ay, by == ay ? -1 : by};
-      for (int j = 0; j < 2; j++) {
-        int q2 = xs[j];
-        if (q2 < 0 || q2 > q1) continue;
-        const int i2 = v.pin_inst[q2];
-        if (i1 == i2) continue;
-        ip_pair<EMIT>(x1, v.pin_cx[q2], k1, v.pin_k[q2], v.icx, netWeight, minDiff, v.min_diff, cnt[0], cnt[2], emx);
-      }
-      for (int j = 0; j < 2; j++) {
-        int q2 = ys[j];
-        if (q2 < 0 || q2 > q1) continue;
-        const int i2 = v.pin_inst[q2];
-        if (i1 == i2) continue;
-        ip_pair<EMIT>(y1, v.pin_cy[q2], k1, v.pin_k[q2], v.icy, netWeight, minDiff, v.min_diff, cnt[1], cnt[3], emy);
-      }
+  const int i1 = v.pin_inst[q1], k1 = v.pin_k[q1];
+  const int x1 = v.pin_cx[q1], y1 = v.pin_cy[q1];
+  const bool xe1 = q1 == mnx || q1 == mxx, ye1 = q1 == mny || q1 == mxy;
+  if (xe1 || ye1) {
+    for (int q2 = s; q2 <= q1; q2++) {
+      const int i2 = v.pin_inst[q2];
+      if (i1 == i2) continue;
+      const int k2 = v.pin_k[q2];
+      if (xe1 || q2 == mnx || q2 == mxx)
+        ip_pair<EMIT>(x1, v.pin_cx[q2], k1, k2, v.icx, netWeight, minDiff, v.min_diff, cnt[0], cnt[2], emx);
+      if (ye1 || q2 == mny || q2 == mxy)
+        ip_pair<EMIT>(y1, v.pin_cy[q2], k1, k2, v.icy, netWeight, minDiff, v.min_diff, cnt[1], cnt[3], emy);
+    }
+  } else {
+    // only extreme partners with q2 <= q1, ascending; x and y lists are independent
+    const int ax = min(mnx, mxx), bx = max(mnx, mxx);
+    const int ay = min(mny, mxy), by = max(mny, mxy);
+    const int xs[2] = {ax, bx == ax ? -1 : bx};
+    const int ys[2] = {ay, by == ay ? -1 : by};
+    for (int j = 0; j < 2; j++) {
+      const int q2 = xs[j];
+      if (q2 < 0 || q2 > q1) continue;
+      if (i1 == v.pin_inst[q2]) continue;
+      ip_pair<EMIT>(x1, v.pin_cx[q2], k1, v.pin_k[q2], v.icx, netWeight, minDiff, v.min_diff, cnt[0], cnt[2], emx);
+    }
+    for (int j = 0; j < 2; j++) {
+      const int q2 = ys[j];
+      if (q2 < 0 || q2 > q1) continue;
+      if (i1 == v.pin_inst[q2]) continue;
+      ip_pair<EMIT>(y1, v.pin_cy[q2], k1, v.pin_k[q2], v.icy, netWeight, minDiff, v.min_diff, cnt[1], cnt[3], emy);
     }
   }
 }
 
-__global__ void __launch_bounds__(RP_BLOCK) k_ip_count(IpView v, int* cnt) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  const int stride = v.n_net + 1;
-  if (e == v.n_net) {
-    for (int a = 0; a < 4; a++) cnt[a * stride + e] = 0;
+// per pin: triplet and right-hand-side counts of both axes
+__global__ void __launch_bounds__(RP_BLOCK) k_ip_count(IpView v, const int* __restrict__ pin_net,
+                                                       const int4* __restrict__ ext, int* cnt) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int stride = v.n_pin + 1;
+  if (q == v.n_pin) {
+    for (int a = 0; a < 4; a++) cnt[a * stride + q] = 0;
   }
-  if (e >= v.n_net) return;
+  if (q >= v.n_pin) return;
+  const int e = pin_net[q];
   int c[4];
-  ip_net<false>(v, e, c, nullptr, nullptr);
-  for (int a = 0; a < 4; a++) cnt[a * stride + e] = c[a];
+  ip_pin<false>(v, q, v.net_start[e], v.net_start[e + 1], ext[e], c, nullptr, nullptr);
+  for (int a = 0; a < 4; a++) cnt[a * stride + q] = c[a];
 }
 
-__global__ void __launch_bounds__(RP_BLOCK) k_ip_emit(IpView v, IpOut o) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= v.n_net) return;
-  const int stride = v.n_net + 1;
-  Emit ex{o.tk[0], o.tv[0], o.rk[0], o.rv[0], o.off[0 * stride + e], o.off[2 * stride + e], o.cb};
-  Emit ey{o.tk[1], o.tv[1], o.rk[1], o.rv[1], o.off[1 * stride + e], o.off[3 * stride + e], o.cb};
+__global__ void __launch_bounds__(RP_BLOCK) k_ip_emit(IpView v, const int* __restrict__ pin_net,
+                                                      const int4* __restrict__ ext, IpOut o) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= v.n_pin) return;
+  const int stride = v.n_pin + 1;
+  Emit ex{o.tk[0], o.tv[0], o.rk[0], o.rv[0], o.off[0 * stride + q], o.off[2 * stride + q], o.cb};
+  Emit ey{o.tk[1], o.tv[1], o.rk[1], o.rv[1], o.off[1 * stride + q], o.off[3 * stride + q], o.cb};
+  const int e = pin_net[q];
   int c[4];
-  ip_net<true>(v, e, c, &ex, &ey);
+  ip_pin<true>(v, q, v.net_start[e], v.net_start[e + 1], ext[e], c, &ex, &ey);
 }
 
 // run heads of a sorted key array
@@ -270,6 +288,11 @@ int rp_ip_asm_init(rp_ctx* c, RpIpAsm& a) {
     int k = i >= 0 ? c->inst2place[i] : -1;
     pk[q] = k >= 0 ? a.pos[k] : -1;
   }
+  std::vector<int> pin_net(c->n_pin);
+  for (int e = 0; e < c->n_net; e++)
+    for (int q = c->net_start[e]; q < c->net_start[e + 1]; q++) pin_net[q] = e;
+  HIPCHK(c, dupload(a.pin_net, pin_net, st));
+  HIPCHK(c, dalloc(a.ext, std::max(1, c->n_net)));
   HIPCHK(c, dupload(a.net_start, c->net_start, st));
   HIPCHK(c, dupload(a.pin_inst, c->pin_inst, st));
   HIPCHK(c, dupload(a.pin_k, pk, st));
@@ -277,8 +300,8 @@ int rp_ip_asm_init(rp_ctx* c, RpIpAsm& a) {
   HIPCHK(c, dalloc(a.pin_cy, c->n_pin));
   HIPCHK(c, dalloc(a.icx, n));
   HIPCHK(c, dalloc(a.icy, n));
-  HIPCHK(c, dalloc(a.cnt, 4 * ((size_t)c->n_net + 1)));
-  HIPCHK(c, dalloc(a.off, 4 * ((size_t)c->n_net + 1)));
+  HIPCHK(c, dalloc(a.cnt, 4 * ((size_t)c->n_pin + 1)));
+  HIPCHK(c, dalloc(a.off, 4 * ((size_t)c->n_pin + 1)));
   for (int ax = 0; ax < 2; ax++) {
     HIPCHK(c, dalloc(a.rowp[ax], (size_t)n + 1));
     HIPCHK(c, dalloc(a.b[ax], n));
@@ -292,14 +315,16 @@ int rp_ip_asm_init(rp_ctx* c, RpIpAsm& a) {
 // (a.pin_cx / a.pin_cy / a.icx / a.icy).
 int rp_ip_assemble(rp_ctx* c, RpIpAsm& a, const int*, const int*) {
   hipStream_t st = c->stream;
-  const int n = a.n, nn = a.n_net;
-  const size_t stride = (size_t)nn + 1;
-  IpView v{nn, n, a.net_start.p, a.pin_inst.p, a.pin_k.p, a.pin_cx.p, a.pin_cy.p, a.icx.p, a.icy.p,
+  const int n = a.n, nn = a.n_net, np = c->n_pin;
+  const size_t stride = (size_t)np + 1;
+  IpView v{nn, n, np, a.net_start.p, a.pin_inst.p, a.pin_k.p, a.pin_cx.p, a.pin_cy.p, a.icx.p, a.icy.p,
            c->prm.initial_place_min_diff_length, c->prm.initial_place_max_fanout,
            c->prm.initial_place_net_weight_scale};
-  hipLaunchKernelGGL(k_ip_count, grid_of((long)nn + 1), dim3(RP_BLOCK), 0, st, v, a.cnt.p);
+  hipLaunchKernelGGL(k_ip_ext, grid_of(nn), dim3(RP_BLOCK), 0, st, v, a.ext.p);
+  hipLaunchKernelGGL(k_ip_count, grid_of((long)np + 1), dim3(RP_BLOCK), 0, st, v, (const int*)a.pin_net.p,
+                     (const int4*)a.ext.p, a.cnt.p);
   HIPCHK(c, hipGetLastError());
-  // exclusive scans of the four count lists (length nn + 1: the last entry is the total)
+  // exclusive scans of the four count lists (length np + 1: the last entry is the total)
   size_t need = 0;
   HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, need, a.cnt.p, a.off.p, (int)stride, st));
   HIPCHK(c, ensure(a.temp, need));
@@ -309,7 +334,7 @@ int rp_ip_assemble(rp_ctx* c, RpIpAsm& a, const int*, const int*) {
                                                (int)stride, st));
   }
   for (int q = 0; q < 4; q++)
-    HIPCHK(c, hipMemcpyAsync(a.h_tot + q, a.off.p + q * stride + nn, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(a.h_tot + q, a.off.p + q * stride + np, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(c, hipStreamSynchronize(st));
   int mt[2] = {a.h_tot[0], a.h_tot[1]}, mr[2] = {a.h_tot[2], a.h_tot[3]};
   for (int ax = 0; ax < 2; ax++) {
@@ -331,7 +356,8 @@ int rp_ip_assemble(rp_ctx* c, RpIpAsm& a, const int*, const int*) {
   HIPCHK(c, ensure(a.srow, mmax));
   IpOut o{a.off.p, {a.tk[0].p, a.tk[1].p}, {a.tv[0].p, a.tv[1].p}, {a.rk[0].p, a.rk[1].p},
           {a.rv[0].p, a.rv[1].p}, a.cb};
-  hipLaunchKernelGGL(k_ip_emit, grid_of(nn), dim3(RP_BLOCK), 0, st, v, o);
+  hipLaunchKernelGGL(k_ip_emit, grid_of(np), dim3(RP_BLOCK), 0, st, v, (const int*)a.pin_net.p,
+                     (const int4*)a.ext.p, o);
   HIPCHK(c, hipGetLastError());
   for (int ax = 0; ax < 2; ax++) {
     const int m = mt[ax], r = mr[ax];
