@@ -402,46 +402,21 @@ int rp_ip_assemble(rp_ctx* c, RpIpAsm& a, const int*, const int*) {
 // Identity preconditioner, the published algorithm (van der Vorst 1992) in
 // the variant that restarts when rho collapses (the reference calls Eigen's
 // BiCGSTAB, src/initialPlace.cpp:78-86).  Both axes run in the same launches
-// (blockIdx.y = axis).
-//
-// Grid form: four launches per iteration k.
-//   restart  only when the head of k asks for it: r = b - A x, r0 = r
-//   pv       p = r + beta (p' - w' v') for its rows; v = A p, where the p of
-//            a neighbour row is recomputed from r, p', v' with the same
-//            expression (so it equals what its owner writes); partial r0.v
-//   st       s = r - alpha v for its rows; t = A s with s recomputed the same
-//            way; partials t.t, t.s
-//   xr       x += alpha p + w s ; r = s - w t ; partials r.r, r0.r
-// p and v alternate between two buffers by iteration parity (p', v' are the
-// previous iteration's).  No block waits for another: every block of a
-// launch derives the scalars it needs from the per-block partials of the
-// earlier launches (summed in block order, so every block gets the same
-// values) and from the state the previous iteration committed (written by
-// block 0 of its xr launch, by parity).  The head of k (termination test,
-// rho, restart test) is therefore recomputed in each launch of k.  A
-// finished axis sets its sticky done flag, after which every launch of that
-// axis returns at once.
+// (blockIdx.y = axis).  FP64 dot products are reduced in a fixed order:
+// per-block partials of a fixed grid, summed sequentially per thread and
+// across the block by one fixed tree, so every block that needs a scalar
+// computes the same value and results do not depend on timing.
 namespace {
 
-constexpr int BC_MAX_PARTS = 256;
+constexpr int BC_MAX_PARTS = 1024;
 constexpr int BC_G = 8;  // lanes per CSR row in the SpMV kernels
-// partial-sum slots per axis; r.r and r0.r alternate by iteration parity
-enum { BC_RR0 = 0, BC_BB, BC_R0V, BC_TT, BC_TS, BC_RST, BC_RR, BC_R0R = BC_RR + 2, BC_SLOTS = BC_R0R + 2 };
+constexpr int BC_SLOTS = 8;  // partial-sum slots per axis
+enum { BC_RR0 = 0, BC_BB, BC_R0V, BC_TT, BC_TS, BC_RR, BC_R0R, BC_RST };
 
-// Final record of one solve, read by the host (the one-workgroup solver writes
-// the same fields).
 struct BcState {
-  double tol2, eps2;  // inputs: tol^2, eps^2
-  int max_iter;       // input
-  int done, zero, i, restarts, pending;
-  double rr, rhs_sq;
+  double rho, rho_old, alpha, w, rr, r0r, r0_sq, rhs_sq, tol2, eps2;
+  int i, restarts, done, restart, zero, max_iter, started, pending;  // pending: an iteration awaits its head
   float err;
-};
-
-// State after an iteration, as the next one needs it.
-struct BcCommit {
-  double rho, alpha, w, r0_sq, rhs_sq;
-  int i, restarts;
 };
 
 struct BcAxis {
@@ -449,12 +424,9 @@ struct BcAxis {
   const int* col;
   const float* val;
   const float* b;
-  float *x, *r, *r0, *s, *t;
-  float* p[2];
-  float* v[2];
+  float *x, *r, *r0, *v, *p, *s, *t;
   double* parts;  // BC_SLOTS x nparts
   BcState* st;
-  BcCommit* cm;   // [2], by parity of the iteration that reads it
 };
 
 struct BcArgs {
@@ -479,79 +451,11 @@ __device__ double bsum(double v, double* sh) {
   return r;
 }
 
-// sums of M partial slots in block order; results valid in every thread
-template <int M>
-__device__ void psums(const BcAxis& a, const int (&slot)[M], int nparts, double (&out)[M], double* sh) {
-  double v[M];
-#pragma unroll
-  for (int m = 0; m < M; m++) v[m] = 0.0;
-  for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
-#pragma unroll
-    for (int m = 0; m < M; m++) v[m] += a.parts[slot[m] * nparts + k];
-  }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
-#pragma unroll
-  for (int m = 0; m < M; m++) v[m] = wsum(v[m]);
-  __syncthreads();
-  if (lane == 0) {
-#pragma unroll
-    for (int m = 0; m < M; m++) sh[m * nw + w] = v[m];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int m = 0; m < M; m++) {
-    double r = 0.0;
-    for (int k = 0; k < nw; k++) r += sh[m * nw + k];
-    out[m] = r;
-  }
-}
-
-// Head of iteration k, identical in every block.
-struct BcHead {
-  double rr, rho, rho_old, alpha_prev, w_prev, r0_sq, rhs_sq;
-  int i, restarts, done, restart, zero;
-};
-
-__device__ BcHead bc_head(const BcAxis& a, int k, int nparts, double* sh) {
-  const BcState& S = *a.st;
-  BcHead h;
-  if (k == 0) {
-    const int sl[2] = {BC_RR0, BC_BB};
-    double o[2];
-    psums<2>(a, sl, nparts, o, sh);
-    h.rr = h.rho = h.r0_sq = o[0];
-    h.rhs_sq = o[1];
-    h.rho_old = h.alpha_prev = h.w_prev = 1.0;
-    h.i = h.restarts = 0;
-  } else {
-    const int q = (k - 1) & 1;
-    const int sl[2] = {BC_RR + q, BC_R0R + q};
-    double o[2];
-    psums<2>(a, sl, nparts, o, sh);
-    const BcCommit& C = a.cm[k & 1];
-    h.rr = o[0];
-    h.rho = o[1];
-    h.rho_old = C.rho;
-    h.alpha_prev = C.alpha;
-    h.w_prev = C.w;
-    h.r0_sq = C.r0_sq;
-    h.rhs_sq = C.rhs_sq;
-    h.i = C.i + 1;
-    h.restarts = C.restarts;
-  }
-  h.zero = h.rhs_sq == 0.0;
-  h.done = h.zero || !(h.rr > S.tol2 * h.rhs_sq && h.i < S.max_iter);
-  h.restart = !h.done && fabs(h.rho) < S.eps2 * h.r0_sq;
-  return h;
-}
-
-// after a restart of iteration k (its r.r in slot BC_RST)
-__device__ void bc_restarted(const BcAxis& a, BcHead& h, int nparts, double* sh) {
-  const int sl[1] = {BC_RST};
-  double o[1];
-  psums<1>(a, sl, nparts, o, sh);
-  h.rho = h.r0_sq = o[0];
-  if (h.restarts++ == 0) h.i = 0;
+// sum of one partial slot, identical in every block
+__device__ double psum(const BcAxis& a, int slot, int nparts, double* sh) {
+  double v = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) v += a.parts[slot * nparts + k];
+  return bsum(v, sh);
 }
 
 // y_i = sum_k val_k x_col_k over BC_G lanes (strided entries, FP64), then a
@@ -571,7 +475,7 @@ __device__ __forceinline__ float spmv_row(const BcAxis& a, int i, const float* _
        i += (int)(gridDim.x * blockDim.x / BC_G))
 #define BC_LEAD ((threadIdx.x & (BC_G - 1)) == 0)
 
-// r = b - A x ; r0 = r ; p' = v' = 0 ; partials r.r and b.b
+// r = b - A x ; r0 = r ; v = p = 0 ; partials r.r and b.b
 __global__ void __launch_bounds__(RP_BLOCK) k_bc_init(BcArgs g) {
   __shared__ double sh[RP_BLOCK / 64];
   const BcAxis& a = g.ax[blockIdx.y];
@@ -582,8 +486,8 @@ __global__ void __launch_bounds__(RP_BLOCK) k_bc_init(BcArgs g) {
       float r = (float)((double)a.b[i] + -1.0 * (double)t);
       a.r[i] = r;
       a.r0[i] = r;
-      a.v[1][i] = 0.0f;
-      a.p[1][i] = 0.0f;
+      a.v[i] = 0.0f;
+      a.p[i] = 0.0f;
       d0 += (double)r * (double)r;
       d1 += (double)a.b[i] * (double)a.b[i];
     }
@@ -596,28 +500,56 @@ __global__ void __launch_bounds__(RP_BLOCK) k_bc_init(BcArgs g) {
   }
 }
 
-// Head of k: a finished axis gets its final record (block 0) and the sticky
-// flag; a restart recomputes r = b - A x ; r0 = r and the partial r.r.
-__global__ void __launch_bounds__(RP_BLOCK) k_bc_restart(BcArgs g, int k) {
-  __shared__ double sh[4 * (RP_BLOCK / 64)];
+// Scalar head of an iteration: finishes the previous iteration (or the
+// set-up), decides termination, and prepares rho / the restart test.
+__global__ void __launch_bounds__(1024) k_bc_head(BcArgs g) {
+  __shared__ double sh[1024 / 64];
   const BcAxis& a = g.ax[blockIdx.y];
   BcState& S = *a.st;
-  if (S.done) return;
-  const BcHead h = bc_head(a, k, g.nparts, sh);
-  if (h.done) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      S.zero = h.zero;
-      S.i = h.i;
-      S.restarts = h.restarts;
-      S.rr = h.rr;
-      S.rhs_sq = h.rhs_sq;
-      S.err = h.zero ? 0.0f : (float)sqrt(h.rr / h.rhs_sq);
-      S.pending = k;  // iterations run
-      S.done = 1;
+  if (S.done || (S.started && !S.pending)) return;  // idempotent: nothing to close
+  double rr, r0r;
+  int i;
+  if (!S.started) {
+    double r0_sq = psum(a, BC_RR0, g.nparts, sh);
+    double rhs_sq = psum(a, BC_BB, g.nparts, sh);
+    rr = r0r = r0_sq;
+    i = 0;
+    if (threadIdx.x == 0) {
+      S.r0_sq = r0_sq;
+      S.rhs_sq = rhs_sq;
+      S.zero = rhs_sq == 0.0;
+      S.tol2 = S.tol2 * rhs_sq;  // tol^2 was stored by the host
+      S.rho = S.alpha = S.w = 1.0;
+      S.restarts = 0;
+      S.started = 1;
     }
+  } else {
+    rr = psum(a, BC_RR, g.nparts, sh);
+    r0r = psum(a, BC_R0R, g.nparts, sh);
+    i = S.i + 1;
+  }
+  if (threadIdx.x != 0) return;
+  const bool zero = S.zero;
+  S.pending = 0;
+  S.rr = rr;
+  S.r0r = r0r;
+  S.i = i;
+  if (zero || !(rr > S.tol2 && i < S.max_iter)) {
+    S.done = 1;
+    S.err = zero ? 0.0f : (float)sqrt(rr / S.rhs_sq);
     return;
   }
-  if (!h.restart) return;
+  S.rho_old = S.rho;
+  S.rho = r0r;
+  S.restart = fabs(S.rho) < S.eps2 * S.r0_sq;
+}
+
+// restart: r = b - A x ; r0 = r ; partial r.r
+__global__ void __launch_bounds__(RP_BLOCK) k_bc_restart(BcArgs g) {
+  __shared__ double sh[RP_BLOCK / 64];
+  const BcAxis& a = g.ax[blockIdx.y];
+  const BcState& S = *a.st;
+  if (S.done || !S.restart) return;
   double d0 = 0.0;
   BC_GROUP_ROWS(i) {
     float t = spmv_row(a, i, a.x);
@@ -632,69 +564,66 @@ __global__ void __launch_bounds__(RP_BLOCK) k_bc_restart(BcArgs g, int k) {
   if (threadIdx.x == 0) a.parts[BC_RST * g.nparts + blockIdx.x] = d0;
 }
 
-// p = r + beta (p' - w' v') ; v = A p ; partial r0.v
-__global__ void __launch_bounds__(RP_BLOCK) k_bc_pv(BcArgs g, int k) {
-  __shared__ double sh[4 * (RP_BLOCK / 64)];
+// p = r + beta (p - w v)   (after a restart, rho = r0_sq = r.r)
+__global__ void __launch_bounds__(RP_BLOCK) k_bc_p(BcArgs g) {
+  __shared__ double sh[RP_BLOCK / 64];
+  const BcAxis& a = g.ax[blockIdx.y];
+  BcState& S = *a.st;
+  if (S.done) return;
+  double rho = S.rho;
+  if (S.restart) {
+    rho = psum(a, BC_RST, g.nparts, sh);
+    __syncthreads();  // every thread has read S before block 0 writes it
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // no other block reads these here
+      S.rho = S.r0_sq = rho;
+      if (S.restarts++ == 0) S.i = 0;
+    }
+  }
+  const double beta = (rho / S.rho_old) * (S.alpha / S.w), w = S.w;
+  BC_ROWS(i) a.p[i] = (float)((double)a.r[i] + beta * ((double)a.p[i] - w * (double)a.v[i]));
+}
+
+// v = A p ; partial r0.v
+__global__ void __launch_bounds__(RP_BLOCK) k_bc_v(BcArgs g) {
+  __shared__ double sh[RP_BLOCK / 64];
   const BcAxis& a = g.ax[blockIdx.y];
   if (a.st->done) return;
-  BcHead h = bc_head(a, k, g.nparts, sh);
-  if (h.restart) bc_restarted(a, h, g.nparts, sh);
-  const int ph = k & 1;
-  const double beta = (h.rho / h.rho_old) * (h.alpha_prev / h.w_prev), w = h.w_prev;
-  const float* __restrict__ po = a.p[ph ^ 1];
-  const float* __restrict__ vo = a.v[ph ^ 1];
-  float* __restrict__ pn = a.p[ph];
-  float* __restrict__ vn = a.v[ph];
   double d = 0.0;
   BC_GROUP_ROWS(i) {
-    const int lane = threadIdx.x & (BC_G - 1);
-    double acc = 0.0;
-    for (int e = a.rowp[i] + lane; e < a.rowp[i + 1]; e += BC_G) {
-      const int j = a.col[e];
-      const float pj = (float)((double)a.r[j] + beta * ((double)po[j] - w * (double)vo[j]));
-      acc += (double)a.val[e] * (double)pj;
-    }
-    for (int o = BC_G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    float v = spmv_row(a, i, a.p);
     if (BC_LEAD) {
-      const float vi = (float)acc;
-      pn[i] = (float)((double)a.r[i] + beta * ((double)po[i] - w * (double)vo[i]));
-      vn[i] = vi;
-      d += (double)a.r0[i] * (double)vi;
+      a.v[i] = v;
+      d += (double)a.r0[i] * (double)v;
     }
   }
   d = bsum(d, sh);
   if (threadIdx.x == 0) a.parts[BC_R0V * g.nparts + blockIdx.x] = d;
 }
 
-// alpha = rho / r0.v ; s = r - alpha v ; t = A s ; partials t.t, t.s
-__global__ void __launch_bounds__(RP_BLOCK) k_bc_st(BcArgs g, int k) {
-  __shared__ double sh[4 * (RP_BLOCK / 64)];
+// alpha = rho / (r0.v) ; s = r - alpha v
+__global__ void __launch_bounds__(RP_BLOCK) k_bc_s(BcArgs g) {
+  __shared__ double sh[RP_BLOCK / 64];
+  const BcAxis& a = g.ax[blockIdx.y];
+  BcState& S = *a.st;
+  if (S.done) return;
+  const double alpha = S.rho / psum(a, BC_R0V, g.nparts, sh);
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) S.alpha = alpha;
+  BC_ROWS(i) a.s[i] = (float)((double)a.r[i] + -alpha * (double)a.v[i]);
+}
+
+// t = A s ; partials t.t and t.s
+__global__ void __launch_bounds__(RP_BLOCK) k_bc_t(BcArgs g) {
+  __shared__ double sh[RP_BLOCK / 64];
   const BcAxis& a = g.ax[blockIdx.y];
   if (a.st->done) return;
-  BcHead h = bc_head(a, k, g.nparts, sh);
-  if (h.restart) bc_restarted(a, h, g.nparts, sh);
-  const int sl[1] = {BC_R0V};
-  double o[1];
-  psums<1>(a, sl, g.nparts, o, sh);
-  const double alpha = h.rho / o[0];
-  const float* __restrict__ vn = a.v[k & 1];
   double tt = 0.0, ts = 0.0;
   BC_GROUP_ROWS(i) {
-    const int lane = threadIdx.x & (BC_G - 1);
-    double acc = 0.0;
-    for (int e = a.rowp[i] + lane; e < a.rowp[i + 1]; e += BC_G) {
-      const int j = a.col[e];
-      const float sj = (float)((double)a.r[j] + -alpha * (double)vn[j]);
-      acc += (double)a.val[e] * (double)sj;
-    }
-    for (int o2 = BC_G / 2; o2 > 0; o2 >>= 1) acc += __shfl_xor(acc, o2, 64);
+    float t = spmv_row(a, i, a.s);
     if (BC_LEAD) {
-      const float ti = (float)acc;
-      const float si = (float)((double)a.r[i] + -alpha * (double)vn[i]);
-      a.s[i] = si;
-      a.t[i] = ti;
-      tt += (double)ti * (double)ti;
-      ts += (double)ti * (double)si;
+      a.t[i] = t;
+      tt += (double)t * (double)t;
+      ts += (double)t * (double)a.s[i];
     }
   }
   tt = bsum(tt, sh);
@@ -705,34 +634,23 @@ __global__ void __launch_bounds__(RP_BLOCK) k_bc_st(BcArgs g, int k) {
   }
 }
 
-// w = t.s / t.t ; x += alpha p + w s ; r = s - w t ; partials r.r, r0.r;
-// block 0 commits the iteration's state for k + 1
-__global__ void __launch_bounds__(RP_BLOCK) k_bc_xr(BcArgs g, int k) {
-  __shared__ double sh[4 * (RP_BLOCK / 64)];
+// w = (t.s)/(t.t) ; x += alpha p + w s ; r = s - w t ; partials r.r and r0.r
+__global__ void __launch_bounds__(RP_BLOCK) k_bc_xr(BcArgs g) {
+  __shared__ double sh[RP_BLOCK / 64];
   const BcAxis& a = g.ax[blockIdx.y];
-  if (a.st->done) return;
-  BcHead h = bc_head(a, k, g.nparts, sh);
-  if (h.restart) bc_restarted(a, h, g.nparts, sh);
-  const int sl[3] = {BC_R0V, BC_TT, BC_TS};
-  double o[3];
-  psums<3>(a, sl, g.nparts, o, sh);
-  const double alpha = h.rho / o[0];
-  const double w = o[1] > 0.0 ? o[2] / o[1] : 0.0;
-  const int ph = k & 1;
+  BcState& S = *a.st;
+  if (S.done) return;
+  const double tt = psum(a, BC_TT, g.nparts, sh);
+  const double ts = psum(a, BC_TS, g.nparts, sh);
+  const double w = tt > 0.0 ? ts / tt : 0.0, alpha = S.alpha;
+  __syncthreads();
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    BcCommit& C = a.cm[ph ^ 1];
-    C.rho = h.rho;
-    C.alpha = alpha;
-    C.w = w;
-    C.r0_sq = h.r0_sq;
-    C.rhs_sq = h.rhs_sq;
-    C.i = h.i;
-    C.restarts = h.restarts;
+    S.w = w;
+    S.pending = 1;
   }
-  const float* __restrict__ pn = a.p[ph];
   double d0 = 0.0, d1 = 0.0;
   BC_ROWS(i) {
-    a.x[i] = (float)((double)a.x[i] + alpha * (double)pn[i] + w * (double)a.s[i]);
+    a.x[i] = (float)((double)a.x[i] + alpha * (double)a.p[i] + w * (double)a.s[i]);
     float r = (float)((double)a.s[i] - w * (double)a.t[i]);
     a.r[i] = r;
     d0 += (double)r * (double)r;
@@ -741,8 +659,8 @@ __global__ void __launch_bounds__(RP_BLOCK) k_bc_xr(BcArgs g, int k) {
   d0 = bsum(d0, sh);
   d1 = bsum(d1, sh);
   if (threadIdx.x == 0) {
-    a.parts[(BC_RR + ph) * g.nparts + blockIdx.x] = d0;
-    a.parts[(BC_R0R + ph) * g.nparts + blockIdx.x] = d1;
+    a.parts[BC_RR * g.nparts + blockIdx.x] = d0;
+    a.parts[BC_R0R * g.nparts + blockIdx.x] = d1;
   }
 }
 
@@ -1349,20 +1267,15 @@ int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve) 
   HIPCHK(c, hipMemcpyAsync(A.pin_cx.p, c->pin_cx.data(), sizeof(int) * np, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(A.pin_cy.p, c->pin_cy.data(), sizeof(int) * np, hipMemcpyHostToDevice, st));
   // solver buffers (both axes)
-  // blocks per axis: one row per lane group up to BC_MAX_PARTS blocks (every
-  // block reads all partial sums of the earlier launches, so few blocks)
-  const int nparts =
-      (int)std::max<long>(1, std::min<long>(BC_MAX_PARTS, ((long)n * BC_G + RP_BLOCK - 1) / RP_BLOCK));
-  DevBuf<float> vec[2][9];  // x, r, r0, s, t, p[2], v[2]
+  const int nparts = (int)std::max<long>(1, std::min<long>(BC_MAX_PARTS, ((long)n * BC_G + RP_BLOCK - 1) / RP_BLOCK));
+  DevBuf<float> vec[2][7];
   DevBuf<double> parts[2];
   DevBuf<IpRec> d_rec;
-  DevBuf<BcCommit> d_commit;  // [axis][parity]
   for (int ax = 0; ax < 2; ax++) {
     for (auto& v : vec[ax]) HIPCHK(c, dalloc(v, n));
     HIPCHK(c, dalloc(parts[ax], (size_t)BC_SLOTS * nparts));
   }
   HIPCHK(c, dalloc(d_rec, 1));
-  HIPCHK(c, dalloc(d_commit, 4));
   IpRec* h_rec = nullptr;
   int* h_done = nullptr;
   HIPCHK(c, hipHostMalloc((void**)&h_rec, sizeof(IpRec), hipHostMallocDefault));
@@ -1384,9 +1297,9 @@ int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve) 
   init.tol2 = (double)tol * (double)tol;
   init.eps2 = 1.1920928955078125e-07 * 1.1920928955078125e-07;
   init.max_iter = p.initial_place_max_solver_iter;
-  const dim3 gb(nparts, 2), blk(RP_BLOCK);
+  const dim3 gb(nparts, 2), g1(1, 2), blk(RP_BLOCK);
   const dim3 gn((n + RP_BLOCK - 1) / RP_BLOCK), gp((np + RP_BLOCK - 1) / RP_BLOCK);
-  const int CH = 16;  // iterations launched between polls of the done flags
+  const int CH = 8;  // iterations launched between polls of the done flags
   const int cap = 2 * std::max(0, init.max_iter) + 1;  // i restarts from 0 at most once
   // one workgroup per axis for small systems; RP_IP_SOLVER=blocks keeps the
   // grid-wide kernels at every size
@@ -1435,15 +1348,12 @@ int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve) 
       a.x = vec[ax][0].p;
       a.r = vec[ax][1].p;
       a.r0 = vec[ax][2].p;
-      a.s = vec[ax][3].p;
-      a.t = vec[ax][4].p;
-      a.p[0] = vec[ax][5].p;
-      a.p[1] = vec[ax][6].p;
-      a.v[0] = vec[ax][7].p;
-      a.v[1] = vec[ax][8].p;
+      a.v = vec[ax][3].p;
+      a.p = vec[ax][4].p;
+      a.s = vec[ax][5].p;
+      a.t = vec[ax][6].p;
       a.parts = parts[ax].p;
       a.st = &d_rec.p->st[ax];
-      a.cm = d_commit.p + 2 * ax;
     }
     IpRec r0{};
     r0.st[0] = r0.st[1] = init;
@@ -1503,16 +1413,19 @@ int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve) 
       HIPCHK(c, hipGetLastError());
     } else {
     hipLaunchKernelGGL(k_bc_init, gb, blk, 0, st, g);
-    hipLaunchKernelGGL(k_bc_restart, gb, blk, 0, st, g, 0);  // head of iteration 0
     int done_iters = 0;
     for (;;) {
       for (int j = 0; j < CH; j++) {
-        hipLaunchKernelGGL(k_bc_pv, gb, blk, 0, st, g, done_iters);
-        hipLaunchKernelGGL(k_bc_st, gb, blk, 0, st, g, done_iters);
-        hipLaunchKernelGGL(k_bc_xr, gb, blk, 0, st, g, done_iters);
-        done_iters++;
-        hipLaunchKernelGGL(k_bc_restart, gb, blk, 0, st, g, done_iters);  // head of the next iteration
+        hipLaunchKernelGGL(k_bc_head, g1, dim3(1024), 0, st, g);
+        hipLaunchKernelGGL(k_bc_restart, gb, blk, 0, st, g);
+        hipLaunchKernelGGL(k_bc_p, gb, blk, 0, st, g);
+        hipLaunchKernelGGL(k_bc_v, gb, blk, 0, st, g);
+        hipLaunchKernelGGL(k_bc_s, gb, blk, 0, st, g);
+        hipLaunchKernelGGL(k_bc_t, gb, blk, 0, st, g);
+        hipLaunchKernelGGL(k_bc_xr, gb, blk, 0, st, g);
       }
+      done_iters += CH;
+      hipLaunchKernelGGL(k_bc_head, g1, dim3(1024), 0, st, g);  // closes the chunk's last iteration
       HIPCHK(c, hipGetLastError());
       HIPCHK(c, hipMemcpyAsync(&h_done[0], &d_rec.p->st[0].done, sizeof(int), hipMemcpyDeviceToHost, st));
       HIPCHK(c, hipMemcpyAsync(&h_done[1], &d_rec.p->st[1].done, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1534,9 +1447,9 @@ int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve) 
     if (!h_rec->st[0].done || !h_rec->st[1].done)
       return fail(c, RP_ERR_STATE, "initial place: BiCGSTAB did not terminate");
     float err = std::max(h_rec->st[0].err, h_rec->st[1].err);
-    if (getenv("RP_IP_TIMING"))
-      fprintf(stderr, "[rp] initial place %d: %s solves of %d rows, %d / %d iterations\n", it,
-              small_r ? "one-workgroup" : "grid", n, h_rec->st[0].pending, h_rec->st[1].pending);
+    if (small_r && getenv("RP_IP_TIMING"))
+      fprintf(stderr, "[rp] initial place %d: one-workgroup solves of %d rows, %d / %d iterations\n", it, n,
+              h_rec->st[0].pending, h_rec->st[1].pending);
     c->ip_err.push_back(err);
     c->ip_hpwl.push_back(h_rec->hpwl);
     if (err <= 1e-5 && it >= 5) break;
