@@ -185,10 +185,29 @@ struct RpIpAsm {
 };
 int rp_ip_asm_init(rp_ctx* c, RpIpAsm& a);
 const std::vector<int>& rp_movable_order(rp_ctx* c);  // BFS locality order of movable instances
-int rp_ip_assemble(rp_ctx* c, RpIpAsm& a, const int* unused_x, const int* unused_y);
+int rp_ip_assemble_a(rp_ctx* c, RpIpAsm& a, hipStream_t st);  // counts; wait for a.h_tot, then
+int rp_ip_assemble_b(rp_ctx* c, RpIpAsm& a, hipStream_t st);  // the CSR and right-hand sides
 // sequential host assembly of the same systems from the device's centres (rp_host.cpp)
 int rp_ip_host_assemble(rp_ctx* c, RpIpAsm& a);
-// the InitialPlace iterations on the device (rp_ip.hip)
+
+// The InitialPlace iterations on the device (rp_ip.hip), one context's run
+// advanced phase by phase so that a driver can interleave many runs on one
+// stream; rp_ip_device_loop drives a single run.  Every phase enqueues on the
+// run's stream; "wait" means the caller synchronises that stream in between.
+struct RpIpRun;
+struct RpIpBatch;  // tables and staging of a driver that solves many runs together
+int rp_ipr_begin(rp_ctx* c, hipStream_t st, bool host_asm, RpIpRun** out);  // uploads on c->stream, then waits
+int rp_ipr_assemble_a(RpIpRun* r);  // wait, then
+int rp_ipr_assemble_b(RpIpRun* r);  // CSR, warm start, solver inputs
+// the solves of k runs (all on stream st); returns once every one has ended
+int rp_ipr_solve(RpIpRun* const* runs, int k, hipStream_t st, RpIpBatch* b);
+int rp_ipr_finish_a(RpIpRun* r);  // HPWL record, updateCoordi; wait, then
+int rp_ipr_finish_b(RpIpRun* r, bool* stop);  // records; stop after the reference's rule
+int rp_ipr_end(RpIpRun* r);                   // positions back to the host model
+void rp_ipr_free(RpIpRun* r);
+rp_ctx* rp_ipr_ctx(RpIpRun* r);
+RpIpBatch* rp_ipb_create();
+void rp_ipb_free(RpIpBatch* b);
 int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve);
 
 // launch wrappers (rp_kernels.hip)

@@ -19,6 +19,7 @@
 // Integer keys and in-order float sums make the result bit-identical to a
 // sequential host assembly, independent of launch shape.
 #include <chrono>
+#include <memory>
 #include <cstdlib>
 #include <cstring>
 
@@ -313,8 +314,9 @@ int rp_ip_asm_init(rp_ctx* c, RpIpAsm& a) {
 
 // Assemble both axes from the device's current pin and instance centres
 // (a.pin_cx / a.pin_cy / a.icx / a.icy).
-int rp_ip_assemble(rp_ctx* c, RpIpAsm& a, const int*, const int*) {
-  hipStream_t st = c->stream;
+// First half: per-pin counts and their scans; the four totals are copied to
+// a.h_tot, which the caller waits for before the second half.
+int rp_ip_assemble_a(rp_ctx* c, RpIpAsm& a, hipStream_t st) {
   const int n = a.n, nn = a.n_net, np = c->n_pin;
   const size_t stride = (size_t)np + 1;
   IpView v{nn, n, np, a.net_start.p, a.pin_inst.p, a.pin_k.p, a.pin_cx.p, a.pin_cy.p, a.icx.p, a.icy.p,
@@ -335,7 +337,15 @@ int rp_ip_assemble(rp_ctx* c, RpIpAsm& a, const int*, const int*) {
   }
   for (int q = 0; q < 4; q++)
     HIPCHK(c, hipMemcpyAsync(a.h_tot + q, a.off.p + q * stride + np, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(c, hipStreamSynchronize(st));
+  return 0;
+}
+
+// Second half: triplets, CSR and right-hand sides of both axes.
+int rp_ip_assemble_b(rp_ctx* c, RpIpAsm& a, hipStream_t st) {
+  const int n = a.n, nn = a.n_net, np = c->n_pin;
+  IpView v{nn, n, np, a.net_start.p, a.pin_inst.p, a.pin_k.p, a.pin_cx.p, a.pin_cy.p, a.icx.p, a.icy.p,
+           c->prm.initial_place_min_diff_length, c->prm.initial_place_max_fanout,
+           c->prm.initial_place_net_weight_scale};
   int mt[2] = {a.h_tot[0], a.h_tot[1]}, mr[2] = {a.h_tot[2], a.h_tot[3]};
   for (int ax = 0; ax < 2; ax++) {
     if (mt[ax] < 0 || mr[ax] < 0) return fail(c, RP_ERR_NOMEM, "initial place: B2B system exceeds 2^31 entries");
@@ -468,17 +478,20 @@ __device__ __forceinline__ float spmv_row(const BcAxis& a, int i, const float* _
   return (float)s;
 }
 
-#define BC_ROWS(i) for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += gridDim.x * blockDim.x)
+// Kernel bodies take (axis, block), so that one body serves a launch per
+// context and a launch over many contexts (a job table); g.nparts blocks of
+// RP_BLOCK threads cover an axis either way.
+#define BC_ROWS(i) for (int i = blk * RP_BLOCK + (int)threadIdx.x; i < g.n; i += g.nparts * RP_BLOCK)
 // one row per BC_G-lane group; all lanes of a group run the same trip count
 #define BC_GROUP_ROWS(i)                                                                  \
-  for (int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) / BC_G); i < g.n;            \
-       i += (int)(gridDim.x * blockDim.x / BC_G))
+  for (int i = (blk * RP_BLOCK + (int)threadIdx.x) / BC_G; i < g.n;                        \
+       i += g.nparts * RP_BLOCK / BC_G)
 #define BC_LEAD ((threadIdx.x & (BC_G - 1)) == 0)
 
 // r = b - A x ; r0 = r ; v = p = 0 ; partials r.r and b.b
-__global__ void __launch_bounds__(RP_BLOCK) k_bc_init(BcArgs g) {
+__device__ __forceinline__ void bc_init_body(const BcArgs& g, int axis, int blk) {
   __shared__ double sh[RP_BLOCK / 64];
-  const BcAxis& a = g.ax[blockIdx.y];
+  const BcAxis& a = g.ax[axis];
   double d0 = 0.0, d1 = 0.0;
   BC_GROUP_ROWS(i) {
     float t = spmv_row(a, i, a.x);
@@ -495,16 +508,16 @@ __global__ void __launch_bounds__(RP_BLOCK) k_bc_init(BcArgs g) {
   d0 = bsum(d0, sh);
   d1 = bsum(d1, sh);
   if (threadIdx.x == 0) {
-    a.parts[BC_RR0 * g.nparts + blockIdx.x] = d0;
-    a.parts[BC_BB * g.nparts + blockIdx.x] = d1;
+    a.parts[BC_RR0 * g.nparts + blk] = d0;
+    a.parts[BC_BB * g.nparts + blk] = d1;
   }
 }
 
 // Scalar head of an iteration: finishes the previous iteration (or the
 // set-up), decides termination, and prepares rho / the restart test.
-__global__ void __launch_bounds__(1024) k_bc_head(BcArgs g) {
+__device__ __forceinline__ void bc_head_body(const BcArgs& g, int axis, int blk) {
   __shared__ double sh[1024 / 64];
-  const BcAxis& a = g.ax[blockIdx.y];
+  const BcAxis& a = g.ax[axis];
   BcState& S = *a.st;
   if (S.done || (S.started && !S.pending)) return;  // idempotent: nothing to close
   double rr, r0r;
@@ -545,9 +558,9 @@ __global__ void __launch_bounds__(1024) k_bc_head(BcArgs g) {
 }
 
 // restart: r = b - A x ; r0 = r ; partial r.r
-__global__ void __launch_bounds__(RP_BLOCK) k_bc_restart(BcArgs g) {
+__device__ __forceinline__ void bc_restart_body(const BcArgs& g, int axis, int blk) {
   __shared__ double sh[RP_BLOCK / 64];
-  const BcAxis& a = g.ax[blockIdx.y];
+  const BcAxis& a = g.ax[axis];
   const BcState& S = *a.st;
   if (S.done || !S.restart) return;
   double d0 = 0.0;
@@ -561,20 +574,20 @@ __global__ void __launch_bounds__(RP_BLOCK) k_bc_restart(BcArgs g) {
     }
   }
   d0 = bsum(d0, sh);
-  if (threadIdx.x == 0) a.parts[BC_RST * g.nparts + blockIdx.x] = d0;
+  if (threadIdx.x == 0) a.parts[BC_RST * g.nparts + blk] = d0;
 }
 
 // p = r + beta (p - w v)   (after a restart, rho = r0_sq = r.r)
-__global__ void __launch_bounds__(RP_BLOCK) k_bc_p(BcArgs g) {
+__device__ __forceinline__ void bc_p_body(const BcArgs& g, int axis, int blk) {
   __shared__ double sh[RP_BLOCK / 64];
-  const BcAxis& a = g.ax[blockIdx.y];
+  const BcAxis& a = g.ax[axis];
   BcState& S = *a.st;
   if (S.done) return;
   double rho = S.rho;
   if (S.restart) {
     rho = psum(a, BC_RST, g.nparts, sh);
     __syncthreads();  // every thread has read S before block 0 writes it
-    if (blockIdx.x == 0 && threadIdx.x == 0) {  // no other block reads these here
+    if (blk == 0 && threadIdx.x == 0) {  // no other block reads these here
       S.rho = S.r0_sq = rho;
       if (S.restarts++ == 0) S.i = 0;
     }
@@ -584,9 +597,9 @@ __global__ void __launch_bounds__(RP_BLOCK) k_bc_p(BcArgs g) {
 }
 
 // v = A p ; partial r0.v
-__global__ void __launch_bounds__(RP_BLOCK) k_bc_v(BcArgs g) {
+__device__ __forceinline__ void bc_v_body(const BcArgs& g, int axis, int blk) {
   __shared__ double sh[RP_BLOCK / 64];
-  const BcAxis& a = g.ax[blockIdx.y];
+  const BcAxis& a = g.ax[axis];
   if (a.st->done) return;
   double d = 0.0;
   BC_GROUP_ROWS(i) {
@@ -597,25 +610,25 @@ __global__ void __launch_bounds__(RP_BLOCK) k_bc_v(BcArgs g) {
     }
   }
   d = bsum(d, sh);
-  if (threadIdx.x == 0) a.parts[BC_R0V * g.nparts + blockIdx.x] = d;
+  if (threadIdx.x == 0) a.parts[BC_R0V * g.nparts + blk] = d;
 }
 
 // alpha = rho / (r0.v) ; s = r - alpha v
-__global__ void __launch_bounds__(RP_BLOCK) k_bc_s(BcArgs g) {
+__device__ __forceinline__ void bc_s_body(const BcArgs& g, int axis, int blk) {
   __shared__ double sh[RP_BLOCK / 64];
-  const BcAxis& a = g.ax[blockIdx.y];
+  const BcAxis& a = g.ax[axis];
   BcState& S = *a.st;
   if (S.done) return;
   const double alpha = S.rho / psum(a, BC_R0V, g.nparts, sh);
   __syncthreads();
-  if (blockIdx.x == 0 && threadIdx.x == 0) S.alpha = alpha;
+  if (blk == 0 && threadIdx.x == 0) S.alpha = alpha;
   BC_ROWS(i) a.s[i] = (float)((double)a.r[i] + -alpha * (double)a.v[i]);
 }
 
 // t = A s ; partials t.t and t.s
-__global__ void __launch_bounds__(RP_BLOCK) k_bc_t(BcArgs g) {
+__device__ __forceinline__ void bc_t_body(const BcArgs& g, int axis, int blk) {
   __shared__ double sh[RP_BLOCK / 64];
-  const BcAxis& a = g.ax[blockIdx.y];
+  const BcAxis& a = g.ax[axis];
   if (a.st->done) return;
   double tt = 0.0, ts = 0.0;
   BC_GROUP_ROWS(i) {
@@ -629,22 +642,22 @@ __global__ void __launch_bounds__(RP_BLOCK) k_bc_t(BcArgs g) {
   tt = bsum(tt, sh);
   ts = bsum(ts, sh);
   if (threadIdx.x == 0) {
-    a.parts[BC_TT * g.nparts + blockIdx.x] = tt;
-    a.parts[BC_TS * g.nparts + blockIdx.x] = ts;
+    a.parts[BC_TT * g.nparts + blk] = tt;
+    a.parts[BC_TS * g.nparts + blk] = ts;
   }
 }
 
 // w = (t.s)/(t.t) ; x += alpha p + w s ; r = s - w t ; partials r.r and r0.r
-__global__ void __launch_bounds__(RP_BLOCK) k_bc_xr(BcArgs g) {
+__device__ __forceinline__ void bc_xr_body(const BcArgs& g, int axis, int blk) {
   __shared__ double sh[RP_BLOCK / 64];
-  const BcAxis& a = g.ax[blockIdx.y];
+  const BcAxis& a = g.ax[axis];
   BcState& S = *a.st;
   if (S.done) return;
   const double tt = psum(a, BC_TT, g.nparts, sh);
   const double ts = psum(a, BC_TS, g.nparts, sh);
   const double w = tt > 0.0 ? ts / tt : 0.0, alpha = S.alpha;
   __syncthreads();
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blk == 0 && threadIdx.x == 0) {
     S.w = w;
     S.pending = 1;
   }
@@ -659,17 +672,37 @@ __global__ void __launch_bounds__(RP_BLOCK) k_bc_xr(BcArgs g) {
   d0 = bsum(d0, sh);
   d1 = bsum(d1, sh);
   if (threadIdx.x == 0) {
-    a.parts[BC_RR * g.nparts + blockIdx.x] = d0;
-    a.parts[BC_R0R * g.nparts + blockIdx.x] = d1;
+    a.parts[BC_RR * g.nparts + blk] = d0;
+    a.parts[BC_R0R * g.nparts + blk] = d1;
   }
 }
 
 // a zero right-hand side gives x = 0
-__global__ void __launch_bounds__(RP_BLOCK) k_bc_zero_x(BcArgs g) {
-  const BcAxis& a = g.ax[blockIdx.y];
+__device__ __forceinline__ void bc_zero_x_body(const BcArgs& g, int axis, int blk) {
+  const BcAxis& a = g.ax[axis];
   if (!a.st->zero) return;
   BC_ROWS(i) a.x[i] = 0.0f;
 }
+
+// one launch per context (blockIdx.y = axis) and one over many contexts
+// (entry blockIdx.x of tab = (context, block))
+#define BC_KERNELS(NAME, NT)                                                             \
+  __global__ void __launch_bounds__(NT) k_bc_##NAME(BcArgs g) {                         \
+    bc_##NAME##_body(g, blockIdx.y, blockIdx.x);                                         \
+  }                                                                                      \
+  __global__ void __launch_bounds__(NT) kb_bc_##NAME(const BcArgs* gs, const int2* tab) { \
+    const int2 e = tab[blockIdx.x];                                                      \
+    bc_##NAME##_body(gs[e.x], blockIdx.y, e.y);                                          \
+  }
+BC_KERNELS(init, RP_BLOCK)
+BC_KERNELS(head, 1024)
+BC_KERNELS(restart, RP_BLOCK)
+BC_KERNELS(p, RP_BLOCK)
+BC_KERNELS(v, RP_BLOCK)
+BC_KERNELS(s, RP_BLOCK)
+BC_KERNELS(t, RP_BLOCK)
+BC_KERNELS(xr, RP_BLOCK)
+BC_KERNELS(zero_x, RP_BLOCK)
 
 // ---------------------------------------------------------------------------
 // Small systems: the whole solve in one workgroup per axis
@@ -874,10 +907,10 @@ __device__ __forceinline__ void bcs_spmv(const SellAxis& a, const int (&len)[R],
 // starts it, later ones resume from the carried state.  Launches stay short
 // (a queue shared with other work is not held for a whole solve).
 template <int R>
-__global__ void __launch_bounds__(BCS_THREADS) k_bc_small(SellArgs g, int first, int npass) {
+__device__ __forceinline__ void bcs_body(const SellArgs& g, int axis, int first, int npass) {
   extern __shared__ float bcs_lds[];
   __shared__ double red[2 * (BCS_THREADS / 64)];
-  const SellAxis& a = g.ax[blockIdx.x];
+  const SellAxis& a = g.ax[axis];
   BcState& S = *a.st;
   if (S.done) return;
   const int n = g.n, tid = threadIdx.x;
@@ -1088,6 +1121,16 @@ __global__ void __launch_bounds__(BCS_THREADS) k_bc_small(SellArgs g, int first,
   }
 }
 
+// one context per launch (blockIdx.x = axis) and many (blockIdx.x = 2 context + axis)
+template <int R>
+__global__ void __launch_bounds__(BCS_THREADS) k_bc_small(SellArgs g, int first, int npass) {
+  bcs_body<R>(g, blockIdx.x, first, npass);
+}
+template <int R>
+__global__ void __launch_bounds__(BCS_THREADS) kb_bc_small(const SellArgs* gs, int first, int npass) {
+  bcs_body<R>(gs[blockIdx.x >> 1], blockIdx.x & 1, first, npass);
+}
+
 // rows per thread of the one-workgroup solve, 0 when n is too large for it
 inline int bcs_rows_per_thread(int n) {
   for (int R = 1; R <= BCS_MAX_R; R *= 2)
@@ -1211,28 +1254,89 @@ struct IpRec {
   long long hpwl;
 };
 
+constexpr int BC_CHUNK = 8;  // grid iterations launched between polls of the done flags
+
 }  // namespace
 
-// The InitialPlace loop (src/initialPlace.cpp:56-105) with every iteration on
-// the device; the host reads one record (errors, HPWL) per iteration.
-int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve) {
+// One context's InitialPlace run (src/initialPlace.cpp:56-105).
+struct RpIpRun {
+  rp_ctx* c = nullptr;
+  hipStream_t st = nullptr;
+  bool host_asm = false;
+  int n = 0, np = 0, nn = 0, nparts = 0, hblocks = 0;
+  int small_r = 0, sell_rows = 0, sell_slices = 0;
+  int it = 0;  // IP iterations done
+  RpIpAsm A;
+  std::vector<int> icx, icy, dlx, dly, hx, hy;
+  DevBuf<int> d_dlx, d_dly, d_hx, d_hy, d_mw, d_mh, d_offx, d_offy;
+  DevBuf<int4> d_sbox;
+  DevBuf<long long> d_hparts;
+  DevBuf<float> vec[2][7];
+  DevBuf<double> parts[2];
+  DevBuf<IpRec> d_rec;
+  IpRec* h_rec = nullptr;
+  IpMove mv{};
+  BcArgs g{};
+  BcState init{};
+  SellBufs sb[2];
+  SellArgs gs{};
+  DevBuf<float> keep[2];
+  DevBuf<BcsCarry> carry;
+  ~RpIpRun() {
+    if (h_rec) (void)hipHostFree(h_rec);
+  }
+};
+
+// Tables and pinned staging of a driver solving many runs at once.
+struct RpIpBatch {
+  DevBuf<SellArgs> d_sell;
+  DevBuf<BcArgs> d_bc;
+  DevBuf<int2> d_tab, d_htab;
+  SellArgs* h_sell = nullptr;
+  BcArgs* h_bc = nullptr;
+  int2* h_tab = nullptr;
+  int* h_done = nullptr;
+  size_t cap_sell = 0, cap_bc = 0, cap_tab = 0;
+  ~RpIpBatch() {
+    for (void* p : {(void*)h_sell, (void*)h_bc, (void*)h_tab, (void*)h_done})
+      if (p) (void)hipHostFree(p);
+  }
+};
+
+rp_ctx* rp_ipr_ctx(RpIpRun* r) { return r->c; }
+void rp_ipr_free(RpIpRun* r) { delete r; }
+RpIpBatch* rp_ipb_create() { return new RpIpBatch(); }
+void rp_ipb_free(RpIpBatch* b) { delete b; }
+
+int rp_ipr_begin(rp_ctx* c, hipStream_t run_stream, bool host_asm, RpIpRun** out) {
+  std::unique_ptr<RpIpRun> R(new RpIpRun());
+  R->c = c;
+  R->st = run_stream;
+  R->host_asm = host_asm;
   hipStream_t st = c->stream;
   const rp_params& p = c->prm;
   const int n = (int)c->place.size(), np = c->n_pin, nn = c->n_net;
-  using clk = std::chrono::steady_clock;
-  auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
-  RpIpAsm A;
+  R->n = n;
+  R->np = np;
+  R->nn = nn;
+  RpIpAsm& A = R->A;
   if (int r = rp_ip_asm_init(c, A)) return r;
   // movable state and static per-pin data
-  std::vector<int> icx(n), icy(n), dlx(n), dly(n), hx(n), hy(n), mw(n), mh(n);
+  R->icx.resize(n);
+  R->icy.resize(n);
+  R->dlx.resize(n);
+  R->dly.resize(n);
+  R->hx.resize(n);
+  R->hy.resize(n);
+  std::vector<int> mw(n), mh(n);
   for (int j = 0; j < n; j++) {  // row j = movable instance order[j]
     const int k = j, i = c->place[rp_movable_order(c)[j]];
-    icx[k] = (c->ilx[i] + c->iux[i]) / 2;
-    icy[k] = (c->ily[i] + c->iuy[i]) / 2;
-    hx[k] = (c->iux[i] - c->ilx[i]) / 2;
-    hy[k] = (c->iuy[i] - c->ily[i]) / 2;
-    dlx[k] = c->dblx[i];
-    dly[k] = c->dbly[i];
+    R->icx[k] = (c->ilx[i] + c->iux[i]) / 2;
+    R->icy[k] = (c->ily[i] + c->iuy[i]) / 2;
+    R->hx[k] = (c->iux[i] - c->ilx[i]) / 2;
+    R->hy[k] = (c->iuy[i] - c->ily[i]) / 2;
+    R->dlx[k] = c->dblx[i];
+    R->dly[k] = c->dbly[i];
     mw[k] = c->mw[i];
     mh[k] = c->mh[i];
   }
@@ -1248,228 +1352,370 @@ int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve) 
       sbox[q] = make_int4(INT_MAX, INT_MAX, INT_MIN, INT_MIN);
     }
   }
-  DevBuf<int> d_dlx, d_dly, d_hx, d_hy, d_mw, d_mh, d_offx, d_offy;
-  DevBuf<int4> d_sbox;
-  DevBuf<long long> d_hparts;
-  HIPCHK(c, dupload(d_dlx, dlx, st));
-  HIPCHK(c, dupload(d_dly, dly, st));
-  HIPCHK(c, dupload(d_hx, hx, st));
-  HIPCHK(c, dupload(d_hy, hy, st));
-  HIPCHK(c, dupload(d_mw, mw, st));
-  HIPCHK(c, dupload(d_mh, mh, st));
-  HIPCHK(c, dupload(d_offx, c->pin_offx, st));
-  HIPCHK(c, dupload(d_offy, c->pin_offy, st));
-  HIPCHK(c, dupload(d_sbox, sbox, st));
-  const int hblocks = std::max(1, (nn + RP_BLOCK - 1) / RP_BLOCK);
-  HIPCHK(c, dalloc(d_hparts, hblocks));
-  HIPCHK(c, hipMemcpyAsync(A.icx.p, icx.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-  HIPCHK(c, hipMemcpyAsync(A.icy.p, icy.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(c, dupload(R->d_dlx, R->dlx, st));
+  HIPCHK(c, dupload(R->d_dly, R->dly, st));
+  HIPCHK(c, dupload(R->d_hx, R->hx, st));
+  HIPCHK(c, dupload(R->d_hy, R->hy, st));
+  HIPCHK(c, dupload(R->d_mw, mw, st));
+  HIPCHK(c, dupload(R->d_mh, mh, st));
+  HIPCHK(c, dupload(R->d_offx, c->pin_offx, st));
+  HIPCHK(c, dupload(R->d_offy, c->pin_offy, st));
+  HIPCHK(c, dupload(R->d_sbox, sbox, st));
+  R->hblocks = std::max(1, (nn + RP_BLOCK - 1) / RP_BLOCK);
+  HIPCHK(c, dalloc(R->d_hparts, R->hblocks));
+  HIPCHK(c, hipMemcpyAsync(A.icx.p, R->icx.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(A.icy.p, R->icy.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(A.pin_cx.p, c->pin_cx.data(), sizeof(int) * np, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(A.pin_cy.p, c->pin_cy.data(), sizeof(int) * np, hipMemcpyHostToDevice, st));
   // solver buffers (both axes)
-  const int nparts = (int)std::max<long>(1, std::min<long>(BC_MAX_PARTS, ((long)n * BC_G + RP_BLOCK - 1) / RP_BLOCK));
-  DevBuf<float> vec[2][7];
-  DevBuf<double> parts[2];
-  DevBuf<IpRec> d_rec;
+  R->nparts = (int)std::max<long>(1, std::min<long>(BC_MAX_PARTS, ((long)n * BC_G + RP_BLOCK - 1) / RP_BLOCK));
   for (int ax = 0; ax < 2; ax++) {
-    for (auto& v : vec[ax]) HIPCHK(c, dalloc(v, n));
-    HIPCHK(c, dalloc(parts[ax], (size_t)BC_SLOTS * nparts));
+    for (auto& v : R->vec[ax]) HIPCHK(c, dalloc(v, n));
+    HIPCHK(c, dalloc(R->parts[ax], (size_t)BC_SLOTS * R->nparts));
   }
-  HIPCHK(c, dalloc(d_rec, 1));
-  IpRec* h_rec = nullptr;
-  int* h_done = nullptr;
-  HIPCHK(c, hipHostMalloc((void**)&h_rec, sizeof(IpRec), hipHostMallocDefault));
-  struct Pinned {
-    void* p;
-    ~Pinned() {
-      if (p) (void)hipHostFree(p);
-    }
-  } pin_rec{h_rec};
-  HIPCHK(c, hipHostMalloc((void**)&h_done, 2 * sizeof(int), hipHostMallocDefault));
-  Pinned pin_done{h_done};
-  IpMove mv{n, nn, np, A.icx.p, A.icy.p, d_dlx.p, d_dly.p, d_hx.p, d_hy.p, d_mw.p, d_mh.p, A.net_start.p,
-            A.pin_k.p, d_offx.p, d_offy.p, A.pin_cx.p, A.pin_cy.p, d_sbox.p, d_hparts.p, vec[0][0].p, vec[1][0].p};
-  BcArgs g{};
+  HIPCHK(c, dalloc(R->d_rec, 1));
+  HIPCHK(c, hipHostMalloc((void**)&R->h_rec, sizeof(IpRec), hipHostMallocDefault));
+  R->mv = IpMove{n, nn, np, A.icx.p, A.icy.p, R->d_dlx.p, R->d_dly.p, R->d_hx.p, R->d_hy.p, R->d_mw.p, R->d_mh.p,
+                 A.net_start.p, A.pin_k.p, R->d_offx.p, R->d_offy.p, A.pin_cx.p, A.pin_cy.p, R->d_sbox.p,
+                 R->d_hparts.p, R->vec[0][0].p, R->vec[1][0].p};
+  BcArgs& g = R->g;
   g.n = n;
-  g.nparts = nparts;
+  g.nparts = R->nparts;
+  for (int ax = 0; ax < 2; ax++) {
+    BcAxis& a = g.ax[ax];
+    a.x = R->vec[ax][0].p;
+    a.r = R->vec[ax][1].p;
+    a.r0 = R->vec[ax][2].p;
+    a.v = R->vec[ax][3].p;
+    a.p = R->vec[ax][4].p;
+    a.s = R->vec[ax][5].p;
+    a.t = R->vec[ax][6].p;
+    a.parts = R->parts[ax].p;
+    a.st = &R->d_rec.p->st[ax];
+  }
   const float tol = 1.1920928955078125e-07f;  // NumTraits<float>::epsilon()
-  BcState init{};
-  init.tol2 = (double)tol * (double)tol;
-  init.eps2 = 1.1920928955078125e-07 * 1.1920928955078125e-07;
-  init.max_iter = p.initial_place_max_solver_iter;
-  const dim3 gb(nparts, 2), g1(1, 2), blk(RP_BLOCK);
-  const dim3 gn((n + RP_BLOCK - 1) / RP_BLOCK), gp((np + RP_BLOCK - 1) / RP_BLOCK);
-  const int CH = 8;  // iterations launched between polls of the done flags
-  const int cap = 2 * std::max(0, init.max_iter) + 1;  // i restarts from 0 at most once
+  R->init.tol2 = (double)tol * (double)tol;
+  R->init.eps2 = 1.1920928955078125e-07 * 1.1920928955078125e-07;
+  R->init.max_iter = p.initial_place_max_solver_iter;
   // one workgroup per axis for small systems; RP_IP_SOLVER=blocks keeps the
   // grid-wide kernels at every size
   const char* solver = getenv("RP_IP_SOLVER");
-  const int small_r = (solver && strcmp(solver, "blocks") == 0) ? 0 : bcs_rows_per_thread(n);
-  const int sell_rows = BCS_THREADS * small_r, sell_slices = sell_rows / SELL_C;
-  SellBufs sb[2];
-  SellArgs gs{};
-  gs.n = n;
-  DevBuf<float> keep[2];
-  DevBuf<BcsCarry> carry;
-  if (small_r) {
-    HIPCHK(c, dalloc(carry, 2));
+  R->small_r = (solver && strcmp(solver, "blocks") == 0) ? 0 : bcs_rows_per_thread(n);
+  R->sell_rows = BCS_THREADS * R->small_r;
+  R->sell_slices = R->sell_rows / SELL_C;
+  if (R->small_r) {
+    HIPCHK(c, dalloc(R->carry, 2));
     for (int ax = 0; ax < 2; ax++) {
-      HIPCHK(c, dalloc(keep[ax], 5 * (size_t)n));
-      HIPCHK(c, dalloc(sb[ax].key, n));
-      HIPCHK(c, dalloc(sb[ax].key2, n));
-      HIPCHK(c, dalloc(sb[ax].id, n));
-      HIPCHK(c, dalloc(sb[ax].perm, n));
-      HIPCHK(c, dalloc(sb[ax].iperm, n));
-      HIPCHK(c, dalloc(sb[ax].slen, n));
-      HIPCHK(c, dalloc(sb[ax].wslots, sell_slices));
-      HIPCHK(c, dalloc(sb[ax].soff, sell_slices));
+      SellBufs& B = R->sb[ax];
+      HIPCHK(c, dalloc(R->keep[ax], 5 * (size_t)n));
+      HIPCHK(c, dalloc(B.key, n));
+      HIPCHK(c, dalloc(B.key2, n));
+      HIPCHK(c, dalloc(B.id, n));
+      HIPCHK(c, dalloc(B.perm, n));
+      HIPCHK(c, dalloc(B.iperm, n));
+      HIPCHK(c, dalloc(B.slen, n));
+      HIPCHK(c, dalloc(B.wslots, R->sell_slices));
+      HIPCHK(c, dalloc(B.soff, R->sell_slices));
       size_t t1 = 0, t2 = 0;
-      HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, t1, sb[ax].key.p, sb[ax].key2.p, sb[ax].id.p,
-                                                   sb[ax].perm.p, n, 0, 16, st));
-      HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, t2, sb[ax].wslots.p, sb[ax].soff.p, sell_slices, st));
-      HIPCHK(c, dalloc(sb[ax].temp, std::max(t1, t2)));
+      HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, t1, B.key.p, B.key2.p, B.id.p, B.perm.p, n, 0, 16,
+                                                   st));
+      HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, t2, B.wslots.p, B.soff.p, R->sell_slices, st));
+      HIPCHK(c, dalloc(B.temp, std::max(t1, t2)));
     }
   }
-  for (int it = 1; it <= p.initial_place_max_iter; it++) {
-    auto ta = clk::now();
-    if (host_asm) {
-      if (int r = rp_ip_host_assemble(c, A)) return r;
-    } else {
-      if (int r = rp_ip_assemble(c, A, nullptr, nullptr)) return r;
+  HIPCHK(c, hipStreamSynchronize(st));
+  *out = R.release();
+  return 0;
+}
+
+int rp_ipr_assemble_a(RpIpRun* R) {
+  if (R->host_asm) return 0;
+  return rp_ip_assemble_a(R->c, R->A, R->st);
+}
+
+int rp_ipr_assemble_b(RpIpRun* R) {
+  rp_ctx* c = R->c;
+  hipStream_t st = R->st;
+  RpIpAsm& A = R->A;
+  const int n = R->n;
+  if (R->host_asm) {
+    if (int r = rp_ip_host_assemble(c, A)) return r;
+  } else {
+    if (int r = rp_ip_assemble_b(c, A, st)) return r;
+  }
+  BcArgs& g = R->g;
+  for (int ax = 0; ax < 2; ax++) {
+    BcAxis& a = g.ax[ax];
+    a.rowp = A.rowp[ax].p;
+    a.col = A.col[ax].p;
+    a.val = A.val[ax].p;
+    a.b = A.b[ax].p;
+  }
+  IpRec r0{};
+  r0.st[0] = r0.st[1] = R->init;
+  *R->h_rec = r0;
+  HIPCHK(c, hipMemcpyAsync(R->d_rec.p, R->h_rec, sizeof(IpRec), hipMemcpyHostToDevice, st));
+  const dim3 gn((n + RP_BLOCK - 1) / RP_BLOCK), blk(RP_BLOCK);
+  hipLaunchKernelGGL(k_ip_warm, gn, blk, 0, st, R->mv);
+  if (!R->small_r) return 0;
+  // sorted SELL copy of each axis' matrix
+  for (int ax = 0; ax < 2; ax++) {
+    SellBufs& B = R->sb[ax];
+    // nnz + one slice of the longest row + the rounding of every slice
+    const size_t cap = A.col[ax].n + (size_t)SELL_C * n + (size_t)SELL_C * (SELL_PAD - 1) * R->sell_slices;
+    if (B.sval.n < cap) {
+      HIPCHK(c, dalloc(B.sval, cap));
+      HIPCHK(c, dalloc(B.scol, cap));
     }
-    *t_asm += since(ta);
-    auto ts = clk::now();
-    for (int ax = 0; ax < 2; ax++) {
-      BcAxis& a = g.ax[ax];
-      a.rowp = A.rowp[ax].p;
-      a.col = A.col[ax].p;
-      a.val = A.val[ax].p;
-      a.b = A.b[ax].p;
-      a.x = vec[ax][0].p;
-      a.r = vec[ax][1].p;
-      a.r0 = vec[ax][2].p;
-      a.v = vec[ax][3].p;
-      a.p = vec[ax][4].p;
-      a.s = vec[ax][5].p;
-      a.t = vec[ax][6].p;
-      a.parts = parts[ax].p;
-      a.st = &d_rec.p->st[ax];
-    }
-    IpRec r0{};
-    r0.st[0] = r0.st[1] = init;
-    *h_rec = r0;
-    HIPCHK(c, hipMemcpyAsync(d_rec.p, h_rec, sizeof(IpRec), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_ip_warm, gn, blk, 0, st, mv);
-    if (small_r) {
-      // sorted SELL copy of each axis' matrix
-      for (int ax = 0; ax < 2; ax++) {
-        SellBufs& B = sb[ax];
-        // nnz + one slice of the longest row + the rounding of every slice
-        const size_t cap = A.col[ax].n + (size_t)SELL_C * n + (size_t)SELL_C * (SELL_PAD - 1) * sell_slices;
-        if (B.sval.n < cap) {
-          HIPCHK(c, dalloc(B.sval, cap));
-          HIPCHK(c, dalloc(B.scol, cap));
-        }
-        hipLaunchKernelGGL(k_sell_keys, gn, blk, 0, st, (const int*)A.rowp[ax].p, n, B.key.p, B.id.p);
-        size_t tb = B.temp.n;
-        HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(B.temp.p, tb, B.key.p, B.key2.p, B.id.p, B.perm.p, n, 0, 16,
-                                                     st));
-        hipLaunchKernelGGL(k_sell_meta, grid_of(std::max(n, sell_slices)), blk, 0, st, (const int*)A.rowp[ax].p,
-                           (const int*)B.perm.p, n, sell_slices, B.iperm.p, B.slen.p, B.wslots.p);
-        tb = B.temp.n;
-        HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(B.temp.p, tb, B.wslots.p, B.soff.p, sell_slices, st));
-        hipLaunchKernelGGL(k_sell_fill, grid_of(sell_rows), blk, 0, st, (const int*)A.rowp[ax].p,
-                           (const int*)A.col[ax].p, (const float*)A.val[ax].p, (const int*)B.perm.p,
-                           (const int*)B.iperm.p, (const int*)B.soff.p, n, sell_rows, B.sval.p, B.scol.p);
-        HIPCHK(c, hipGetLastError());
-        SellAxis& a = gs.ax[ax];
-        a.rowp = g.ax[ax].rowp;
-        a.col = g.ax[ax].col;
-        a.val = g.ax[ax].val;
-        a.b = g.ax[ax].b;
-        a.x = g.ax[ax].x;
-        a.perm = B.perm.p;
-        a.slen = B.slen.p;
-        a.width = B.wslots.p;
-        a.soff = B.soff.p;
-        a.sval = B.sval.p;
-        a.scol = B.scol.p;
-        a.st = g.ax[ax].st;
-        a.keep = keep[ax].p;
-        a.carry = carry.p + ax;
-      }
-      // the solve in launches of BCS_PASSES iterations; launches after the
-      // solve has finished return at once
-      const size_t lds = 4 * sizeof(float) * (size_t)n;
-      const int guard = 2 * std::max(0, init.max_iter) + 2;
-      for (int first = 0; first < guard; first += BCS_PASSES) {
-        switch (small_r) {
-          case 1: hipLaunchKernelGGL(k_bc_small<1>, dim3(2), dim3(BCS_THREADS), lds, st, gs, first, BCS_PASSES); break;
-          case 2: hipLaunchKernelGGL(k_bc_small<2>, dim3(2), dim3(BCS_THREADS), lds, st, gs, first, BCS_PASSES); break;
-          case 4: hipLaunchKernelGGL(k_bc_small<4>, dim3(2), dim3(BCS_THREADS), lds, st, gs, first, BCS_PASSES); break;
-          default: hipLaunchKernelGGL(k_bc_small<8>, dim3(2), dim3(BCS_THREADS), lds, st, gs, first, BCS_PASSES); break;
-        }
-      }
-      HIPCHK(c, hipGetLastError());
-    } else {
-    hipLaunchKernelGGL(k_bc_init, gb, blk, 0, st, g);
-    int done_iters = 0;
-    for (;;) {
-      for (int j = 0; j < CH; j++) {
-        hipLaunchKernelGGL(k_bc_head, g1, dim3(1024), 0, st, g);
-        hipLaunchKernelGGL(k_bc_restart, gb, blk, 0, st, g);
-        hipLaunchKernelGGL(k_bc_p, gb, blk, 0, st, g);
-        hipLaunchKernelGGL(k_bc_v, gb, blk, 0, st, g);
-        hipLaunchKernelGGL(k_bc_s, gb, blk, 0, st, g);
-        hipLaunchKernelGGL(k_bc_t, gb, blk, 0, st, g);
-        hipLaunchKernelGGL(k_bc_xr, gb, blk, 0, st, g);
-      }
-      done_iters += CH;
-      hipLaunchKernelGGL(k_bc_head, g1, dim3(1024), 0, st, g);  // closes the chunk's last iteration
-      HIPCHK(c, hipGetLastError());
-      HIPCHK(c, hipMemcpyAsync(&h_done[0], &d_rec.p->st[0].done, sizeof(int), hipMemcpyDeviceToHost, st));
-      HIPCHK(c, hipMemcpyAsync(&h_done[1], &d_rec.p->st[1].done, sizeof(int), hipMemcpyDeviceToHost, st));
-      HIPCHK(c, hipStreamSynchronize(st));
-      if ((h_done[0] && h_done[1]) || done_iters >= cap) break;
-    }
-    hipLaunchKernelGGL(k_bc_zero_x, gb, blk, 0, st, g);
-    }
-    // log record: HPWL before updateCoordi (src/initialPlace.cpp:88-91)
-    hipLaunchKernelGGL(k_ip_hpwl, dim3(hblocks), blk, 0, st, mv);
-    hipLaunchKernelGGL(k_ip_hpwl_final, dim3(1), dim3(1024), 0, st, (const long long*)d_hparts.p, hblocks,
-                       &d_rec.p->hpwl);
-    hipLaunchKernelGGL(k_ip_update, gn, blk, 0, st, mv);
-    hipLaunchKernelGGL(k_ip_pins, gp, blk, 0, st, mv);
+    hipLaunchKernelGGL(k_sell_keys, gn, blk, 0, st, (const int*)A.rowp[ax].p, n, B.key.p, B.id.p);
+    size_t tb = B.temp.n;
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(B.temp.p, tb, B.key.p, B.key2.p, B.id.p, B.perm.p, n, 0, 16, st));
+    hipLaunchKernelGGL(k_sell_meta, grid_of(std::max(n, R->sell_slices)), blk, 0, st, (const int*)A.rowp[ax].p,
+                       (const int*)B.perm.p, n, R->sell_slices, B.iperm.p, B.slen.p, B.wslots.p);
+    tb = B.temp.n;
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(B.temp.p, tb, B.wslots.p, B.soff.p, R->sell_slices, st));
+    hipLaunchKernelGGL(k_sell_fill, grid_of(R->sell_rows), blk, 0, st, (const int*)A.rowp[ax].p,
+                       (const int*)A.col[ax].p, (const float*)A.val[ax].p, (const int*)B.perm.p,
+                       (const int*)B.iperm.p, (const int*)B.soff.p, n, R->sell_rows, B.sval.p, B.scol.p);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(h_rec, d_rec.p, sizeof(IpRec), hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipStreamSynchronize(st));
-    *t_solve += since(ts);
-    if (!h_rec->st[0].done || !h_rec->st[1].done)
-      return fail(c, RP_ERR_STATE, "initial place: BiCGSTAB did not terminate");
-    float err = std::max(h_rec->st[0].err, h_rec->st[1].err);
-    if (small_r && getenv("RP_IP_TIMING"))
-      fprintf(stderr, "[rp] initial place %d: one-workgroup solves of %d rows, %d / %d iterations\n", it, n,
-              h_rec->st[0].pending, h_rec->st[1].pending);
-    c->ip_err.push_back(err);
-    c->ip_hpwl.push_back(h_rec->hpwl);
-    if (err <= 1e-5 && it >= 5) break;
+    SellAxis& a = R->gs.ax[ax];
+    a.rowp = g.ax[ax].rowp;
+    a.col = g.ax[ax].col;
+    a.val = g.ax[ax].val;
+    a.b = g.ax[ax].b;
+    a.x = g.ax[ax].x;
+    a.perm = B.perm.p;
+    a.slen = B.slen.p;
+    a.width = B.wslots.p;
+    a.soff = B.soff.p;
+    a.sval = B.sval.p;
+    a.scol = B.scol.p;
+    a.st = g.ax[ax].st;
+    a.keep = R->keep[ax].p;
+    a.carry = R->carry.p + ax;
   }
+  R->gs.n = n;
+  return 0;
+}
+
+namespace {
+
+template <typename T>
+int grow_pinned(rp_ctx* c, T*& h, DevBuf<T>& d, size_t& cap, size_t need) {
+  if (cap >= need && h) return 0;
+  size_t n = std::max<size_t>(need, 2 * cap);
+  if (h) (void)hipHostFree(h);
+  h = nullptr;
+  HIPCHK(c, hipHostMalloc((void**)&h, n * sizeof(T), hipHostMallocDefault));
+  HIPCHK(c, dalloc(d, n));
+  cap = n;
+  return 0;
+}
+
+int solve_guard(const RpIpRun* R) { return 2 * std::max(0, R->init.max_iter) + 2; }
+
+void launch_small(int r, dim3 grid, size_t lds, hipStream_t st, const SellArgs* gs, int first) {
+  switch (r) {
+    case 1: hipLaunchKernelGGL(kb_bc_small<1>, grid, dim3(BCS_THREADS), lds, st, gs, first, BCS_PASSES); break;
+    case 2: hipLaunchKernelGGL(kb_bc_small<2>, grid, dim3(BCS_THREADS), lds, st, gs, first, BCS_PASSES); break;
+    case 4: hipLaunchKernelGGL(kb_bc_small<4>, grid, dim3(BCS_THREADS), lds, st, gs, first, BCS_PASSES); break;
+    default: hipLaunchKernelGGL(kb_bc_small<8>, grid, dim3(BCS_THREADS), lds, st, gs, first, BCS_PASSES); break;
+  }
+}
+
+void launch_small1(int r, size_t lds, hipStream_t st, const SellArgs& g, int first) {
+  switch (r) {
+    case 1: hipLaunchKernelGGL(k_bc_small<1>, dim3(2), dim3(BCS_THREADS), lds, st, g, first, BCS_PASSES); break;
+    case 2: hipLaunchKernelGGL(k_bc_small<2>, dim3(2), dim3(BCS_THREADS), lds, st, g, first, BCS_PASSES); break;
+    case 4: hipLaunchKernelGGL(k_bc_small<4>, dim3(2), dim3(BCS_THREADS), lds, st, g, first, BCS_PASSES); break;
+    default: hipLaunchKernelGGL(k_bc_small<8>, dim3(2), dim3(BCS_THREADS), lds, st, g, first, BCS_PASSES); break;
+  }
+}
+
+}  // namespace
+
+// The solves of k runs.  One-workgroup runs of one row class share a launch
+// (2 workgroups per run); grid runs share each of the seven launches through
+// a (run, block) table, each run keeping its own block count, so every run
+// computes exactly what it computes alone.
+int rp_ipr_solve(RpIpRun* const* runs, int k, hipStream_t st, RpIpBatch* bt) {
+  if (k <= 0) return 0;
+  rp_ctx* c0 = runs[0]->c;
+  const dim3 blk(RP_BLOCK);
+  if (k == 1 || !bt) {  // one run: launches of its own
+    for (int j = 0; j < k; j++) {
+      RpIpRun* R = runs[j];
+      rp_ctx* c = R->c;
+      if (R->small_r) {
+        const size_t lds = 4 * sizeof(float) * (size_t)R->n;
+        for (int first = 0; first < solve_guard(R); first += BCS_PASSES) launch_small1(R->small_r, lds, st, R->gs, first);
+        HIPCHK(c, hipGetLastError());
+        continue;
+      }
+      const BcArgs& g = R->g;
+      const dim3 gb(R->nparts, 2), g1(1, 2);
+      const int cap = 2 * std::max(0, R->init.max_iter) + 1;  // i restarts from 0 at most once
+      hipLaunchKernelGGL(k_bc_init, gb, blk, 0, st, g);
+      int done_iters = 0;
+      int* h_done = &R->h_rec->st[0].done;  // scratch: the record is rewritten by finish_a
+      for (;;) {
+        for (int i = 0; i < BC_CHUNK; i++) {
+          hipLaunchKernelGGL(k_bc_head, g1, dim3(1024), 0, st, g);
+          hipLaunchKernelGGL(k_bc_restart, gb, blk, 0, st, g);
+          hipLaunchKernelGGL(k_bc_p, gb, blk, 0, st, g);
+          hipLaunchKernelGGL(k_bc_v, gb, blk, 0, st, g);
+          hipLaunchKernelGGL(k_bc_s, gb, blk, 0, st, g);
+          hipLaunchKernelGGL(k_bc_t, gb, blk, 0, st, g);
+          hipLaunchKernelGGL(k_bc_xr, gb, blk, 0, st, g);
+        }
+        done_iters += BC_CHUNK;
+        hipLaunchKernelGGL(k_bc_head, g1, dim3(1024), 0, st, g);  // closes the chunk's last iteration
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipMemcpyAsync(&h_done[0], &R->d_rec.p->st[0].done, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipMemcpyAsync(&R->h_rec->st[1].done, &R->d_rec.p->st[1].done, sizeof(int),
+                                 hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        if ((R->h_rec->st[0].done && R->h_rec->st[1].done) || done_iters >= cap) break;
+      }
+      hipLaunchKernelGGL(k_bc_zero_x, gb, blk, 0, st, g);
+      HIPCHK(c, hipGetLastError());
+    }
+    return 0;
+  }
+  // one-workgroup runs, one launch set per row class
+  std::vector<RpIpRun*> small[BCS_MAX_R + 1], grid;
+  for (int j = 0; j < k; j++) {
+    if (runs[j]->small_r)
+      small[runs[j]->small_r].push_back(runs[j]);
+    else
+      grid.push_back(runs[j]);
+  }
+  size_t nsmall = 0;
+  for (auto& v : small) nsmall += v.size();
+  if (nsmall) {
+    if (int r = grow_pinned(c0, bt->h_sell, bt->d_sell, bt->cap_sell, nsmall)) return r;
+    size_t at = 0;
+    for (int rc = 1; rc <= BCS_MAX_R; rc++) {
+      if (small[rc].empty()) continue;
+      int nmax = 0, guard = 0;
+      for (size_t j = 0; j < small[rc].size(); j++) {
+        bt->h_sell[at + j] = small[rc][j]->gs;
+        nmax = std::max(nmax, small[rc][j]->n);
+        guard = std::max(guard, solve_guard(small[rc][j]));
+      }
+      HIPCHK(c0, hipMemcpyAsync(bt->d_sell.p + at, bt->h_sell + at, sizeof(SellArgs) * small[rc].size(),
+                                hipMemcpyHostToDevice, st));
+      const size_t lds = 4 * sizeof(float) * (size_t)nmax;
+      for (int first = 0; first < guard; first += BCS_PASSES)
+        launch_small(rc, dim3(2 * (unsigned)small[rc].size()), lds, st, bt->d_sell.p + at, first);
+      HIPCHK(c0, hipGetLastError());
+      at += small[rc].size();
+    }
+  }
+  if (grid.empty()) return 0;
+  // grid runs: a (run, block) table for the row kernels, (run, 0) for the head
+  const int ng = (int)grid.size();
+  size_t ntab = 0;
+  int cap = 0;
+  for (RpIpRun* R : grid) {
+    ntab += R->nparts;
+    cap = std::max(cap, 2 * std::max(0, R->init.max_iter) + 1);
+  }
+  if (int r = grow_pinned(c0, bt->h_bc, bt->d_bc, bt->cap_bc, ng)) return r;
+  if (int r = grow_pinned(c0, bt->h_tab, bt->d_tab, bt->cap_tab, ntab + ng)) return r;
+  if (!bt->h_done) HIPCHK(c0, hipHostMalloc((void**)&bt->h_done, sizeof(int) * 2 * 4096, hipHostMallocDefault));
+  if (ng > 4096) return fail(c0, RP_ERR_ARG, "initial place: more than 4096 grid solves in one batch");
+  size_t t = 0;
+  for (int j = 0; j < ng; j++) {
+    bt->h_bc[j] = grid[j]->g;
+    for (int b = 0; b < grid[j]->nparts; b++) bt->h_tab[t++] = make_int2(j, b);
+  }
+  for (int j = 0; j < ng; j++) bt->h_tab[ntab + j] = make_int2(j, 0);
+  HIPCHK(c0, hipMemcpyAsync(bt->d_bc.p, bt->h_bc, sizeof(BcArgs) * ng, hipMemcpyHostToDevice, st));
+  HIPCHK(c0, hipMemcpyAsync(bt->d_tab.p, bt->h_tab, sizeof(int2) * (ntab + ng), hipMemcpyHostToDevice, st));
+  const BcArgs* gsd = bt->d_bc.p;
+  const int2* tab = bt->d_tab.p;
+  const int2* htab = bt->d_tab.p + ntab;
+  const dim3 gb((unsigned)ntab, 2), g1((unsigned)ng, 2);
+  hipLaunchKernelGGL(kb_bc_init, gb, blk, 0, st, gsd, tab);
+  int done_iters = 0;
+  for (;;) {
+    for (int i = 0; i < BC_CHUNK; i++) {
+      hipLaunchKernelGGL(kb_bc_head, g1, dim3(1024), 0, st, gsd, htab);
+      hipLaunchKernelGGL(kb_bc_restart, gb, blk, 0, st, gsd, tab);
+      hipLaunchKernelGGL(kb_bc_p, gb, blk, 0, st, gsd, tab);
+      hipLaunchKernelGGL(kb_bc_v, gb, blk, 0, st, gsd, tab);
+      hipLaunchKernelGGL(kb_bc_s, gb, blk, 0, st, gsd, tab);
+      hipLaunchKernelGGL(kb_bc_t, gb, blk, 0, st, gsd, tab);
+      hipLaunchKernelGGL(kb_bc_xr, gb, blk, 0, st, gsd, tab);
+    }
+    done_iters += BC_CHUNK;
+    hipLaunchKernelGGL(kb_bc_head, g1, dim3(1024), 0, st, gsd, htab);
+    HIPCHK(c0, hipGetLastError());
+    for (int j = 0; j < ng; j++)
+      for (int ax = 0; ax < 2; ax++)
+        HIPCHK(c0, hipMemcpyAsync(&bt->h_done[2 * j + ax], &grid[j]->d_rec.p->st[ax].done, sizeof(int),
+                                  hipMemcpyDeviceToHost, st));
+    HIPCHK(c0, hipStreamSynchronize(st));
+    bool all = true;
+    for (int j = 0; j < 2 * ng; j++) all = all && bt->h_done[j];
+    if (all || done_iters >= cap) break;
+  }
+  hipLaunchKernelGGL(kb_bc_zero_x, gb, blk, 0, st, gsd, tab);
+  HIPCHK(c0, hipGetLastError());
+  return 0;
+}
+
+int rp_ipr_finish_a(RpIpRun* R) {
+  rp_ctx* c = R->c;
+  hipStream_t st = R->st;
+  const dim3 blk(RP_BLOCK), gn((R->n + RP_BLOCK - 1) / RP_BLOCK), gp((R->np + RP_BLOCK - 1) / RP_BLOCK);
+  // log record: HPWL before updateCoordi (src/initialPlace.cpp:88-91)
+  hipLaunchKernelGGL(k_ip_hpwl, dim3(R->hblocks), blk, 0, st, R->mv);
+  hipLaunchKernelGGL(k_ip_hpwl_final, dim3(1), dim3(1024), 0, st, (const long long*)R->d_hparts.p, R->hblocks,
+                     &R->d_rec.p->hpwl);
+  hipLaunchKernelGGL(k_ip_update, gn, blk, 0, st, R->mv);
+  hipLaunchKernelGGL(k_ip_pins, gp, blk, 0, st, R->mv);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(R->h_rec, R->d_rec.p, sizeof(IpRec), hipMemcpyDeviceToHost, st));
+  return 0;
+}
+
+int rp_ipr_finish_b(RpIpRun* R, bool* stop) {
+  rp_ctx* c = R->c;
+  const IpRec& rec = *R->h_rec;
+  R->it++;
+  if (!rec.st[0].done || !rec.st[1].done) return fail(c, RP_ERR_STATE, "initial place: BiCGSTAB did not terminate");
+  const float err = std::max(rec.st[0].err, rec.st[1].err);
+  if (R->small_r && getenv("RP_IP_TIMING"))
+    fprintf(stderr, "[rp] initial place %d: one-workgroup solves of %d rows, %d / %d iterations\n", R->it, R->n,
+            rec.st[0].pending, rec.st[1].pending);
+  c->ip_err.push_back(err);
+  c->ip_hpwl.push_back(rec.hpwl);
+  *stop = (err <= 1e-5 && R->it >= 5) || R->it >= c->prm.initial_place_max_iter;
+  return 0;
+}
+
+int rp_ipr_end(RpIpRun* R) {
+  rp_ctx* c = R->c;
+  hipStream_t st = R->st;
+  const int n = R->n, np = R->np;
   // final positions back to the host model
-  HIPCHK(c, hipMemcpyAsync(icx.data(), A.icx.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
-  HIPCHK(c, hipMemcpyAsync(icy.data(), A.icy.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
-  HIPCHK(c, hipMemcpyAsync(dlx.data(), d_dlx.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
-  HIPCHK(c, hipMemcpyAsync(dly.data(), d_dly.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipMemcpyAsync(R->icx.data(), R->A.icx.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipMemcpyAsync(R->icy.data(), R->A.icy.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipMemcpyAsync(R->dlx.data(), R->d_dlx.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipMemcpyAsync(R->dly.data(), R->d_dly.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
   HIPCHK(c, hipStreamSynchronize(st));
   const std::vector<int>& order = rp_movable_order(c);
   for (int k = 0; k < n; k++) {
     int i = c->place[order[k]];
-    int h0 = hx[k], h1 = hy[k];
-    c->ilx[i] = icx[k] - h0;
-    c->iux[i] = icx[k] + h0;
-    c->ily[i] = icy[k] - h1;
-    c->iuy[i] = icy[k] + h1;
-    c->dblx[i] = dlx[k];
-    c->dbly[i] = dly[k];
+    int h0 = R->hx[k], h1 = R->hy[k];
+    c->ilx[i] = R->icx[k] - h0;
+    c->iux[i] = R->icx[k] + h0;
+    c->ily[i] = R->icy[k] - h1;
+    c->iuy[i] = R->icy[k] + h1;
+    c->dblx[i] = R->dlx[k];
+    c->dbly[i] = R->dly[k];
   }
   for (int q = 0; q < np; q++) {
     int i = c->pin_inst[q];
@@ -1478,4 +1724,33 @@ int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve) 
     c->pin_cy[q] = (c->ily[i] + c->iuy[i]) / 2 + c->pin_offy[q];
   }
   return 0;
+}
+
+// The InitialPlace loop (src/initialPlace.cpp:56-105) of one context with
+// every iteration on the device; the host reads one record (errors, HPWL)
+// per iteration.
+int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve) {
+  using clk = std::chrono::steady_clock;
+  auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
+  hipStream_t st = c->stream;
+  RpIpRun* raw = nullptr;
+  if (int r = rp_ipr_begin(c, st, host_asm, &raw)) return r;
+  std::unique_ptr<RpIpRun> R(raw);
+  for (;;) {
+    auto ta = clk::now();
+    if (int r = rp_ipr_assemble_a(R.get())) return r;
+    if (!host_asm) HIPCHK(c, hipStreamSynchronize(st));
+    if (int r = rp_ipr_assemble_b(R.get())) return r;
+    *t_asm += since(ta);
+    auto ts = clk::now();
+    RpIpRun* one = R.get();
+    if (int r = rp_ipr_solve(&one, 1, st, nullptr)) return r;
+    if (int r = rp_ipr_finish_a(R.get())) return r;
+    HIPCHK(c, hipStreamSynchronize(st));
+    bool stop = false;
+    if (int r = rp_ipr_finish_b(R.get(), &stop)) return r;
+    *t_solve += since(ts);
+    if (stop) break;
+  }
+  return rp_ipr_end(R.get());
 }
