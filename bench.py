@@ -118,6 +118,7 @@ print(json.dumps(dict(init_s=t1 - t0, iter_s=t2 - t1, evals=st.evals, poisson=o.
 
 _PHASES = {"create": 0.0, "initial_place": 0.0, "setup": 0.0, "nesterov": 0.0}
 SETUP_THREADS = [16]  # host workers of the placement service (--setup-threads)
+BATCHED_IP = [0]  # the service's batched initial place (--batched-ip)
 TILES = [None]  # tile shapes the service's tuner chose
 
 
@@ -147,7 +148,7 @@ def _served_common(jobs, arrivals, steps_per_poll):
     service times)."""
     from replace_amd.capi import Server
     _plain_job(*jobs[0])  # warm-up: module load, first allocations
-    srv = Server(steps_per_poll=steps_per_poll, setup_threads=SETUP_THREADS[0])
+    srv = Server(steps_per_poll=steps_per_poll, setup_threads=SETUP_THREADS[0], batched_ip=BATCHED_IP[0])
     ids = [srv.submit(nl, p, a) for (nl, p), a in zip(jobs, arrivals)]
     t = time.perf_counter()
     srv.run()
@@ -210,6 +211,7 @@ def bench_batch(args):
         "speedup_vs_plain": round(plain_s / wall, 2),
         "batched_steps": int(stats.steps), "table_rebuilds": int(stats.rebuilds),
         "netgen_s": round(gen_s, 2), "timeline": _timeline(res), "setup_threads": args.setup_threads,
+        "batched_ip": args.batched_ip,
         "tiles": TILES[0],
     }
     out.update(_parity(res, plain, jobs))
@@ -253,7 +255,7 @@ def bench_serve(args):
                                "service_s_total": round(sum(svc), 3)},
         "speedup_vs_plain": round(done / makespan, 2),
         "batched_steps": int(stats.steps), "table_rebuilds": int(stats.rebuilds),
-        "netgen_s": round(gen_s, 2), "timeline": _timeline(res),
+        "netgen_s": round(gen_s, 2), "timeline": _timeline(res), "batched_ip": args.batched_ip,
     }
     out.update(_parity(res, plain, jobs))
     print(json.dumps(out), flush=True)
@@ -326,8 +328,11 @@ def main():
                     help="c4: arrival rate, jobs/s (default: ~80%% of the measured 34/s batched capacity)")
     ap.add_argument("--steps-per-poll", type=int, default=8, help="c3/c4: batched steps per graph replay")
     ap.add_argument("--setup-threads", type=int, default=16, help="c3/c4: host set-up workers of the service")
+    ap.add_argument("--batched-ip", type=int, default=0, choices=[0, 1],
+                    help="c3/c4: 1 runs the initial place of all jobs in set-up in shared launches")
     args = ap.parse_args()
     SETUP_THREADS[0] = args.setup_threads
+    BATCHED_IP[0] = args.batched_ip
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build places one instance per GPU and "
               "has no multi-GPU path (see DESIGN.md for how it would shard)", file=sys.stderr)

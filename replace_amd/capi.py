@@ -232,12 +232,12 @@ class Server:
     """Placement service: many jobs in continuous batches on one GPU."""
 
     def __init__(self, hbm_budget_bytes=None, max_active=None, setup_threads=None,
-                 steps_per_poll=None, use_graph=None, autotune=None):
+                 steps_per_poll=None, use_graph=None, autotune=None, batched_ip=None):
         o = RpServerOpts()
         lib().rp_server_opts_default(ctypes.byref(o))
         for k, v in dict(hbm_budget_bytes=hbm_budget_bytes, max_active=max_active,
                          setup_threads=setup_threads, steps_per_poll=steps_per_poll,
-                         use_graph=use_graph, autotune=autotune).items():
+                         use_graph=use_graph, autotune=autotune, batched_ip=batched_ip).items():
             if v is not None:
                 setattr(o, k, int(v))
         self.opts = o

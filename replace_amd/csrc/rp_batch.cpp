@@ -52,6 +52,8 @@ struct Job {
   rp_netlist nl{};
   rp_params prm{};
   rp_ctx* ctx = nullptr;
+  RpIpRun* ip_run = nullptr;  // initial place handed to the batched driver
+  int ip_code = 0;            // its result, when it comes back
   enum { QUEUED, SETUP, READY, RUNNING, DONE, FAILED } status = QUEUED;
   int code = 0;
   std::string err;
@@ -149,8 +151,16 @@ struct rp_server {
   std::deque<int> setup_queue;
   bool stop = false;
   std::vector<std::thread> workers;
+  // batched initial place: set-up workers hand their runs to one driver
+  std::mutex ip_mu;
+  std::condition_variable ip_cv;
+  std::deque<Job*> ip_queue;
+  std::thread ip_thread;
+  hipStream_t ip_stream = nullptr;
+  RpIpBatch* ipb = nullptr;
   rp_server_stats stats{};
 };
+
 
 static int sfail(rp_server* s, int code, const char* msg) {
   s->err = msg;
@@ -173,6 +183,7 @@ extern "C" void rp_server_opts_default(rp_server_opts* o) {
   o->steps_per_poll = 8;
   o->use_graph = 1;
   o->autotune = 1;
+  o->batched_ip = 0;
 }
 
 static void tiles_default(rp_server* s) {
@@ -270,9 +281,30 @@ static void setup_worker(rp_server* s) {
       s->setup_queue.pop_front();
     }
     Job& j = *s->jobs[id];
+    int r = 0;
     rp_ctx* c = nullptr;
-    int r = rp_create(&j.nl, &j.prm, &c);
-    if (r == 0) r = rp_initial_place(c, nullptr);
+    if (j.ip_run) {  // back from the batched initial place
+      c = rp_ipr_ctx(j.ip_run);
+      r = j.ip_code;
+      rp_ipr_free(j.ip_run);
+      j.ip_run = nullptr;
+    } else {
+      r = rp_create(&j.nl, &j.prm, &c);
+      if (r == 0 && s->ip_stream) {
+        RpIpRun* run = nullptr;
+        r = rp_initial_place_begin(c, s->ip_stream, &run);
+        if (r == 0 && run) {  // the driver iterates it and queues the job here again
+          j.ip_run = run;
+          std::lock_guard<std::mutex> lk(s->ip_mu);
+          s->ip_queue.push_back(&j);
+          s->ip_cv.notify_one();
+          continue;
+        }
+        if (r == 0) r = rp_initial_place_end(c, nullptr);
+      } else if (r == 0) {
+        r = rp_initial_place(c, nullptr);
+      }
+    }
     if (r == 0) r = rp_nesterov_setup(c, nullptr);
     std::lock_guard<std::mutex> lk(s->mu);
     j.ctx = c;
@@ -283,6 +315,74 @@ static void setup_worker(rp_server* s) {
     } else {
       j.status = Job::READY;
     }
+  }
+}
+
+// The batched initial place: every round runs one IP iteration of every job
+// handed in (assembly halves back to back, the solves in shared launches, the
+// records after one wait), so a round costs about what one job's iteration
+// costs while all jobs advance.  A finished job, its positions written to the
+// host model, goes back to the set-up queue for the rest of its set-up.
+static void ip_worker(rp_server* s) {
+  (void)hipSetDevice(s->device);
+  std::vector<Job*> active;
+  const bool timing = getenv("RP_IP_TIMING") != nullptr;
+  auto ms = [](clk::time_point a) { return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
+  for (int round = 0;; round++) {
+    {
+      std::unique_lock<std::mutex> lk(s->ip_mu);
+      if (active.empty()) s->ip_cv.wait(lk, [&] { return s->stop || !s->ip_queue.empty(); });
+      if (active.empty() && s->ip_queue.empty()) return;  // stopping
+      while (!s->ip_queue.empty()) {
+        active.push_back(s->ip_queue.front());
+        s->ip_queue.pop_front();
+      }
+    }
+    hipStream_t st = s->ip_stream;
+    const auto t0 = clk::now();
+    for (Job* j : active)
+      if (!j->ip_code) j->ip_code = rp_ipr_assemble_a(j->ip_run);
+    const double t_la = ms(t0);
+    const bool synced = hipStreamSynchronize(st) == hipSuccess;
+    const double t_a = ms(t0);
+    std::vector<RpIpRun*> live;
+    std::vector<Job*> lj;
+    for (Job* j : active) {
+      if (!synced && !j->ip_code) j->ip_code = RP_ERR_HIP;
+      if (!j->ip_code) j->ip_code = rp_ipr_assemble_b(j->ip_run);
+      if (!j->ip_code) {
+        live.push_back(j->ip_run);
+        lj.push_back(j);
+      }
+    }
+    const double t_b = ms(t0);
+    if (int r = rp_ipr_solve(live.data(), (int)live.size(), st, s->ipb))
+      for (Job* j : lj) j->ip_code = r;
+    const double t_s = ms(t0);
+    for (Job* j : lj)
+      if (!j->ip_code) j->ip_code = rp_ipr_finish_a(j->ip_run);
+    const bool synced2 = hipStreamSynchronize(st) == hipSuccess;
+    if (timing)
+      fprintf(stderr, "[rp] ip round %d: %zu runs, assembly launches %.1f ms, counts %.1f ms, CSR %.1f ms, "
+              "solves %.1f ms, records %.1f ms\n", round, live.size(), t_la, t_a - t_la, t_b - t_a, t_s - t_b,
+              ms(t0) - t_s);
+    std::vector<Job*> keep, back;
+    for (Job* j : active) {
+      bool stop = false;
+      if (!synced2 && !j->ip_code) j->ip_code = RP_ERR_HIP;
+      if (!j->ip_code) j->ip_code = rp_ipr_finish_b(j->ip_run, &stop);
+      if (!j->ip_code && stop) j->ip_code = rp_initial_place_end(rp_ipr_ctx(j->ip_run), j->ip_run);
+      if (j->ip_code || stop)
+        back.push_back(j);
+      else
+        keep.push_back(j);
+    }
+    if (!back.empty()) {
+      std::lock_guard<std::mutex> lk(s->mu);
+      for (Job* j : back) s->setup_queue.push_back(j->id);
+    }
+    for (size_t k = 0; k < back.size(); k++) s->cv.notify_one();
+    active.swap(keep);
   }
 }
 
@@ -304,8 +404,16 @@ extern "C" int rp_server_create(const rp_server_opts* opt, rp_server** out) {
     return sfail(*out, RP_ERR_HIP, "hipStreamCreate failed");
   }
   tiles_default(s.get());
-  int nt = std::max(1, s->opt.setup_threads);
   rp_server* raw = s.get();
+  if (s->opt.batched_ip) {
+    if (hipStreamCreateWithFlags(&s->ip_stream, hipStreamNonBlocking) != hipSuccess) {
+      *out = s.release();
+      return sfail(*out, RP_ERR_HIP, "hipStreamCreate failed");
+    }
+    s->ipb = rp_ipb_create();
+    s->ip_thread = std::thread(ip_worker, raw);
+  }
+  int nt = std::max(1, s->opt.setup_threads);
   for (int k = 0; k < nt; k++) s->workers.emplace_back(setup_worker, raw);
   *out = s.release();
   return RP_OK;
@@ -656,6 +764,15 @@ extern "C" void rp_server_destroy(rp_server* s) {
   }
   s->cv.notify_all();
   for (auto& t : s->workers) t.join();
+  if (s->ip_thread.joinable()) {
+    {
+      std::lock_guard<std::mutex> lk(s->ip_mu);  // the workers are gone: no run is waiting
+    }
+    s->ip_cv.notify_all();
+    s->ip_thread.join();
+  }
+  if (s->ipb) rp_ipb_free(s->ipb);
+  if (s->ip_stream) (void)hipStreamDestroy(s->ip_stream);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (auto& j : s->jobs)
     if (j->ctx) rp_destroy(j->ctx);

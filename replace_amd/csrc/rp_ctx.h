@@ -209,6 +209,9 @@ rp_ctx* rp_ipr_ctx(RpIpRun* r);
 RpIpBatch* rp_ipb_create();
 void rp_ipb_free(RpIpBatch* b);
 int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve);
+// rp_initial_place split around its iterations (rp_host.cpp)
+int rp_initial_place_begin(rp_ctx* c, hipStream_t run_stream, RpIpRun** run);
+int rp_initial_place_end(rp_ctx* c, RpIpRun* run);
 
 // launch wrappers (rp_kernels.hip)
 hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_t* ev, const RpMfmaDev* mf,

@@ -507,18 +507,39 @@ int rp_ip_host_assemble(rp_ctx* c, RpIpAsm& a) {
   return 0;
 }
 
+// The steps of rp_initial_place around its iterations, for a driver that
+// runs the iterations of many contexts together (rp_batch.cpp): *run is the
+// context's run, or null when there is nothing to iterate.
+static void ip_prepare(rp_ctx* c) {
+  const int n = (int)c->place.size();
+  c->ip_err.clear();
+  c->ip_hpwl.clear();
+  // placeInstsCenter (src/initialPlace.cpp:108-115)
+  if (!c->prm.incremental_place_mode) {
+    int ccx = (c->core_lx + c->core_ux) / 2, ccy = (c->core_ly + c->core_uy) / 2;
+    for (int k = 0; k < n; k++) inst_set_center(c, c->place[k], ccx, ccy);
+  }
+}
+
+int rp_initial_place_begin(rp_ctx* c, hipStream_t run_stream, RpIpRun** run) {
+  *run = nullptr;
+  ip_prepare(c);
+  if (c->prm.initial_place_max_iter > 0 && !c->place.empty()) return rp_ipr_begin(c, run_stream, false, run);
+  return 0;
+}
+
+int rp_initial_place_end(rp_ctx* c, RpIpRun* run) {
+  const int r = run ? rp_ipr_end(run) : 0;
+  c->nb_ready = false;
+  return r;
+}
+
 extern "C" int rp_initial_place(rp_ctx* c, rp_ip_stats* stats) {
   if (!c) return RP_ERR_ARG;
   auto t0 = std::chrono::steady_clock::now();
   const rp_params& p = c->prm;
   const int n = (int)c->place.size();
-  c->ip_err.clear();
-  c->ip_hpwl.clear();
-  // placeInstsCenter (src/initialPlace.cpp:108-115)
-  if (!p.incremental_place_mode) {
-    int ccx = (c->core_lx + c->core_ux) / 2, ccy = (c->core_ly + c->core_uy) / 2;
-    for (int k = 0; k < n; k++) inst_set_center(c, c->place[k], ccx, ccy);
-  }
+  ip_prepare(c);
   if (p.initial_place_max_iter > 0 && n > 0) {
     // B2B assembly on the device; RP_IP_HOST_ASSEMBLY=1 selects the sequential
     // host assembly of the same systems

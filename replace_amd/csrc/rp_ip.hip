@@ -1304,6 +1304,7 @@ struct RpIpBatch {
 };
 
 rp_ctx* rp_ipr_ctx(RpIpRun* r) { return r->c; }
+
 void rp_ipr_free(RpIpRun* r) { delete r; }
 RpIpBatch* rp_ipb_create() { return new RpIpBatch(); }
 void rp_ipb_free(RpIpBatch* b) { delete b; }

@@ -326,6 +326,7 @@ class RpServerOpts(ctypes.Structure):
         ("steps_per_poll", ctypes.c_int32),
         ("use_graph", ctypes.c_int32),
         ("autotune", ctypes.c_int32),
+        ("batched_ip", ctypes.c_int32),
     ]
 
 

@@ -110,3 +110,28 @@ def test_tuner_times_every_shape_of_the_buckets_present():
         assert min(t["scatter_us"]) > 0 and min(t["wa_us"]) > 0
         assert t["scatter_cells"] == 256 << int(np.argmin(t["scatter_us"]))
         assert t["wa_pins"] == 256 << int(np.argmin(t["wa_us"]))
+
+
+def test_batched_initial_place_identical_to_alone():
+    """With batched_ip the service runs the initial place of every job in
+    set-up in shared launches (one-workgroup solves of small systems two
+    workgroups per job, grid solves through one (job, block) table); each
+    job still computes exactly what it computes alone, so its placement is
+    bit-identical to the plain path's."""
+    jobs = []
+    for i, (nc, ip) in enumerate(((1500, 6), (6000, 5), (12000, 4), (900, 1), (3000, 0))):
+        nl = netgen.generate(nc, int(nc * 1.05), seed=700 + i, util=0.65, td=0.7)
+        nb = netgen.next_pow2(int(np.ceil(np.sqrt(nc))))
+        p = netgen.params_default(target_density=0.7, bin_grid_cnt_x=max(32, nb),
+                                  bin_grid_cnt_y=max(32, nb), initial_place_max_iter=ip)
+        jobs.append((nl, p))
+    alone = [_alone(nl, p) for nl, p in jobs]
+    srv = Server(steps_per_poll=4, batched_ip=1)
+    ids = [srv.submit(nl, p, 0.0) for nl, p in jobs]
+    srv.run()
+    for jid, (code, st, rec, pos) in zip(ids, alone):
+        r = srv.result(jid)
+        assert r.done == 1 and r.code == code
+        assert srv.records(jid) == rec
+        lx, ly = srv.positions(jid)
+        assert np.array_equal(lx, pos[0]) and np.array_equal(ly, pos[1])
