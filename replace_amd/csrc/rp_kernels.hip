@@ -889,7 +889,23 @@ __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
   const int tid = threadIdx.x;
   long long h = 0;
   if (npw <= CAP) {
-    for (int j = tid; j < npw; j += blockDim.x) s_pos[j] = d.ppin[p0 + j];
+    // Every phase issues its loads before it consumes them (the pins of a
+    // thread are tid + RP_BLOCK r); the per-net loops read four pins ahead.
+    // The arithmetic and its order are those of one pin at a time.
+    constexpr int PER = (CAP + RP_BLOCK - 1) / RP_BLOCK;
+    {
+      int2 lp[PER];
+#pragma unroll
+      for (int r = 0; r < PER; r++) {
+        const int j = tid + r * RP_BLOCK;
+        if (j < npw) lp[r] = d.ppin[p0 + j];
+      }
+#pragma unroll
+      for (int r = 0; r < PER; r++) {
+        const int j = tid + r * RP_BLOCK;
+        if (j < npw) s_pos[j] = lp[r];
+      }
+    }
     __syncthreads();
     const int e = rng.x + tid;
     const bool owner = e < rng.y;
@@ -898,7 +914,21 @@ __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
     if (owner) {
       s = d.nstart[e] - p0;
       t = d.nstart[e + 1] - p0;
-      for (int q = s; q < t; q++) {
+      int q = s;
+      for (; q + 4 <= t; q += 4) {
+        int2 p[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) p[u] = s_pos[q + u];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          box.x = min(p[u].x, box.x);
+          box.y = min(p[u].y, box.y);
+          box.z = max(p[u].x, box.z);
+          box.w = max(p[u].y, box.w);
+          s_net[q + u] = (uint8_t)tid;
+        }
+      }
+      for (; q < t; q++) {
         int2 p = s_pos[q];
         box.x = min(p.x, box.x);
         box.y = min(p.y, box.y);
@@ -910,11 +940,39 @@ __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
       if (t > s) h = (long long)((box.z - box.x) + (box.w - box.y));
     }
     __syncthreads();
-    for (int j = tid; j < npw; j += blockDim.x) s_ex[j] = wa_exps(s_pos[j], s_box[s_net[j]], cx, cy, bar);
+    {
+      int net[PER];
+      int2 pos[PER];
+#pragma unroll
+      for (int r = 0; r < PER; r++) {
+        const int j = tid + r * RP_BLOCK;
+        if (j < npw) {
+          net[r] = s_net[j];
+          pos[r] = s_pos[j];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < PER; r++) {
+        const int j = tid + r * RP_BLOCK;
+        if (j < npw) s_ex[j] = wa_exps(pos[r], s_box[net[r]], cx, cy, bar);
+      }
+    }
     __syncthreads();
     if (owner) {
       WaSums w = {0, 0, 0, 0, 0, 0, 0, 0};
-      for (int q = s; q < t; q++) wa_accumulate(w, s_pos[q], s_ex[q]);
+      int q = s;
+      for (; q + 4 <= t; q += 4) {
+        int2 p[4];
+        float4 x[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          p[u] = s_pos[q + u];
+          x[u] = s_ex[q + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) wa_accumulate(w, p[u], x[u]);
+      }
+      for (; q < t; q++) wa_accumulate(w, s_pos[q], s_ex[q]);
       float4 sa = make_float4(w.s0, w.s1, w.s2, w.s3), sb = make_float4(w.s4, w.s5, w.s6, w.s7);
       s_sum[2 * tid] = sa;
       s_sum[2 * tid + 1] = sb;
@@ -925,9 +983,27 @@ __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
       }
     }
     __syncthreads();
-    for (int j = tid; j < npw; j += blockDim.x) {
-      int L = s_net[j];
-      d.pgrad[p0 + j] = wa_pin_grad(s_pos[j], s_ex[j], s_sum[2 * L], s_sum[2 * L + 1], cx, cy);
+    {
+      int net[PER];
+      int2 pos[PER];
+      float4 ex[PER];
+#pragma unroll
+      for (int r = 0; r < PER; r++) {
+        const int j = tid + r * RP_BLOCK;
+        if (j < npw) {
+          net[r] = s_net[j];
+          pos[r] = s_pos[j];
+          ex[r] = s_ex[j];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < PER; r++) {
+        const int j = tid + r * RP_BLOCK;
+        if (j < npw) {
+          const int L = net[r];
+          d.pgrad[p0 + j] = wa_pin_grad(pos[r], ex[r], s_sum[2 * L], s_sum[2 * L + 1], cx, cy);
+        }
+      }
     }
   } else {
     // one net with more than CAP pins: bbox by a block min/max, the
