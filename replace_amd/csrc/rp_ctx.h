@@ -78,7 +78,7 @@ struct rp_ctx {
   DevBuf<int> d_sord, d_srank, d_scnt, d_soff;  // spatial cell order of the scatter and its counting sort
   RpSortBufs sortb{};
   DevBuf<int> d_cp_start, d_cp_pin, d_nstart, d_pcell;
-  DevBuf<int2> d_wa_rng[RP_WA_VARIANTS];
+  DevBuf<int4> d_wa_rng[RP_WA_VARIANTS];
   DevBuf<int2> d_ppos, d_ppin;
   DevBuf<float2> d_pgrad;
   DevBuf<int4> d_nbox;

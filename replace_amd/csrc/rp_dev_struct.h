@@ -29,9 +29,9 @@ struct RpDev {
   float2* pgrad;       // per-pin WA gradient gMin - gMax, device pin order
   const int* nstart;
   int nwa;             // wirelength workgroups of the packing in use
-  const int2* wa_rng;  // their net ranges [x, y)
+  const int4* wa_rng;  // their net ranges [x, y) and pin ranges [z, w)
   int nwa_v[RP_WA_VARIANTS];              // the packings for every tile-shape variant
-  const int2* wa_rng_v[RP_WA_VARIANTS];
+  const int4* wa_rng_v[RP_WA_VARIANTS];
   const int* pcell;
   const int2* ppos;
   int2* ppin;        // absolute pin centres, rewritten every evaluation (k_pin_pos)

@@ -872,18 +872,18 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   int nwa_v[RP_WA_VARIANTS];
   for (int v = 0; v < RP_WA_VARIANTS; v++) {
     const int cap = 256 << v;
-    std::vector<int2> wa_rng;
+    std::vector<int4> wa_rng;  // nets [x, y), their pins [z, w)
     for (int e = 0; e < nn;) {
       const int e0 = e;
       if (nstart[e + 1] - nstart[e] > cap) {
-        wa_rng.push_back(make_int2(e, e + 1));
+        wa_rng.push_back(make_int4(e, e + 1, nstart[e], nstart[e + 1]));
         e++;
         continue;
       }
       while (e < nn && e - e0 < RP_WA_NETS && nstart[e + 1] - nstart[e0] <= cap) e++;
-      wa_rng.push_back(make_int2(e0, e));
+      wa_rng.push_back(make_int4(e0, e, nstart[e0], nstart[e]));
     }
-    if (wa_rng.empty()) wa_rng.push_back(make_int2(0, 0));
+    if (wa_rng.empty()) wa_rng.push_back(make_int4(0, 0, 0, 0));
     nwa_v[v] = (int)wa_rng.size();
     HIPCHK(c, dupload(c->d_wa_rng[v], wa_rng, s));
   }

@@ -883,8 +883,8 @@ __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
   const RpState* st = d.st;
   if (st->phase != 0) return;
   const float cx = st->wl_coef_x, cy = st->wl_coef_y, bar = d.bar;
-  const int2 rng = d.wa_rng[blk];
-  const int p0 = d.nstart[rng.x], p1 = d.nstart[rng.y];
+  const int4 rng = d.wa_rng[blk];
+  const int p0 = rng.z, p1 = rng.w;
   const int npw = p1 - p0;
   const int tid = threadIdx.x;
   long long h = 0;
@@ -893,6 +893,13 @@ __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
     // thread are tid + RP_BLOCK r); the per-net loops read four pins ahead.
     // The arithmetic and its order are those of one pin at a time.
     constexpr int PER = (CAP + RP_BLOCK - 1) / RP_BLOCK;
+    const int e = rng.x + tid;
+    const bool owner = e < rng.y;
+    int s = 0, t = 0;
+    if (owner) {  // issued beside the pin loads
+      s = d.nstart[e] - p0;
+      t = d.nstart[e + 1] - p0;
+    }
     {
       int2 lp[PER];
 #pragma unroll
@@ -907,13 +914,8 @@ __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
       }
     }
     __syncthreads();
-    const int e = rng.x + tid;
-    const bool owner = e < rng.y;
-    int s = 0, t = 0;
     int4 box = make_int4(INT_MAX, INT_MAX, INT_MIN, INT_MIN);
     if (owner) {
-      s = d.nstart[e] - p0;
-      t = d.nstart[e + 1] - p0;
       int q = s;
       for (; q + 4 <= t; q += 4) {
         int2 p[4];
