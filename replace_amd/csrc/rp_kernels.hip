@@ -1077,7 +1077,7 @@ __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
 // last-block-done fusion needs an agent-scope release per block, which on
 // MI355X writes back the XCD's L2 and costs far more than one launch.
 __device__ void grad_body(const RpDev& d, int blk) {
-  __shared__ double sh_red[RP_BLOCK / 64];
+  __shared__ double sh_red[RP_P_COUNT][RP_BLOCK / 64];
   const RpState* st = d.st;
   if (st->phase != 0) return;
   int c = blk * blockDim.x + threadIdx.x;
@@ -1086,6 +1086,11 @@ __device__ void grad_body(const RpDev& d, int blk) {
     float lam = st->density_penalty;
     int dst = st->dst, ref = st->ring.cur;
     int2 icc = d.ic[c];
+    // the distance terms' operands, loaded before the gradient store below
+    // (which the compiler cannot tell apart from them)
+    const float2 a = d.slp[(size_t)ref * d.nc + c];
+    const float2 b = d.slp[(size_t)dst * d.nc + c];
+    const float2 ga = d.sumg[(size_t)ref * d.nc + c];
     // wire length
     float gx = 0.0f, gy = 0.0f;
     int k0 = d.cp_start[c], k1 = d.cp_start[c + 1];
@@ -1147,25 +1152,32 @@ __device__ void grad_body(const RpDev& d, int blk) {
     v_dens = (double)fabsf(fx) + (double)fabsf(fy);
     v_grad = (double)(fabsf(sx) + fabsf(sy));
     // getDistance terms (src/nesterovPlace.cpp:598-607)
-    float2 a = d.slp[(size_t)ref * d.nc + c];
-    float2 b = d.slp[(size_t)dst * d.nc + c];
-    float2 ga = d.sumg[(size_t)ref * d.nc + c];
     float ddx = a.x - b.x, ddy = a.y - b.y;
     float gdx = ga.x - sx, gdy = ga.y - sy;
     v_dc = (double)(ddx * ddx) + (double)(ddy * ddy);
     v_dg = (double)(gdx * gdx) + (double)(gdy * gdy);
   }
-  double r;
-  r = block_sum_d(v_wl, sh_red);
-  if (threadIdx.x == 0) d.part_grad[(size_t)blk * RP_P_COUNT + RP_P_WL] = r;
-  r = block_sum_d(v_dens, sh_red);
-  if (threadIdx.x == 0) d.part_grad[(size_t)blk * RP_P_COUNT + RP_P_DENS] = r;
-  r = block_sum_d(v_grad, sh_red);
-  if (threadIdx.x == 0) d.part_grad[(size_t)blk * RP_P_COUNT + RP_P_GRAD] = r;
-  r = block_sum_d(v_dc, sh_red);
-  if (threadIdx.x == 0) d.part_grad[(size_t)blk * RP_P_COUNT + RP_P_DCOORD] = r;
-  r = block_sum_d(v_dg, sh_red);
-  if (threadIdx.x == 0) d.part_grad[(size_t)blk * RP_P_COUNT + RP_P_DGRAD] = r;
+  // the five block sums in one pass: per value the same wavefront tree and
+  // wavefront order as block_sum_d
+  double v[RP_P_COUNT];
+  v[RP_P_WL] = v_wl;
+  v[RP_P_DENS] = v_dens;
+  v[RP_P_GRAD] = v_grad;
+  v[RP_P_DCOORD] = v_dc;
+  v[RP_P_DGRAD] = v_dg;
+#pragma unroll
+  for (int m = 0; m < RP_P_COUNT; m++) v[m] = wave_sum_d(v[m]);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int m = 0; m < RP_P_COUNT; m++) sh_red[m][wv] = v[m];
+  __syncthreads();
+  if (threadIdx.x < RP_P_COUNT) {
+    const int m = threadIdx.x;
+    double r = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) r += sh_red[m][k];
+    d.part_grad[(size_t)blk * RP_P_COUNT + m] = r;
+  }
 }
 
 // ---------------------------------------------------------------------------
