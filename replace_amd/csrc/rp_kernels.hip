@@ -1229,49 +1229,69 @@ struct Totals {
 __device__ void reduce_all(const RpDev& d, Totals& t) {
   __shared__ long long shl[RP_CTRL_BLOCK / 64];
   const RpState* st = d.st;
-  // one pass over the gradient partials: each thread sums a fixed subset
+  // Each thread sums a fixed strided subset of the gradient, bin (overflow)
+  // and net (HPWL) partials, batch by batch in index order; the first batch
+  // of all three arrays is loaded before any of them is added.
   double v[RP_P_COUNT] = {0, 0, 0, 0, 0};
-  const int ng = st->num_partials_grad;
-  constexpr int GB = 4;  // iterations whose loads are issued together
-  for (int k0 = threadIdx.x; k0 < ng; k0 += GB * blockDim.x) {
-    double x[GB][RP_P_COUNT];
-#pragma unroll
-    for (int u = 0; u < GB; u++) {
-      int k = k0 + u * blockDim.x;
-      const double* q = d.part_grad + (size_t)k * RP_P_COUNT;
-#pragma unroll
-      for (int j = 0; j < RP_P_COUNT; j++) x[u][j] = k < ng ? q[j] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < GB; u++)
-      if (k0 + u * (int)blockDim.x < ng)
-#pragma unroll
-        for (int j = 0; j < RP_P_COUNT; j++) v[j] += x[u][j];
-  }
-  // bin (overflow) and net (HPWL) partials, each thread a fixed strided subset
   double vb = 0.0;
   long long vh = 0;
+  const int ng = st->num_partials_grad;
   const int nbp = st->num_partials_bin, nnp = d.nwa;  // one HPWL partial per wirelength workgroup
-  for (int k0 = threadIdx.x; k0 < nbp; k0 += RP_RED_BATCH * blockDim.x) {
-    double x[RP_RED_BATCH];
+  constexpr int GB = 4;  // gradient iterations whose loads are issued together
+  const int T = blockDim.x, k0 = threadIdx.x;
+  double xg[GB][RP_P_COUNT], xb[RP_RED_BATCH];
+  long long xn[RP_RED_BATCH];
 #pragma unroll
-    for (int u = 0; u < RP_RED_BATCH; u++) {
-      int k = k0 + u * blockDim.x;
-      x[u] = k < nbp ? d.part_bin[k] : 0.0;
+  for (int u = 0; u < GB; u++) {
+    const int k = k0 + u * T;
+    const double* q = d.part_grad + (size_t)k * RP_P_COUNT;
+#pragma unroll
+    for (int j = 0; j < RP_P_COUNT; j++) xg[u][j] = k < ng ? q[j] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < RP_RED_BATCH; u++) {
+    const int k = k0 + u * T;
+    xb[u] = k < nbp ? d.part_bin[k] : 0.0;
+    xn[u] = k < nnp ? d.part_net[k] : 0;
+  }
+  for (int kb = k0;;) {  // gradient partials
+#pragma unroll
+    for (int u = 0; u < GB; u++)
+      if (kb + u * T < ng)
+#pragma unroll
+        for (int j = 0; j < RP_P_COUNT; j++) v[j] += xg[u][j];
+    kb += GB * T;
+    if (kb >= ng) break;
+#pragma unroll
+    for (int u = 0; u < GB; u++) {
+      const int k = kb + u * T;
+      const double* q = d.part_grad + (size_t)k * RP_P_COUNT;
+#pragma unroll
+      for (int j = 0; j < RP_P_COUNT; j++) xg[u][j] = k < ng ? q[j] : 0.0;
     }
+  }
+  for (int kb = k0;;) {  // bin partials
 #pragma unroll
     for (int u = 0; u < RP_RED_BATCH; u++)
-      if (k0 + u * (int)blockDim.x < nbp) vb += x[u];
-  }
-  for (int k0 = threadIdx.x; k0 < nnp; k0 += RP_RED_BATCH * blockDim.x) {
-    long long x[RP_RED_BATCH];
+      if (kb + u * T < nbp) vb += xb[u];
+    kb += RP_RED_BATCH * T;
+    if (kb >= nbp) break;
 #pragma unroll
     for (int u = 0; u < RP_RED_BATCH; u++) {
-      int k = k0 + u * blockDim.x;
-      x[u] = k < nnp ? d.part_net[k] : 0;
+      const int k = kb + u * T;
+      xb[u] = k < nbp ? d.part_bin[k] : 0.0;
     }
+  }
+  for (int kb = k0;;) {  // net partials
 #pragma unroll
-    for (int u = 0; u < RP_RED_BATCH; u++) vh += x[u];
+    for (int u = 0; u < RP_RED_BATCH; u++) vh += xn[u];
+    kb += RP_RED_BATCH * T;
+    if (kb >= nnp) break;
+#pragma unroll
+    for (int u = 0; u < RP_RED_BATCH; u++) {
+      const int k = kb + u * T;
+      xn[u] = k < nnp ? d.part_net[k] : 0;
+    }
   }
   // all seven totals in one pass: wavefront trees, then the wavefronts in
   // order (the same arithmetic as one block_sum per value)
