@@ -492,14 +492,9 @@ __device__ void sorted_scatter_body(const RpDev& d, int blk, const RpSortBufs& s
   __shared__ int t_key[RP_TILE_SLOTS], t_cnt[RP_TILE_SLOTS], t_base[RP_TILE_SLOTS];
   const RpState* st = d.st;
   if (st->phase != 0) return;
-  win_clear(t);
-  for (int k = threadIdx.x; k < RP_TILE_SLOTS; k += blockDim.x) {
-    t_key[k] = -1;
-    t_cnt[k] = 0;
-  }
-  __syncthreads();
   int cell[RP_SS_PER_THREAD], slot[RP_SS_PER_THREAD], lrank[RP_SS_PER_THREAD];
-  // all loads before the first atomic (which could alias them)
+  // all loads before the first atomic (which could alias them), and before
+  // the table clear and its barrier, which they overlap
   int2 v_ic[RP_SS_PER_THREAD], v_h[RP_SS_PER_THREAD];
   float v_sc[RP_SS_PER_THREAD];
   uint8_t v_kd[RP_SS_PER_THREAD];
@@ -517,6 +512,12 @@ __device__ void sorted_scatter_body(const RpDev& d, int blk, const RpSortBufs& s
     v_sc[r] = d.scale[c];
     v_kd[r] = d.kind[c];
   }
+  win_clear(t);
+  for (int k = threadIdx.x; k < RP_TILE_SLOTS; k += blockDim.x) {
+    t_key[k] = -1;
+    t_cnt[k] = 0;
+  }
+  __syncthreads();
 #pragma unroll
   for (int r = 0; r < RP_SS_PER_THREAD; r++) {
     int c = cell[r];
