@@ -212,6 +212,18 @@ __device__ __forceinline__ void dct2_post(const float2* Z, int n, int k, const f
   xb = vb.x * w.x - vb.y * w.y;
 }
 
+// dct2_post with the twiddle tw2[k] already loaded (w)
+__device__ __forceinline__ void dct2_post_w(const float2* Z, int n, int k, float2 w, float& xa, float& xb) {
+  float2 zk = Z[k];
+  float2 zm = Z[(n - k) & (n - 1)];
+  zm.y = -zm.y;  // conj
+  float2 va = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
+  float dr = zk.x - zm.x, di = zk.y - zm.y;
+  float2 vb = make_float2(0.5f * di, -0.5f * dr);
+  xa = va.x * w.x - va.y * w.y;
+  xb = vb.x * w.x - vb.y * w.y;
+}
+
 // Pre-twiddle of the inverse Makhoul (DCT-III, Ooura "IDCT" without scale):
 // W[k] = 1/2 e^{i pi k/2N} (a[k] - i a[N-k]) (k >= 1), W[0] = a[0];
 // returns W_a + i W_b for two real coefficient rows read through getters.
@@ -223,6 +235,20 @@ __device__ __forceinline__ float2 dct3_pre(float a_k, float a_nk, float b_k, flo
     wb = make_float2(b_k, 0.0f);
   } else {
     float2 w = tw2[k];
+    float c = w.x, s = -w.y;  // e^{+i t} = (c, s)
+    wa = make_float2(0.5f * (a_k * c + a_nk * s), 0.5f * (a_k * s - a_nk * c));
+    wb = make_float2(0.5f * (b_k * c + b_nk * s), 0.5f * (b_k * s - b_nk * c));
+  }
+  return make_float2(wa.x - wb.y, wa.y + wb.x);
+}
+
+// dct3_pre with the twiddle tw2[k] already loaded (w; unused at k = 0)
+__device__ __forceinline__ float2 dct3_pre_w(float a_k, float a_nk, float b_k, float b_nk, int k, float2 w) {
+  float2 wa, wb;
+  if (k == 0) {
+    wa = make_float2(a_k, 0.0f);
+    wb = make_float2(b_k, 0.0f);
+  } else {
     float c = w.x, s = -w.y;  // e^{+i t} = (c, s)
     wa = make_float2(0.5f * (a_k * c + a_nk * s), 0.5f * (a_k * s - a_nk * c));
     wb = make_float2(0.5f * (b_k * c + b_nk * s), 0.5f * (b_k * s - b_nk * c));
@@ -663,18 +689,41 @@ __device__ void col_body(const RpDev& d, int blk) {
   float* eb = ea + n;
   int ua = 2 * blk, ub = ua + 1;
   const size_t nx = (size_t)d.nbx;
+  // Everything this thread reads from global memory in any phase is loaded
+  // first (its rows v = threadIdx.x + r blockDim.x): the two columns, the
+  // twiddles of the DCT post/pre steps and the spectral weights.
+  constexpr int MAXC = RP_MAX_FFT / RP_BLOCK;
+  float2 colv[MAXC], tw[MAXC];
+  float wyv[MAXC], wy2v[MAXC];
+#pragma unroll
+  for (int r = 0; r < MAXC; r++) {
+    const int v = threadIdx.x + r * blockDim.x;
+    if (v < n) {
+      // columns ua, ub of the row-transformed grid t (row-major): one 8-byte
+      // load per row (the workgroups sharing a cache line run on one XCD)
+      colv[r] = *(const float2*)(d.t + v * nx + ua);
+      tw[r] = d.tw2y[v];
+      wyv[r] = d.wy[v];
+      wy2v[r] = d.wy2[v];
+    }
+  }
+  const float wx2u[2] = {d.wx2[ua], d.wx2[ub]};
   load_twiddles(T, d.twy, n);
-  // columns ua, ub of the row-transformed grid t (row-major): one 8-byte load
-  // per row (the workgroups sharing a cache line run on one XCD)
-  for (int v = threadIdx.x; v < n; v += blockDim.x)
-    A[mk_pos(v, n)] = *(const float2*)(d.t + v * nx + ua);
+#pragma unroll
+  for (int r = 0; r < MAXC; r++) {
+    const int v = threadIdx.x + r * blockDim.x;
+    if (v < n) A[mk_pos(v, n)] = colv[r];
+  }
   __syncthreads();
   float2* Z = lds_fft(A, B, n, T, -1);
   const float scale4 = (float)(4.0 / (double)d.nbx / (double)n);
-  for (int v = threadIdx.x; v < n; v += blockDim.x) {
+#pragma unroll
+  for (int rr = 0; rr < MAXC; rr++) {
+    const int v = threadIdx.x + rr * blockDim.x;
+    if (v >= n) continue;
     float xa, xb;
-    dct2_post(Z, n, v, d.tw2y, xa, xb);
-    float wy = d.wy[v], wy2 = d.wy2[v];
+    dct2_post_w(Z, n, v, tw[rr], xa, xb);
+    float wy = wyv[rr], wy2 = wy2v[rr];
     float f[2] = {xa, xb};
     int us[2] = {ua, ub};
     for (int r = 0; r < 2; r++) {
@@ -685,7 +734,7 @@ __device__ void col_body(const RpDev& d, int blk) {
       F = F * scale4;
       float ph = 0.0f, ey = 0.0f;
       if (!(u == 0 && v == 0)) {
-        ph = F / (d.wx2[u] + wy2);
+        ph = F / (wx2u[r] + wy2);
         ey = ph * wy;
       }
       (r ? pb : pa)[v] = ph;
@@ -694,10 +743,13 @@ __device__ void col_body(const RpDev& d, int blk) {
   }
   __syncthreads();
   // DCT-III of phi hat (columns a, b) -> y1
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+#pragma unroll
+  for (int rr = 0; rr < MAXC; rr++) {
+    const int k = threadIdx.x + rr * blockDim.x;
+    if (k >= n) continue;
     int nk = (n - k) & (n - 1);
     float ank = k ? pa[nk] : 0.0f, bnk = k ? pb[nk] : 0.0f;
-    A[k] = dct3_pre(pa[k], ank, pb[k], bnk, k, d.tw2y);
+    A[k] = dct3_pre_w(pa[k], ank, pb[k], bnk, k, tw[rr]);
   }
   __syncthreads();
   Z = lds_fft(A, B, n, T, +1);
@@ -708,13 +760,16 @@ __device__ void col_body(const RpDev& d, int blk) {
   }
   __syncthreads();
   // DST-III of Ey hat: b'[0] = a[0] (the A[N] slot), b'[j] = a[N-j]; sign (-1)^k
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+#pragma unroll
+  for (int rr = 0; rr < MAXC; rr++) {
+    const int k = threadIdx.x + rr * blockDim.x;
+    if (k >= n) continue;
     int nk = (n - k) & (n - 1);
     float ak = k ? ea[n - k] : ea[0];
     float bk = k ? eb[n - k] : eb[0];
     float ank = k ? (nk ? ea[n - nk] : ea[0]) : 0.0f;
     float bnk = k ? (nk ? eb[n - nk] : eb[0]) : 0.0f;
-    A[k] = dct3_pre(ak, ank, bk, bnk, k, d.tw2y);
+    A[k] = dct3_pre_w(ak, ank, bk, bnk, k, tw[rr]);
   }
   __syncthreads();
   Z = lds_fft(A, B, n, T, +1);
