@@ -795,23 +795,46 @@ __device__ void rowinv_body(const RpDev& d, int blk, int want_phi) {
   float* qa = rb + n;  // second input rows a, b (Y2, then Y1 for phi)
   float* qb = qa + n;
   int ja = 2 * blk, jb = ja + 1;
+  // the thread's loads of every phase first (its entries u = threadIdx.x +
+  // r blockDim.x): the two rows of y1 and y2, wx and the DCT-III twiddles
+  constexpr int MAXC = RP_MAX_FFT / RP_BLOCK;
+  float wv[MAXC], y1a[MAXC], y1b[MAXC], y2a[MAXC], y2b[MAXC];
+  float2 tw[MAXC];
+#pragma unroll
+  for (int r = 0; r < MAXC; r++) {
+    const int u = threadIdx.x + r * blockDim.x;
+    if (u < n) {
+      wv[r] = d.wx[u];
+      y1a[r] = d.y1[(size_t)ja * n + u];
+      y1b[r] = d.y1[(size_t)jb * n + u];
+      y2a[r] = d.y2[(size_t)ja * n + u];
+      y2b[r] = d.y2[(size_t)jb * n + u];
+      tw[r] = d.tw2x[u];
+    }
+  }
   load_twiddles(T, d.twx, n);
   // Ex: DST-III of (wx[u] * Y1[j][u]); Ey: DCT-III of Y2 rows
-  for (int u = threadIdx.x; u < n; u += blockDim.x) {
-    float w = d.wx[u];
-    ra[u] = w * d.y1[(size_t)ja * n + u];
-    rb[u] = w * d.y1[(size_t)jb * n + u];
-    qa[u] = d.y2[(size_t)ja * n + u];
-    qb[u] = d.y2[(size_t)jb * n + u];
+#pragma unroll
+  for (int r = 0; r < MAXC; r++) {
+    const int u = threadIdx.x + r * blockDim.x;
+    if (u < n) {
+      ra[u] = wv[r] * y1a[r];
+      rb[u] = wv[r] * y1b[r];
+      qa[u] = y2a[r];
+      qb[u] = y2b[r];
+    }
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+#pragma unroll
+  for (int r = 0; r < MAXC; r++) {
+    const int k = threadIdx.x + r * blockDim.x;
+    if (k >= n) continue;
     int nk = (n - k) & (n - 1);
     float ak = k ? ra[n - k] : ra[0];
     float bk = k ? rb[n - k] : rb[0];
     float ank = k ? (nk ? ra[n - nk] : ra[0]) : 0.0f;
     float bnk = k ? (nk ? rb[n - nk] : rb[0]) : 0.0f;
-    A[k] = dct3_pre(ak, ank, bk, bnk, k, d.tw2x);
+    A[k] = dct3_pre_w(ak, ank, bk, bnk, k, tw[r]);
   }
   __syncthreads();
   float2* Z = lds_fft(A, B, n, T, +1);
@@ -822,9 +845,12 @@ __device__ void rowinv_body(const RpDev& d, int blk, int want_phi) {
     rb[m] = sg * z.y;
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+#pragma unroll
+  for (int r = 0; r < MAXC; r++) {
+    const int k = threadIdx.x + r * blockDim.x;
+    if (k >= n) continue;
     int nk = (n - k) & (n - 1);
-    A[k] = dct3_pre(qa[k], k ? qa[nk] : 0.0f, qb[k], k ? qb[nk] : 0.0f, k, d.tw2x);
+    A[k] = dct3_pre_w(qa[k], k ? qa[nk] : 0.0f, qb[k], k ? qb[nk] : 0.0f, k, tw[r]);
   }
   __syncthreads();
   Z = lds_fft(A, B, n, T, +1);
