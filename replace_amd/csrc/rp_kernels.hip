@@ -610,14 +610,24 @@ __device__ void binfin_body(const RpDev& d, int blk, int from_rho) {
   float2* T = smem + 2 * n;
   int ja = 2 * blk, jb = ja + 1;
   double ov_sum = 0.0;
-  if (!d.mfma) load_twiddles(T, d.twx, n);
+  constexpr int MAXC = RP_MAX_FFT / RP_BLOCK;
+  // the DCT-II post-twiddles of the thread's outputs k = threadIdx.x + r
+  // blockDim.x, loaded now rather than after the FFT
+  float2 tw[MAXC];
+  if (!d.mfma) {
+#pragma unroll
+    for (int r = 0; r < MAXC; r++) {
+      const int k = threadIdx.x + r * blockDim.x;
+      if (k < n) tw[r] = d.tw2x[k];
+    }
+    load_twiddles(T, d.twx, n);
+  }
   if (from_rho) {  // Poisson-only entry: density given
     for (int i = threadIdx.x; i < n; i += blockDim.x)
       A[mk_pos(i, n)] = make_float2(d.rho[(size_t)ja * n + i], d.rho[(size_t)jb * n + i]);
   } else {
     // every load of the thread's bins is issued before any store (the zeroed
     // accumulators could otherwise alias later loads and serialise them)
-    constexpr int MAXC = RP_MAX_FFT / RP_BLOCK;
     long long v_in[2][MAXC], v_fi[2][MAXC], v_np[2][MAXC];
 #pragma unroll
     for (int t = 0; t < MAXC; t++) {
@@ -663,9 +673,12 @@ __device__ void binfin_body(const RpDev& d, int blk, int from_rho) {
   if (d.mfma) return;  // the matrix-core solve takes rho from d.rho
   // block_sum_d synchronised the LDS writes above
   float2* Z = lds_fft(A, B, n, T, -1);
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+#pragma unroll
+  for (int r = 0; r < MAXC; r++) {
+    const int k = threadIdx.x + r * blockDim.x;
+    if (k >= n) continue;
     float xa, xb;
-    dct2_post(Z, n, k, d.tw2x, xa, xb);
+    dct2_post_w(Z, n, k, tw[r], xa, xb);
     d.t[(size_t)ja * n + k] = xa;
     d.t[(size_t)jb * n + k] = xb;
   }
