@@ -718,9 +718,10 @@ BC_KERNELS(zero_x, RP_BLOCK)
 // and x live in LDS.  The matrix is held as sliced ELL (SELL-C-sigma,
 // Kreutzer et al., SIAM J. Sci. Comput. 36(5), 2014) with C = 64: each
 // 64-row slice, one per wave, stores its rows slot-major at the slice's
-// width, so every load instruction of a wave reads 64 consecutive entries,
-// and the sort keeps rows of one slice at nearly the same length; within a
-// slice, each lane's slots are grouped in fours (16-byte loads).  The
+// width (a multiple of 8 slots), so every load instruction of a wave reads 64
+// consecutive entries, and the sort keeps rows of one slice at nearly the same
+// length; within a slice each lane's slots are grouped in fours (16-byte value
+// loads, 8-byte loads of 16-bit column positions).  The
 // entries of a row keep their CSR order and are summed in FP64 in that order;
 // dot products are per-thread FP64 sums over its rows in k order, then one
 // fixed tree (wave xor, then the waves in order), so every thread holds the
@@ -729,7 +730,8 @@ constexpr int BCS_THREADS = 1024;
 constexpr int BCS_MAX_R = 8;
 constexpr int BCS_PASSES = 16;  // iterations per launch
 constexpr int SELL_C = 64;
-constexpr int SELL_PAD = 16;  // slice widths are multiples of this many slots
+constexpr int SELL_PAD = 8;  // slice widths are multiples of this many slots
+static_assert(BCS_THREADS * BCS_MAX_R <= 65536, "column positions are stored in 16 bits");
 
 struct SellAxis {
   const int* rowp;
@@ -742,7 +744,7 @@ struct SellAxis {
   const int* width;   // per slice: entries, SELL_C x its slots (its first row's length, rounded up)
   const int* soff;    // per slice: first entry
   const float* sval;  // SELL entries, slot-major per slice
-  const int* scol;    // sorted column positions (pads: the row itself)
+  const uint16_t* scol;  // sorted column positions, 16 bits (pads: the row itself)
   BcState* st;
   float* keep;        // 5 n: p, r0, x, r, v carried from one launch to the next
   struct BcsCarry* carry;
@@ -792,7 +794,7 @@ __global__ void __launch_bounds__(RP_BLOCK) k_sell_fill(const int* __restrict__ 
                                                         const float* __restrict__ val, const int* __restrict__ perm,
                                                         const int* __restrict__ iperm, const int* __restrict__ soff,
                                                         int n, int nrows_pad, float* __restrict__ sval,
-                                                        int* __restrict__ scol) {
+                                                        uint16_t* __restrict__ scol) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nrows_pad) return;
   const int s = q / SELL_C, lane = q % SELL_C;
@@ -806,7 +808,7 @@ __global__ void __launch_bounds__(RP_BLOCK) k_sell_fill(const int* __restrict__ 
     const int e = soff[s] + (j / 4) * (4 * SELL_C) + lane * 4 + (j % 4);
     const bool in = j < len;
     sval[e] = in ? val[rb + j] : 0.0f;
-    scol[e] = in ? iperm[col[rb + j]] : own;
+    scol[e] = (uint16_t)(in ? iperm[col[rb + j]] : own);
   }
 }
 
@@ -860,8 +862,8 @@ __device__ __forceinline__ void bcs_sum2(double& a, double& b, double* red) {
 // SB slots (SB divides SELL_PAD, so slice widths are whole rounds) and issues
 // its loads and gathers unconditionally; an entry past the row's length is
 // dropped by a select, so each row's sum is exactly the CSR-order sum of its
-// entries.  Wider rounds keep more loads in flight where registers allow.
-template <int R, int SB = (R <= 4 ? 16 : 8)>
+// entries.
+template <int R, int SB = 8>
 __device__ __forceinline__ void bcs_spmv(const SellAxis& a, const int (&len)[R], const float* X, float (&y)[R]) {
   static_assert(SELL_PAD % SB == 0 && SB % 4 == 0, "rounds must tile the slice widths in quads");
   const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -872,7 +874,7 @@ __device__ __forceinline__ void bcs_spmv(const SellAxis& a, const int (&len)[R],
     const int w = __builtin_amdgcn_readfirstlane(a.width[s]) / SELL_C;
     const unsigned base = __builtin_amdgcn_readfirstlane(a.soff[s]);
     const float* __restrict__ pv = a.sval + base;
-    const int* __restrict__ pc = a.scol + base;
+    const uint16_t* __restrict__ pc = a.scol + base;
     double acc = 0.0;
     for (int j = 0; j < w; j += SB) {
       float vv[SB], xx[SB];
@@ -881,7 +883,7 @@ __device__ __forceinline__ void bcs_spmv(const SellAxis& a, const int (&len)[R],
       for (int u = 0; u < SB; u += 4) {
         const unsigned e = (unsigned)(j + u) * SELL_C + lane * 4;
         const float4 v4 = *reinterpret_cast<const float4*>(pv + e);
-        const int4 c4 = *reinterpret_cast<const int4*>(pc + e);
+        const ushort4 c4 = *reinterpret_cast<const ushort4*>(pc + e);
         vv[u] = v4.x;
         vv[u + 1] = v4.y;
         vv[u + 2] = v4.z;
@@ -1141,8 +1143,9 @@ inline int bcs_rows_per_thread(int n) {
 // Work arrays of the sorted SELL copy of one axis.
 struct SellBufs {
   DevBuf<unsigned> key, key2;
-  DevBuf<int> id, perm, iperm, slen, wslots, soff, scol;
+  DevBuf<int> id, perm, iperm, slen, wslots, soff;
   DevBuf<float> sval;
+  DevBuf<uint16_t> scol;
   DevBuf<char> temp;
 };
 

@@ -34,10 +34,13 @@ def _cases():
     nl, p = netgen.workload("c0", scale=0.25)
     p.initial_place_max_iter = 5
     out.append((nl, p))
+    # 14000 movable rows (grid-wide solver)
+    nl = netgen.generate(14000, 14700, seed=78, name="ip14k")
+    out.append((nl, netgen.params_default(target_density=0.7, initial_place_max_iter=5)))
     return out
 
 
-@pytest.mark.parametrize("case", range(6))
+@pytest.mark.parametrize("case", range(7))
 def test_device_assembly_matches_host_assembly(case):
     nl, p = _cases()[case]
     d = _ip(nl, p, host=False)
