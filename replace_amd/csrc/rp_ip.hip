@@ -480,7 +480,11 @@ __device__ __forceinline__ float spmv_row(const BcAxis& a, int i, const float* _
 
 // Kernel bodies take (axis, block), so that one body serves a launch per
 // context and a launch over many contexts (a job table); g.nparts blocks of
-// RP_BLOCK threads cover an axis either way.
+// RP_BLOCK threads cover an axis either way.  The row stride is
+// g.nparts * RP_BLOCK, so the one-thread-per-row kernels (p, s, xr, zero_x)
+// have rows only in their first min(nparts, ceil(n / RP_BLOCK)) blocks and are
+// launched with those alone; the r.r and r0.r partials of the blocks past them
+// are the 0.0 that init writes.
 #define BC_ROWS(i) for (int i = blk * RP_BLOCK + (int)threadIdx.x; i < g.n; i += g.nparts * RP_BLOCK)
 // one row per BC_G-lane group; all lanes of a group run the same trip count
 #define BC_GROUP_ROWS(i)                                                                  \
@@ -510,6 +514,8 @@ __device__ __forceinline__ void bc_init_body(const BcArgs& g, int axis, int blk)
   if (threadIdx.x == 0) {
     a.parts[BC_RR0 * g.nparts + blk] = d0;
     a.parts[BC_BB * g.nparts + blk] = d1;
+    a.parts[BC_RR * g.nparts + blk] = 0.0;  // xr rewrites those of the blocks with rows
+    a.parts[BC_R0R * g.nparts + blk] = 0.0;
   }
 }
 
@@ -1266,7 +1272,7 @@ struct RpIpRun {
   rp_ctx* c = nullptr;
   hipStream_t st = nullptr;
   bool host_asm = false;
-  int n = 0, np = 0, nn = 0, nparts = 0, hblocks = 0;
+  int n = 0, np = 0, nn = 0, nparts = 0, eblocks = 0, hblocks = 0;
   int small_r = 0, sell_rows = 0, sell_slices = 0;
   int it = 0;  // IP iterations done
   RpIpAsm A;
@@ -1373,6 +1379,7 @@ int rp_ipr_begin(rp_ctx* c, hipStream_t run_stream, bool host_asm, RpIpRun** out
   HIPCHK(c, hipMemcpyAsync(A.pin_cy.p, c->pin_cy.data(), sizeof(int) * np, hipMemcpyHostToDevice, st));
   // solver buffers (both axes)
   R->nparts = (int)std::max<long>(1, std::min<long>(BC_MAX_PARTS, ((long)n * BC_G + RP_BLOCK - 1) / RP_BLOCK));
+  R->eblocks = std::max(1, std::min(R->nparts, (n + RP_BLOCK - 1) / RP_BLOCK));  // blocks with rows at stride nparts
   for (int ax = 0; ax < 2; ax++) {
     for (auto& v : R->vec[ax]) HIPCHK(c, dalloc(v, n));
     HIPCHK(c, dalloc(R->parts[ax], (size_t)BC_SLOTS * R->nparts));
@@ -1557,7 +1564,7 @@ int rp_ipr_solve(RpIpRun* const* runs, int k, hipStream_t st, RpIpBatch* bt) {
         continue;
       }
       const BcArgs& g = R->g;
-      const dim3 gb(R->nparts, 2), g1(1, 2);
+      const dim3 gb(R->nparts, 2), ge(R->eblocks, 2), g1(1, 2);
       const int cap = 2 * std::max(0, R->init.max_iter) + 1;  // i restarts from 0 at most once
       hipLaunchKernelGGL(k_bc_init, gb, blk, 0, st, g);
       int done_iters = 0;
@@ -1566,11 +1573,11 @@ int rp_ipr_solve(RpIpRun* const* runs, int k, hipStream_t st, RpIpBatch* bt) {
         for (int i = 0; i < BC_CHUNK; i++) {
           hipLaunchKernelGGL(k_bc_head, g1, dim3(1024), 0, st, g);
           hipLaunchKernelGGL(k_bc_restart, gb, blk, 0, st, g);
-          hipLaunchKernelGGL(k_bc_p, gb, blk, 0, st, g);
+          hipLaunchKernelGGL(k_bc_p, ge, blk, 0, st, g);
           hipLaunchKernelGGL(k_bc_v, gb, blk, 0, st, g);
-          hipLaunchKernelGGL(k_bc_s, gb, blk, 0, st, g);
+          hipLaunchKernelGGL(k_bc_s, ge, blk, 0, st, g);
           hipLaunchKernelGGL(k_bc_t, gb, blk, 0, st, g);
-          hipLaunchKernelGGL(k_bc_xr, gb, blk, 0, st, g);
+          hipLaunchKernelGGL(k_bc_xr, ge, blk, 0, st, g);
         }
         done_iters += BC_CHUNK;
         hipLaunchKernelGGL(k_bc_head, g1, dim3(1024), 0, st, g);  // closes the chunk's last iteration
@@ -1581,7 +1588,7 @@ int rp_ipr_solve(RpIpRun* const* runs, int k, hipStream_t st, RpIpBatch* bt) {
         HIPCHK(c, hipStreamSynchronize(st));
         if ((R->h_rec->st[0].done && R->h_rec->st[1].done) || done_iters >= cap) break;
       }
-      hipLaunchKernelGGL(k_bc_zero_x, gb, blk, 0, st, g);
+      hipLaunchKernelGGL(k_bc_zero_x, ge, blk, 0, st, g);
       HIPCHK(c, hipGetLastError());
     }
     return 0;
@@ -1617,16 +1624,18 @@ int rp_ipr_solve(RpIpRun* const* runs, int k, hipStream_t st, RpIpBatch* bt) {
     }
   }
   if (grid.empty()) return 0;
-  // grid runs: a (run, block) table for the row kernels, (run, 0) for the head
+  // grid runs: (run, block) tables for the group-per-row kernels (nparts
+  // blocks a run) and the thread-per-row kernels (eblocks), (run, 0) for the head
   const int ng = (int)grid.size();
-  size_t ntab = 0;
+  size_t ntab = 0, netab = 0;
   int cap = 0;
   for (RpIpRun* R : grid) {
     ntab += R->nparts;
+    netab += R->eblocks;
     cap = std::max(cap, 2 * std::max(0, R->init.max_iter) + 1);
   }
   if (int r = grow_pinned(c0, bt->h_bc, bt->d_bc, bt->cap_bc, ng)) return r;
-  if (int r = grow_pinned(c0, bt->h_tab, bt->d_tab, bt->cap_tab, ntab + ng)) return r;
+  if (int r = grow_pinned(c0, bt->h_tab, bt->d_tab, bt->cap_tab, ntab + ng + netab)) return r;
   if (!bt->h_done) HIPCHK(c0, hipHostMalloc((void**)&bt->h_done, sizeof(int) * 2 * 4096, hipHostMallocDefault));
   if (ng > 4096) return fail(c0, RP_ERR_ARG, "initial place: more than 4096 grid solves in one batch");
   size_t t = 0;
@@ -1634,24 +1643,27 @@ int rp_ipr_solve(RpIpRun* const* runs, int k, hipStream_t st, RpIpBatch* bt) {
     bt->h_bc[j] = grid[j]->g;
     for (int b = 0; b < grid[j]->nparts; b++) bt->h_tab[t++] = make_int2(j, b);
   }
-  for (int j = 0; j < ng; j++) bt->h_tab[ntab + j] = make_int2(j, 0);
+  for (int j = 0; j < ng; j++) bt->h_tab[t++] = make_int2(j, 0);
+  for (int j = 0; j < ng; j++)
+    for (int b = 0; b < grid[j]->eblocks; b++) bt->h_tab[t++] = make_int2(j, b);
   HIPCHK(c0, hipMemcpyAsync(bt->d_bc.p, bt->h_bc, sizeof(BcArgs) * ng, hipMemcpyHostToDevice, st));
-  HIPCHK(c0, hipMemcpyAsync(bt->d_tab.p, bt->h_tab, sizeof(int2) * (ntab + ng), hipMemcpyHostToDevice, st));
+  HIPCHK(c0, hipMemcpyAsync(bt->d_tab.p, bt->h_tab, sizeof(int2) * t, hipMemcpyHostToDevice, st));
   const BcArgs* gsd = bt->d_bc.p;
   const int2* tab = bt->d_tab.p;
   const int2* htab = bt->d_tab.p + ntab;
-  const dim3 gb((unsigned)ntab, 2), g1((unsigned)ng, 2);
+  const int2* etab = bt->d_tab.p + ntab + ng;
+  const dim3 gb((unsigned)ntab, 2), ge((unsigned)netab, 2), g1((unsigned)ng, 2);
   hipLaunchKernelGGL(kb_bc_init, gb, blk, 0, st, gsd, tab);
   int done_iters = 0;
   for (;;) {
     for (int i = 0; i < BC_CHUNK; i++) {
       hipLaunchKernelGGL(kb_bc_head, g1, dim3(1024), 0, st, gsd, htab);
       hipLaunchKernelGGL(kb_bc_restart, gb, blk, 0, st, gsd, tab);
-      hipLaunchKernelGGL(kb_bc_p, gb, blk, 0, st, gsd, tab);
+      hipLaunchKernelGGL(kb_bc_p, ge, blk, 0, st, gsd, etab);
       hipLaunchKernelGGL(kb_bc_v, gb, blk, 0, st, gsd, tab);
-      hipLaunchKernelGGL(kb_bc_s, gb, blk, 0, st, gsd, tab);
+      hipLaunchKernelGGL(kb_bc_s, ge, blk, 0, st, gsd, etab);
       hipLaunchKernelGGL(kb_bc_t, gb, blk, 0, st, gsd, tab);
-      hipLaunchKernelGGL(kb_bc_xr, gb, blk, 0, st, gsd, tab);
+      hipLaunchKernelGGL(kb_bc_xr, ge, blk, 0, st, gsd, etab);
     }
     done_iters += BC_CHUNK;
     hipLaunchKernelGGL(kb_bc_head, g1, dim3(1024), 0, st, gsd, htab);
@@ -1665,7 +1677,7 @@ int rp_ipr_solve(RpIpRun* const* runs, int k, hipStream_t st, RpIpBatch* bt) {
     for (int j = 0; j < 2 * ng; j++) all = all && bt->h_done[j];
     if (all || done_iters >= cap) break;
   }
-  hipLaunchKernelGGL(kb_bc_zero_x, gb, blk, 0, st, gsd, tab);
+  hipLaunchKernelGGL(kb_bc_zero_x, ge, blk, 0, st, gsd, etab);
   HIPCHK(c0, hipGetLastError());
   return 0;
 }
