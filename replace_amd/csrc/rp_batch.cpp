@@ -113,6 +113,8 @@ constexpr int kTuneJobs = 16;   // jobs per bucket the tuner times
 constexpr int kTuneMinJobs = 4;  // running jobs a bucket needs before it is tuned
 constexpr int kTuneReps = 5;     // timed launches per shape (the fastest counts: other
                                  // streams' set-up kernels can share the GPU meanwhile)
+constexpr int kIpAux = 4;        // batched initial place: streams (and host threads) for the
+                                 // per-run assembly and records (the hardware queues per process)
 
 int size_bucket(int ncells) {
   if (ncells < 2000) return 0;
@@ -151,12 +153,17 @@ struct rp_server {
   std::deque<int> setup_queue;
   bool stop = false;
   std::vector<std::thread> workers;
-  // batched initial place: set-up workers hand their runs to one driver
+  // batched initial place: set-up workers hand their runs to one driver; a
+  // run's assembly and record work go on ip_aux[job % kIpAux] (one host
+  // thread per stream issues them), the shared solves on ip_stream
   std::mutex ip_mu;
   std::condition_variable ip_cv;
   std::deque<Job*> ip_queue;
   std::thread ip_thread;
   hipStream_t ip_stream = nullptr;
+  hipStream_t ip_aux[kIpAux] = {};
+  hipEvent_t ip_asm_done[kIpAux] = {};
+  hipEvent_t ip_solved = nullptr;
   RpIpBatch* ipb = nullptr;
   rp_server_stats stats{};
 };
@@ -292,7 +299,7 @@ static void setup_worker(rp_server* s) {
       r = rp_create(&j.nl, &j.prm, &c);
       if (r == 0 && s->ip_stream) {
         RpIpRun* run = nullptr;
-        r = rp_initial_place_begin(c, s->ip_stream, &run);
+        r = rp_initial_place_begin(c, s->ip_aux[j.id % kIpAux], &run);
         if (r == 0 && run) {  // the driver iterates it and queues the job here again
           j.ip_run = run;
           std::lock_guard<std::mutex> lk(s->ip_mu);
@@ -340,28 +347,57 @@ static void ip_worker(rp_server* s) {
     }
     hipStream_t st = s->ip_stream;
     const auto t0 = clk::now();
-    for (Job* j : active)
+    // fn(job) for every active job: one host thread per aux stream issues the
+    // launches of that stream's runs
+    auto per_stream = [&](auto&& fn) {
+      std::thread th[kIpAux];
+      for (int h = 0; h < kIpAux; h++)
+        th[h] = std::thread([&, h] {
+          (void)hipSetDevice(s->device);
+          for (Job* j : active)
+            if (j->id % kIpAux == h) fn(j);
+        });
+      for (auto& t : th) t.join();
+    };
+    auto sync_aux = [&] {
+      bool ok = true;
+      for (int h = 0; h < kIpAux; h++) ok = hipStreamSynchronize(s->ip_aux[h]) == hipSuccess && ok;
+      return ok;
+    };
+    per_stream([](Job* j) {
       if (!j->ip_code) j->ip_code = rp_ipr_assemble_a(j->ip_run);
+    });
     const double t_la = ms(t0);
-    const bool synced = hipStreamSynchronize(st) == hipSuccess;
+    const bool synced = sync_aux();
     const double t_a = ms(t0);
-    std::vector<RpIpRun*> live;
-    std::vector<Job*> lj;
-    for (Job* j : active) {
+    per_stream([&](Job* j) {
       if (!synced && !j->ip_code) j->ip_code = RP_ERR_HIP;
       if (!j->ip_code) j->ip_code = rp_ipr_assemble_b(j->ip_run);
+    });
+    std::vector<RpIpRun*> live;
+    std::vector<Job*> lj;
+    for (Job* j : active)
       if (!j->ip_code) {
         live.push_back(j->ip_run);
         lj.push_back(j);
       }
+    // the shared solves wait for every run's assembly; the records for the solves
+    bool evok = true;
+    for (int h = 0; h < kIpAux; h++) {
+      evok = hipEventRecord(s->ip_asm_done[h], s->ip_aux[h]) == hipSuccess && evok;
+      evok = hipStreamWaitEvent(st, s->ip_asm_done[h], 0) == hipSuccess && evok;
     }
     const double t_b = ms(t0);
-    if (int r = rp_ipr_solve(live.data(), (int)live.size(), st, s->ipb))
+    if (int r = evok ? rp_ipr_solve(live.data(), (int)live.size(), st, s->ipb) : RP_ERR_HIP)
       for (Job* j : lj) j->ip_code = r;
+    evok = hipEventRecord(s->ip_solved, st) == hipSuccess;
+    for (int h = 0; h < kIpAux; h++) evok = hipStreamWaitEvent(s->ip_aux[h], s->ip_solved, 0) == hipSuccess && evok;
     const double t_s = ms(t0);
-    for (Job* j : lj)
+    per_stream([&](Job* j) {
+      if (!evok && !j->ip_code) j->ip_code = RP_ERR_HIP;
       if (!j->ip_code) j->ip_code = rp_ipr_finish_a(j->ip_run);
-    const bool synced2 = hipStreamSynchronize(st) == hipSuccess;
+    });
+    const bool synced2 = sync_aux();
     if (timing)
       fprintf(stderr, "[rp] ip round %d: %zu runs, assembly launches %.1f ms, counts %.1f ms, CSR %.1f ms, "
               "solves %.1f ms, records %.1f ms\n", round, live.size(), t_la, t_a - t_la, t_b - t_a, t_s - t_b,
@@ -406,9 +442,14 @@ extern "C" int rp_server_create(const rp_server_opts* opt, rp_server** out) {
   tiles_default(s.get());
   rp_server* raw = s.get();
   if (s->opt.batched_ip) {
-    if (hipStreamCreateWithFlags(&s->ip_stream, hipStreamNonBlocking) != hipSuccess) {
+    bool ok = hipStreamCreateWithFlags(&s->ip_stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&s->ip_solved, hipEventDisableTiming) == hipSuccess;
+    for (int h = 0; h < kIpAux && ok; h++)
+      ok = hipStreamCreateWithFlags(&s->ip_aux[h], hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&s->ip_asm_done[h], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
       *out = s.release();
-      return sfail(*out, RP_ERR_HIP, "hipStreamCreate failed");
+      return sfail(*out, RP_ERR_HIP, "hipStreamCreate / hipEventCreate failed");
     }
     s->ipb = rp_ipb_create();
     s->ip_thread = std::thread(ip_worker, raw);
@@ -773,6 +814,11 @@ extern "C" void rp_server_destroy(rp_server* s) {
   }
   if (s->ipb) rp_ipb_free(s->ipb);
   if (s->ip_stream) (void)hipStreamDestroy(s->ip_stream);
+  for (int h = 0; h < kIpAux; h++) {
+    if (s->ip_aux[h]) (void)hipStreamDestroy(s->ip_aux[h]);
+    if (s->ip_asm_done[h]) (void)hipEventDestroy(s->ip_asm_done[h]);
+  }
+  if (s->ip_solved) (void)hipEventDestroy(s->ip_solved);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (auto& j : s->jobs)
     if (j->ctx) rp_destroy(j->ctx);
