@@ -118,7 +118,7 @@ print(json.dumps(dict(init_s=t1 - t0, iter_s=t2 - t1, evals=st.evals, poisson=o.
 
 _PHASES = {"create": 0.0, "initial_place": 0.0, "setup": 0.0, "nesterov": 0.0}
 SETUP_THREADS = [16]  # host workers of the placement service (--setup-threads)
-BATCHED_IP = [0]  # the service's batched initial place (--batched-ip)
+BATCHED_IP = [2]  # the service's batched initial place (--batched-ip)
 TILES = [None]  # tile shapes the service's tuner chose
 
 
@@ -328,8 +328,9 @@ def main():
                     help="c4: arrival rate, jobs/s (default: ~80%% of the measured 34/s batched capacity)")
     ap.add_argument("--steps-per-poll", type=int, default=8, help="c3/c4: batched steps per graph replay")
     ap.add_argument("--setup-threads", type=int, default=16, help="c3/c4: host set-up workers of the service")
-    ap.add_argument("--batched-ip", type=int, default=0, choices=[0, 1],
-                    help="c3/c4: 1 runs the initial place of all jobs in set-up in shared launches")
+    ap.add_argument("--batched-ip", type=int, default=2, choices=[0, 1, 2],
+                    help="c3/c4: initial place in shared launches for the jobs above 8192 movable "
+                         "instances (2, default), for every job (1) or none (0)")
     args = ap.parse_args()
     SETUP_THREADS[0] = args.setup_threads
     BATCHED_IP[0] = args.batched_ip

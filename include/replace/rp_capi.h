@@ -294,12 +294,13 @@ typedef struct rp_server_opts {
   int32_t autotune;          /* 1: time the tile shapes of the batched scatter and
                                 wirelength kernels per job size bucket on the
                                 bucket's first jobs and keep the fastest */
-  int32_t batched_ip;        /* 1: the initial place of every job in set-up runs
-                                in rounds of shared launches (one driver thread,
-                                one stream): lower latency under load, lower
-                                throughput; 0 (default): each set-up worker runs
-                                its job's initial place on the job's stream.  Each
-                                job's result is the same either way */
+  int32_t batched_ip;        /* 2 (default): the initial place of jobs of more
+                                than 8192 movable instances (the grid-wide
+                                solver) runs in rounds of shared launches (one
+                                driver thread), smaller jobs' on their set-up
+                                worker; 1: every job's in the rounds; 0: every
+                                job's on its set-up worker.  Each job's result
+                                is the same in every mode */
 } rp_server_opts;
 
 /* Tile shapes the service uses for one job size bucket (rp_server_tuning).

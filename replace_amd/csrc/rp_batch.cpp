@@ -190,7 +190,7 @@ extern "C" void rp_server_opts_default(rp_server_opts* o) {
   o->steps_per_poll = 8;
   o->use_graph = 1;
   o->autotune = 1;
-  o->batched_ip = 0;
+  o->batched_ip = 2;
 }
 
 static void tiles_default(rp_server* s) {
@@ -297,7 +297,8 @@ static void setup_worker(rp_server* s) {
       j.ip_run = nullptr;
     } else {
       r = rp_create(&j.nl, &j.prm, &c);
-      if (r == 0 && s->ip_stream) {
+      // batched_ip 2: only the systems of the grid-wide solver go to the rounds
+      if (r == 0 && s->ip_stream && (s->opt.batched_ip != 2 || !rp_ip_one_workgroup((int)c->place.size()))) {
         RpIpRun* run = nullptr;
         r = rp_initial_place_begin(c, s->ip_aux[j.id % kIpAux], &run);
         if (r == 0 && run) {  // the driver iterates it and queues the job here again

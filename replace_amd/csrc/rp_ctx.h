@@ -207,6 +207,7 @@ int rp_ipr_end(RpIpRun* r);                   // positions back to the host mode
 void rp_ipr_free(RpIpRun* r);
 rp_ctx* rp_ipr_ctx(RpIpRun* r);
 RpIpBatch* rp_ipb_create();
+bool rp_ip_one_workgroup(int n);  // n rows: the initial place solves in one workgroup per axis
 void rp_ipb_free(RpIpBatch* b);
 int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve);
 // rp_initial_place split around its iterations (rp_host.cpp)

@@ -1318,6 +1318,13 @@ void rp_ipr_free(RpIpRun* r) { delete r; }
 RpIpBatch* rp_ipb_create() { return new RpIpBatch(); }
 void rp_ipb_free(RpIpBatch* b) { delete b; }
 
+// one workgroup per axis for small systems; RP_IP_SOLVER=blocks keeps the
+// grid-wide kernels at every size
+bool rp_ip_one_workgroup(int n) {
+  const char* solver = getenv("RP_IP_SOLVER");
+  return !(solver && strcmp(solver, "blocks") == 0) && bcs_rows_per_thread(n) != 0;
+}
+
 int rp_ipr_begin(rp_ctx* c, hipStream_t run_stream, bool host_asm, RpIpRun** out) {
   std::unique_ptr<RpIpRun> R(new RpIpRun());
   R->c = c;
@@ -1408,10 +1415,7 @@ int rp_ipr_begin(rp_ctx* c, hipStream_t run_stream, bool host_asm, RpIpRun** out
   R->init.tol2 = (double)tol * (double)tol;
   R->init.eps2 = 1.1920928955078125e-07 * 1.1920928955078125e-07;
   R->init.max_iter = p.initial_place_max_solver_iter;
-  // one workgroup per axis for small systems; RP_IP_SOLVER=blocks keeps the
-  // grid-wide kernels at every size
-  const char* solver = getenv("RP_IP_SOLVER");
-  R->small_r = (solver && strcmp(solver, "blocks") == 0) ? 0 : bcs_rows_per_thread(n);
+  R->small_r = rp_ip_one_workgroup(n) ? bcs_rows_per_thread(n) : 0;
   R->sell_rows = BCS_THREADS * R->small_r;
   R->sell_slices = R->sell_rows / SELL_C;
   if (R->small_r) {

@@ -50,6 +50,19 @@ def test_params_default_matches_library():
         assert getattr(p, name) == getattr(q, name), name
 
 
+def test_server_opts_default():
+    """The service defaults (rp_server_opts_default fills the struct only,
+    no device): the initial place of jobs above 8192 movable instances in
+    shared rounds (batched_ip 2), 16 set-up workers, graph replay on."""
+    if not os.path.exists(capi.LIB_PATH):
+        pytest.skip("library not built")
+    from replace_amd.netlist import RpServerOpts
+    o = RpServerOpts()
+    capi.lib().rp_server_opts_default(ctypes.byref(o))
+    assert o.batched_ip == 2
+    assert o.setup_threads == 16 and o.steps_per_poll == 8 and o.use_graph == 1
+
+
 def test_struct_sizes_match_header_layout():
     # sizes of the C structs on x86-64 (pointers 8 bytes, natural alignment)
     assert ctypes.sizeof(RpParams) == 22 * 4

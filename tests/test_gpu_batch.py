@@ -112,12 +112,14 @@ def test_tuner_times_every_shape_of_the_buckets_present():
         assert t["wa_pins"] == 256 << int(np.argmin(t["wa_us"]))
 
 
-def test_batched_initial_place_identical_to_alone():
-    """With batched_ip the service runs the initial place of every job in
+@pytest.mark.parametrize("batched_ip", [0, 1, 2])
+def test_batched_initial_place_identical_to_alone(batched_ip):
+    """With batched_ip 1 the service runs the initial place of every job in
     set-up in shared launches (one-workgroup solves of small systems two
-    workgroups per job, grid solves through one (job, block) table); each
-    job still computes exactly what it computes alone, so its placement is
-    bit-identical to the plain path's."""
+    workgroups per job, grid solves through one (job, block) table); with 2
+    only the grid-solver jobs (the 12000-cell one here) go to those rounds.
+    Each job still computes exactly what it computes alone, so its placement
+    is bit-identical to the plain path's."""
     jobs = []
     for i, (nc, ip) in enumerate(((1500, 6), (6000, 5), (12000, 4), (900, 1), (3000, 0))):
         nl = netgen.generate(nc, int(nc * 1.05), seed=700 + i, util=0.65, td=0.7)
@@ -126,7 +128,7 @@ def test_batched_initial_place_identical_to_alone():
                                   bin_grid_cnt_y=max(32, nb), initial_place_max_iter=ip)
         jobs.append((nl, p))
     alone = [_alone(nl, p) for nl, p in jobs]
-    srv = Server(steps_per_poll=4, batched_ip=1)
+    srv = Server(steps_per_poll=4, batched_ip=batched_ip)
     ids = [srv.submit(nl, p, 0.0) for nl, p in jobs]
     srv.run()
     for jid, (code, st, rec, pos) in zip(ids, alone):
