@@ -710,6 +710,12 @@ BC_KERNELS(t, RP_BLOCK)
 BC_KERNELS(xr, RP_BLOCK)
 BC_KERNELS(zero_x, RP_BLOCK)
 
+// done flags of ng grid runs (both axes) into one array, read back in one copy
+__global__ void __launch_bounds__(RP_BLOCK) k_bc_done_gather(const BcArgs* gs, int ng, int* out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < 2 * ng) out[k] = gs[k >> 1].ax[k & 1].st->done;
+}
+
 // ---------------------------------------------------------------------------
 // Small systems: the whole solve in one workgroup per axis
 // ---------------------------------------------------------------------------
@@ -1301,6 +1307,7 @@ struct RpIpBatch {
   DevBuf<SellArgs> d_sell;
   DevBuf<BcArgs> d_bc;
   DevBuf<int2> d_tab, d_htab;
+  DevBuf<int> d_done;  // the grid runs' done flags, gathered for one read
   SellArgs* h_sell = nullptr;
   BcArgs* h_bc = nullptr;
   int2* h_tab = nullptr;
@@ -1641,6 +1648,7 @@ int rp_ipr_solve(RpIpRun* const* runs, int k, hipStream_t st, RpIpBatch* bt) {
   if (int r = grow_pinned(c0, bt->h_bc, bt->d_bc, bt->cap_bc, ng)) return r;
   if (int r = grow_pinned(c0, bt->h_tab, bt->d_tab, bt->cap_tab, ntab + ng + netab)) return r;
   if (!bt->h_done) HIPCHK(c0, hipHostMalloc((void**)&bt->h_done, sizeof(int) * 2 * 4096, hipHostMallocDefault));
+  if (!bt->d_done.p) HIPCHK(c0, dalloc(bt->d_done, 2 * 4096));
   if (ng > 4096) return fail(c0, RP_ERR_ARG, "initial place: more than 4096 grid solves in one batch");
   size_t t = 0;
   for (int j = 0; j < ng; j++) {
@@ -1672,10 +1680,9 @@ int rp_ipr_solve(RpIpRun* const* runs, int k, hipStream_t st, RpIpBatch* bt) {
     done_iters += BC_CHUNK;
     hipLaunchKernelGGL(kb_bc_head, g1, dim3(1024), 0, st, gsd, htab);
     HIPCHK(c0, hipGetLastError());
-    for (int j = 0; j < ng; j++)
-      for (int ax = 0; ax < 2; ax++)
-        HIPCHK(c0, hipMemcpyAsync(&bt->h_done[2 * j + ax], &grid[j]->d_rec.p->st[ax].done, sizeof(int),
-                                  hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(k_bc_done_gather, dim3((2 * ng + RP_BLOCK - 1) / RP_BLOCK), blk, 0, st, gsd, ng,
+                       bt->d_done.p);
+    HIPCHK(c0, hipMemcpyAsync(bt->h_done, bt->d_done.p, sizeof(int) * 2 * ng, hipMemcpyDeviceToHost, st));
     HIPCHK(c0, hipStreamSynchronize(st));
     bool all = true;
     for (int j = 0; j < 2 * ng; j++) all = all && bt->h_done[j];
