@@ -1331,15 +1331,34 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
   int evals_before = hs->evals;
   // tile counts left by a run that stopped after its last counting slot
   HIPCHK(c, hipMemsetAsync(c->sortb.cnt, 0, sizeof(int) * RP_SORT_TILES, s));
+  // Batches after the first (sort refresh + K slots, the same launches every
+  // time) are captured once into a HIP graph and replayed, unless per-kernel
+  // events are recorded or the matrix-core transform (library GEMMs) is in use.
+  const bool use_graph = !c->timing && !c->mf() && getenv("RP_NO_GRAPH") == nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
   for (int b = 0; b < max_batches && hs->phase == 0; b++) {
-    // Spatial order of the scatter from the centres counted by the last slot
-    // of the previous batch (it ran: the state polled after it still runs).
-    if (sev) HIPCHK(c, hipEventRecord(sev[0], s));
-    if (b > 0) HIPCHK(c, rp_launch_sort_refresh(d, c->sortb, s));
-    if (sev) HIPCHK(c, hipEventRecord(sev[1], s));
-    for (int k = 0; k < K; k++)
-      HIPCHK(c, rp_launch_eval(d, s, 0, c->timing ? &kev[(size_t)k * EV] : nullptr, c->mf(),
-                               k == K - 1 ? &c->sortb : nullptr, &launched));
+    if (use_graph && b > 0) {
+      if (!gexec) {
+        HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        hipError_t e = rp_launch_sort_refresh(d, c->sortb, s);
+        for (int k = 0; k < K && e == hipSuccess; k++)
+          e = rp_launch_eval(d, s, 0, nullptr, nullptr, k == K - 1 ? &c->sortb : nullptr, &launched);
+        const hipError_t e2 = hipStreamEndCapture(s, &graph);
+        HIPCHK(c, e != hipSuccess ? e : e2);
+        HIPCHK(c, hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+      }
+      HIPCHK(c, hipGraphLaunch(gexec, s));
+    } else {
+      // Spatial order of the scatter from the centres counted by the last slot
+      // of the previous batch (it ran: the state polled after it still runs).
+      if (sev) HIPCHK(c, hipEventRecord(sev[0], s));
+      if (b > 0) HIPCHK(c, rp_launch_sort_refresh(d, c->sortb, s));
+      if (sev) HIPCHK(c, hipEventRecord(sev[1], s));
+      for (int k = 0; k < K; k++)
+        HIPCHK(c, rp_launch_eval(d, s, 0, c->timing ? &kev[(size_t)k * EV] : nullptr, c->mf(),
+                                 k == K - 1 ? &c->sortb : nullptr, &launched));
+    }
     HIPCHK(c, hipMemcpyAsync(hs, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipEventRecord(polled, s));
     HIPCHK(c, hipEventSynchronize(polled));
@@ -1363,6 +1382,8 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
     evals_before = hs->evals;
   }
   for (auto& e : kev) (void)hipEventDestroy(e);
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
   HIPCHK(c, hipEventRecord(e1, s));
   HIPCHK(c, hipEventSynchronize(e1));
   float ms = 0;
