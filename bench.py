@@ -396,7 +396,10 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tpath):
-        ent = json.load(open(tpath))["kernels"].get(ROCPROF_NAME.get(dom, "k_" + dom))
+        name = ROCPROF_NAME.get(dom, "k_" + dom)
+        kern = json.load(open(tpath))["kernels"]
+        # the kernel, or the template instance of it this workload launches
+        ent = kern.get(name) or next((v for k, v in kern.items() if k.startswith(name + "<")), None)
         traffic = ent["hbm_bytes_per_launch"] if ent else None
     code, iters, evals, ovf, hpwl, dev_s, fin = runs[-1]
     result = {

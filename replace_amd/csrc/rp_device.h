@@ -30,10 +30,10 @@
 
 #define RP_BLOCK 256
 #define RP_CTRL_BLOCK 1024  // controller workgroup (fixed-order reductions)
-#ifndef RP_SS_PER_THREAD
-#define RP_SS_PER_THREAD 1
-#endif
-#define RP_SS_CELLS (RP_BLOCK * RP_SS_PER_THREAD)  // cells per spatial-order scatter workgroup
+// spatial-order scatter: two cells per thread from this many cells on (half
+// the workgroups flush each dense bin), one below (enough workgroups to fill
+// the GPU)
+#define RP_SS_WIDE_CELLS (1 << 17)
 #define RP_MAX_FFT 2048
 #ifndef RP_WA_CAP
 #define RP_WA_CAP 512         // pins per wirelength workgroup (a larger net gets one of its own)

@@ -500,7 +500,7 @@ __device__ void step_body(const RpDev& d, int blk) {
   (void)eval_centre(d, st, c, d.dhalf[c]);
 }
 
-// Scatter only, over RP_SS_CELLS cells of the spatial order d.sord
+// Scatter only, over PER * RP_BLOCK cells of the spatial order d.sord
 // (cells sorted by the Morton code of their bin tile), so a workgroup's
 // cells share bins and nearly every update lands in the LDS table.  Any
 // permutation gives the same integer sums; the order only decides how many
@@ -513,24 +513,25 @@ __device__ void step_body(const RpDev& d, int blk) {
 // kept in sb.rank.  rp_launch_sort_refresh turns counts and ranks into the
 // new order.
 #define RP_TILE_SLOTS 256
+template <int PER>
 __device__ void sorted_scatter_body(const RpDev& d, int blk, const RpSortBufs& sb, int count) {
   __shared__ WinTable t;
   __shared__ int t_key[RP_TILE_SLOTS], t_cnt[RP_TILE_SLOTS], t_base[RP_TILE_SLOTS];
   const RpState* st = d.st;
   if (st->phase != 0) return;
-  int cell[RP_SS_PER_THREAD], slot[RP_SS_PER_THREAD], lrank[RP_SS_PER_THREAD];
+  int cell[PER], slot[PER], lrank[PER];
   // all loads before the first atomic (which could alias them), and before
   // the table clear and its barrier, which they overlap
-  int2 v_ic[RP_SS_PER_THREAD], v_h[RP_SS_PER_THREAD];
-  float v_sc[RP_SS_PER_THREAD];
-  uint8_t v_kd[RP_SS_PER_THREAD];
+  int2 v_ic[PER], v_h[PER];
+  float v_sc[PER];
+  uint8_t v_kd[PER];
 #pragma unroll
-  for (int r = 0; r < RP_SS_PER_THREAD; r++) {
-    int k = blk * RP_SS_CELLS + r * RP_BLOCK + threadIdx.x;
+  for (int r = 0; r < PER; r++) {
+    int k = blk * (PER * RP_BLOCK) + r * RP_BLOCK + threadIdx.x;
     cell[r] = k < d.nc ? d.sord[k] : -1;
   }
 #pragma unroll
-  for (int r = 0; r < RP_SS_PER_THREAD; r++) {
+  for (int r = 0; r < PER; r++) {
     int c = cell[r];
     if (c < 0) continue;
     v_ic[r] = d.ic[c];
@@ -545,7 +546,7 @@ __device__ void sorted_scatter_body(const RpDev& d, int blk, const RpSortBufs& s
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < RP_SS_PER_THREAD; r++) {
+  for (int r = 0; r < PER; r++) {
     int c = cell[r];
     if (c < 0) continue;
     int2 icc = v_ic[r];
@@ -574,7 +575,7 @@ __device__ void sorted_scatter_body(const RpDev& d, int blk, const RpSortBufs& s
       if (t_key[k] >= 0) t_base[k] = atomicAdd(&sb.cnt[t_key[k]], t_cnt[k]);
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < RP_SS_PER_THREAD; r++)
+    for (int r = 0; r < PER; r++)
       if (cell[r] >= 0 && slot[r] >= 0) sb.rank[cell[r]] = t_base[slot[r]] + lrank[r];
   }
 }
@@ -1617,18 +1618,29 @@ __global__ void __launch_bounds__(RP_BLOCK) k_step(RpDev d) { step_body(d, xcd_b
 // Density scatter (blocks [0, nss)) and pin centres (the rest) in one
 // launch: both read only the integer centres the step wrote.  nss is a
 // multiple of the XCD count, so both ranges keep the XCD-contiguous order.
+template <int PER>
 __global__ void __launch_bounds__(RP_BLOCK) k_scatter_pins(RpDev d, RpSortBufs sb, int count, int nss) {
   const int b = blockIdx.x;
   if (b < nss)
-    sorted_scatter_body(d, xcd_block(b, nss), sb, count);
+    sorted_scatter_body<PER>(d, xcd_block(b, nss), sb, count);
   else
     pinpos_body(d, xcd_block(b - nss, gridDim.x - nss));
 }
+static int pin_blocks(const RpDev& d) { return std::max(1, (d.np + RP_BLOCK - 1) / RP_BLOCK); }
+static int scatter_per_thread(const RpDev& d) { return d.nc >= RP_SS_WIDE_CELLS ? 2 : 1; }
 static int scatter_pins_blocks(const RpDev& d) {
-  int n = (d.nc + RP_SS_CELLS - 1) / RP_SS_CELLS;
+  const int cells = RP_BLOCK * scatter_per_thread(d);
+  int n = (d.nc + cells - 1) / cells;
   return (n + RP_NUM_XCD - 1) / RP_NUM_XCD * RP_NUM_XCD;
 }
-static int pin_blocks(const RpDev& d) { return std::max(1, (d.np + RP_BLOCK - 1) / RP_BLOCK); }
+static void launch_scatter_pins(const RpDev& d, hipStream_t s, const RpSortBufs& sb, int count) {
+  const int nss = scatter_pins_blocks(d);
+  const dim3 grid(nss + pin_blocks(d));
+  if (scatter_per_thread(d) == 2)
+    hipLaunchKernelGGL(k_scatter_pins<2>, grid, dim3(RP_BLOCK), 0, s, d, sb, count, nss);
+  else
+    hipLaunchKernelGGL(k_scatter_pins<1>, grid, dim3(RP_BLOCK), 0, s, d, sb, count, nss);
+}
 __global__ void __launch_bounds__(RP_BLOCK) k_pin_pos(RpDev d) { pinpos_body(d, xcd_block(blockIdx.x, gridDim.x)); }
 __global__ void __launch_bounds__(RP_BLOCK) k_binfin_rowdct(RpDev d, int from_rho) {
   binfin_body(d, blockIdx.x, from_rho);
@@ -1703,9 +1715,7 @@ hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_
   hipLaunchKernelGGL(k_step, dim3(nb_c), dim3(RP_BLOCK), 0, s, d);
   end(RP_K_STEP);
   beg(RP_K_SCATTER);
-  const int nss = scatter_pins_blocks(d);
-  hipLaunchKernelGGL(k_scatter_pins, dim3(nss + pin_blocks(d)), dim3(RP_BLOCK), 0, s, d, sb ? *sb : RpSortBufs{},
-                     sb ? 1 : 0, nss);
+  launch_scatter_pins(d, s, sb ? *sb : RpSortBufs{}, sb ? 1 : 0);
   end(RP_K_SCATTER);
   beg(RP_K_BINFIN_ROWDCT);
   hipLaunchKernelGGL(k_binfin_rowdct, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), rp_lds_binfin(d.nbx), s, d, 0);
@@ -1746,8 +1756,7 @@ hipError_t rp_launch_ctrl_init(const RpDev& d, hipStream_t s, int stage) {
 // Split stages for the stage-level test entry points.
 hipError_t rp_launch_scatter(const RpDev& d, hipStream_t s) {
   hipLaunchKernelGGL(k_step, dim3((d.nc + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, d);
-  const int nss = scatter_pins_blocks(d);
-  hipLaunchKernelGGL(k_scatter_pins, dim3(nss + pin_blocks(d)), dim3(RP_BLOCK), 0, s, d, RpSortBufs{}, 0, nss);
+  launch_scatter_pins(d, s, RpSortBufs{}, 0);
   return hipGetLastError();
 }
 
