@@ -436,6 +436,12 @@ extern "C" int rp_server_create(const rp_server_opts* opt, rp_server** out) {
     return sfail(*out, RP_ERR_NO_DEVICE, "no HIP device visible");
   }
   (void)hipGetDevice(&s->device);
+  // the library's code objects are loaded here, on one thread, before the
+  // set-up threads make their first launches concurrently
+  if (rp_kernels_load_code() != hipSuccess || rp_ip_load_code() != hipSuccess) {
+    *out = s.release();
+    return sfail(*out, RP_ERR_HIP, "loading the kernels' code objects failed");
+  }
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
     *out = s.release();
     return sfail(*out, RP_ERR_HIP, "hipStreamCreate failed");

@@ -10,6 +10,7 @@
 struct RpDev {
   int nc, nn, np, nbx, nby, bsx, bsy;
   int mfma;  // Poisson on matrix cores: the bin kernel only produces rho
+  int scatter_per;  // cells per thread of the spatial-order scatter (1 or 2)
   int core_lx, core_ly, core_ux, core_uy;
   float td;
   float bar;          // minWireLengthForceBar
@@ -111,3 +112,8 @@ int rp_sort_tile_shift(int nbx, int nby);
 // Finishes the counting sort started by the last counting step launch:
 // offsets, then d.sord.
 hipError_t rp_launch_sort_refresh(const RpDev& d, const RpSortBufs& b, hipStream_t s);
+
+// Load the code objects of rp_kernels.hip / rp_ip.hip on the current device
+// (before many host threads start launching their kernels).
+hipError_t rp_kernels_load_code();
+hipError_t rp_ip_load_code();

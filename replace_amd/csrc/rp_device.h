@@ -32,7 +32,8 @@
 #define RP_CTRL_BLOCK 1024  // controller workgroup (fixed-order reductions)
 // spatial-order scatter: two cells per thread from this many cells on (half
 // the workgroups flush each dense bin), one below (enough workgroups to fill
-// the GPU)
+// the GPU).  RP_SCATTER_PER=1|2 in the environment overrides the choice for
+// contexts set up afterwards (both shapes give identical integer sums).
 #define RP_SS_WIDE_CELLS (1 << 17)
 #define RP_MAX_FFT 2048
 #ifndef RP_WA_CAP

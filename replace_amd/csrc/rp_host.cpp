@@ -939,6 +939,11 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   d.core_ux = c->core_ux;
   d.core_uy = c->core_uy;
   d.td = c->td;
+  d.scatter_per = ng >= RP_SS_WIDE_CELLS ? 2 : 1;
+  if (const char* e = getenv("RP_SCATTER_PER")) {
+    const int v = atoi(e);
+    if (v == 1 || v == 2) d.scatter_per = v;
+  }
   d.bar = p.min_wire_length_force_bar;
   d.min_precond = p.min_preconditioner;
   d.dhalf = c->d_dhalf.p;
@@ -1305,7 +1310,23 @@ int rp_nesterov_collect(rp_ctx* c, double seconds) {
 extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
   if (!c) return RP_ERR_ARG;
   hipStream_t s = c->stream;
-  hipEvent_t e0, e1;
+  // events and the captured graph, released on every exit path
+  struct Res {
+    hipEvent_t e0 = nullptr, e1 = nullptr, polled = nullptr;
+    std::vector<hipEvent_t> kev;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    ~Res() {
+      for (hipEvent_t e : kev)
+        if (e) (void)hipEventDestroy(e);
+      if (gexec) (void)hipGraphExecDestroy(gexec);
+      if (graph) (void)hipGraphDestroy(graph);
+      for (hipEvent_t e : {e0, e1, polled})
+        if (e) (void)hipEventDestroy(e);
+    }
+  } res;
+  hipEvent_t& e0 = res.e0;
+  hipEvent_t& e1 = res.e1;
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
   HIPCHK(c, hipEventRecord(e0, s));
@@ -1315,13 +1336,13 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
   // main loop: queue evaluation slots, poll the state between batches
   const int K = slots_per_poll(c);
   RpState* hs = c->h_state;
-  hipEvent_t polled;
+  hipEvent_t& polled = res.polled;
   HIPCHK(c, hipEventCreateWithFlags(&polled, hipEventDisableTiming));
   int max_batches = (c->prm.nesterov_place_max_iter * std::max(1, c->prm.max_back_track)) / K + 4;
-  std::vector<hipEvent_t> kev;
+  std::vector<hipEvent_t>& kev = res.kev;
   constexpr int EV = 2 * RP_NUM_EVAL_KERNELS;  // events per slot
   if (c->timing) {
-    kev.resize((size_t)K * EV + 2);  // + the sort refresh bracket
+    kev.assign((size_t)K * EV + 2, nullptr);  // + the sort refresh bracket
     for (auto& e : kev) HIPCHK(c, hipEventCreate(&e));
   }
   hipEvent_t* sev = c->timing ? &kev[(size_t)K * EV] : nullptr;
@@ -1335,8 +1356,8 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
   // time) are captured once into a HIP graph and replayed, unless per-kernel
   // events are recorded or the matrix-core transform (library GEMMs) is in use.
   const bool use_graph = !c->timing && !c->mf() && getenv("RP_NO_GRAPH") == nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  hipGraph_t& graph = res.graph;
+  hipGraphExec_t& gexec = res.gexec;
   for (int b = 0; b < max_batches && hs->phase == 0; b++) {
     if (use_graph && b > 0) {
       if (!gexec) {
@@ -1381,16 +1402,10 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
     }
     evals_before = hs->evals;
   }
-  for (auto& e : kev) (void)hipEventDestroy(e);
-  if (gexec) (void)hipGraphExecDestroy(gexec);
-  if (graph) (void)hipGraphDestroy(graph);
   HIPCHK(c, hipEventRecord(e1, s));
   HIPCHK(c, hipEventSynchronize(e1));
   float ms = 0;
   HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipEventDestroy(polled);
   int code = rp_nesterov_collect(c, ms * 1e-3);
   if (st) *st = c->last_stats;
   return code;

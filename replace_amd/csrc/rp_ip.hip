@@ -364,6 +364,20 @@ int rp_ip_assemble_b(rp_ctx* c, RpIpAsm& a, hipStream_t st) {
   HIPCHK(c, ensure(a.flag, mmax));
   HIPCHK(c, ensure(a.seg, mmax));
   HIPCHK(c, ensure(a.srow, mmax));
+  // the sort / scan temporary storage of both axes, sized before anything is
+  // queued: growing it between the axes would free a buffer that the first
+  // axis' queued sorts still use
+  size_t tneed = 0;
+  for (int ax = 0; ax < 2; ax++) {
+    size_t t1 = 0, t2 = 0, t3 = 0;
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, t1, a.tk[ax].p, a.tk2[ax].p, a.tv[ax].p, a.tv2[ax].p,
+                                                 std::max(1, mt[ax]), 0, 2 * a.cb, st));
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, t2, a.rk[ax].p, a.rk2[ax].p, a.rv[ax].p, a.rv2[ax].p,
+                                                 std::max(1, mr[ax]), 0, a.cb, st));
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, t3, a.flag.p, a.seg.p, std::max(1, mt[ax]), st));
+    tneed = std::max(tneed, std::max(t1, std::max(t2, t3)));
+  }
+  HIPCHK(c, ensure(a.temp, tneed));
   IpOut o{a.off.p, {a.tk[0].p, a.tk[1].p}, {a.tv[0].p, a.tv[1].p}, {a.rk[0].p, a.rk[1].p},
           {a.rv[0].p, a.rv[1].p}, a.cb};
   hipLaunchKernelGGL(k_ip_emit, grid_of(np), dim3(RP_BLOCK), 0, st, v, (const int*)a.pin_net.p,
@@ -372,13 +386,6 @@ int rp_ip_assemble_b(rp_ctx* c, RpIpAsm& a, hipStream_t st) {
   for (int ax = 0; ax < 2; ax++) {
     const int m = mt[ax], r = mr[ax];
     // stable sorts: equal keys keep triplet order
-    size_t t1 = 0, t2 = 0, t3 = 0;
-    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, t1, a.tk[ax].p, a.tk2[ax].p, a.tv[ax].p, a.tv2[ax].p,
-                                                 std::max(1, m), 0, 2 * a.cb, st));
-    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, t2, a.rk[ax].p, a.rk2[ax].p, a.rv[ax].p, a.rv2[ax].p,
-                                                 std::max(1, r), 0, a.cb, st));
-    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, t3, a.flag.p, a.seg.p, std::max(1, m), st));
-    HIPCHK(c, ensure(a.temp, std::max(t1, std::max(t2, t3))));
     if (m > 0) {
       size_t tb = a.temp.n;
       HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(a.temp.p, tb, a.tk[ax].p, a.tk2[ax].p, a.tv[ax].p,
@@ -1780,4 +1787,13 @@ int rp_ip_device_loop(rp_ctx* c, bool host_asm, double* t_asm, double* t_solve) 
     if (stop) break;
   }
   return rp_ipr_end(R.get());
+}
+
+// Loads this file's code object (its kernels and the hipCUB ones it
+// instantiates) on the current device.  The placement service calls it once
+// before it starts its set-up threads, so the first launches of many threads
+// at once do not each trigger the load.
+hipError_t rp_ip_load_code() {
+  hipFuncAttributes fa;
+  return hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_ip_segsum));
 }

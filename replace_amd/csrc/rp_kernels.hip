@@ -1627,7 +1627,7 @@ __global__ void __launch_bounds__(RP_BLOCK) k_scatter_pins(RpDev d, RpSortBufs s
     pinpos_body(d, xcd_block(b - nss, gridDim.x - nss));
 }
 static int pin_blocks(const RpDev& d) { return std::max(1, (d.np + RP_BLOCK - 1) / RP_BLOCK); }
-static int scatter_per_thread(const RpDev& d) { return d.nc >= RP_SS_WIDE_CELLS ? 2 : 1; }
+static int scatter_per_thread(const RpDev& d) { return d.scatter_per == 2 ? 2 : 1; }
 static int scatter_pins_blocks(const RpDev& d) {
   const int cells = RP_BLOCK * scatter_per_thread(d);
   int n = (d.nc + cells - 1) / cells;
@@ -1891,4 +1891,10 @@ hipError_t rp_launch_eval_batch(const RpBatchLaunch& b, hipStream_t s) {
     hipLaunchKernelGGL(kb_control_main, dim3(n[RP_K_CONTROL]), dim3(RP_CTRL_BLOCK), 0, s, b.devs,
                        b.tab[RP_K_CONTROL]);
   return hipGetLastError();
+}
+
+// Loads this file's code object on the current device (see rp_ip_load_code).
+hipError_t rp_kernels_load_code() {
+  hipFuncAttributes fa;
+  return hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_grad));
 }

@@ -17,8 +17,11 @@ container, where the reference tree exists; the fixtures are data only).
                       reference's golden result DEFs (test/<design>.defok,
                       written by write_def after global_placement), in the
                       component order of test/<design>.def
+  medium01.npz        the netlist of the reference's larger regression
+                      design test/medium01.def (17,782 instances, a
+                      2,251-pin net); it has no golden log or result DEF
 
-Usage: python tests/golden/make_fixtures.py [/root/reference] [--defok-only]
+Usage: python tests/golden/make_fixtures.py [/root/reference] [--defok-only] [--extra-only]
 """
 import json
 import os
@@ -100,11 +103,24 @@ def write_defok(test, lef):
     np.savez_compressed(os.path.join(HERE, "defok.npz"), **out)
 
 
+# designs stored as netlists only (no golden log / result DEF in the reference)
+EXTRA_NETLISTS = ["medium01"]
+
+
+def write_extra(test, lef):
+    for name in EXTRA_NETLISTS:
+        nl = read_def(os.path.join(test, name + ".def"), lef, name=name)
+        nl.save(os.path.join(HERE, name + ".npz"))
+
+
 def main():
     args = [a for a in sys.argv[1:] if not a.startswith("--")]
     ref = args[0] if args else "/root/reference"
     test = os.path.join(ref, "test")
     lef = read_lef(os.path.join(test, "nangate45.lef"))
+    write_extra(test, lef)
+    if "--extra-only" in sys.argv:
+        return
     write_defok(test, lef)
     if "--defok-only" in sys.argv:
         return

@@ -47,3 +47,106 @@ def rel_err(a, b):
     b = np.asarray(b, np.float64)
     d = np.abs(b).max()
     return float(np.abs(a - b).max() / (d if d > 0 else 1.0))
+
+
+def clamped_random_centres(gc, info, seed):
+    """Random density centres inside the core, clamped so every density box
+    stays in it (the state a Nesterov step leaves after its layout clamp)."""
+    rng = np.random.default_rng(seed)
+    cx = rng.uniform(info.core_lx, info.core_ux, info.num_gcells).astype(np.float32)
+    cy = rng.uniform(info.core_ly, info.core_uy, info.num_gcells).astype(np.float32)
+    cx = np.clip(cx, info.core_lx + gc["dhalf_x"], info.core_ux - gc["dhalf_x"]).astype(np.float32)
+    cy = np.clip(cy, info.core_ly + gc["dhalf_y"], info.core_uy - gc["dhalf_y"]).astype(np.float32)
+    return cx, cy
+
+
+def bin_areas(info):
+    """Bin areas of BinGrid::initBins (src/nesterovBase.cpp:674-711): regular
+    bins of binSize, the last column / row truncated at the core edge
+    (y-major, j * Nx + i)."""
+    nx, ny = info.bin_cnt_x, info.bin_cnt_y
+    lx = info.core_lx + np.arange(nx, dtype=np.int64) * info.bin_size_x
+    ly = info.core_ly + np.arange(ny, dtype=np.int64) * info.bin_size_y
+    w = np.minimum(lx + info.bin_size_x, info.core_ux) - lx
+    h = np.minimum(ly + info.bin_size_y, info.core_uy) - ly
+    return (h[:, None] * w[None, :]).reshape(-1)
+
+
+def overflow_exact(bin_inst, nonplace, info):
+    """Exact sum of the per-bin overflow terms max(0, float(inst) +
+    float(nonPlace) - float(binArea * td)) (src/nesterovBase.cpp:812-816),
+    each term in float as the reference computes it, truncated to int64."""
+    at = (bin_areas(info).astype(np.float32) * np.float32(info.target_density)).astype(np.float32)
+    t = (bin_inst.astype(np.float32) + nonplace.astype(np.float32)).astype(np.float32) - at
+    t = np.maximum(np.float32(0), t.astype(np.float32))
+    return int(np.sum(t.astype(np.float64)))
+
+
+def check_eval(a, b, lam, info=None, nonplace=None):
+    """One gradient evaluation of the HIP path (a) against the oracle (b):
+    integer work and the float sums kept in the reference's order bit-exact,
+    the Poisson solve and what follows it within 1e-4 relative, the gradient
+    norms (FP64 here, sequential float in the oracle) within the float
+    summation bound."""
+    assert np.array_equal(a["bin_inst"], b["bin_inst"])
+    assert np.array_equal(a["bin_filler"], b["bin_filler"])
+    assert np.array_equal(a["density"], b["density"])
+    # overflow area (src/nesterovBase.cpp:800-816): the reference adds each
+    # bin's float term to an int64 through float (int64 += float), so its
+    # running sum carries up to one rounding of the running value per bin:
+    # within nb (u S + 1) of the exact sum, u = 2^-24.  The HIP path sums
+    # the same float terms in FP64, so it is checked against their exact sum
+    # (from its own bins, which equal the oracle's) and against the oracle's
+    # value within that bound.
+    nb = a["bin_inst"].size
+    ov_b = int(b["overflow_area"][0])
+    assert abs(int(a["overflow_area"][0]) - ov_b) <= max(64, nb * (2.0 ** -24 * abs(ov_b) + 1)), (
+        a["overflow_area"], b["overflow_area"])
+    if info is not None:
+        exact = overflow_exact(a["bin_inst"], nonplace, info)
+        assert abs(int(a["overflow_area"][0]) - exact) <= 1 + 1e-12 * exact, (a["overflow_area"], exact)
+    assert np.array_equal(a["net_box"], b["net_box"])
+    assert int(a["hpwl"][0]) == int(b["hpwl"][0])
+    assert np.array_equal(a["net_sums"], b["net_sums"])
+    assert np.array_equal(a["wl_grad"], b["wl_grad"])
+    for k in ("phi", "ex", "ey", "dens_grad"):
+        assert rel_err(a[k], b[k]) < 1e-4, (k, rel_err(a[k], b[k]))
+    if lam > 0:
+        assert rel_err(a["sum_grad"], b["sum_grad"]) < 1e-4
+    else:
+        assert np.array_equal(a["sum_grad"], b["sum_grad"])
+    # sum |wl| and sum |dens| (src/nesterovPlace.cpp:225-235): the oracle sums
+    # the n cells' nonnegative terms sequentially in float, as the reference
+    # does, the HIP path in FP64, so the two differ by at most the sequential
+    # summation bound (n - 1) u / (1 - (n - 1) u), u = 2^-24 (Higham, Accuracy
+    # and Stability of Numerical Algorithms, 2nd ed., section 4.2), besides the
+    # 1e-4 of the field for the density term
+    n = a["wl_grad"].size // 2
+    gam = (n - 1) * 2.0 ** -24 / (1 - (n - 1) * 2.0 ** -24)
+    assert abs(a["sums"][0] - b["sums"][0]) <= max(1e-5, gam) * abs(b["sums"][0]), (a["sums"], b["sums"])
+    assert abs(a["sums"][1] - b["sums"][1]) <= (1e-4 + gam) * abs(b["sums"][1]), (a["sums"], b["sums"])
+
+
+def hpwl_at(rec, ovf):
+    """HPWL of the first record at or below overflow ovf (else the last)."""
+    for r in rec:
+        if r[0] <= ovf:
+            return r[1]
+    return rec[-1][1]
+
+
+def assert_tracks_trace(name, rec, iterations, final_overflow, ip):
+    """A run's records against the reference's golden [NesterovSolve] lines:
+    the first 50 iterations close (2e-3 overflow / 0.2 % HPWL; 5e-3 / 0.5 %
+    for runs seeded by the parity-unpinned BiCGSTAB iterates), the finish
+    within +-12 iterations, HPWL within 0.5 % at equal overflow."""
+    t = traces()[name]
+    for it, ovf, hpwl in t["nesterov"]:
+        if it > 50 or it - 1 >= len(rec):
+            break
+        assert abs(rec[it - 1][0] - float(ovf)) < (5e-3 if ip else 2e-3)
+        assert abs(rec[it - 1][1] - hpwl) <= (5e-3 if ip else 2e-3) * hpwl
+    last_it, last_o, last_h = t["nesterov"][-1]
+    assert abs(iterations - last_it) <= 12
+    assert abs(hpwl_at(rec, float(last_o)) - last_h) <= 0.005 * last_h
+    assert final_overflow <= 0.1

@@ -5,7 +5,7 @@ the HBM budget."""
 import numpy as np
 import pytest
 
-from helpers import design
+from helpers import assert_tracks_trace, design
 from replace_amd import netgen
 from replace_amd.capi import Placer, Server
 
@@ -134,6 +134,55 @@ def test_batched_initial_place_identical_to_alone(batched_ip):
     for jid, (code, st, rec, pos) in zip(ids, alone):
         r = srv.result(jid)
         assert r.done == 1 and r.code == code
+        assert srv.records(jid) == rec
+        lx, ly = srv.positions(jid)
+        assert np.array_equal(lx, pos[0]) and np.array_equal(ly, pos[1])
+
+
+def test_batched_reference_designs_track_golden_traces():
+    """Reference designs placed by the service, checked against the
+    reference's own golden logs (not only against the plain GPU path)."""
+    names = ("simple01", "ar02", "incremental01", "simple02", "core01")
+    srv = Server(steps_per_poll=4)
+    ids = [srv.submit(*design(n), 0.0) for n in names]
+    srv.run()
+    for name, jid in zip(names, ids):
+        r = srv.result(jid)
+        assert r.done == 1 and r.code == 0 and r.stats.finished == 1
+        assert_tracks_trace(name, srv.records(jid), r.stats.iterations, r.stats.final_overflow, False)
+
+
+def test_batch_workload_sample_identical_to_alone():
+    """Sixteen instances of the 256-instance batched workload (500-50k cells,
+    own target densities and bin grids, initial place on), every one
+    bit-identical in the batch and alone."""
+    idx = [0, 1, 3, 5, 13, 17, 18, 31, 47, 64, 90, 111, 128, 150, 200, 255]
+    jobs = [netgen.batch_instance(i) for i in idx]
+    alone = [_alone(nl, p) for nl, p in jobs]
+    srv = Server()
+    ids = [srv.submit(nl, p, 0.0) for nl, p in jobs]
+    srv.run()
+    for jid, (code, st, rec, pos) in zip(ids, alone):
+        r = srv.result(jid)
+        assert r.done == 1 and r.code == code
+        assert srv.records(jid) == rec
+        lx, ly = srv.positions(jid)
+        assert np.array_equal(lx, pos[0]) and np.array_equal(ly, pos[1])
+
+
+def test_served_stream_with_a_large_job_identical_to_alone():
+    """A served stream (staggered arrivals) holding a large job (200,915
+    cells, 512 x 512 bins) among instances of the batched distribution: every
+    job bit-identical to its plain run, and no job starts before it arrives."""
+    jobs = [netgen.large_instance(4)] + [netgen.batch_instance(i) for i in (2, 7, 19, 40, 77, 133)]
+    arrivals = [0.0, 0.05, 0.1, 0.15, 0.3, 0.45, 0.6]
+    alone = [_alone(nl, p) for nl, p in jobs]
+    srv = Server(steps_per_poll=8)
+    ids = [srv.submit(nl, p, a) for (nl, p), a in zip(jobs, arrivals)]
+    srv.run()
+    for jid, (code, st, rec, pos) in zip(ids, alone):
+        r = srv.result(jid)
+        assert r.done == 1 and r.code == code and r.start_s >= r.arrival_s
         assert srv.records(jid) == rec
         lx, ly = srv.positions(jid)
         assert np.array_equal(lx, pos[0]) and np.array_equal(ly, pos[1])
