@@ -203,7 +203,7 @@ def bench_batch(args):
         "value": round(n / wall, 3), "unit": "placements/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{n} instances, cells log-uniform 500-50k (sum {cells}), td U[0.6,0.9], "
-                               "bins clamp(nextPow2(sqrt(Nc)),32,256)",
+                               "bins clamp(nextPow2(ceil(sqrt(Nc)))/2,32,256)",
                    "steps_per_poll": args.steps_per_poll},
         "wall_s": round(wall, 3),
         "plain_back_to_back": {"value": round(n / plain_s, 3), "unit": "placements/s", "wall_s": round(plain_s, 3),
@@ -244,7 +244,7 @@ def bench_serve(args):
         "value": round(n / makespan, 3), "unit": "placements/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{n} jobs, Poisson rate {args.rate}/s (seed 7), 95% batched distribution, "
-                               "5% large (200k-1M cells, 512/1024 bins)",
+                               "5% large (200k-1M cells, 512x512 bins)",
                    "large_jobs": sum(1 for _, k, _ in stream if k == "large"),
                    "steps_per_poll": args.steps_per_poll},
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),

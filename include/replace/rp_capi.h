@@ -145,6 +145,10 @@ typedef struct rp_nesterov_stats {
   float final_overflow;
   float final_step_length;
   float final_density_penalty;
+  int32_t diverged;       /* 1 when the loop stopped on divergence: code 2, 3, 4,
+                             or 0 for a NaN / inf gradient sum in the loop
+                             (the reference's "[ERROR]  (REPL-0)" exit,
+                             src/nesterovPlace.cpp:371-374, 466-471) */
   int64_t final_hpwl;     /* pin-centre HPWL of the last accepted iterate */
   double seconds;         /* device time of the loop, init included */
 } rp_nesterov_stats;
@@ -172,6 +176,54 @@ int rp_initial_place(rp_ctx* ctx, rp_ip_stats* stats);
  * Returns 0 or REPL code 1..4; positions are updated in every case, as the
  * reference calls updateDb before exiting (src/nesterovPlace.cpp:465-471). */
 int rp_nesterov_place(rp_ctx* ctx, rp_nesterov_stats* stats);
+
+/* ------------------------------------------------------------------------
+ * The Nesterov loop in pieces: init, a bounded number of outer iterations,
+ * and the loop state between two iterations.  rp_nesterov_place is
+ * rp_nesterov_init followed by rp_nesterov_iterate(max_iter) (with its
+ * positions written back).  Getting and setting the state lets a run be
+ * checkpointed and resumed, and a single iteration be checked from a given
+ * state (teacher forcing against the oracle).
+ * ---------------------------------------------------------------------- */
+
+/* NesterovPlace's state between two outer iterations (the members of
+ * src/nesterovPlace.h:76-127 the loop reads, and the loop locals of
+ * doNesterovPlace, src/nesterovPlace.cpp:300-313), per-gcell arrays in the
+ * reference's gcell order with x, y interleaved (2 * num_gcells floats).
+ * Array pointers may be NULL (not read / not written). */
+typedef struct rp_nesterov_state {
+  float* cur_slp;          /* curSLPCoordi_ */
+  float* cur_slp_grad;     /* curSLPSumGrads_ (preconditioned sum gradient) */
+  float* prev_slp;         /* prevSLPCoordi_ */
+  float* prev_slp_grad;    /* prevSLPSumGrads_ */
+  float* cur_coord;        /* curCoordi_ */
+  float step_length;       /* stepLength_ */
+  float density_penalty;   /* densityPenalty_ */
+  float wl_coef_x, wl_coef_y;  /* wireLengthCoefX_/Y_ */
+  float base_wl_coef;      /* baseWireLengthCoef_ */
+  float cur_a;             /* curA: a_k of the next iteration's prevA */
+  float sum_overflow;      /* sumOverflow_ */
+  float min_sum_overflow;  /* loop local minSumOverflow */
+  float hpwl_with_min_sum_overflow;  /* loop local hpwlWithMinSumOverflow */
+  float max_phi_coef;      /* npVars_.maxPhiCoef (scaled once by 0.99) */
+  int32_t is_max_phi_coef_changed;   /* loop local */
+  int32_t iter;            /* outer iterations done (the next one is iter + 1) */
+  int64_t prev_hpwl;       /* prevHpwl_ */
+} rp_nesterov_state;
+
+/* NesterovPlace::init (src/nesterovPlace.cpp:64-175): the two lambda-0
+ * evaluations, initial penalty and step length.  Returns 0, 2 (diverged in
+ * init) or < 0. */
+int rp_nesterov_init(rp_ctx* ctx);
+/* Up to n_iters more outer iterations of doNesterovPlace
+ * (src/nesterovPlace.cpp:315-463) from the current state, stopping early on
+ * the overflow target, divergence or nesterov_place_max_iter; positions are
+ * updated (updateDb).  stats covers the run since rp_nesterov_init. */
+int rp_nesterov_iterate(rp_ctx* ctx, int32_t n_iters, rp_nesterov_stats* stats);
+int rp_nesterov_get_state(rp_ctx* ctx, rp_nesterov_state* state);
+/* Replaces the loop state (valid after rp_nesterov_init); the next
+ * rp_nesterov_iterate starts iteration state->iter + 1 from it. */
+int rp_nesterov_set_state(rp_ctx* ctx, const rp_nesterov_state* state);
 
 /* [InitialPlace] lines of the last rp_initial_place call: CG error and the
  * inst-bbox HPWL printed per iteration (src/initialPlace.cpp:88-90). */

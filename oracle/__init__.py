@@ -12,7 +12,8 @@ from typing import Optional
 
 import numpy as np
 
-from replace_amd.netlist import (Netlist, RpEvalOut, RpIterRecord, RpNbInfo, RpNesterovStats,
+from replace_amd.netlist import (Netlist, RpEvalOut, RpIterRecord, RpNbInfo, RpNesterovState, RpNesterovStats,
+                                 nesterov_state_buffers, nesterov_state_values,
                                  RpParams)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -58,6 +59,8 @@ def lib() -> ctypes.CDLL:
         L.ro_nesterov_place.argtypes = [vp, ctypes.POINTER(RpNesterovStats)]
         L.ro_nesterov_init.argtypes = [vp]
         L.ro_nesterov_iterate.argtypes = [vp, ctypes.c_int, ctypes.POINTER(RpNesterovStats)]
+        L.ro_get_state.argtypes = [vp, ctypes.POINTER(RpNesterovState)]
+        L.ro_set_state.argtypes = [vp, ctypes.POINTER(RpNesterovState)]
         L.ro_get_iter_records.argtypes = [vp, ctypes.POINTER(RpIterRecord), ctypes.c_int32,
                                           ctypes.POINTER(ctypes.c_int32)]
         L.ro_get_positions.argtypes = [vp, vp, vp]
@@ -218,6 +221,19 @@ class Oracle:
         st = RpNesterovStats()
         code = lib().ro_nesterov_iterate(self.h, n, ctypes.byref(st))
         return code, st
+
+    def get_state(self) -> dict:
+        st, bufs = nesterov_state_buffers(self.info.num_gcells)
+        r = lib().ro_get_state(self.h, ctypes.byref(st))
+        if r:
+            raise RuntimeError(f"oracle get_state failed {r}")
+        return nesterov_state_values(st, bufs)
+
+    def set_state(self, values: dict) -> None:
+        st, _bufs = nesterov_state_buffers(self.info.num_gcells, values)
+        r = lib().ro_set_state(self.h, ctypes.byref(st))
+        if r:
+            raise RuntimeError(f"oracle set_state failed {r}")
 
     def records(self):
         cnt = ctypes.c_int32(0)

@@ -1767,6 +1767,7 @@ int ro_nesterov_iterate(ro_ctx* c, int n_iters, rp_nesterov_stats* st) {
     st->final_overflow = c->sumOverflow;
     st->final_step_length = c->stepLength;
     st->final_density_penalty = c->densityPenalty;
+    st->diverged = c->isDiverged;
     st->final_hpwl = c->prevHpwl;
     st->seconds = 0;
   }
@@ -1789,12 +1790,71 @@ int ro_nesterov_place(ro_ctx* c, rp_nesterov_stats* st) {
   if (c->isDiverged) {
     if (st) memset(st, 0, sizeof(*st));
     if (st) st->code = 2;
+    if (st) st->diverged = 1;
     return 2;
   }
   r = ro_nesterov_iterate(c, c->p.nesterov_place_max_iter, st);
   update_db(c);
   /* a NaN divergence breaks the loop with code 0 (src/nesterovPlace.cpp:371-374, 411-413) */
   return c->code;
+}
+
+/* The loop state between two outer iterations (members of NesterovPlace,
+ * src/nesterovPlace.h:76-127, and the doNesterovPlace locals kept across
+ * ro_nesterov_iterate calls), per gcell in the reference's gcell order. */
+static void copy_xy(float* dst, const float* src, int ng) {
+  if (dst && src) memcpy(dst, src, sizeof(float) * 2 * (size_t)ng);
+}
+
+int ro_get_state(ro_ctx* c, rp_nesterov_state* s) {
+  if (!c->np_ready) return RP_ERR_STATE;
+  int ng = c->n_g;
+  copy_xy(s->cur_slp, c->curSLP, ng);
+  copy_xy(s->cur_slp_grad, c->curSLPSum, ng);
+  copy_xy(s->prev_slp, c->prevSLP, ng);
+  copy_xy(s->prev_slp_grad, c->prevSLPSum, ng);
+  copy_xy(s->cur_coord, c->curCo, ng);
+  s->step_length = c->stepLength;
+  s->density_penalty = c->densityPenalty;
+  s->wl_coef_x = c->wlCoefX;
+  s->wl_coef_y = c->wlCoefY;
+  s->base_wl_coef = c->baseWLCoef;
+  s->cur_a = c->curA;
+  s->sum_overflow = c->sumOverflow;
+  s->min_sum_overflow = c->minSumOverflow;
+  s->hpwl_with_min_sum_overflow = c->hpwlWithMinSumOverflow;
+  s->max_phi_coef = c->maxPhiCoef;
+  s->is_max_phi_coef_changed = c->isMaxPhiCoefChanged;
+  s->iter = c->iter;
+  s->prev_hpwl = c->prevHpwl;
+  return 0;
+}
+
+int ro_set_state(ro_ctx* c, const rp_nesterov_state* s) {
+  if (!c->np_ready) return RP_ERR_STATE;
+  int ng = c->n_g;
+  copy_xy(c->curSLP, s->cur_slp, ng);
+  copy_xy(c->curSLPSum, s->cur_slp_grad, ng);
+  copy_xy(c->prevSLP, s->prev_slp, ng);
+  copy_xy(c->prevSLPSum, s->prev_slp_grad, ng);
+  copy_xy(c->curCo, s->cur_coord, ng);
+  c->stepLength = s->step_length;
+  c->densityPenalty = s->density_penalty;
+  c->wlCoefX = s->wl_coef_x;
+  c->wlCoefY = s->wl_coef_y;
+  c->baseWLCoef = s->base_wl_coef;
+  c->curA = s->cur_a;
+  c->sumOverflow = s->sum_overflow;
+  c->minSumOverflow = s->min_sum_overflow;
+  c->hpwlWithMinSumOverflow = s->hpwl_with_min_sum_overflow;
+  c->maxPhiCoef = s->max_phi_coef;
+  c->isMaxPhiCoefChanged = s->is_max_phi_coef_changed;
+  c->iter = s->iter;
+  c->prevHpwl = s->prev_hpwl;
+  c->finished = 0;
+  c->isDiverged = 0;
+  c->code = 0;
+  return 0;
 }
 
 int ro_get_iter_records(ro_ctx* c, rp_iter_record* out, int32_t cap, int32_t* count) {

@@ -70,6 +70,9 @@ int ro_nesterov_place(ro_ctx* c, rp_nesterov_stats* st);
 int ro_nesterov_init(ro_ctx* c);
 int ro_nesterov_iterate(ro_ctx* c, int n_iters, rp_nesterov_stats* st);
 int ro_get_iter_records(ro_ctx* c, rp_iter_record* out, int32_t cap, int32_t* count);
+/* Loop state between two outer iterations (rp_nesterov_state layout). */
+int ro_get_state(ro_ctx* c, rp_nesterov_state* s);
+int ro_set_state(ro_ctx* c, const rp_nesterov_state* s);
 int ro_get_positions(ro_ctx* c, int32_t* lx, int32_t* ly);
 int ro_set_positions(ro_ctx* c, const int32_t* lx, const int32_t* ly);
 int64_t ro_inst_hpwl(ro_ctx* c);

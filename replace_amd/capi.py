@@ -12,8 +12,9 @@ from typing import Optional
 
 import numpy as np
 
-from .netlist import (Netlist, RpEvalOut, RpIpStats, RpIterRecord, RpJobResult, RpNbInfo,
-                      RpNesterovStats, RpParams, RpServerOpts, RpServerStats, RpTileChoice, params_default)
+from .netlist import (Netlist, RpEvalOut, RpIpStats, RpIterRecord, RpJobResult, RpNbInfo, RpNesterovState,
+                      RpNesterovStats, RpParams, RpServerOpts, RpServerStats, RpTileChoice, nesterov_state_buffers,
+                      nesterov_state_values, params_default)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libreplace_amd.so")
@@ -39,6 +40,10 @@ def lib() -> ctypes.CDLL:
         L.rp_create.argtypes = [vp, ctypes.POINTER(RpParams), ctypes.POINTER(vp)]
         L.rp_initial_place.argtypes = [vp, ctypes.POINTER(RpIpStats)]
         L.rp_nesterov_place.argtypes = [vp, ctypes.POINTER(RpNesterovStats)]
+        L.rp_nesterov_init.argtypes = [vp]
+        L.rp_nesterov_iterate.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(RpNesterovStats)]
+        L.rp_nesterov_get_state.argtypes = [vp, ctypes.POINTER(RpNesterovState)]
+        L.rp_nesterov_set_state.argtypes = [vp, ctypes.POINTER(RpNesterovState)]
         L.rp_get_iter_records.argtypes = [vp, ctypes.POINTER(RpIterRecord), ctypes.c_int32,
                                           ctypes.POINTER(ctypes.c_int32)]
         L.rp_get_positions.argtypes = [vp, vp, vp]
@@ -181,6 +186,24 @@ class Placer:
         st = RpNesterovStats()
         code = self._check(lib().rp_nesterov_place(self.h, ctypes.byref(st)))
         return code, st
+
+    def nesterov_init(self) -> int:
+        return self._check(lib().rp_nesterov_init(self.h))
+
+    def nesterov_iterate(self, n: int):
+        st = RpNesterovStats()
+        code = self._check(lib().rp_nesterov_iterate(self.h, int(n), ctypes.byref(st)))
+        return code, st
+
+    def get_state(self) -> dict:
+        """The loop state between two iterations (rp_nesterov_state) as a dict."""
+        st, bufs = nesterov_state_buffers(self.info.num_gcells)
+        self._check(lib().rp_nesterov_get_state(self.h, ctypes.byref(st)))
+        return nesterov_state_values(st, bufs)
+
+    def set_state(self, values: dict) -> None:
+        st, _bufs = nesterov_state_buffers(self.info.num_gcells, values)
+        self._check(lib().rp_nesterov_set_state(self.h, ctypes.byref(st)))
 
     def records(self):
         cnt = ctypes.c_int32(0)

@@ -111,6 +111,8 @@ struct rp_ctx {
   std::vector<RpIterRec> records;
   rp_nesterov_stats last_stats;
   bool placed = false;
+  bool np_ready = false;      // rp_nesterov_init ran: rp_nesterov_iterate may continue the loop
+  double loop_seconds = 0.0;  // device time of the loop since rp_nesterov_init
 
   // device bytes held by this context (HBM admission accounting)
   size_t device_bytes() const {

@@ -100,6 +100,7 @@ struct RpState {
   float step_length;
   float coeff;
   float cur_a;
+  float prev_a;          // a_k of the iteration in progress (cur_a is a_(k+1))
   float density_penalty;
   float wl_coef_x, wl_coef_y;
   float base_wl_coef;

@@ -1302,43 +1302,42 @@ int rp_nesterov_collect(rp_ctx* c, double seconds) {
   out.final_overflow = hs->sum_overflow;
   out.final_step_length = hs->step_length;
   out.final_density_penalty = hs->density_penalty;
+  out.diverged = hs->phase == 2 ? 1 : 0;
   out.final_hpwl = hs->prev_hpwl;
   out.seconds = seconds;
   return code;
 }
 
-extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
-  if (!c) return RP_ERR_ARG;
-  hipStream_t s = c->stream;
-  // events and the captured graph, released on every exit path
-  struct Res {
-    hipEvent_t e0 = nullptr, e1 = nullptr, polled = nullptr;
-    std::vector<hipEvent_t> kev;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    ~Res() {
-      for (hipEvent_t e : kev)
-        if (e) (void)hipEventDestroy(e);
-      if (gexec) (void)hipGraphExecDestroy(gexec);
-      if (graph) (void)hipGraphDestroy(graph);
-      for (hipEvent_t e : {e0, e1, polled})
-        if (e) (void)hipEventDestroy(e);
-    }
-  } res;
-  hipEvent_t& e0 = res.e0;
-  hipEvent_t& e1 = res.e1;
-  HIPCHK(c, hipEventCreate(&e0));
-  HIPCHK(c, hipEventCreate(&e1));
-  HIPCHK(c, hipEventRecord(e0, s));
-  int r = rp_nesterov_begin(c, s);
-  if (r) return r;
+namespace {
+
+// Events and the captured graph of one run of the slot loop, released on
+// every exit path.
+struct LoopRes {
+  hipEvent_t e0 = nullptr, e1 = nullptr, polled = nullptr;
+  std::vector<hipEvent_t> kev;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  ~LoopRes() {
+    for (hipEvent_t e : kev)
+      if (e) (void)hipEventDestroy(e);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (hipEvent_t e : {e0, e1, polled})
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+// The doNesterovPlace loop (src/nesterovPlace.cpp:315-463) on the device:
+// evaluation slots are queued in batches and the state is polled between
+// batches until the device state leaves phase 0 (stop, divergence or the
+// iteration cap in the state).  The host copy c->h_state holds the final
+// state.
+int run_slots(rp_ctx* c, hipStream_t s, LoopRes& res) {
   const RpDev& d = c->dev;
-  // main loop: queue evaluation slots, poll the state between batches
   const int K = slots_per_poll(c);
   RpState* hs = c->h_state;
-  hipEvent_t& polled = res.polled;
-  HIPCHK(c, hipEventCreateWithFlags(&polled, hipEventDisableTiming));
-  int max_batches = (c->prm.nesterov_place_max_iter * std::max(1, c->prm.max_back_track)) / K + 4;
+  HIPCHK(c, hipEventCreateWithFlags(&res.polled, hipEventDisableTiming));
+  const int max_batches = (c->prm.nesterov_place_max_iter * std::max(1, c->prm.max_back_track)) / K + 4;
   std::vector<hipEvent_t>& kev = res.kev;
   constexpr int EV = 2 * RP_NUM_EVAL_KERNELS;  // events per slot
   if (c->timing) {
@@ -1356,20 +1355,18 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
   // time) are captured once into a HIP graph and replayed, unless per-kernel
   // events are recorded or the matrix-core transform (library GEMMs) is in use.
   const bool use_graph = !c->timing && !c->mf() && getenv("RP_NO_GRAPH") == nullptr;
-  hipGraph_t& graph = res.graph;
-  hipGraphExec_t& gexec = res.gexec;
   for (int b = 0; b < max_batches && hs->phase == 0; b++) {
     if (use_graph && b > 0) {
-      if (!gexec) {
+      if (!res.gexec) {
         HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         hipError_t e = rp_launch_sort_refresh(d, c->sortb, s);
         for (int k = 0; k < K && e == hipSuccess; k++)
           e = rp_launch_eval(d, s, 0, nullptr, nullptr, k == K - 1 ? &c->sortb : nullptr, &launched);
-        const hipError_t e2 = hipStreamEndCapture(s, &graph);
+        const hipError_t e2 = hipStreamEndCapture(s, &res.graph);
         HIPCHK(c, e != hipSuccess ? e : e2);
-        HIPCHK(c, hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        HIPCHK(c, hipGraphInstantiate(&res.gexec, res.graph, nullptr, nullptr, 0));
       }
-      HIPCHK(c, hipGraphLaunch(gexec, s));
+      HIPCHK(c, hipGraphLaunch(res.gexec, s));
     } else {
       // Spatial order of the scatter from the centres counted by the last slot
       // of the previous batch (it ran: the state polled after it still runs).
@@ -1381,8 +1378,8 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
                                  k == K - 1 ? &c->sortb : nullptr, &launched));
     }
     HIPCHK(c, hipMemcpyAsync(hs, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipEventRecord(polled, s));
-    HIPCHK(c, hipEventSynchronize(polled));
+    HIPCHK(c, hipEventRecord(res.polled, s));
+    HIPCHK(c, hipEventSynchronize(res.polled));
     if (c->timing) {
       // only slots that evaluated (the ones after the stop are no-ops)
       int ran = std::min(K, hs->evals - evals_before);
@@ -1402,13 +1399,184 @@ extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
     }
     evals_before = hs->evals;
   }
-  HIPCHK(c, hipEventRecord(e1, s));
-  HIPCHK(c, hipEventSynchronize(e1));
+  return 0;
+}
+
+// begin_iteration of the device controller (src/nesterovPlace.cpp:318-326)
+// on the host copy of the state: the same double expressions, so the same
+// floats.
+void host_begin_iteration(RpState& S) {
+  const float prevA = S.cur_a;
+  S.prev_a = prevA;
+  S.cur_a = (float)((1.0 + std::sqrt(4.0 * (double)prevA * (double)prevA + 1.0)) * 0.5);
+  S.coeff = (float)(((double)prevA - 1.0) / (double)S.cur_a);
+  S.bt = 0;
+  S.mode = 0;
+  S.dst = S.ring.next;
+}
+
+}  // namespace
+
+extern "C" int rp_nesterov_place(rp_ctx* c, rp_nesterov_stats* st) {
+  if (!c) return RP_ERR_ARG;
+  hipStream_t s = c->stream;
+  LoopRes res;
+  HIPCHK(c, hipEventCreate(&res.e0));
+  HIPCHK(c, hipEventCreate(&res.e1));
+  HIPCHK(c, hipEventRecord(res.e0, s));
+  int r = rp_nesterov_begin(c, s);
+  if (r) return r;
+  if (int q = run_slots(c, s, res)) return q;
+  HIPCHK(c, hipEventRecord(res.e1, s));
+  HIPCHK(c, hipEventSynchronize(res.e1));
   float ms = 0;
-  HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
-  int code = rp_nesterov_collect(c, ms * 1e-3);
+  HIPCHK(c, hipEventElapsedTime(&ms, res.e0, res.e1));
+  c->np_ready = true;
+  c->loop_seconds = ms * 1e-3;
+  int code = rp_nesterov_collect(c, c->loop_seconds);
   if (st) *st = c->last_stats;
   return code;
+}
+
+extern "C" int rp_nesterov_init(rp_ctx* c) {
+  if (!c) return RP_ERR_ARG;
+  hipStream_t s = c->stream;
+  LoopRes res;
+  HIPCHK(c, hipEventCreate(&res.e0));
+  HIPCHK(c, hipEventCreate(&res.e1));
+  HIPCHK(c, hipEventRecord(res.e0, s));
+  int r = rp_nesterov_begin(c, s);
+  if (r) return r;
+  HIPCHK(c, hipEventRecord(res.e1, s));
+  HIPCHK(c, hipMemcpyAsync(c->h_state, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, res.e0, res.e1));
+  c->loop_seconds = ms * 1e-3;
+  c->np_ready = true;
+  const RpState& S = *c->h_state;
+  return S.phase == 2 ? S.code : 0;
+}
+
+extern "C" int rp_nesterov_iterate(rp_ctx* c, int32_t n_iters, rp_nesterov_stats* st) {
+  if (!c || n_iters < 0) return RP_ERR_ARG;
+  if (!c->np_ready) return fail(c, RP_ERR_STATE, "rp_nesterov_iterate before rp_nesterov_init");
+  hipStream_t s = c->stream;
+  RpState& S = *c->h_state;
+  HIPCHK(c, hipMemcpy(&S, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost));
+  const int cap = c->prm.nesterov_place_max_iter;
+  // a pause at the bounded iteration count of an earlier call: the next
+  // iteration begins now
+  if (S.phase == 3 && S.iter < cap) {
+    S.phase = 0;
+    host_begin_iteration(S);
+  }
+  if (S.phase == 0 && n_iters > 0) {
+    S.max_iter = std::min(cap, S.iter + n_iters);
+    HIPCHK(c, hipMemcpy(c->d_state.p, &S, sizeof(RpState), hipMemcpyHostToDevice));
+    LoopRes res;
+    HIPCHK(c, hipEventCreate(&res.e0));
+    HIPCHK(c, hipEventCreate(&res.e1));
+    HIPCHK(c, hipEventRecord(res.e0, s));
+    int r = run_slots(c, s, res);
+    // the iteration cap of the context again (the state keeps phase 3 for a pause)
+    HIPCHK(c, hipMemcpy(&S, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost));
+    S.max_iter = cap;
+    HIPCHK(c, hipMemcpy(c->d_state.p, &S, sizeof(RpState), hipMemcpyHostToDevice));
+    if (r) return r;
+    HIPCHK(c, hipEventRecord(res.e1, s));
+    HIPCHK(c, hipEventSynchronize(res.e1));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, res.e0, res.e1));
+    c->loop_seconds += ms * 1e-3;
+  }
+  int code = rp_nesterov_collect(c, c->loop_seconds);
+  if (st) *st = c->last_stats;
+  return code;
+}
+
+extern "C" int rp_nesterov_get_state(rp_ctx* c, rp_nesterov_state* out) {
+  if (!c || !out) return RP_ERR_ARG;
+  if (!c->np_ready) return fail(c, RP_ERR_STATE, "rp_nesterov_get_state before rp_nesterov_init");
+  const int ng = c->n_g;
+  RpState& S = *c->h_state;
+  HIPCHK(c, hipMemcpy(&S, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost));
+  auto get = [&](float* dst, const float2* base, int slot) -> int {
+    if (!dst) return 0;
+    std::vector<float2> v(ng);
+    HIPCHK(c, hipMemcpy(v.data(), base + (size_t)slot * ng, sizeof(float2) * ng, hipMemcpyDeviceToHost));
+    for (int k = 0; k < ng; k++) {
+      const int g = c->gperm[k];
+      dst[2 * g] = v[k].x;
+      dst[2 * g + 1] = v[k].y;
+    }
+    return 0;
+  };
+  if (int r = get(out->cur_slp, c->d_slp.p, S.ring.cur)) return r;
+  if (int r = get(out->cur_slp_grad, c->d_sumg.p, S.ring.cur)) return r;
+  if (int r = get(out->prev_slp, c->d_slp.p, S.ring.prev)) return r;
+  if (int r = get(out->prev_slp_grad, c->d_sumg.p, S.ring.prev)) return r;
+  if (int r = get(out->cur_coord, c->d_co.p, S.ring.co_cur)) return r;
+  out->step_length = S.step_length;
+  out->density_penalty = S.density_penalty;
+  out->wl_coef_x = S.wl_coef_x;
+  out->wl_coef_y = S.wl_coef_y;
+  out->base_wl_coef = S.base_wl_coef;
+  // a running state has begun its next iteration (cur_a advanced)
+  out->cur_a = S.phase == 0 ? S.prev_a : S.cur_a;
+  out->sum_overflow = S.sum_overflow;
+  out->min_sum_overflow = S.min_sum_overflow;
+  out->hpwl_with_min_sum_overflow = S.hpwl_with_min_sum_overflow;
+  out->max_phi_coef = S.max_phi_coef;
+  out->is_max_phi_coef_changed = S.is_max_phi_coef_changed;
+  out->iter = S.iter;
+  out->prev_hpwl = S.prev_hpwl;
+  return RP_OK;
+}
+
+extern "C" int rp_nesterov_set_state(rp_ctx* c, const rp_nesterov_state* in) {
+  if (!c || !in) return RP_ERR_ARG;
+  if (!c->np_ready) return fail(c, RP_ERR_STATE, "rp_nesterov_set_state before rp_nesterov_init");
+  if (in->iter < 0 || in->iter >= c->prm.nesterov_place_max_iter)
+    return fail(c, RP_ERR_ARG, "state iteration %d outside [0, %d)", in->iter, c->prm.nesterov_place_max_iter);
+  const int ng = c->n_g;
+  RpState& S = *c->h_state;
+  HIPCHK(c, hipMemcpy(&S, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost));
+  auto put = [&](const float* src, float2* base, int slot) -> int {
+    if (!src) return 0;
+    std::vector<float2> v(ng);
+    for (int k = 0; k < ng; k++) {
+      const int g = c->gperm[k];
+      v[k] = make_float2(src[2 * g], src[2 * g + 1]);
+    }
+    HIPCHK(c, hipMemcpy(base + (size_t)slot * ng, v.data(), sizeof(float2) * ng, hipMemcpyHostToDevice));
+    return 0;
+  };
+  if (int r = put(in->cur_slp, c->d_slp.p, S.ring.cur)) return r;
+  if (int r = put(in->cur_slp_grad, c->d_sumg.p, S.ring.cur)) return r;
+  if (int r = put(in->prev_slp, c->d_slp.p, S.ring.prev)) return r;
+  if (int r = put(in->prev_slp_grad, c->d_sumg.p, S.ring.prev)) return r;
+  if (int r = put(in->cur_coord, c->d_co.p, S.ring.co_cur)) return r;
+  S.step_length = in->step_length;
+  S.density_penalty = in->density_penalty;
+  S.wl_coef_x = in->wl_coef_x;
+  S.wl_coef_y = in->wl_coef_y;
+  S.base_wl_coef = in->base_wl_coef;
+  S.cur_a = in->cur_a;
+  S.sum_overflow = in->sum_overflow;
+  S.min_sum_overflow = in->min_sum_overflow;
+  S.hpwl_with_min_sum_overflow = in->hpwl_with_min_sum_overflow;
+  S.max_phi_coef = in->max_phi_coef;
+  S.is_max_phi_coef_changed = in->is_max_phi_coef_changed;
+  S.iter = in->iter;
+  S.prev_hpwl = in->prev_hpwl;
+  S.phase = 0;
+  S.code = 0;
+  S.diverged = 0;
+  S.max_iter = c->prm.nesterov_place_max_iter;
+  host_begin_iteration(S);
+  HIPCHK(c, hipMemcpy(c->d_state.p, &S, sizeof(RpState), hipMemcpyHostToDevice));
+  return RP_OK;
 }
 
 extern "C" int rp_get_iter_records(rp_ctx* c, rp_iter_record* out, int32_t cap, int32_t* count) {

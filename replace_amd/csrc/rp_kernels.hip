@@ -1454,6 +1454,7 @@ __device__ bool bad_sum(double v) {
 __device__ void begin_iteration(RpState* s) {
   // src/nesterovPlace.cpp:318-326
   float prevA = s->cur_a;
+  s->prev_a = prevA;
   s->cur_a = (float)((1.0 + sqrt(4.0 * (double)prevA * (double)prevA + 1.0)) * 0.5);
   s->coeff = (float)(((double)prevA - 1.0) / (double)s->cur_a);
   s->bt = 0;

@@ -5,7 +5,8 @@ Workloads (sizes as named in the benchmark configurations):
     site 380 x 2800 DBU = 0.19 x 1.4 um at 2000 DBU/um), 21,000 nets,
     square core at 70 % utilisation, 128 x 128 bins.
   * ``c1``: 1,000,000 cells, 1,050,000 nets, 40 fixed macros, 1024 x 1024 bins.
-  * ``c3``: batch instances, cells log-uniform in [500, 50k].
+  * ``c3``: batch instances, cells log-uniform in [500, 50k], bin grids
+    sized to the cell count (bins_for_cells).
 
 Net degrees: 2 pins 60 %, 3 pins 18 %, U{4..10} 19 %, U{11..50} 3 %.
 Pins come from a locality window: cells are shuffled onto a virtual
@@ -201,15 +202,25 @@ def workload(name: str, scale: float = 1.0):
     raise KeyError(name)
 
 
+def bins_for_cells(nc: int, lo: int, hi: int) -> int:
+    """Bin count per side sized to the cell count: the largest power of two
+    not above sqrt(Nc) (what the reference's automatic rule, BinGrid::initBins
+    src/nesterovBase.cpp:651-659, gives when utilisation is near the target
+    density: at most one bin per cell), clamped to [lo, hi].  A grid with
+    more bins than cells (nextPow2 rounding up) leaves many instances unable
+    to reach overflow 0.10."""
+    return min(hi, max(lo, next_pow2(int(math.ceil(math.sqrt(nc)))) // 2))
+
+
 def batch_instance(i: int):
     """Instance i of the batched workload: cells log-uniform in [500, 50k],
     target density U[0.6, 0.9], utilisation min(0.70, td - 0.01), bins
-    clamp(nextPow2(ceil(sqrt(Nc))), 32, 256) per side, seed 1000 + i."""
+    bins_for_cells(Nc, 32, 256) per side, seed 1000 + i."""
     rng = np.random.default_rng(1000 + i)
     nc = int(round(math.exp(rng.uniform(math.log(500), math.log(50000)))))
     td = float(rng.uniform(0.6, 0.9))
     util = min(0.70, td - 0.01)
-    nb = min(256, max(32, next_pow2(int(math.ceil(math.sqrt(nc))))))
+    nb = bins_for_cells(nc, 32, 256)
     nl = generate(nc, int(round(nc * 1.05)), seed=1000 + i, util=util, td=td, name=f"c3_{i}")
     p = params_default(target_density=td, bin_grid_cnt_x=nb, bin_grid_cnt_y=nb,
                        target_overflow=0.10)
@@ -219,13 +230,13 @@ def batch_instance(i: int):
 def large_instance(i: int):
     """A large served job: cells uniform in [200k, 1M], target density
     U[0.6, 0.9] and utilisation min(0.70, td - 0.01) as in the batched
-    distribution, bins nextPow2(sqrt(Nc)) clamped to [512, 1024] per side,
-    seed 5000 + i."""
+    distribution, bins bins_for_cells(Nc, 512, 1024) per side (512 for every
+    size drawn), seed 5000 + i."""
     rng = np.random.default_rng(5000 + i)
     nc = int(rng.integers(200000, 1000001))
     td = float(rng.uniform(0.6, 0.9))
     util = min(0.70, td - 0.01)
-    nb = min(1024, max(512, next_pow2(int(math.ceil(math.sqrt(nc))))))
+    nb = bins_for_cells(nc, 512, 1024)
     nl = generate(nc, int(round(nc * 1.05)), seed=5000 + i, util=util, td=td, name=f"large_{i}")
     p = params_default(target_density=td, bin_grid_cnt_x=nb, bin_grid_cnt_y=nb, target_overflow=0.10)
     return nl, p

@@ -17,6 +17,9 @@ INT_MAX = 2**31 - 1
 INT_MIN = -(2**31)
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_f64p = ctypes.POINTER(ctypes.c_double)
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 
 
@@ -95,9 +98,66 @@ class RpNesterovStats(ctypes.Structure):
         ("final_overflow", ctypes.c_float),
         ("final_step_length", ctypes.c_float),
         ("final_density_penalty", ctypes.c_float),
+        ("diverged", ctypes.c_int32),
         ("final_hpwl", ctypes.c_int64),
         ("seconds", ctypes.c_double),
     ]
+
+
+class RpNesterovState(ctypes.Structure):
+    """ctypes mirror of ``rp_nesterov_state``."""
+
+    _fields_ = [
+        ("cur_slp", _f32p),
+        ("cur_slp_grad", _f32p),
+        ("prev_slp", _f32p),
+        ("prev_slp_grad", _f32p),
+        ("cur_coord", _f32p),
+        ("step_length", ctypes.c_float),
+        ("density_penalty", ctypes.c_float),
+        ("wl_coef_x", ctypes.c_float),
+        ("wl_coef_y", ctypes.c_float),
+        ("base_wl_coef", ctypes.c_float),
+        ("cur_a", ctypes.c_float),
+        ("sum_overflow", ctypes.c_float),
+        ("min_sum_overflow", ctypes.c_float),
+        ("hpwl_with_min_sum_overflow", ctypes.c_float),
+        ("max_phi_coef", ctypes.c_float),
+        ("is_max_phi_coef_changed", ctypes.c_int32),
+        ("iter", ctypes.c_int32),
+        ("prev_hpwl", ctypes.c_int64),
+    ]
+
+    ARRAYS = ("cur_slp", "cur_slp_grad", "prev_slp", "prev_slp_grad", "cur_coord")
+    SCALARS = ("step_length", "density_penalty", "wl_coef_x", "wl_coef_y", "base_wl_coef", "cur_a",
+               "sum_overflow", "min_sum_overflow", "hpwl_with_min_sum_overflow", "max_phi_coef",
+               "is_max_phi_coef_changed", "iter", "prev_hpwl")
+
+
+def nesterov_state_buffers(num_gcells: int, values: Optional[dict] = None):
+    """An ``rp_nesterov_state`` whose arrays point into numpy buffers
+    (2 * num_gcells floats each, x/y interleaved), filled from ``values``
+    when given.  Returns (state, dict of the buffers and scalars)."""
+    st = RpNesterovState()
+    d = {}
+    for k in RpNesterovState.ARRAYS:
+        a = np.zeros(2 * num_gcells, np.float32)
+        if values is not None:
+            a[:] = values[k]
+        d[k] = a
+        setattr(st, k, a.ctypes.data_as(_f32p))
+    if values is not None:
+        for k in RpNesterovState.SCALARS:
+            setattr(st, k, values[k])
+    return st, d
+
+
+def nesterov_state_values(st: "RpNesterovState", bufs: dict) -> dict:
+    """The filled state as a dict (arrays copied)."""
+    out = {k: bufs[k].copy() for k in RpNesterovState.ARRAYS}
+    for k in RpNesterovState.SCALARS:
+        out[k] = getattr(st, k)
+    return out
 
 
 class RpIpStats(ctypes.Structure):
@@ -134,9 +194,6 @@ class RpNbInfo(ctypes.Structure):
     ]
 
 
-_f32p = ctypes.POINTER(ctypes.c_float)
-_i64p = ctypes.POINTER(ctypes.c_int64)
-_f64p = ctypes.POINTER(ctypes.c_double)
 
 
 class RpEvalOut(ctypes.Structure):

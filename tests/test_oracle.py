@@ -289,3 +289,51 @@ def test_def_writer_matches_reference_components(name, tmp_path):
     out = str(tmp_path / (name + ".def"))
     write_def(os.path.join(REF_TEST, name + ".def"), out, lx, ly)
     assert components_section(out) == components_section(os.path.join(REF_TEST, name + ".defok"))
+
+
+def test_state_round_trip_resumes_the_same_run():
+    """ro_get_state / ro_set_state: a run stopped after k iterations and
+    resumed in a fresh context from its state continues exactly as the
+    uninterrupted run (records and final state equal)."""
+    nl, p = design("simple01")
+
+    def fresh():
+        o = oracle.Oracle(nl, p)
+        o.initial_place()
+        o.setup()
+        assert o.nesterov_init() == 0
+        return o
+
+    a = fresh()
+    a.nesterov_iterate(40)
+    s40 = a.get_state()
+    a.nesterov_iterate(15)
+    want = a.records()[40:]
+    s55 = a.get_state()
+    b = fresh()
+    b.set_state(s40)
+    assert b.get_state()["iter"] == 40
+    b.nesterov_iterate(15)
+    assert b.records() == want
+    got = b.get_state()
+    for k, v in s55.items():
+        assert np.array_equal(np.asarray(got[k]), np.asarray(v)), k
+
+
+def test_repl1_paths():
+    """REPL-1 (src/placerBase.cpp:952-956 utilisation above 100.1 %,
+    src/nesterovBase.cpp:1116-1121 negative filler area)."""
+    nl, p = design("simple01")
+    # a target density below the utilisation: negative filler area
+    p.target_density = 0.3
+    o = oracle.Oracle(nl, p)
+    with pytest.raises(RuntimeError, match="code 1"):
+        o.setup()
+    # a quarter of the rows: the cells no longer fit the core
+    nl2, p2 = design("simple01")
+    rows = nl2.row_box[: len(nl2.row_box) // 4]
+    small = Netlist(dbu_per_micron=nl2.dbu_per_micron, die=nl2.die, site_w=nl2.site_w, site_h=nl2.site_h,
+                    row_box=rows, inst_box=nl2.inst_box, inst_fixed=nl2.inst_fixed,
+                    net_pin_start=nl2.net_pin_start, pin_inst=nl2.pin_inst, pin_off=nl2.pin_off,
+                    pin_box=nl2.pin_box)
+    assert oracle.Oracle(small, p2).create_code == 1
