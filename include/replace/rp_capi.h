@@ -327,6 +327,27 @@ int rp_eval_gradient(rp_ctx* ctx, const float* cx, const float* cy, float densit
 int rp_eval_poisson(rp_ctx* ctx, const float* density, float* phi, float* ex, float* ey);
 
 /* ------------------------------------------------------------------------
+ * Poisson solve without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It solves
+ * FFT::doFFT (src/fft.cpp:112-177) like rp_eval_poisson, on the caller's
+ * device buffers (y-major, j * nx + i): density -> phi (may be NULL), ex,
+ * ey, for an nx x ny grid of bins of bin_size_x x bin_size_y DBU, queued on
+ * `stream` (a hipStream_t; NULL = the null stream) and not waited for.
+ * Its work memory is a fixed scratch array in device memory, one per GPU,
+ * held from the first call on that GPU until the process ends
+ * (RP_NOALLOC_SCRATCH_MIB); it calls no hipMalloc / hipFree.  Calls that
+ * overlap on one GPU share that array and race: order them on one stream.
+ * Returns 0, RP_ERR_ARG (a NULL buffer, a non-positive bin size, a side
+ * that is not a power of two in [4, 2048], or more than
+ * RP_NOALLOC_MAX_BINS bins), RP_ERR_NO_DEVICE or RP_ERR_HIP.
+ * ---------------------------------------------------------------------- */
+#define RP_NOALLOC_MAX_BINS (1 << 20)  /* 1024 x 1024 */
+#define RP_NOALLOC_SCRATCH_MIB 21      /* scratch held per GPU (rounded up) */
+int rp_poisson_solve_noalloc(const float* density, int32_t nx, int32_t ny, int32_t bin_size_x,
+                             int32_t bin_size_y, float* phi, float* ex, float* ey, void* stream);
+
+/* ------------------------------------------------------------------------
  * Placement service (no reference counterpart: the reference places one
  * design per process).  Many jobs share batched evaluation steps on one GPU:
  * jobs are admitted in arrival order against an HBM budget, set up on host

@@ -33,7 +33,8 @@ COMMON = [
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
 ]
 
-LIB_SOURCES = ["rp_kernels.hip", "rp_ip.hip", "rp_dct_mfma.hip", "rp_host.cpp", "rp_batch.cpp", "Replace.cpp"]
+LIB_SOURCES = ["rp_kernels.hip", "rp_ip.hip", "rp_dct_mfma.hip", "rp_noalloc.hip", "rp_host.cpp", "rp_batch.cpp",
+               "Replace.cpp"]
 
 
 def _run(cmd):

@@ -21,6 +21,14 @@ LIB_PATH = os.path.join(HERE, "libreplace_amd.so")
 
 _lib = None
 
+# return codes of include/replace/rp_capi.h
+RP_OK = 0
+RP_ERR_ARG = -1
+RP_ERR_HIP = -2
+RP_ERR_NOMEM = -3
+RP_ERR_STATE = -4
+RP_ERR_NO_DEVICE = -5
+
 
 class ReplaceError(RuntimeError):
     def __init__(self, code: int, msg: str):
@@ -61,6 +69,8 @@ def lib() -> ctypes.CDLL:
         L.rp_set_kernel_timing.argtypes = [vp, ctypes.c_int]
         L.rp_get_kernel_timing.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int64)]
         L.rp_get_ip_records.argtypes = [vp, vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+        L.rp_poisson_solve_noalloc.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                               vp, vp, vp, vp]
         L.rp_server_opts_default.argtypes = [ctypes.POINTER(RpServerOpts)]
         L.rp_server_tuning.argtypes = [vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
         L.rp_server_set_tiles.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
@@ -82,6 +92,15 @@ def lib() -> ctypes.CDLL:
 
 def _p(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def poisson_solve_noalloc(density_ptr: int, nx: int, ny: int, bin_size_x: int, bin_size_y: int,
+                          phi_ptr: Optional[int], ex_ptr: int, ey_ptr: int, stream: Optional[int] = None) -> int:
+    """rp_poisson_solve_noalloc on device buffers given by address (queued on
+    `stream`, a hipStream_t handle; not waited for).  Returns the C code."""
+    return lib().rp_poisson_solve_noalloc(ctypes.c_void_p(density_ptr), nx, ny, bin_size_x, bin_size_y,
+                                          ctypes.c_void_p(phi_ptr) if phi_ptr else None, ctypes.c_void_p(ex_ptr),
+                                          ctypes.c_void_p(ey_ptr), ctypes.c_void_p(stream) if stream else None)
 
 
 def alloc_eval_out(info: RpNbInfo):

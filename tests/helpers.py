@@ -150,3 +150,51 @@ def assert_tracks_trace(name, rec, iterations, final_overflow, ip):
     assert abs(iterations - last_it) <= 12
     assert abs(hpwl_at(rec, float(last_o)) - last_h) <= 0.005 * last_h
     assert final_overflow <= 0.1
+
+
+class DeviceArray:
+    """A device buffer for the tests of entry points that take device
+    pointers, allocated through the HIP runtime the library itself uses
+    (libamdhip64.so.7, resolved to the copy already loaded with
+    libreplace_amd.so), so no second runtime enters the process."""
+
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            import ctypes
+            from replace_amd.capi import lib
+            lib()  # the library first: its runtime is the one reused below
+            h = ctypes.CDLL("libamdhip64.so.7", mode=ctypes.RTLD_GLOBAL)
+            vp = ctypes.c_void_p
+            h.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+            h.hipFree.argtypes = [vp]
+            h.hipMemcpy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+            h.hipDeviceSynchronize.argtypes = []
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, host: np.ndarray):
+        import ctypes
+        self.host = np.ascontiguousarray(host)
+        self.nbytes = self.host.nbytes
+        p = ctypes.c_void_p()
+        assert self.hip().hipMalloc(ctypes.byref(p), max(1, self.nbytes)) == 0
+        self.ptr = p.value
+        assert self.hip().hipMemcpy(ctypes.c_void_p(self.ptr), self.host.ctypes.data_as(ctypes.c_void_p),
+                                    self.nbytes, 1) == 0  # host to device
+
+    def get(self) -> np.ndarray:
+        import ctypes
+        out = np.empty_like(self.host)
+        assert self.hip().hipDeviceSynchronize() == 0
+        assert self.hip().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.ptr),
+                                    self.nbytes, 2) == 0  # device to host
+        return out
+
+    def free(self):
+        if self.ptr:
+            import ctypes
+            self.hip().hipFree(ctypes.c_void_p(self.ptr))
+            self.ptr = None
