@@ -84,7 +84,7 @@ struct rp_ctx {
   DevBuf<int4> d_nbox;
   DevBuf<float4> d_nsum;
   DevBuf<unsigned long long> d_bin_inst, d_bin_filler;
-  DevBuf<long long> d_bin_nonplace;
+  DevBuf<float> d_bin_npf;
   DevBuf<float> d_rho, d_phi, d_t, d_y1, d_y2;
   DevBuf<float2> d_field;
   DevBuf<float2> d_twx, d_tw2x, d_twy, d_tw2y;
@@ -120,7 +120,7 @@ struct rp_ctx {
     auto add = [&b](const auto& buf) { b += buf.n * sizeof(*buf.p); };
     add(d_dhalf); add(d_scale); add(d_area); add(d_npin); add(d_kind); add(d_slp); add(d_sumg); add(d_co);
     add(d_ic); add(d_cp_start); add(d_cp_pin); add(d_nstart); add(d_pcell); add(d_ppos); add(d_ppin); add(d_pgrad);
-    add(d_nbox); add(d_nsum); add(d_bin_inst); add(d_bin_filler); add(d_bin_nonplace); add(d_rho); add(d_phi);
+    add(d_nbox); add(d_nsum); add(d_bin_inst); add(d_bin_filler); add(d_bin_npf); add(d_rho); add(d_phi);
     add(d_t); add(d_y1); add(d_y2); add(d_field); add(d_twx); add(d_tw2x);
     add(d_twy); add(d_tw2y); add(d_wx); add(d_wx2); add(d_wy); add(d_wy2); add(d_part_grad); add(d_part_bin);
     add(d_part_net); add(d_state); add(d_rec); add(d_dbg_wl); add(d_dbg_dens);

@@ -40,7 +40,8 @@ struct RpDev {
   float4* nsum;
   unsigned long long* bin_inst;
   unsigned long long* bin_filler;
-  const long long* bin_nonplace;
+  const float* bin_npf;  // float(nonPlace area) per bin (the value the density formulas convert it to)
+  int store_rho;         // bin kernel writes rho (stage-level outputs, matrix-core solve)
   float* rho;
   float2* field;
   float* phi;
@@ -65,11 +66,12 @@ struct RpDev {
 };
 
 // Dynamic LDS bytes of the FFT kernels for an n-point row / column: two
-// ping-pong buffers and the twiddle circle (n float2 each), plus the staged
-// rows of the column and inverse passes.
-inline size_t rp_lds_binfin(int n) { return sizeof(float2) * 3 * (size_t)n; }
-inline size_t rp_lds_col(int n) { return sizeof(float2) * 3 * (size_t)n + sizeof(float) * 4 * (size_t)n; }
-inline size_t rp_lds_rinv(int n) { return sizeof(float2) * 3 * (size_t)n + sizeof(float) * 4 * (size_t)n; }
+// ping-pong buffers (n float2 plus one float2 of padding per 16) and the
+// stage twiddles (n float2), plus the staged rows of the column and inverse
+// passes.
+inline size_t rp_lds_binfin(int n) { return sizeof(float2) * (2 * (size_t)(n + n / 16) + (size_t)n); }
+inline size_t rp_lds_col(int n) { return rp_lds_binfin(n) + sizeof(float) * 4 * (size_t)n; }
+inline size_t rp_lds_rinv(int n) { return rp_lds_binfin(n) + sizeof(float) * 4 * (size_t)n; }
 
 // One batched evaluation step over many instances (kb_* kernels): per kernel
 // a block table of (instance, logical block) entries (instance -1 = idle).

@@ -898,8 +898,9 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   HIPCHK(c, dalloc(c->d_bin_filler, nb));
   HIPCHK(c, hipMemsetAsync(c->d_bin_inst.p, 0, sizeof(unsigned long long) * nb, s));
   HIPCHK(c, hipMemsetAsync(c->d_bin_filler.p, 0, sizeof(unsigned long long) * nb, s));
-  std::vector<long long> npl(c->nonplace_bins.begin(), c->nonplace_bins.end());
-  HIPCHK(c, dupload(c->d_bin_nonplace, npl, s));
+  std::vector<float> npf(nb);
+  for (int b = 0; b < nb; b++) npf[b] = (float)c->nonplace_bins[b];  // int64 -> float as the density formulas convert it
+  HIPCHK(c, dupload(c->d_bin_npf, npf, s));
   HIPCHK(c, dalloc(c->d_rho, nb));
   HIPCHK(c, dalloc(c->d_phi, nb));
   HIPCHK(c, dalloc(c->d_field, nb));
@@ -973,7 +974,8 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   d.nsum = c->d_nsum.p;
   d.bin_inst = c->d_bin_inst.p;
   d.bin_filler = c->d_bin_filler.p;
-  d.bin_nonplace = c->d_bin_nonplace.p;
+  d.bin_npf = c->d_bin_npf.p;
+  d.store_rho = p.poisson_transform == 1 ? 1 : 0;
   d.rho = c->d_rho.p;
   d.field = c->d_field.p;
   d.phi = c->d_phi.p;
@@ -1102,6 +1104,7 @@ extern "C" int rp_eval_gradient(rp_ctx* c, const float* cx, const float* cy, flo
   HIPCHK(c, hipMemsetAsync(c->d_bin_inst.p, 0, sizeof(unsigned long long) * nb, s));
   HIPCHK(c, hipMemsetAsync(c->d_bin_filler.p, 0, sizeof(unsigned long long) * nb, s));
   RpDev d = c->dev;
+  d.store_rho = 1;  // the density is an output here
   d.dbg_wl = c->d_dbg_wl.p;
   d.dbg_dens = c->d_dbg_dens.p;
   HIPCHK(c, rp_launch_scatter(d, s));

@@ -28,9 +28,6 @@
 // ---------------------------------------------------------------------------
 // small helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
 
 // fastExp, reference src/nesterovBase.cpp:1568-1582.  Its first step,
 // (float)(1.0 + (double)a / 1024.0), is computed in float: a / 1024 is exact,
@@ -121,139 +118,160 @@ __device__ __forceinline__ int xcd_block(int b, int nblocks) {
 }
 
 // ---------------------------------------------------------------------------
-// Stockham FFT in LDS (natural order in, natural order out): radix-4
-// stages, preceded by one radix-2 stage when log2(n) is odd, so a 1024-point
-// transform takes 5 barriers.  T holds the full circle of twiddles
-// e^{-2 pi i m / n}, m < n (load_twiddles).  dir = -1 forward (e^{-i}),
-// +1 inverse (e^{+i}), unnormalised.  Returns the buffer holding the result.
+// Stockham FFT in LDS, specialised per transform length N (natural order in,
+// natural order out): radix-4 stages, preceded by one radix-2 stage when
+// log2(N) is odd, so a 1024-point transform takes 5 barriers.  The work
+// arrays are indexed through lpad (one float2 of padding per 16 entries),
+// which spreads the power-of-two strides of the stage reads and writes over
+// the LDS banks.  The twiddles of every radix-4 stage with ns > 1 sit in a
+// table of their own (w1, w2, w3 for k < ns, stage after stage), so a
+// stage reads them contiguously.  DIR = -1 forward (e^{-i}), +1 inverse
+// (e^{+i}), unnormalised.  The transform's products may contract to FMAs:
+// its parity bar is a tolerance against the reference's Ooura transform.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void load_twiddles(float2* T, const float2* __restrict__ tw, int n) {
-  int half = n >> 1;
-  for (int m = threadIdx.x; m < half; m += blockDim.x) {
-    float2 w = tw[m];
-    T[m] = w;
-    T[m + half] = make_float2(-w.x, -w.y);
-  }
-}
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+
+// float2 entries of one padded work array of n points
+__host__ __device__ constexpr int fft_buf(int n) { return n + (n >> 4); }
+
+constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n >> 1); }
 
 // z * (dir * i)
-__device__ __forceinline__ float2 mul_dir_i(float2 z, int dir) {
-  return dir > 0 ? make_float2(-z.y, z.x) : make_float2(z.y, -z.x);
+template <int DIR>
+__device__ __forceinline__ float2 mul_dir_i(float2 z) {
+  return DIR > 0 ? make_float2(-z.y, z.x) : make_float2(z.y, -z.x);
 }
 
-__device__ float2* lds_fft(float2* a, float2* b, int n, const float2* T, int dir) {
-  float2* in = a;
-  float2* out = b;
-  int ns = 1;
-  if ((__ffs(n) - 1) & 1) {  // odd log2(n): radix-2 first stage (no twiddles)
-    int half = n >> 1;
-    for (int j = threadIdx.x; j < half; j += blockDim.x) {
-      float2 u = in[j], v = in[j + half];
-      out[2 * j] = make_float2(u.x + v.x, u.y + v.y);
-      out[2 * j + 1] = make_float2(u.x - v.x, u.y - v.y);
+// Stage twiddle tables of an N-point transform from the global half circle
+// tw[m] = e^{-2 pi i m / N}, m < N/2 (e^{-2 pi i m / N} = -tw[m - N/2] above).
+template <int N>
+__device__ __forceinline__ void load_stage_twiddles(float2* TW, const float2* __restrict__ tw) {
+  constexpr int LOG = ilog2c(N);
+  int off = 0;
+#pragma unroll
+  for (int ns = (LOG & 1) ? 2 : 4; ns < N; ns *= 4) {
+    const int t = N / (4 * ns);
+    for (int e = threadIdx.x; e < 3 * ns; e += blockDim.x) {
+      const int r = e / ns + 1, k = e - (r - 1) * ns;
+      const int m = r * k * t;
+      float2 w = m < N / 2 ? tw[m] : tw[m - N / 2];
+      if (m >= N / 2) w = make_float2(-w.x, -w.y);
+      TW[off + e] = w;
     }
-    __syncthreads();
-    float2* t = in;
-    in = out;
-    out = t;
-    ns = 2;
+    off += 3 * ns;
   }
-  const int q = n >> 2;
-  for (; ns < n; ns <<= 2) {
-    int tstride = n / (4 * ns);
-    for (int j = threadIdx.x; j < q; j += blockDim.x) {
-      int k = j & (ns - 1);
-      float2 a0 = in[j], a1 = in[j + q], a2 = in[j + 2 * q], a3 = in[j + 3 * q];
-      if (k) {
-        int m = k * tstride;
-        float2 w1 = T[m], w2 = T[2 * m], w3 = T[3 * m];
-        if (dir > 0) {
-          w1.y = -w1.y;
-          w2.y = -w2.y;
-          w3.y = -w3.y;
-        }
-        a1 = cmul(a1, w1);
-        a2 = cmul(a2, w2);
-        a3 = cmul(a3, w3);
+}
+
+// One radix-4 stage of stride ns (twiddles at TW + off), in -> out.
+template <int N, int NS, int DIR>
+__device__ __forceinline__ void fft_stage4(const float2* in, float2* out, const float2* TW, int off, int tid,
+                                           int nt) {
+#pragma clang fp contract(fast)
+  constexpr int Q = N / 4;
+  for (int j = tid; j < Q; j += nt) {
+    const int k = j & (NS - 1);
+    float2 a0 = in[lpad(j)], a1 = in[lpad(j + Q)], a2 = in[lpad(j + 2 * Q)], a3 = in[lpad(j + 3 * Q)];
+    if (NS > 1) {
+      float2 w1 = TW[off + k], w2 = TW[off + NS + k], w3 = TW[off + 2 * NS + k];
+      if (DIR > 0) {
+        w1.y = -w1.y;
+        w2.y = -w2.y;
+        w3.y = -w3.y;
       }
-      float2 s02 = make_float2(a0.x + a2.x, a0.y + a2.y);
-      float2 d02 = make_float2(a0.x - a2.x, a0.y - a2.y);
-      float2 s13 = make_float2(a1.x + a3.x, a1.y + a3.y);
-      float2 d13 = mul_dir_i(make_float2(a1.x - a3.x, a1.y - a3.y), dir);
-      int dst = (j - k) * 4 + k;
-      out[dst] = make_float2(s02.x + s13.x, s02.y + s13.y);
-      out[dst + ns] = make_float2(d02.x + d13.x, d02.y + d13.y);
-      out[dst + 2 * ns] = make_float2(s02.x - s13.x, s02.y - s13.y);
-      out[dst + 3 * ns] = make_float2(d02.x - d13.x, d02.y - d13.y);
+      auto cm = [](float2 x, float2 w) { return make_float2(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x); };
+      a1 = cm(a1, w1);
+      a2 = cm(a2, w2);
+      a3 = cm(a3, w3);
+    }
+    const float2 s02 = make_float2(a0.x + a2.x, a0.y + a2.y);
+    const float2 d02 = make_float2(a0.x - a2.x, a0.y - a2.y);
+    const float2 s13 = make_float2(a1.x + a3.x, a1.y + a3.y);
+    const float2 d13 = mul_dir_i<DIR>(make_float2(a1.x - a3.x, a1.y - a3.y));
+    const int dst = (j - k) * 4 + k;
+    out[lpad(dst)] = make_float2(s02.x + s13.x, s02.y + s13.y);
+    out[lpad(dst + NS)] = make_float2(d02.x + d13.x, d02.y + d13.y);
+    out[lpad(dst + 2 * NS)] = make_float2(s02.x - s13.x, s02.y - s13.y);
+    out[lpad(dst + 3 * NS)] = make_float2(d02.x - d13.x, d02.y - d13.y);
+  }
+}
+
+template <int N, int NS, int DIR>
+__device__ __forceinline__ float2* fft_stages(float2* in, float2* out, const float2* TW, int off, int tid, int nt) {
+  if constexpr (NS >= N) {
+    return in;
+  } else {
+    fft_stage4<N, NS, DIR>(in, out, TW, off, tid, nt);
+    __syncthreads();
+    return fft_stages<N, NS * 4, DIR>(out, in, TW, NS > 1 ? off + 3 * NS : off, tid, nt);
+  }
+}
+
+// N-point transform of a (b is the other work array) by threads tid of nt;
+// returns the array holding the result.  Every thread of the workgroup
+// reaches its barriers (groups of threads may run transforms of the same
+// length side by side on their own arrays).
+template <int N, int DIR>
+__device__ __forceinline__ float2* fft_n(float2* a, float2* b, const float2* TW, int tid, int nt) {
+  constexpr int LOG = ilog2c(N);
+  if constexpr (LOG & 1) {  // radix-2 first stage (no twiddles)
+    constexpr int H = N / 2;
+    for (int j = tid; j < H; j += nt) {
+      const float2 u = a[lpad(j)], v = a[lpad(j + H)];
+      b[lpad(2 * j)] = make_float2(u.x + v.x, u.y + v.y);
+      b[lpad(2 * j + 1)] = make_float2(u.x - v.x, u.y - v.y);
     }
     __syncthreads();
-    float2* t = in;
-    in = out;
-    out = t;
+    return fft_stages<N, 2, DIR>(b, a, TW, 0, tid, nt);
+  } else {
+    return fft_stages<N, 1, DIR>(a, b, TW, 0, tid, nt);
   }
-  return in;
+}
+
+template <int N, int DIR>
+__device__ __forceinline__ float2* fft_n(float2* a, float2* b, const float2* TW) {
+  return fft_n<N, DIR>(a, b, TW, threadIdx.x, blockDim.x);
 }
 
 // Makhoul input reorder index: x[i] goes to v[pos(i)]
 __device__ __forceinline__ int mk_pos(int i, int n) { return (i & 1) ? (n - 1 - (i >> 1)) : (i >> 1); }
 
-// After a forward FFT of z = v_a + i v_b (each Makhoul-reordered), output
-// the two DCT-II spectra: X[k] = sum_n x[n] cos(pi (n+1/2) k / N).
-__device__ __forceinline__ void dct2_post(const float2* Z, int n, int k, const float2* __restrict__ tw2,
-                                          float& xa, float& xb) {
-  float2 zk = Z[k];
-  float2 zm = Z[(n - k) & (n - 1)];
+// After a forward FFT of z = v_a + i v_b (each Makhoul-reordered), the two
+// DCT-II spectra X[k] = sum_n x[n] cos(pi (n+1/2) k / N), with the twiddle
+// tw2[k] = (cos t, -sin t), t = pi k / 2N, already loaded (w).
+template <int N>
+__device__ __forceinline__ void dct2_post(const float2* Z, int k, float2 w, float& xa, float& xb) {
+  float2 zk = Z[lpad(k)];
+  float2 zm = Z[lpad((N - k) & (N - 1))];
   zm.y = -zm.y;  // conj
-  float2 va = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
-  float dr = zk.x - zm.x, di = zk.y - zm.y;
-  float2 vb = make_float2(0.5f * di, -0.5f * dr);
-  float2 w = tw2[k];  // (cos t, -sin t), t = pi k / 2N
-  xa = va.x * w.x - va.y * w.y;
-  xb = vb.x * w.x - vb.y * w.y;
-}
-
-// dct2_post with the twiddle tw2[k] already loaded (w)
-__device__ __forceinline__ void dct2_post_w(const float2* Z, int n, int k, float2 w, float& xa, float& xb) {
-  float2 zk = Z[k];
-  float2 zm = Z[(n - k) & (n - 1)];
-  zm.y = -zm.y;  // conj
-  float2 va = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
-  float dr = zk.x - zm.x, di = zk.y - zm.y;
-  float2 vb = make_float2(0.5f * di, -0.5f * dr);
+  const float2 va = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y + zm.y));
+  const float dr = zk.x - zm.x, di = zk.y - zm.y;
+  const float2 vb = make_float2(0.5f * di, -0.5f * dr);
   xa = va.x * w.x - va.y * w.y;
   xb = vb.x * w.x - vb.y * w.y;
 }
 
 // Pre-twiddle of the inverse Makhoul (DCT-III, Ooura "IDCT" without scale):
 // W[k] = 1/2 e^{i pi k/2N} (a[k] - i a[N-k]) (k >= 1), W[0] = a[0];
-// returns W_a + i W_b for two real coefficient rows read through getters.
-__device__ __forceinline__ float2 dct3_pre(float a_k, float a_nk, float b_k, float b_nk, int k,
-                                           const float2* __restrict__ tw2) {
+// returns W_a + i W_b for two real coefficient rows, with the twiddle
+// tw2[k] already loaded (w; unused at k = 0).
+__device__ __forceinline__ float2 dct3_pre(float a_k, float a_nk, float b_k, float b_nk, int k, float2 w) {
   float2 wa, wb;
   if (k == 0) {
     wa = make_float2(a_k, 0.0f);
     wb = make_float2(b_k, 0.0f);
   } else {
-    float2 w = tw2[k];
-    float c = w.x, s = -w.y;  // e^{+i t} = (c, s)
+    const float c = w.x, s = -w.y;  // e^{+i t} = (c, s)
     wa = make_float2(0.5f * (a_k * c + a_nk * s), 0.5f * (a_k * s - a_nk * c));
     wb = make_float2(0.5f * (b_k * c + b_nk * s), 0.5f * (b_k * s - b_nk * c));
   }
   return make_float2(wa.x - wb.y, wa.y + wb.x);
 }
 
-// dct3_pre with the twiddle tw2[k] already loaded (w; unused at k = 0)
-__device__ __forceinline__ float2 dct3_pre_w(float a_k, float a_nk, float b_k, float b_nk, int k, float2 w) {
-  float2 wa, wb;
-  if (k == 0) {
-    wa = make_float2(a_k, 0.0f);
-    wb = make_float2(b_k, 0.0f);
-  } else {
-    float c = w.x, s = -w.y;  // e^{+i t} = (c, s)
-    wa = make_float2(0.5f * (a_k * c + a_nk * s), 0.5f * (a_k * s - a_nk * c));
-    wb = make_float2(0.5f * (b_k * c + b_nk * s), 0.5f * (b_k * s - b_nk * c));
-  }
-  return make_float2(wa.x - wb.y, wa.y + wb.x);
+// Output m of an inverse Makhoul transform held in Z: y[2q] = w[q],
+// y[2q+1] = w[N-1-q]
+template <int N>
+__device__ __forceinline__ float2 dct3_out(const float2* Z, int m) {
+  return Z[lpad((m & 1) ? (N - 1 - (m >> 1)) : (m >> 1))];
 }
 
 // ---------------------------------------------------------------------------
@@ -601,17 +619,19 @@ __device__ void pinpos_body(const RpDev& d, int blk) {
 // ---------------------------------------------------------------------------
 // 2. bin density + overflow + DCT-II along x (two rows per workgroup)
 // ---------------------------------------------------------------------------
+// LDS of the FFT kernels (dynamic): the two padded work arrays and the stage
+// twiddles (N float2 each at most), then the kernel's staged rows.
+template <int N>
 __device__ void binfin_body(const RpDev& d, int blk, int from_rho) {
   extern __shared__ float2 smem[];
   __shared__ double sh_red[RP_BLOCK / 64];
   if (d.st->phase != 0) return;
-  const int n = d.nbx;
   float2* A = smem;
-  float2* B = smem + n;
-  float2* T = smem + 2 * n;
-  int ja = 2 * blk, jb = ja + 1;
+  float2* B = smem + fft_buf(N);
+  float2* TW = smem + 2 * fft_buf(N);
+  const int ja = 2 * blk, jb = ja + 1;
   double ov_sum = 0.0;
-  constexpr int MAXC = RP_MAX_FFT / RP_BLOCK;
+  constexpr int MAXC = (N + RP_BLOCK - 1) / RP_BLOCK;
   // the DCT-II post-twiddles of the thread's outputs k = threadIdx.x + r
   // blockDim.x, loaded now rather than after the FFT
   float2 tw[MAXC];
@@ -619,33 +639,34 @@ __device__ void binfin_body(const RpDev& d, int blk, int from_rho) {
 #pragma unroll
     for (int r = 0; r < MAXC; r++) {
       const int k = threadIdx.x + r * blockDim.x;
-      if (k < n) tw[r] = d.tw2x[k];
+      if (k < N) tw[r] = d.tw2x[k];
     }
-    load_twiddles(T, d.twx, n);
+    load_stage_twiddles<N>(TW, d.twx);
   }
   if (from_rho) {  // Poisson-only entry: density given
-    for (int i = threadIdx.x; i < n; i += blockDim.x)
-      A[mk_pos(i, n)] = make_float2(d.rho[(size_t)ja * n + i], d.rho[(size_t)jb * n + i]);
+    for (int i = threadIdx.x; i < N; i += blockDim.x)
+      A[lpad(mk_pos(i, N))] = make_float2(d.rho[(size_t)ja * N + i], d.rho[(size_t)jb * N + i]);
   } else {
     // every load of the thread's bins is issued before any store (the zeroed
     // accumulators could otherwise alias later loads and serialise them)
-    long long v_in[2][MAXC], v_fi[2][MAXC], v_np[2][MAXC];
+    long long v_in[2][MAXC], v_fi[2][MAXC];
+    float v_np[2][MAXC];
 #pragma unroll
     for (int t = 0; t < MAXC; t++) {
-      int i = threadIdx.x + t * blockDim.x;
-      if (i < n)
+      const int i = threadIdx.x + t * blockDim.x;
+      if (i < N)
 #pragma unroll
         for (int r = 0; r < 2; r++) {
-          size_t b = (size_t)(ja + r) * n + i;
+          const size_t b = (size_t)(ja + r) * N + i;
           v_in[r][t] = (long long)d.bin_inst[b];
           v_fi[r][t] = (long long)d.bin_filler[b];
-          v_np[r][t] = d.bin_nonplace[b];
+          v_np[r][t] = d.bin_npf[b];
         }
     }
 #pragma unroll
     for (int t = 0; t < MAXC; t++) {
-      int i = threadIdx.x + t * blockDim.x;
-      if (i >= n) continue;
+      const int i = threadIdx.x + t * blockDim.x;
+      if (i >= N) continue;
       int blx, bux;
       bin_x(d, i, blx, bux);
       float r2[2];
@@ -653,35 +674,35 @@ __device__ void binfin_body(const RpDev& d, int blk, int from_rho) {
       for (int r = 0; r < 2; r++) {
         int bly, buy;
         bin_y(d, ja + r, bly, buy);
-        size_t b = (size_t)(ja + r) * n + i;
-        long long bin_area = (long long)(bux - blx) * (long long)(buy - bly);
-        float at = (float)bin_area * d.td;
+        const size_t b = (size_t)(ja + r) * N + i;
+        const long long bin_area = (long long)(bux - blx) * (long long)(buy - bly);
+        const float at = (float)bin_area * d.td;
         // src/nesterovBase.cpp:806-816
-        float rho = ((float)v_in[r][t] + (float)v_fi[r][t] + (float)v_np[r][t]) / at;
-        float ov = (float)v_in[r][t] + (float)v_np[r][t] - at;
+        const float rho = ((float)v_in[r][t] + (float)v_fi[r][t] + v_np[r][t]) / at;
+        float ov = (float)v_in[r][t] + v_np[r][t] - at;
         ov = (0.0f < ov) ? ov : 0.0f;
         ov_sum += (double)ov;
-        d.rho[b] = rho;
+        if (d.store_rho) d.rho[b] = rho;
         d.bin_inst[b] = 0ull;
         d.bin_filler[b] = 0ull;
         r2[r] = rho;
       }
-      A[mk_pos(i, n)] = make_float2(r2[0], r2[1]);
+      A[lpad(mk_pos(i, N))] = make_float2(r2[0], r2[1]);
     }
   }
-  double tot = block_sum_d(ov_sum, sh_red);
+  const double tot = block_sum_d(ov_sum, sh_red);
   if (threadIdx.x == 0) d.part_bin[blk] = tot;
   if (d.mfma) return;  // the matrix-core solve takes rho from d.rho
   // block_sum_d synchronised the LDS writes above
-  float2* Z = lds_fft(A, B, n, T, -1);
+  const float2* Z = fft_n<N, -1>(A, B, TW);
 #pragma unroll
   for (int r = 0; r < MAXC; r++) {
     const int k = threadIdx.x + r * blockDim.x;
-    if (k >= n) continue;
+    if (k >= N) continue;
     float xa, xb;
-    dct2_post_w(Z, n, k, tw[r], xa, xb);
-    d.t[(size_t)ja * n + k] = xa;
-    d.t[(size_t)jb * n + k] = xb;
+    dct2_post<N>(Z, k, tw[r], xa, xb);
+    d.t[(size_t)ja * N + k] = xa;
+    d.t[(size_t)jb * N + k] = xb;
   }
 }
 
@@ -690,29 +711,29 @@ __device__ void binfin_body(const RpDev& d, int blk, int from_rho) {
 // the row-major grids: DCT-II along y, FFT::doFFT's scaling and spectral
 // divide (src/fft.cpp:117-166), then DCT-III (phi) and DST-III (Ey) along y.
 // ---------------------------------------------------------------------------
+template <int N>
 __device__ void col_body(const RpDev& d, int blk) {
   extern __shared__ float2 smem[];
   if (d.st->phase != 0) return;
-  const int n = d.nby;
   float2* A = smem;
-  float2* B = smem + n;
-  float2* T = smem + 2 * n;
-  float* pa = (float*)(smem + 3 * n);  // phi hat rows a, b; Ey hat rows a, b
-  float* pb = pa + n;
-  float* ea = pb + n;
-  float* eb = ea + n;
-  int ua = 2 * blk, ub = ua + 1;
+  float2* B = smem + fft_buf(N);
+  float2* TW = smem + 2 * fft_buf(N);
+  float* pa = (float*)(TW + N);  // phi hat columns a, b; Ey hat columns a, b
+  float* pb = pa + N;
+  float* ea = pb + N;
+  float* eb = ea + N;
+  const int ua = 2 * blk, ub = ua + 1;
   const size_t nx = (size_t)d.nbx;
   // Everything this thread reads from global memory in any phase is loaded
   // first (its rows v = threadIdx.x + r blockDim.x): the two columns, the
   // twiddles of the DCT post/pre steps and the spectral weights.
-  constexpr int MAXC = RP_MAX_FFT / RP_BLOCK;
+  constexpr int MAXC = (N + RP_BLOCK - 1) / RP_BLOCK;
   float2 colv[MAXC], tw[MAXC];
   float wyv[MAXC], wy2v[MAXC];
 #pragma unroll
   for (int r = 0; r < MAXC; r++) {
     const int v = threadIdx.x + r * blockDim.x;
-    if (v < n) {
+    if (v < N) {
       // columns ua, ub of the row-transformed grid t (row-major): one 8-byte
       // load per row (the workgroups sharing a cache line run on one XCD)
       colv[r] = *(const float2*)(d.t + v * nx + ua);
@@ -722,26 +743,26 @@ __device__ void col_body(const RpDev& d, int blk) {
     }
   }
   const float wx2u[2] = {d.wx2[ua], d.wx2[ub]};
-  load_twiddles(T, d.twy, n);
+  load_stage_twiddles<N>(TW, d.twy);
 #pragma unroll
   for (int r = 0; r < MAXC; r++) {
     const int v = threadIdx.x + r * blockDim.x;
-    if (v < n) A[mk_pos(v, n)] = colv[r];
+    if (v < N) A[lpad(mk_pos(v, N))] = colv[r];
   }
   __syncthreads();
-  float2* Z = lds_fft(A, B, n, T, -1);
-  const float scale4 = (float)(4.0 / (double)d.nbx / (double)n);
+  const float2* Z = fft_n<N, -1>(A, B, TW);
+  const float scale4 = (float)(4.0 / (double)d.nbx / (double)N);
 #pragma unroll
   for (int rr = 0; rr < MAXC; rr++) {
     const int v = threadIdx.x + rr * blockDim.x;
-    if (v >= n) continue;
+    if (v >= N) continue;
     float xa, xb;
-    dct2_post_w(Z, n, v, tw[rr], xa, xb);
-    float wy = wyv[rr], wy2 = wy2v[rr];
-    float f[2] = {xa, xb};
-    int us[2] = {ua, ub};
+    dct2_post<N>(Z, v, tw[rr], xa, xb);
+    const float wy = wyv[rr], wy2 = wy2v[rr];
+    const float f[2] = {xa, xb};
+    const int us[2] = {ua, ub};
     for (int r = 0; r < 2; r++) {
-      int u = us[r];
+      const int u = us[r];
       float F = f[r];
       if (v == 0) F *= 0.5f;
       if (u == 0) F *= 0.5f;
@@ -760,36 +781,33 @@ __device__ void col_body(const RpDev& d, int blk) {
 #pragma unroll
   for (int rr = 0; rr < MAXC; rr++) {
     const int k = threadIdx.x + rr * blockDim.x;
-    if (k >= n) continue;
-    int nk = (n - k) & (n - 1);
-    float ank = k ? pa[nk] : 0.0f, bnk = k ? pb[nk] : 0.0f;
-    A[k] = dct3_pre_w(pa[k], ank, pb[k], bnk, k, tw[rr]);
+    if (k >= N) continue;
+    const int nk = (N - k) & (N - 1);
+    const float ank = k ? pa[nk] : 0.0f, bnk = k ? pb[nk] : 0.0f;
+    A[lpad(k)] = dct3_pre(pa[k], ank, pb[k], bnk, k, tw[rr]);
   }
   __syncthreads();
-  Z = lds_fft(A, B, n, T, +1);
-  for (int m = threadIdx.x; m < n; m += blockDim.x) {
-    // y[2q] = w[q], y[2q+1] = w[N-1-q]
-    float2 z = Z[(m & 1) ? (n - 1 - (m >> 1)) : (m >> 1)];
-    *(float2*)(d.y1 + m * nx + ua) = z;  // row-major, as the row pass reads it
-  }
+  Z = fft_n<N, +1>(A, B, TW);
+  for (int m = threadIdx.x; m < N; m += blockDim.x)
+    *(float2*)(d.y1 + m * nx + ua) = dct3_out<N>(Z, m);  // row-major, as the row pass reads it
   __syncthreads();
   // DST-III of Ey hat: b'[0] = a[0] (the A[N] slot), b'[j] = a[N-j]; sign (-1)^k
 #pragma unroll
   for (int rr = 0; rr < MAXC; rr++) {
     const int k = threadIdx.x + rr * blockDim.x;
-    if (k >= n) continue;
-    int nk = (n - k) & (n - 1);
-    float ak = k ? ea[n - k] : ea[0];
-    float bk = k ? eb[n - k] : eb[0];
-    float ank = k ? (nk ? ea[n - nk] : ea[0]) : 0.0f;
-    float bnk = k ? (nk ? eb[n - nk] : eb[0]) : 0.0f;
-    A[k] = dct3_pre_w(ak, ank, bk, bnk, k, tw[rr]);
+    if (k >= N) continue;
+    const int nk = (N - k) & (N - 1);
+    const float ak = k ? ea[N - k] : ea[0];
+    const float bk = k ? eb[N - k] : eb[0];
+    const float ank = k ? (nk ? ea[N - nk] : ea[0]) : 0.0f;
+    const float bnk = k ? (nk ? eb[N - nk] : eb[0]) : 0.0f;
+    A[lpad(k)] = dct3_pre(ak, ank, bk, bnk, k, tw[rr]);
   }
   __syncthreads();
-  Z = lds_fft(A, B, n, T, +1);
-  for (int m = threadIdx.x; m < n; m += blockDim.x) {
-    float2 z = Z[(m & 1) ? (n - 1 - (m >> 1)) : (m >> 1)];
-    float sg = (m & 1) ? -1.0f : 1.0f;
+  Z = fft_n<N, +1>(A, B, TW);
+  for (int m = threadIdx.x; m < N; m += blockDim.x) {
+    const float2 z = dct3_out<N>(Z, m);
+    const float sg = (m & 1) ? -1.0f : 1.0f;
     *(float2*)(d.y2 + m * nx + ua) = make_float2(sg * z.x, sg * z.y);
   }
 }
@@ -797,41 +815,41 @@ __device__ void col_body(const RpDev& d, int blk) {
 // ---------------------------------------------------------------------------
 // 5. inverse along x: Ex = DST-III_x(wx * Y1), Ey = DCT-III_x(Y2), phi optional
 // ---------------------------------------------------------------------------
+template <int N>
 __device__ void rowinv_body(const RpDev& d, int blk, int want_phi) {
   extern __shared__ float2 smem[];
   if (d.st->phase != 0) return;
-  const int n = d.nbx;
   float2* A = smem;
-  float2* B = smem + n;
-  float2* T = smem + 2 * n;
-  float* ra = (float*)(smem + 3 * n);  // input rows a, b, then Ex rows a, b
-  float* rb = ra + n;
-  float* qa = rb + n;  // second input rows a, b (Y2, then Y1 for phi)
-  float* qb = qa + n;
-  int ja = 2 * blk, jb = ja + 1;
+  float2* B = smem + fft_buf(N);
+  float2* TW = smem + 2 * fft_buf(N);
+  float* ra = (float*)(TW + N);  // input rows a, b, then Ex rows a, b
+  float* rb = ra + N;
+  float* qa = rb + N;  // second input rows a, b (Y2, then Y1 for phi)
+  float* qb = qa + N;
+  const int ja = 2 * blk, jb = ja + 1;
   // the thread's loads of every phase first (its entries u = threadIdx.x +
   // r blockDim.x): the two rows of y1 and y2, wx and the DCT-III twiddles
-  constexpr int MAXC = RP_MAX_FFT / RP_BLOCK;
+  constexpr int MAXC = (N + RP_BLOCK - 1) / RP_BLOCK;
   float wv[MAXC], y1a[MAXC], y1b[MAXC], y2a[MAXC], y2b[MAXC];
   float2 tw[MAXC];
 #pragma unroll
   for (int r = 0; r < MAXC; r++) {
     const int u = threadIdx.x + r * blockDim.x;
-    if (u < n) {
+    if (u < N) {
       wv[r] = d.wx[u];
-      y1a[r] = d.y1[(size_t)ja * n + u];
-      y1b[r] = d.y1[(size_t)jb * n + u];
-      y2a[r] = d.y2[(size_t)ja * n + u];
-      y2b[r] = d.y2[(size_t)jb * n + u];
+      y1a[r] = d.y1[(size_t)ja * N + u];
+      y1b[r] = d.y1[(size_t)jb * N + u];
+      y2a[r] = d.y2[(size_t)ja * N + u];
+      y2b[r] = d.y2[(size_t)jb * N + u];
       tw[r] = d.tw2x[u];
     }
   }
-  load_twiddles(T, d.twx, n);
+  load_stage_twiddles<N>(TW, d.twx);
   // Ex: DST-III of (wx[u] * Y1[j][u]); Ey: DCT-III of Y2 rows
 #pragma unroll
   for (int r = 0; r < MAXC; r++) {
     const int u = threadIdx.x + r * blockDim.x;
-    if (u < n) {
+    if (u < N) {
       ra[u] = wv[r] * y1a[r];
       rb[u] = wv[r] * y1b[r];
       qa[u] = y2a[r];
@@ -842,19 +860,19 @@ __device__ void rowinv_body(const RpDev& d, int blk, int want_phi) {
 #pragma unroll
   for (int r = 0; r < MAXC; r++) {
     const int k = threadIdx.x + r * blockDim.x;
-    if (k >= n) continue;
-    int nk = (n - k) & (n - 1);
-    float ak = k ? ra[n - k] : ra[0];
-    float bk = k ? rb[n - k] : rb[0];
-    float ank = k ? (nk ? ra[n - nk] : ra[0]) : 0.0f;
-    float bnk = k ? (nk ? rb[n - nk] : rb[0]) : 0.0f;
-    A[k] = dct3_pre_w(ak, ank, bk, bnk, k, tw[r]);
+    if (k >= N) continue;
+    const int nk = (N - k) & (N - 1);
+    const float ak = k ? ra[N - k] : ra[0];
+    const float bk = k ? rb[N - k] : rb[0];
+    const float ank = k ? (nk ? ra[N - nk] : ra[0]) : 0.0f;
+    const float bnk = k ? (nk ? rb[N - nk] : rb[0]) : 0.0f;
+    A[lpad(k)] = dct3_pre(ak, ank, bk, bnk, k, tw[r]);
   }
   __syncthreads();
-  float2* Z = lds_fft(A, B, n, T, +1);
-  for (int m = threadIdx.x; m < n; m += blockDim.x) {
-    float2 z = Z[(m & 1) ? (n - 1 - (m >> 1)) : (m >> 1)];
-    float sg = (m & 1) ? -1.0f : 1.0f;
+  const float2* Z = fft_n<N, +1>(A, B, TW);
+  for (int m = threadIdx.x; m < N; m += blockDim.x) {
+    const float2 z = dct3_out<N>(Z, m);
+    const float sg = (m & 1) ? -1.0f : 1.0f;
     ra[m] = sg * z.x;  // Ex row a (ra no longer needed as input)
     rb[m] = sg * z.y;
   }
@@ -862,36 +880,284 @@ __device__ void rowinv_body(const RpDev& d, int blk, int want_phi) {
 #pragma unroll
   for (int r = 0; r < MAXC; r++) {
     const int k = threadIdx.x + r * blockDim.x;
-    if (k >= n) continue;
-    int nk = (n - k) & (n - 1);
-    A[k] = dct3_pre_w(qa[k], k ? qa[nk] : 0.0f, qb[k], k ? qb[nk] : 0.0f, k, tw[r]);
+    if (k >= N) continue;
+    const int nk = (N - k) & (N - 1);
+    A[lpad(k)] = dct3_pre(qa[k], k ? qa[nk] : 0.0f, qb[k], k ? qb[nk] : 0.0f, k, tw[r]);
   }
   __syncthreads();
-  Z = lds_fft(A, B, n, T, +1);
-  for (int m = threadIdx.x; m < n; m += blockDim.x) {
-    float2 z = Z[(m & 1) ? (n - 1 - (m >> 1)) : (m >> 1)];
-    d.field[(size_t)ja * n + m] = make_float2(ra[m], z.x);
-    d.field[(size_t)jb * n + m] = make_float2(rb[m], z.y);
+  Z = fft_n<N, +1>(A, B, TW);
+  for (int m = threadIdx.x; m < N; m += blockDim.x) {
+    const float2 z = dct3_out<N>(Z, m);
+    d.field[(size_t)ja * N + m] = make_float2(ra[m], z.x);
+    d.field[(size_t)jb * N + m] = make_float2(rb[m], z.y);
   }
   if (!want_phi) return;
   __syncthreads();
-  for (int u = threadIdx.x; u < n; u += blockDim.x) {
-    qa[u] = d.y1[(size_t)ja * n + u];
-    qb[u] = d.y1[(size_t)jb * n + u];
+  for (int u = threadIdx.x; u < N; u += blockDim.x) {
+    qa[u] = d.y1[(size_t)ja * N + u];
+    qb[u] = d.y1[(size_t)jb * N + u];
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
-    int nk = (n - k) & (n - 1);
-    A[k] = dct3_pre(qa[k], k ? qa[nk] : 0.0f, qb[k], k ? qb[nk] : 0.0f, k, d.tw2x);
+#pragma unroll
+  for (int r = 0; r < MAXC; r++) {
+    const int k = threadIdx.x + r * blockDim.x;
+    if (k >= N) continue;
+    const int nk = (N - k) & (N - 1);
+    A[lpad(k)] = dct3_pre(qa[k], k ? qa[nk] : 0.0f, qb[k], k ? qb[nk] : 0.0f, k, tw[r]);
   }
   __syncthreads();
-  Z = lds_fft(A, B, n, T, +1);
-  for (int m = threadIdx.x; m < n; m += blockDim.x) {
-    float2 z = Z[(m & 1) ? (n - 1 - (m >> 1)) : (m >> 1)];
-    d.phi[(size_t)ja * n + m] = z.x;
-    d.phi[(size_t)jb * n + m] = z.y;
+  Z = fft_n<N, +1>(A, B, TW);
+  for (int m = threadIdx.x; m < N; m += blockDim.x) {
+    const float2 z = dct3_out<N>(Z, m);
+    d.phi[(size_t)ja * N + m] = z.x;
+    d.phi[(size_t)jb * N + m] = z.y;
   }
 }
+
+// ---------------------------------------------------------------------------
+// Wide forms of the column and inverse passes (single instance, RP_WIDE
+// threads per workgroup): the two halves of the workgroup transform side by
+// side.
+//  * col4_body: four adjacent columns per workgroup (two column pairs, one
+//    per half), read and written as 16-byte row segments, so a wavefront's
+//    access touches half as many cache lines as two columns at a time.
+//  * rowinv2_body: the two inverse transforms of a row pair (Ex and Ey) in
+//    parallel, one per half.
+// The arithmetic of every value is that of col_body / rowinv_body.
+// ---------------------------------------------------------------------------
+#define RP_WIDE 512
+
+inline size_t rp_lds_col4(int n) { return sizeof(float2) * (4 * (size_t)(n + n / 16) + (size_t)n) + sizeof(float) * 8 * (size_t)n; }
+inline size_t rp_lds_rinv2(int n) { return rp_lds_col4(n) - sizeof(float) * 4 * (size_t)n; }
+
+template <int N>
+__device__ void col4_body(const RpDev& d, int blk) {
+  extern __shared__ float2 smem[];
+  if (d.st->phase != 0) return;
+  const int h = threadIdx.x / RP_BLOCK, lt = threadIdx.x % RP_BLOCK;
+  float2* A = smem + 2 * h * fft_buf(N);
+  float2* B = A + fft_buf(N);
+  float2* TW = smem + 4 * fft_buf(N);
+  float* stage = (float*)(TW + N) + 4 * h * N;  // this half's phi hat columns a, b; Ey hat columns a, b
+  float* pa = stage;
+  float* pb = pa + N;
+  float* ea = pb + N;
+  float* eb = ea + N;
+  const int u0 = 4 * blk, ua = u0 + 2 * h, ub = ua + 1;
+  const size_t nx = (size_t)d.nbx;
+  constexpr int MAXW = (N + RP_WIDE - 1) / RP_WIDE;   // rows per thread, whole workgroup
+  constexpr int MAXC = (N + RP_BLOCK - 1) / RP_BLOCK; // rows per thread, one half
+  float4 colv[MAXW];
+  float2 tw[MAXC];
+  float wyv[MAXC], wy2v[MAXC];
+#pragma unroll
+  for (int r = 0; r < MAXW; r++) {
+    const int v = threadIdx.x + r * RP_WIDE;
+    if (v < N) colv[r] = *(const float4*)(d.t + v * nx + u0);
+  }
+#pragma unroll
+  for (int r = 0; r < MAXC; r++) {
+    const int v = lt + r * RP_BLOCK;
+    if (v < N) {
+      tw[r] = d.tw2y[v];
+      wyv[r] = d.wy[v];
+      wy2v[r] = d.wy2[v];
+    }
+  }
+  const float wx2u[2] = {d.wx2[ua], d.wx2[ub]};
+  load_stage_twiddles<N>(TW, d.twy);
+  float2* A0 = smem;
+  float2* A1 = smem + 2 * fft_buf(N);
+#pragma unroll
+  for (int r = 0; r < MAXW; r++) {
+    const int v = threadIdx.x + r * RP_WIDE;
+    if (v < N) {
+      const int p = lpad(mk_pos(v, N));
+      A0[p] = make_float2(colv[r].x, colv[r].y);
+      A1[p] = make_float2(colv[r].z, colv[r].w);
+    }
+  }
+  __syncthreads();
+  const float2* Z = fft_n<N, -1>(A, B, TW, lt, RP_BLOCK);
+  const float scale4 = (float)(4.0 / (double)d.nbx / (double)N);
+#pragma unroll
+  for (int rr = 0; rr < MAXC; rr++) {
+    const int v = lt + rr * RP_BLOCK;
+    if (v >= N) continue;
+    float xa, xb;
+    dct2_post<N>(Z, v, tw[rr], xa, xb);
+    const float wy = wyv[rr], wy2 = wy2v[rr];
+    const float f[2] = {xa, xb};
+    const int us[2] = {ua, ub};
+    for (int r = 0; r < 2; r++) {
+      const int u = us[r];
+      float F = f[r];
+      if (v == 0) F *= 0.5f;
+      if (u == 0) F *= 0.5f;
+      F = F * scale4;
+      float ph = 0.0f, ey = 0.0f;
+      if (!(u == 0 && v == 0)) {
+        ph = F / (wx2u[r] + wy2);
+        ey = ph * wy;
+      }
+      (r ? pb : pa)[v] = ph;
+      (r ? eb : ea)[v] = ey;
+    }
+  }
+  __syncthreads();
+  // DCT-III of phi hat -> y1
+#pragma unroll
+  for (int rr = 0; rr < MAXC; rr++) {
+    const int k = lt + rr * RP_BLOCK;
+    if (k >= N) continue;
+    const int nk = (N - k) & (N - 1);
+    const float ank = k ? pa[nk] : 0.0f, bnk = k ? pb[nk] : 0.0f;
+    A[lpad(k)] = dct3_pre(pa[k], ank, pb[k], bnk, k, tw[rr]);
+  }
+  __syncthreads();
+  Z = fft_n<N, +1>(A, B, TW, lt, RP_BLOCK);
+  // both halves' results sit in the same array of their pair of buffers
+  const float2* Z0 = Z - 2 * h * fft_buf(N);
+  const float2* Z1 = Z0 + 2 * fft_buf(N);
+  for (int m = threadIdx.x; m < N; m += RP_WIDE) {
+    const float2 z0 = dct3_out<N>(Z0, m), z1 = dct3_out<N>(Z1, m);
+    *(float4*)(d.y1 + m * nx + u0) = make_float4(z0.x, z0.y, z1.x, z1.y);
+  }
+  __syncthreads();
+  // DST-III of Ey hat: b'[0] = a[0] (the A[N] slot), b'[j] = a[N-j]; sign (-1)^k
+#pragma unroll
+  for (int rr = 0; rr < MAXC; rr++) {
+    const int k = lt + rr * RP_BLOCK;
+    if (k >= N) continue;
+    const int nk = (N - k) & (N - 1);
+    const float ak = k ? ea[N - k] : ea[0];
+    const float bk = k ? eb[N - k] : eb[0];
+    const float ank = k ? (nk ? ea[N - nk] : ea[0]) : 0.0f;
+    const float bnk = k ? (nk ? eb[N - nk] : eb[0]) : 0.0f;
+    A[lpad(k)] = dct3_pre(ak, ank, bk, bnk, k, tw[rr]);
+  }
+  __syncthreads();
+  Z = fft_n<N, +1>(A, B, TW, lt, RP_BLOCK);
+  const float2* W0 = Z - 2 * h * fft_buf(N);
+  const float2* W1 = W0 + 2 * fft_buf(N);
+  for (int m = threadIdx.x; m < N; m += RP_WIDE) {
+    const float2 z0 = dct3_out<N>(W0, m), z1 = dct3_out<N>(W1, m);
+    const float sg = (m & 1) ? -1.0f : 1.0f;
+    *(float4*)(d.y2 + m * nx + u0) = make_float4(sg * z0.x, sg * z0.y, sg * z1.x, sg * z1.y);
+  }
+}
+
+template <int N>
+__device__ void rowinv2_body(const RpDev& d, int blk, int want_phi) {
+  extern __shared__ float2 smem[];
+  if (d.st->phase != 0) return;
+  const int h = threadIdx.x / RP_BLOCK, lt = threadIdx.x % RP_BLOCK;
+  float2* A = smem + 2 * h * fft_buf(N);
+  float2* B = A + fft_buf(N);
+  float2* TW = smem + 4 * fft_buf(N);
+  float* ra = (float*)(TW + N);  // Ex input rows a, b (wx * Y1)
+  float* rb = ra + N;
+  float* qa = rb + N;  // Ey input rows a, b (Y2)
+  float* qb = qa + N;
+  const int ja = 2 * blk, jb = ja + 1;
+  constexpr int MAXW = (N + RP_WIDE - 1) / RP_WIDE;
+  constexpr int MAXC = (N + RP_BLOCK - 1) / RP_BLOCK;
+  float wv[MAXW], y1a[MAXW], y1b[MAXW], y2a[MAXW], y2b[MAXW];
+  float2 tw[MAXC];
+#pragma unroll
+  for (int r = 0; r < MAXW; r++) {
+    const int u = threadIdx.x + r * RP_WIDE;
+    if (u < N) {
+      wv[r] = d.wx[u];
+      y1a[r] = d.y1[(size_t)ja * N + u];
+      y1b[r] = d.y1[(size_t)jb * N + u];
+      y2a[r] = d.y2[(size_t)ja * N + u];
+      y2b[r] = d.y2[(size_t)jb * N + u];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < MAXC; r++) {
+    const int k = lt + r * RP_BLOCK;
+    if (k < N) tw[r] = d.tw2x[k];
+  }
+  load_stage_twiddles<N>(TW, d.twx);
+#pragma unroll
+  for (int r = 0; r < MAXW; r++) {
+    const int u = threadIdx.x + r * RP_WIDE;
+    if (u < N) {
+      ra[u] = wv[r] * y1a[r];
+      rb[u] = wv[r] * y1b[r];
+      qa[u] = y2a[r];
+      qb[u] = y2b[r];
+    }
+  }
+  __syncthreads();
+  // half 0: DST-III of wx * Y1 (Ex); half 1: DCT-III of Y2 (Ey)
+#pragma unroll
+  for (int r = 0; r < MAXC; r++) {
+    const int k = lt + r * RP_BLOCK;
+    if (k >= N) continue;
+    const int nk = (N - k) & (N - 1);
+    if (h == 0) {
+      const float ak = k ? ra[N - k] : ra[0];
+      const float bk = k ? rb[N - k] : rb[0];
+      const float ank = k ? (nk ? ra[N - nk] : ra[0]) : 0.0f;
+      const float bnk = k ? (nk ? rb[N - nk] : rb[0]) : 0.0f;
+      A[lpad(k)] = dct3_pre(ak, ank, bk, bnk, k, tw[r]);
+    } else {
+      A[lpad(k)] = dct3_pre(qa[k], k ? qa[nk] : 0.0f, qb[k], k ? qb[nk] : 0.0f, k, tw[r]);
+    }
+  }
+  __syncthreads();
+  const float2* Z = fft_n<N, +1>(A, B, TW, lt, RP_BLOCK);
+  const float2* Z0 = Z - 2 * h * fft_buf(N);
+  const float2* Z1 = Z0 + 2 * fft_buf(N);
+  for (int m = threadIdx.x; m < N; m += RP_WIDE) {
+    const float2 zx = dct3_out<N>(Z0, m), zy = dct3_out<N>(Z1, m);
+    const float sg = (m & 1) ? -1.0f : 1.0f;
+    d.field[(size_t)ja * N + m] = make_float2(sg * zx.x, zy.x);
+    d.field[(size_t)jb * N + m] = make_float2(sg * zx.y, zy.y);
+  }
+  if (!want_phi) return;
+  // phi: DCT-III of Y1 rows (half 0; half 1 only meets the barriers)
+  __syncthreads();
+  for (int u = threadIdx.x; u < N; u += RP_WIDE) {
+    qa[u] = d.y1[(size_t)ja * N + u];
+    qb[u] = d.y1[(size_t)jb * N + u];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < MAXC; r++) {
+    const int k = lt + r * RP_BLOCK;
+    if (h != 0 || k >= N) continue;
+    const int nk = (N - k) & (N - 1);
+    A[lpad(k)] = dct3_pre(qa[k], k ? qa[nk] : 0.0f, qb[k], k ? qb[nk] : 0.0f, k, tw[r]);
+  }
+  __syncthreads();
+  Z = fft_n<N, +1>(A, B, TW, lt, RP_BLOCK);
+  if (h != 0) return;
+  for (int m = lt; m < N; m += RP_BLOCK) {
+    const float2 z = dct3_out<N>(Z, m);
+    d.phi[(size_t)ja * N + m] = z.x;
+    d.phi[(size_t)jb * N + m] = z.y;
+  }
+}
+
+// Calls BODY<n>(args) for the transform length n of the grid side at hand.
+#define RP_FFT_DISPATCH(n, BODY, ...)        \
+  switch (n) {                               \
+    case 4: BODY<4>(__VA_ARGS__); break;     \
+    case 8: BODY<8>(__VA_ARGS__); break;     \
+    case 16: BODY<16>(__VA_ARGS__); break;   \
+    case 32: BODY<32>(__VA_ARGS__); break;   \
+    case 64: BODY<64>(__VA_ARGS__); break;   \
+    case 128: BODY<128>(__VA_ARGS__); break; \
+    case 256: BODY<256>(__VA_ARGS__); break; \
+    case 512: BODY<512>(__VA_ARGS__); break; \
+    case 1024: BODY<1024>(__VA_ARGS__); break; \
+    case 2048: BODY<2048>(__VA_ARGS__); break; \
+    default: break;                          \
+  }
 
 // ---------------------------------------------------------------------------
 // 6. WA exponent sums per net (src/nesterovBase.cpp:1304-1360) + HPWL
@@ -1644,11 +1910,19 @@ static void launch_scatter_pins(const RpDev& d, hipStream_t s, const RpSortBufs&
 }
 __global__ void __launch_bounds__(RP_BLOCK) k_pin_pos(RpDev d) { pinpos_body(d, xcd_block(blockIdx.x, gridDim.x)); }
 __global__ void __launch_bounds__(RP_BLOCK) k_binfin_rowdct(RpDev d, int from_rho) {
-  binfin_body(d, blockIdx.x, from_rho);
+  RP_FFT_DISPATCH(d.nbx, binfin_body, d, blockIdx.x, from_rho)
 }
-__global__ void __launch_bounds__(RP_BLOCK) k_col_spectral(RpDev d) { col_body(d, xcd_block(blockIdx.x, gridDim.x)); }
+__global__ void __launch_bounds__(RP_BLOCK) k_col_spectral(RpDev d) {
+  RP_FFT_DISPATCH(d.nby, col_body, d, xcd_block(blockIdx.x, gridDim.x))
+}
+__global__ void __launch_bounds__(RP_WIDE) k_col_spectral4(RpDev d) {
+  RP_FFT_DISPATCH(d.nby, col4_body, d, xcd_block(blockIdx.x, gridDim.x))
+}
+__global__ void __launch_bounds__(RP_WIDE) k_row_inverse2(RpDev d, int want_phi) {
+  RP_FFT_DISPATCH(d.nbx, rowinv2_body, d, blockIdx.x, want_phi)
+}
 __global__ void __launch_bounds__(RP_BLOCK) k_row_inverse(RpDev d, int want_phi) {
-  rowinv_body(d, blockIdx.x, want_phi);
+  RP_FFT_DISPATCH(d.nbx, rowinv_body, d, blockIdx.x, want_phi)
 }
 __global__ void __launch_bounds__(RP_BLOCK) k_wa(RpDev d, int store_nets) {
   wa_body<RP_WA_CAP>(d, xcd_block(blockIdx.x, gridDim.x), store_nets);
@@ -1667,15 +1941,15 @@ __global__ void __launch_bounds__(RP_BLOCK) kb_pin_pos(const RpDev* devs, const 
 }
 __global__ void __launch_bounds__(RP_BLOCK) kb_binfin_rowdct(const RpDev* devs, const int2* tab) {
   int2 e = tab[blockIdx.x];
-  if (e.x >= 0) binfin_body(devs[e.x], e.y, 0);
+  if (e.x >= 0) RP_FFT_DISPATCH(devs[e.x].nbx, binfin_body, devs[e.x], e.y, 0)
 }
 __global__ void __launch_bounds__(RP_BLOCK) kb_col_spectral(const RpDev* devs, const int2* tab) {
   int2 e = tab[blockIdx.x];
-  if (e.x >= 0) col_body(devs[e.x], e.y);
+  if (e.x >= 0) RP_FFT_DISPATCH(devs[e.x].nby, col_body, devs[e.x], e.y)
 }
 __global__ void __launch_bounds__(RP_BLOCK) kb_row_inverse(const RpDev* devs, const int2* tab) {
   int2 e = tab[blockIdx.x];
-  if (e.x >= 0) rowinv_body(devs[e.x], e.y, 0);
+  if (e.x >= 0) RP_FFT_DISPATCH(devs[e.x].nbx, rowinv_body, devs[e.x], e.y, 0)
 }
 template <int CAP>
 __global__ void __launch_bounds__(RP_BLOCK) kb_wa(const RpDev* devs, const int2* tab) {
@@ -1697,6 +1971,24 @@ __global__ void __launch_bounds__(RP_CTRL_BLOCK) kb_control_main(const RpDev* de
 // FFT-based kernels always run RP_BLOCK threads: the per-thread partial
 // sums (overflow) then group identically in single and batched launches.
 static int fft_threads(int) { return RP_BLOCK; }
+
+// Column and inverse passes of the single-instance path: the wide forms
+// (two transforms side by side per workgroup) from 256 bins per side, where
+// they measured faster; the one-pair forms below.
+#define RP_WIDE_FROM 256
+static void launch_col(const RpDev& d, hipStream_t s) {
+  if (d.nbx >= RP_WIDE_FROM && d.nby >= RP_WIDE_FROM)
+    hipLaunchKernelGGL(k_col_spectral4, dim3(d.nbx / 4), dim3(RP_WIDE), rp_lds_col4(d.nby), s, d);
+  else
+    hipLaunchKernelGGL(k_col_spectral, dim3(d.nbx / 2), dim3(fft_threads(d.nby)), rp_lds_col(d.nby), s, d);
+}
+static void launch_rowinv(const RpDev& d, hipStream_t s, int want_phi) {
+  if (d.nbx >= RP_WIDE_FROM && d.nby >= RP_WIDE_FROM)
+    hipLaunchKernelGGL(k_row_inverse2, dim3(d.nby / 2), dim3(RP_WIDE), rp_lds_rinv2(d.nbx), s, d, want_phi);
+  else
+    hipLaunchKernelGGL(k_row_inverse, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), rp_lds_rinv(d.nbx), s, d,
+                       want_phi);
+}
 
 // One evaluation slot.  ev (optional) holds 2 * RP_NUM_EVAL_KERNELS events:
 // ev[2q] and ev[2q + 1] bracket kernel q when it is launched on this path;
@@ -1729,11 +2021,10 @@ hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_
     end(RP_K_MFMA_POISSON);
   } else {
     beg(RP_K_COL_SPECTRAL);
-    hipLaunchKernelGGL(k_col_spectral, dim3(d.nbx / 2), dim3(fft_threads(d.nby)), rp_lds_col(d.nby), s, d);
+    launch_col(d, s);
     end(RP_K_COL_SPECTRAL);
     beg(RP_K_ROW_INVERSE);
-    hipLaunchKernelGGL(k_row_inverse, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), rp_lds_rinv(d.nbx), s, d,
-                       want_phi);
+    launch_rowinv(d, s, want_phi);
     end(RP_K_ROW_INVERSE);
   }
   beg(RP_K_WA);
@@ -1765,10 +2056,8 @@ hipError_t rp_launch_poisson(const RpDev& d, hipStream_t s, int want_phi, int fr
   size_t shx = rp_lds_binfin(d.nbx);
   hipLaunchKernelGGL(k_binfin_rowdct, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shx, s, d, from_rho);
   if (mf) return rp_launch_poisson_mfma(d, *mf, s, want_phi);
-  size_t shy = rp_lds_col(d.nby);
-  hipLaunchKernelGGL(k_col_spectral, dim3(d.nbx / 2), dim3(fft_threads(d.nby)), shy, s, d);
-  size_t shr = rp_lds_rinv(d.nbx);
-  hipLaunchKernelGGL(k_row_inverse, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), shr, s, d, want_phi);
+  launch_col(d, s);
+  launch_rowinv(d, s, want_phi);
   return hipGetLastError();
 }
 
