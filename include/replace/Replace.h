@@ -14,6 +14,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "replace/rp_capi.h"
@@ -70,6 +71,11 @@ class Replace {
   // Host SoA netlist that stands in for OpenDB; borrowed until the next
   // doInitialPlace()/doNesterovPlace() returns.
   void setNetlist(const rp_netlist* netlist);
+  // Names of the netlist's pins (by pin index; only top-level ports are
+  // read), for the "[WARN] <port> toplevel port is not placed!" lines of
+  // PlacerBase (src/placerBase.cpp:394-399).  Without names a port is
+  // called "port<k>".
+  void setPinNames(const std::vector<std::string>& names);
   // Instance lower-left corners after placement (what updateDb writes).
   bool getPositions(std::vector<int32_t>& lx, std::vector<int32_t>& ly) const;
   // When false, fatal REPL codes are returned by lastCode() instead of
@@ -111,9 +117,14 @@ class Replace {
   int lastCode_;
   rp_nesterov_stats lastStats_;
 
+  std::vector<std::string> pinNames_;
+
   rp_params params() const;
   bool ensureContext();
   void fatal(int code, const char* msg);
+  void printPlacerInfo();
+  void printNesterovBaseInfo(const rp_nb_info& nb);
+  void printNesterovRun(int code, const std::vector<rp_iter_record>& rec);
 };
 }  // namespace replace
 

@@ -162,12 +162,31 @@ typedef struct rp_ip_stats {
 
 typedef struct rp_ctx rp_ctx;
 
+/* The values PlacerBase::init and PlacerBase::printInfo log
+ * (src/placerBase.cpp:609-626, 902-958): "[INFO] DBU", "SiteSize",
+ * "CoreArea*", "NumInstances" ... "MacroInstsArea".  num_insts counts the
+ * fragmented-row dummy instances; num_fixed_insts the fixed instances that
+ * overlap the core. */
+typedef struct rp_pb_info {
+  int32_t dbu_per_micron, site_w, site_h;
+  int32_t die_lx, die_ly, die_ux, die_uy;
+  int32_t core_lx, core_ly, core_ux, core_uy;
+  int32_t num_insts, num_place_insts, num_fixed_insts, num_dummy_insts, num_nets, num_pins;
+  int32_t num_unplaced_ports;   /* BTerm pins without a placed shape (the "[WARN] ... not placed" lines) */
+  int64_t core_area, nonplace_insts_area, place_insts_area, std_insts_area, macro_insts_area;
+  float util;                   /* percent, as "Util(%)" prints it */
+} rp_pb_info;
+
 void rp_params_default(rp_params* params);
 
 /* PlacerBase::init: instance classification, areas, fragmented-row dummy
  * instances, pins.  Returns 1 (REPL exit 1) when utilisation >= 100.1 %.
  * Device selection: the current HIP device. */
 int rp_create(const rp_netlist* netlist, const rp_params* params, rp_ctx** out);
+
+/* PlacerBase values of a context (valid after rp_create, also when it
+ * returned REPL-1). */
+int rp_get_pb_info(rp_ctx* ctx, rp_pb_info* info);
 
 /* Replace::doInitialPlace (B2B net model + BiCGSTAB on the device). */
 int rp_initial_place(rp_ctx* ctx, rp_ip_stats* stats);
@@ -247,6 +266,27 @@ int rp_set_positions(rp_ctx* ctx, const int32_t* lx, const int32_t* ly);
 void rp_destroy(rp_ctx* ctx);
 const char* rp_last_error(rp_ctx* ctx);
 
+/* Verbose-3 trace (rp_params.verbose >= 3): one record per gradient
+ * evaluation of the last rp_nesterov_place / rp_nesterov_iterate run, with
+ * the values the reference logs at verbose level 3 in updateGradients,
+ * getStepLength and updateNextIter (src/nesterovPlace.cpp:206-273, 514-571):
+ * density penalty, the three gradient sums, the two distances and the new
+ * step length, the new gradient's second norm, sum phi * area (sumPhi,
+ * src/nesterovBase.cpp:1493-1505; 0 with the matrix-core transform), the
+ * overflow and the HPWL of the evaluated iterate.  iter is the outer
+ * iteration (0-based), -1 for the two NesterovPlace::init evaluations;
+ * trial the back-tracking trial.  Sums are FP64 reductions, so values
+ * differ from the reference's float sums in the last digits. */
+typedef struct rp_eval_record {
+  int32_t iter, trial, diverged;
+  float density_penalty, wl_coef;
+  float wl_grad_sum, density_grad_sum, grad_sum;
+  float coordinate_distance, gradient_distance, new_step_length;
+  float gradient_norm, sum_phi, overflow;
+  int64_t hpwl;
+} rp_eval_record;
+int rp_get_eval_records(rp_ctx* ctx, rp_eval_record* out, int32_t capacity, int32_t* count);
+
 /* Per-kernel timing of the evaluation slots (no reference counterpart: the
  * reference has no timers).  When enabled, rp_nesterov_place brackets every
  * kernel of every executed slot with HIP events on the context's stream and
@@ -279,6 +319,10 @@ typedef struct rp_nb_info {
   int32_t filler_dx, filler_dy;
   int64_t std_insts_area, macro_insts_area, place_insts_area, nonplace_insts_area;
   float target_density;
+  /* the values NesterovBase::init logs (src/nesterovBase.cpp:645-649,
+   * 1127-1133): FillerInit areas and the automatic bin-count inputs */
+  int64_t core_area, white_space_area, movable_area, total_filler_area;
+  int64_t average_place_inst_area, ideal_bin_area, ideal_bin_count, total_bin_area;
 } rp_nb_info;
 
 /* NesterovBase::init only (gcells, fillers, bin grid, density sizes). */

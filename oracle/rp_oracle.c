@@ -999,6 +999,14 @@ int ro_nesterov_setup(ro_ctx* c, rp_nb_info* info) {
     info->place_insts_area = c->place_area;
     info->nonplace_insts_area = c->nonplace_area;
     info->target_density = c->td;
+    info->core_area = coreArea;
+    info->white_space_area = whiteSpace;
+    info->movable_area = movable;
+    info->total_filler_area = totalFiller;
+    info->average_place_inst_area = avgArea;
+    info->ideal_bin_area = idealBinArea;
+    info->ideal_bin_count = idealBinCnt;
+    info->total_bin_area = totalBinArea;
   }
   /* updateDensityCoordiLayoutInside for every gcell, NesterovPlace::init
    * src/nesterovPlace.cpp:86-93 -> src/nesterovBase.cpp:1245-1268 */

@@ -12,7 +12,7 @@ from typing import Optional
 
 import numpy as np
 
-from .netlist import (Netlist, RpEvalOut, RpIpStats, RpIterRecord, RpJobResult, RpNbInfo, RpNesterovState,
+from .netlist import (Netlist, RpEvalOut, RpEvalRecord, RpIpStats, RpIterRecord, RpJobResult, RpNbInfo, RpNesterovState, RpPbInfo,
                       RpNesterovStats, RpParams, RpServerOpts, RpServerStats, RpTileChoice, nesterov_state_buffers,
                       nesterov_state_values, params_default)
 
@@ -49,6 +49,8 @@ def lib() -> ctypes.CDLL:
         L.rp_initial_place.argtypes = [vp, ctypes.POINTER(RpIpStats)]
         L.rp_nesterov_place.argtypes = [vp, ctypes.POINTER(RpNesterovStats)]
         L.rp_nesterov_init.argtypes = [vp]
+        L.rp_get_pb_info.argtypes = [vp, ctypes.POINTER(RpPbInfo)]
+        L.rp_get_eval_records.argtypes = [vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
         L.rp_nesterov_iterate.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(RpNesterovStats)]
         L.rp_nesterov_get_state.argtypes = [vp, ctypes.POINTER(RpNesterovState)]
         L.rp_nesterov_set_state.argtypes = [vp, ctypes.POINTER(RpNesterovState)]
@@ -205,6 +207,19 @@ class Placer:
         st = RpNesterovStats()
         code = self._check(lib().rp_nesterov_place(self.h, ctypes.byref(st)))
         return code, st
+
+    def pb_info(self) -> RpPbInfo:
+        info = RpPbInfo()
+        self._check(lib().rp_get_pb_info(self.h, ctypes.byref(info)))
+        return info
+
+    def eval_records(self):
+        """Verbose-3 trace: one dict per gradient evaluation of the last run."""
+        n = ctypes.c_int32(0)
+        self._check(lib().rp_get_eval_records(self.h, None, 0, ctypes.byref(n)))
+        arr = (RpEvalRecord * max(1, n.value))()
+        self._check(lib().rp_get_eval_records(self.h, arr, n.value, ctypes.byref(n)))
+        return [{k: getattr(r, k) for k, _ in RpEvalRecord._fields_} for r in arr[: n.value]]
 
     def nesterov_init(self) -> int:
         return self._check(lib().rp_nesterov_init(self.h))

@@ -44,6 +44,8 @@ struct rp_ctx {
   hipStream_t stream = nullptr;
   // ---- PlacerBase ----
   int core_lx = 0, core_ly = 0, core_ux = 0, core_uy = 0;
+  int die_lx = 0, die_ly = 0, die_ux = 0, die_uy = 0, dbu = 0;
+  int n_unplaced_ports = 0;
   int site_w = 0, site_h = 0;
   int n_real = 0;
   std::vector<int> ilx, ily, iux, iuy, mw, mh, dblx, dbly;  // incl. dummies
@@ -94,6 +96,13 @@ struct rp_ctx {
   DevBuf<RpState> d_state;
   DevBuf<RpIterRec> d_rec;
   DevBuf<float2> d_dbg_wl, d_dbg_dens;
+  // verbose-3 trace
+  bool trace = false;
+  DevBuf<RpEvalRec> d_evrec;
+  DevBuf<double> d_part_gn, d_part_phi;
+  DevBuf<float> d_dbg_area;
+  int64_t nb_core_area = 0, nb_white = 0, nb_movable = 0, nb_total_filler = 0;
+  int64_t nb_avg_area = 0, nb_ideal_bin_area = 0, nb_ideal_bin_cnt = 0;
   RpState* h_state = nullptr;  // pinned
   std::vector<float2> init_xy; // initial centres (device order), kept alive for async upload
   std::vector<int> net_perm;   // device net id -> reference net id
@@ -124,6 +133,7 @@ struct rp_ctx {
     add(d_t); add(d_y1); add(d_y2); add(d_field); add(d_twx); add(d_tw2x);
     add(d_twy); add(d_tw2y); add(d_wx); add(d_wx2); add(d_wy); add(d_wy2); add(d_part_grad); add(d_part_bin);
     add(d_part_net); add(d_state); add(d_rec); add(d_dbg_wl); add(d_dbg_dens);
+    add(d_evrec); add(d_part_gn); add(d_part_phi); add(d_dbg_area);
     add(d_sord); add(d_srank); add(d_scnt); add(d_soff);
     for (const auto& w : d_wa_rng) add(w);
     return b + mfma.bytes();

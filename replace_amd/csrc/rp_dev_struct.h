@@ -63,6 +63,14 @@ struct RpDev {
   RpIterRec* rec;
   float2* dbg_wl;
   float2* dbg_dens;
+  // verbose-3 trace (all NULL unless the context traces): per-eval records,
+  // per-block partials of the gradient's squared norm and of sum phi * area,
+  // the per-bin area the bin kernel leaves for the latter
+  RpEvalRec* evrec;
+  int evrec_cap;
+  double* part_gn;
+  double* part_phi;
+  float* dbg_area;
 };
 
 // Dynamic LDS bytes of the FFT kernels for an n-point row / column: two

@@ -130,6 +130,20 @@ enum {
   RP_P_COUNT = 5
 };
 
+// One gradient evaluation's values for the verbose-3 trace (the layout of
+// rp_eval_record in rp_capi.h): written by the controller when the context
+// traces.
+struct RpEvalRec {
+  int iter;   // outer iteration (0-based); -1 for the two NesterovPlace::init evaluations
+  int trial;  // back-tracking trial within the iteration (init: 0 = at x0, 1 = at the previous SLP)
+  int diverged;
+  float density_penalty, wl_coef;
+  float wl_grad_sum, density_grad_sum, grad_sum;
+  float coordinate_distance, gradient_distance, new_step_length;
+  float gradient_norm, sum_phi, overflow;
+  long long hpwl;
+};
+
 struct RpIterRec {
   float overflow;
   float step_length;

@@ -135,6 +135,11 @@ extern "C" int rp_create(const rp_netlist* nl, const rp_params* prm, rp_ctx** ou
   c->core_uy = maxY;
   c->site_w = nl->site_w;
   c->site_h = nl->site_h;
+  c->die_lx = nl->die_lx;
+  c->die_ly = nl->die_ly;
+  c->die_ux = nl->die_ux;
+  c->die_uy = nl->die_uy;
+  c->dbu = nl->dbu_per_micron;
   const int nr = nl->num_insts;
   // fragmented-row dummy instances (src/placerBase.cpp:760-846)
   int scx = (c->core_ux - c->core_lx) / c->site_w;
@@ -252,6 +257,8 @@ extern "C" int rp_create(const rp_netlist* nl, const rp_params* prm, rp_ctx** ou
       c->pin_box[4 * (size_t)q + 3] = uy;
       c->pin_cx[q] = (int)(((int64_t)lx + ux) / 2);
       c->pin_cy[q] = (int)(((int64_t)ly + uy) / 2);
+      // Pin::updateCoordi(dbBTerm*) warns and takes (0, 0) (src/placerBase.cpp:394-399)
+      if (lx == INT_MAX || ly == INT_MAX || ux == INT_MIN || uy == INT_MIN) c->n_unplaced_ports++;
     }
   }
   // per-instance pin order (src/placerBase.cpp:728-743)
@@ -280,6 +287,38 @@ extern "C" int rp_create(const rp_netlist* nl, const rp_params* prm, rp_ctx** ou
 }
 
 extern "C" const char* rp_last_error(rp_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int rp_get_pb_info(rp_ctx* c, rp_pb_info* o) {
+  if (!c || !o) return RP_ERR_ARG;
+  memset(o, 0, sizeof *o);
+  o->dbu_per_micron = c->dbu;
+  o->site_w = c->site_w;
+  o->site_h = c->site_h;
+  o->die_lx = c->die_lx;
+  o->die_ly = c->die_ly;
+  o->die_ux = c->die_ux;
+  o->die_uy = c->die_uy;
+  o->core_lx = c->core_lx;
+  o->core_ly = c->core_ly;
+  o->core_ux = c->core_ux;
+  o->core_uy = c->core_uy;
+  const int ni = (int)c->ilx.size();
+  o->num_insts = ni;
+  o->num_place_insts = (int)c->place.size();
+  o->num_dummy_insts = ni - c->n_real;
+  o->num_fixed_insts = (int)c->nonplace.size() - o->num_dummy_insts;
+  o->num_nets = c->n_net;
+  o->num_pins = c->n_pin;
+  o->num_unplaced_ports = c->n_unplaced_ports;
+  // PlacerBase::printInfo (src/placerBase.cpp:936-950)
+  o->core_area = (int64_t)(c->core_ux - c->core_lx) * (int64_t)(c->core_uy - c->core_ly);
+  o->nonplace_insts_area = c->nonplace_area;
+  o->place_insts_area = c->place_area;
+  o->std_insts_area = c->std_area;
+  o->macro_insts_area = c->macro_area;
+  o->util = (float)c->place_area / (float)(o->core_area - c->nonplace_area) * 100;
+  return RP_OK;
+}
 
 extern "C" void rp_destroy(rp_ctx* c) {
   if (!c) return;
@@ -621,6 +660,10 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
                 "Filler area is negative!!\n Please put higher target density or \n Re-floorplan to have enough "
                 "coreArea\n");
   int fillerCnt = (avgDx * avgDy) > 0 ? (int)(totalFiller / (int64_t)(avgDx * avgDy)) : 0;
+  c->nb_core_area = coreArea;
+  c->nb_white = white;
+  c->nb_movable = movable;
+  c->nb_total_filler = totalFiller;
   c->filler_dx = avgDx;
   c->filler_dy = avgDy;
   c->n_gi = ngi;
@@ -657,6 +700,9 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   int64_t avgArea = c->place_area / (int64_t)ngi;
   int64_t idealBinArea = (int64_t)std::round((float)avgArea / c->td);
   int idealBinCnt = idealBinArea > 0 ? (int)(totalBinArea / idealBinArea) : 0;
+  c->nb_avg_area = avgArea;
+  c->nb_ideal_bin_area = idealBinArea;
+  c->nb_ideal_bin_cnt = idealBinCnt;
   int found = 2;
   for (found = 2; found <= 1024; found *= 2)
     if (found * found <= idealBinCnt && 4 * found * found > idealBinCnt) break;
@@ -924,6 +970,17 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   HIPCHK(c, dalloc(c->d_rec, std::max(1, p.nesterov_place_max_iter)));
   HIPCHK(c, dalloc(c->d_dbg_wl, ng));
   HIPCHK(c, dalloc(c->d_dbg_dens, ng));
+  // verbose-3 trace buffers: a record per evaluation (init + iterations x trials)
+  c->trace = p.verbose >= 3;
+  if (c->trace) {
+    const size_t cap = (size_t)std::max(1, p.nesterov_place_max_iter) * std::max(1, p.max_back_track) + 2;
+    HIPCHK(c, dalloc(c->d_evrec, cap));
+    HIPCHK(c, dalloc(c->d_part_gn, npg));
+    HIPCHK(c, dalloc(c->d_part_phi, std::max(1, c->nby / 2)));
+    HIPCHK(c, dalloc(c->d_dbg_area, nb));
+    HIPCHK(c, hipMemsetAsync(c->d_part_phi.p, 0, sizeof(double) * std::max(1, c->nby / 2), s));
+    HIPCHK(c, hipMemsetAsync(c->d_part_gn.p, 0, sizeof(double) * npg, s));
+  }
   HIPCHK(c, hipStreamSynchronize(s));
 
   RpDev& d = c->dev;
@@ -997,6 +1054,13 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   d.rec = c->d_rec.p;
   d.dbg_wl = nullptr;
   d.dbg_dens = nullptr;
+  if (c->trace) {
+    d.evrec = c->d_evrec.p;
+    d.evrec_cap = (int)c->d_evrec.n;
+    d.part_gn = c->d_part_gn.p;
+    d.part_phi = p.poisson_transform == 1 ? nullptr : c->d_part_phi.p;
+    d.dbg_area = c->d_dbg_area.p;
+  }
 
   RpState& S = *c->h_state;
   memset(&S, 0, sizeof S);
@@ -1053,6 +1117,14 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
     info->place_insts_area = c->place_area;
     info->nonplace_insts_area = c->nonplace_area;
     info->target_density = c->td;
+    info->core_area = c->nb_core_area;
+    info->white_space_area = c->nb_white;
+    info->movable_area = c->nb_movable;
+    info->total_filler_area = c->nb_total_filler;
+    info->average_place_inst_area = c->nb_avg_area;
+    info->ideal_bin_area = c->nb_ideal_bin_area;
+    info->ideal_bin_count = c->nb_ideal_bin_cnt;
+    info->total_bin_area = totalBinArea;
   }
   return RP_OK;
 }
@@ -1357,14 +1429,15 @@ int run_slots(rp_ctx* c, hipStream_t s, LoopRes& res) {
   // Batches after the first (sort refresh + K slots, the same launches every
   // time) are captured once into a HIP graph and replayed, unless per-kernel
   // events are recorded or the matrix-core transform (library GEMMs) is in use.
-  const bool use_graph = !c->timing && !c->mf() && getenv("RP_NO_GRAPH") == nullptr;
+  const bool use_graph = !c->timing && !c->trace && !c->mf() && getenv("RP_NO_GRAPH") == nullptr;
+  const int want_phi = c->trace ? 1 : 0;  // sumPhi of the trace needs phi
   for (int b = 0; b < max_batches && hs->phase == 0; b++) {
     if (use_graph && b > 0) {
       if (!res.gexec) {
         HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         hipError_t e = rp_launch_sort_refresh(d, c->sortb, s);
         for (int k = 0; k < K && e == hipSuccess; k++)
-          e = rp_launch_eval(d, s, 0, nullptr, nullptr, k == K - 1 ? &c->sortb : nullptr, &launched);
+          e = rp_launch_eval(d, s, want_phi, nullptr, nullptr, k == K - 1 ? &c->sortb : nullptr, &launched);
         const hipError_t e2 = hipStreamEndCapture(s, &res.graph);
         HIPCHK(c, e != hipSuccess ? e : e2);
         HIPCHK(c, hipGraphInstantiate(&res.gexec, res.graph, nullptr, nullptr, 0));
@@ -1377,7 +1450,7 @@ int run_slots(rp_ctx* c, hipStream_t s, LoopRes& res) {
       if (b > 0) HIPCHK(c, rp_launch_sort_refresh(d, c->sortb, s));
       if (sev) HIPCHK(c, hipEventRecord(sev[1], s));
       for (int k = 0; k < K; k++)
-        HIPCHK(c, rp_launch_eval(d, s, 0, c->timing ? &kev[(size_t)k * EV] : nullptr, c->mf(),
+        HIPCHK(c, rp_launch_eval(d, s, want_phi, c->timing ? &kev[(size_t)k * EV] : nullptr, c->mf(),
                                  k == K - 1 ? &c->sortb : nullptr, &launched));
     }
     HIPCHK(c, hipMemcpyAsync(hs, c->d_state.p, sizeof(RpState), hipMemcpyDeviceToHost, s));
@@ -1579,6 +1652,19 @@ extern "C" int rp_nesterov_set_state(rp_ctx* c, const rp_nesterov_state* in) {
   S.max_iter = c->prm.nesterov_place_max_iter;
   host_begin_iteration(S);
   HIPCHK(c, hipMemcpy(c->d_state.p, &S, sizeof(RpState), hipMemcpyHostToDevice));
+  return RP_OK;
+}
+
+static_assert(sizeof(RpEvalRec) == sizeof(rp_eval_record), "rp_eval_record layout");
+
+extern "C" int rp_get_eval_records(rp_ctx* c, rp_eval_record* out, int32_t cap, int32_t* count) {
+  if (!c || !count) return RP_ERR_ARG;
+  *count = 0;
+  if (!c->trace || !c->nb_ready) return RP_OK;
+  const int n = std::min<int>(c->h_state->evals, (int)c->d_evrec.n);
+  *count = n;
+  if (out && cap > 0)
+    HIPCHK(c, hipMemcpy(out, c->d_evrec.p, sizeof(rp_eval_record) * std::min(n, cap), hipMemcpyDeviceToHost));
   return RP_OK;
 }
 

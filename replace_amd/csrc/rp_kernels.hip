@@ -683,6 +683,7 @@ __device__ void binfin_body(const RpDev& d, int blk, int from_rho) {
         ov = (0.0f < ov) ? ov : 0.0f;
         ov_sum += (double)ov;
         if (d.store_rho) d.rho[b] = rho;
+        if (d.dbg_area) d.dbg_area[b] = (float)(v_in[r][t] + v_fi[r][t]) + v_np[r][t];
         d.bin_inst[b] = 0ull;
         d.bin_filler[b] = 0ull;
         r2[r] = rho;
@@ -907,10 +908,18 @@ __device__ void rowinv_body(const RpDev& d, int blk, int want_phi) {
   }
   __syncthreads();
   Z = fft_n<N, +1>(A, B, TW);
+  double vphi = 0.0;
   for (int m = threadIdx.x; m < N; m += blockDim.x) {
     const float2 z = dct3_out<N>(Z, m);
     d.phi[(size_t)ja * N + m] = z.x;
     d.phi[(size_t)jb * N + m] = z.y;
+    if (d.part_phi)
+      vphi += (double)(z.x * d.dbg_area[(size_t)ja * N + m]) + (double)(z.y * d.dbg_area[(size_t)jb * N + m]);
+  }
+  if (d.part_phi) {  // sumPhi (src/nesterovBase.cpp:1493-1505), verbose-3 trace only
+    __shared__ double sh_phi[RP_BLOCK / 64];
+    const double tot = block_sum_d(vphi, sh_phi);
+    if (threadIdx.x == 0) d.part_phi[blk] = tot;
   }
 }
 
@@ -1135,11 +1144,19 @@ __device__ void rowinv2_body(const RpDev& d, int blk, int want_phi) {
   }
   __syncthreads();
   Z = fft_n<N, +1>(A, B, TW, lt, RP_BLOCK);
-  if (h != 0) return;
-  for (int m = lt; m < N; m += RP_BLOCK) {
-    const float2 z = dct3_out<N>(Z, m);
-    d.phi[(size_t)ja * N + m] = z.x;
-    d.phi[(size_t)jb * N + m] = z.y;
+  double vphi = 0.0;
+  if (h == 0)
+    for (int m = lt; m < N; m += RP_BLOCK) {
+      const float2 z = dct3_out<N>(Z, m);
+      d.phi[(size_t)ja * N + m] = z.x;
+      d.phi[(size_t)jb * N + m] = z.y;
+      if (d.part_phi)
+        vphi += (double)(z.x * d.dbg_area[(size_t)ja * N + m]) + (double)(z.y * d.dbg_area[(size_t)jb * N + m]);
+    }
+  if (d.part_phi) {  // sumPhi (src/nesterovBase.cpp:1493-1505), verbose-3 trace only
+    __shared__ double sh_phi[RP_WIDE / 64];
+    const double tot = block_sum_d(vphi, sh_phi);
+    if (threadIdx.x == 0) d.part_phi[blk] = tot;
   }
 }
 
@@ -1443,7 +1460,7 @@ __device__ void grad_body(const RpDev& d, int blk) {
   const RpState* st = d.st;
   if (st->phase != 0) return;
   int c = blk * blockDim.x + threadIdx.x;
-  double v_wl = 0.0, v_dens = 0.0, v_grad = 0.0, v_dc = 0.0, v_dg = 0.0;
+  double v_wl = 0.0, v_dens = 0.0, v_grad = 0.0, v_dc = 0.0, v_dg = 0.0, v_gn = 0.0;
   if (c < d.nc) {
     float lam = st->density_penalty;
     int dst = st->dst, ref = st->ring.cur;
@@ -1518,6 +1535,12 @@ __device__ void grad_body(const RpDev& d, int blk) {
     float gdx = ga.x - sx, gdy = ga.y - sy;
     v_dc = (double)(ddx * ddx) + (double)(ddy * ddy);
     v_dg = (double)(gdx * gdx) + (double)(gdy * gdy);
+    v_gn = (double)(sx * sx + sy * sy);
+  }
+  if (d.part_gn) {  // getSecondNorm of the new gradient (src/nesterovPlace.cpp:609-616), trace only
+    __shared__ double sh_gn[RP_BLOCK / 64];
+    const double tot = block_sum_d(v_gn, sh_gn);
+    if (threadIdx.x == 0) d.part_gn[blk] = tot;
   }
   // the five block sums in one pass: per value the same wavefront tree and
   // wavefront order as block_sum_d
@@ -1689,6 +1712,17 @@ __device__ void reduce_all(const RpDev& d, Totals& t) {
   __syncthreads();
 }
 
+// Trace totals of one evaluation (all threads of the controller take part):
+// the squared norm of the new gradient and sum phi * area.
+struct TraceTotals {
+  double gn, phi;
+};
+__device__ void reduce_trace(const RpDev& d, TraceTotals& tt) {
+  __shared__ double sh[RP_CTRL_BLOCK / 64];
+  tt.gn = reduce_parts_d(d.part_gn, d.st->num_partials_grad, 1, 0, sh);
+  tt.phi = d.part_phi ? reduce_parts_d(d.part_phi, d.nby / 2, 1, 0, sh) : 0.0;
+}
+
 // updateWireLengthCoef, src/nesterovPlace.cpp:474-490
 __device__ void update_wl_coef(RpState* s, float overflow) {
   float c;
@@ -1710,6 +1744,31 @@ __device__ float step_of(const RpState* s, double dc, double dg) {
   float cd = (float)sqrt(dc / n2);
   float gd = (float)sqrt(dg / n2);
   return cd / gd;
+}
+
+// The verbose-3 record of the evaluation just reduced (thread 0).
+__device__ void write_eval_rec(const RpDev& d, const RpState* s, const Totals& t, const TraceTotals& tt, int iter,
+                               int trial, int diverged, float new_step, float overflow) {
+  const int idx = s->evals - 1;
+  if (idx < 0 || idx >= d.evrec_cap) return;
+  const double n2 = 2.0 * (double)s->num_cells;
+  RpEvalRec e;
+  e.iter = iter;
+  e.trial = trial;
+  e.diverged = diverged;
+  e.density_penalty = s->density_penalty;
+  e.wl_coef = s->wl_coef_x;
+  e.wl_grad_sum = (float)t.wl;
+  e.density_grad_sum = (float)t.dens;
+  e.grad_sum = (float)t.grad;
+  e.coordinate_distance = (float)sqrt(t.dc / n2);
+  e.gradient_distance = (float)sqrt(t.dg / n2);
+  e.new_step_length = new_step;
+  e.gradient_norm = (float)sqrt(tt.gn / n2);
+  e.sum_phi = (float)tt.phi;
+  e.overflow = overflow;
+  e.hpwl = t.hpwl;
+  d.evrec[idx] = e;
 }
 
 __device__ bool bad_sum(double v) {
@@ -1734,6 +1793,8 @@ __device__ void begin_iteration(RpState* s) {
 __global__ void __launch_bounds__(RP_CTRL_BLOCK) k_control_init(RpDev d, int stage) {
   Totals t;
   reduce_all(d, t);
+  TraceTotals tt = {0.0, 0.0};
+  if (d.evrec && stage > 0) reduce_trace(d, tt);
   if (threadIdx.x != 0) return;
   RpState* s = d.st;
   if (s->phase != 0) return;
@@ -1750,11 +1811,15 @@ __global__ void __launch_bounds__(RP_CTRL_BLOCK) k_control_init(RpDev d, int sta
   s->wl_grad_sum = t.wl;
   s->dens_grad_sum = t.dens;
   if (bad_sum(t.wl) || bad_sum(t.dens)) {
+    if (d.evrec) write_eval_rec(d, s, t, tt, -1, stage - 1, 1, 0.0f, s->sum_overflow);
     s->phase = 2;
     s->code = 2;  // diverged in init -> REPL-2 (src/nesterovPlace.cpp:280-283)
     s->diverged = 1;
     return;
   }
+  if (d.evrec)
+    write_eval_rec(d, s, t, tt, -1, stage - 1, 0, stage == 2 ? step_of(s, t.dc, t.dg) : 0.0f,
+                   stage == 2 ? overflow_of(s, (long long)t.ovf) : s->sum_overflow);
   if (stage == 1) {
     s->prev_hpwl = t.hpwl;  // src/nesterovPlace.cpp:98-99
     // updateInitialPrevSLPCoordi as a step of length initialPrevCoordiUpdateCoef
@@ -1781,6 +1846,8 @@ __device__ void control_main_body(const RpDev& d) {
   if (d.st->phase != 0) return;
   Totals t;
   reduce_all(d, t);
+  TraceTotals tt = {0.0, 0.0};
+  if (d.evrec) reduce_trace(d, tt);
   if (threadIdx.x != 0) return;
   RpState* s = d.st;
   s->evals++;
@@ -1793,6 +1860,9 @@ __device__ void control_main_body(const RpDev& d) {
   s->hpwl = t.hpwl;
   bool div = bad_sum(t.wl) || bad_sum(t.dens);
   bool accepted = false;
+  if (d.evrec)
+    write_eval_rec(d, s, t, tt, s->iter, s->bt, div ? 1 : 0, div ? 0.0f : step_of(s, t.dc, t.dg),
+                   overflow_of(s, (long long)t.ovf));
   if (!div) {
     float ns = step_of(s, t.dc, t.dg);
     s->last_new_step = ns;

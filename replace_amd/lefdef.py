@@ -298,7 +298,7 @@ def read_def(path: str, lef: Lef, net_iterm_order: str = "def",
             i += 1
 
     comp_index = {c: k for k, c in enumerate(comp_names)}
-    pin_inst, pin_off, pin_box = [], [], []
+    pin_inst, pin_off, pin_box, pin_name = [], [], [], []
     net_start = [0]
     inst_term: Dict[Tuple[int, str], int] = {}
     for conns, use in nets:
@@ -321,11 +321,13 @@ def read_def(path: str, lef: Lef, net_iterm_order: str = "def",
             pin_inst.append(k)
             pin_off.append((ox, oy))
             pin_box.append((0, 0, 0, 0))
+            pin_name.append("")
         for b in bterms:
             bx = bterm.get(b)
             pin_inst.append(-1)
             pin_off.append((0, 0))
             pin_box.append(bx if bx is not None else (INT_MAX, INT_MAX, INT_MIN, INT_MIN))
+            pin_name.append(b)
         net_start.append(len(pin_inst))
     # per-instance ITerm order: master pin order restricted to connected pins
     ips = [0]
@@ -339,6 +341,12 @@ def read_def(path: str, lef: Lef, net_iterm_order: str = "def",
                 ipl.append(q)
         ips.append(len(ipl))
     site_w, site_h = rows[0][4], rows[0][5]
+    # the die PlacerBase reads is OpenDB's block bounding box
+    # (dbBlock::getBBox, src/placerBase.cpp:616-617): the DIEAREA together
+    # with every instance and every placed port shape
+    bx = [die] + [tuple(b) for b in comp_box] + [b for b in pin_box if b[0] != INT_MAX and b[0] <= b[2]]
+    bx = [b for b in bx if b[0] < b[2] or b[1] < b[3]]
+    die = (min(b[0] for b in bx), min(b[1] for b in bx), max(b[2] for b in bx), max(b[3] for b in bx))
     return Netlist(
         dbu_per_micron=dbu,
         die=die,
@@ -353,6 +361,7 @@ def read_def(path: str, lef: Lef, net_iterm_order: str = "def",
         pin_box=np.array(pin_box, dtype=np.int64).clip(INT_MIN, INT_MAX).astype(np.int32).reshape(-1, 4),
         inst_pin_start=np.array(ips, dtype=np.int32),
         inst_pins=np.array(ipl, dtype=np.int32),
+        pin_names=pin_name,
         name=name or path,
     )
 

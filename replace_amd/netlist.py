@@ -191,7 +191,35 @@ class RpNbInfo(ctypes.Structure):
         ("place_insts_area", ctypes.c_int64),
         ("nonplace_insts_area", ctypes.c_int64),
         ("target_density", ctypes.c_float),
+        ("core_area", ctypes.c_int64),
+        ("white_space_area", ctypes.c_int64),
+        ("movable_area", ctypes.c_int64),
+        ("total_filler_area", ctypes.c_int64),
+        ("average_place_inst_area", ctypes.c_int64),
+        ("ideal_bin_area", ctypes.c_int64),
+        ("ideal_bin_count", ctypes.c_int64),
+        ("total_bin_area", ctypes.c_int64),
     ]
+
+
+class RpPbInfo(ctypes.Structure):
+    """ctypes mirror of ``rp_pb_info``."""
+
+    _fields_ = [(k, ctypes.c_int32) for k in (
+        "dbu_per_micron", "site_w", "site_h", "die_lx", "die_ly", "die_ux", "die_uy",
+        "core_lx", "core_ly", "core_ux", "core_uy", "num_insts", "num_place_insts", "num_fixed_insts",
+        "num_dummy_insts", "num_nets", "num_pins", "num_unplaced_ports")] + [
+        (k, ctypes.c_int64) for k in ("core_area", "nonplace_insts_area", "place_insts_area", "std_insts_area",
+                                      "macro_insts_area")] + [("util", ctypes.c_float)]
+
+
+class RpEvalRecord(ctypes.Structure):
+    """ctypes mirror of ``rp_eval_record``."""
+
+    _fields_ = [("iter", ctypes.c_int32), ("trial", ctypes.c_int32), ("diverged", ctypes.c_int32)] + [
+        (k, ctypes.c_float) for k in ("density_penalty", "wl_coef", "wl_grad_sum", "density_grad_sum", "grad_sum",
+                                      "coordinate_distance", "gradient_distance", "new_step_length",
+                                      "gradient_norm", "sum_phi", "overflow")] + [("hpwl", ctypes.c_int64)]
 
 
 
@@ -269,6 +297,7 @@ class Netlist:
     pin_box: Optional[np.ndarray] = None   # (Np, 4) BTerm boxes
     inst_pin_start: Optional[np.ndarray] = None
     inst_pins: Optional[np.ndarray] = None
+    pin_names: Optional[list] = None       # per pin: the port name of a BTerm, "" for an ITerm
     name: str = "netlist"
     _keep: list = field(default_factory=list, repr=False)
 
@@ -334,6 +363,8 @@ class Netlist:
         if self.inst_pin_start is not None:
             arrs["inst_pin_start"] = self.inst_pin_start
             arrs["inst_pins"] = self.inst_pins
+        if self.pin_names is not None:
+            arrs["pin_names"] = np.array(self.pin_names, dtype=np.str_)
         np.savez_compressed(path, **arrs)
 
     @staticmethod
@@ -354,6 +385,7 @@ class Netlist:
             pin_box=z["pin_box"] if "pin_box" in z.files else None,
             inst_pin_start=z["inst_pin_start"] if "inst_pin_start" in z.files else None,
             inst_pins=z["inst_pins"] if "inst_pins" in z.files else None,
+            pin_names=[str(v) for v in z["pin_names"]] if "pin_names" in z.files else None,
             name=name or path,
         )
 
@@ -363,16 +395,19 @@ class Netlist:
         pin_box = self.pin_box if self.pin_box is not None else np.zeros((0, 4), np.int32)
         ips = self.inst_pin_start if self.inst_pin_start is not None else np.zeros(0, np.int32)
         ipn = self.inst_pins if self.inst_pins is not None else np.zeros(0, np.int32)
+        # version 2: a block of NUL-terminated pin names (ports) follows
+        names = b"".join((n or "").encode() + b"\0" for n in self.pin_names) if self.pin_names else b""
         head = np.array(
-            [0x52504E4C, 1, self.dbu_per_micron, *self.die, self.site_w, self.site_h,
+            [0x52504E4C, 2, self.dbu_per_micron, *self.die, self.site_w, self.site_h,
              self.row_box.shape[0], self.num_insts, self.num_nets, npin,
-             pin_box.shape[0], ips.shape[0], ipn.shape[0]],
+             pin_box.shape[0], ips.shape[0], ipn.shape[0], len(names)],
             dtype=np.int32,
         )
         with open(path, "wb") as f:
             for a in (head, self.row_box, self.inst_box, self.inst_fixed.astype(np.int32),
                       self.net_pin_start, self.pin_inst, self.pin_off, pin_box, ips, ipn):
                 f.write(np.ascontiguousarray(a, dtype=np.int32).tobytes())
+            f.write(names)
 
 
 class RpServerOpts(ctypes.Structure):

@@ -57,10 +57,13 @@ DESIGNS = {
 
 def parse_ok(path):
     ip, nes, info = [], [], {}
+    log = []  # the placer's own lines in order ([INFO], [WARN], [InitialPlace], [NesterovSolve], [ERROR] + continuations)
     finished = None
     code = 0
     for line in open(path):
         line = line.rstrip("\n")
+        if line.startswith("[") or (log and line.startswith(" ") and log[-1].startswith(("[WARN]", "[ERROR]", " "))):
+            log.append(line)
         m = re.match(r"\[InitialPlace\]  Iter: (\d+) CG Error: (\S+) HPWL: (\d+)", line)
         if m:
             ip.append([int(m.group(1)), m.group(2), int(m.group(3))])
@@ -79,7 +82,7 @@ def parse_ok(path):
         m = re.search(r"\(REPL-(\d+)\)", line)
         if m:
             code = int(m.group(1))
-    return dict(ip=ip, nesterov=nes, finished=finished, code=code, info=info)
+    return dict(ip=ip, nesterov=nes, finished=finished, code=code, info=info, log=log)
 
 
 MT_SRC = r"""

@@ -63,14 +63,35 @@ def test_server_opts_default():
     assert o.setup_threads == 16 and o.steps_per_poll == 8 and o.use_graph == 1
 
 
-def test_struct_sizes_match_header_layout():
-    # sizes of the C structs on x86-64 (pointers 8 bytes, natural alignment)
-    assert ctypes.sizeof(RpParams) == 22 * 4
-    assert ctypes.sizeof(RpIterRecord) == 24
-    assert ctypes.sizeof(RpNesterovStats) == 48
-    assert ctypes.sizeof(RpEvalOut) == 14 * 8
-    assert ctypes.sizeof(RpNbInfo) == 15 * 4 + 4 + 4 * 8 + 8  # int32 x15, pad, int64 x4, float + pad
-    assert ctypes.sizeof(RpNetlist) % 8 == 0
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes mirror against the C header: a probe compiled with gcc
+    prints sizeof and the offset of every field of each rp_capi.h struct."""
+    from replace_amd import netlist as N
+    mirrors = {"rp_netlist": N.RpNetlist, "rp_params": N.RpParams, "rp_iter_record": N.RpIterRecord,
+               "rp_nesterov_stats": N.RpNesterovStats, "rp_ip_stats": N.RpIpStats, "rp_nb_info": N.RpNbInfo,
+               "rp_pb_info": N.RpPbInfo, "rp_eval_out": N.RpEvalOut, "rp_eval_record": N.RpEvalRecord,
+               "rp_nesterov_state": N.RpNesterovState, "rp_server_opts": N.RpServerOpts,
+               "rp_tile_choice": N.RpTileChoice, "rp_job_result": N.RpJobResult,
+               "rp_server_stats": N.RpServerStats}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "replace/rp_capi.h"', "int main(void) {"]
+    for cname, m in mirrors.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in m._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("  return 0;\n}")
+    src = os.path.join(tmp_path, "probe.c")
+    exe = os.path.join(tmp_path, "probe")
+    open(src, "w").write("\n".join(lines))
+    import subprocess
+    subprocess.check_call(["gcc", "-I" + os.path.join(ROOT, "include"), src, "-o", exe])
+    got = {}
+    for line in subprocess.check_output([exe]).decode().splitlines():
+        a, b, v = line.split()
+        got[(a, b)] = int(v)
+    for cname, m in mirrors.items():
+        assert got[(cname, "size")] == ctypes.sizeof(m), cname
+        for f, _ in m._fields_:
+            assert got[(cname, f)] == getattr(m, f).offset, (cname, f)
 
 
 def test_netlist_roundtrip(tmp_path):
@@ -83,7 +104,8 @@ def test_netlist_roundtrip(tmp_path):
         assert np.array_equal(getattr(nl, a), getattr(nl2, a)), a
     nl.write_bin(os.path.join(tmp_path, "n.bin"))
     head = np.fromfile(os.path.join(tmp_path, "n.bin"), dtype=np.int32, count=17)
-    assert head[0] == 0x52504E4C and head[10] == nl.num_insts and head[12] == nl.num_pins
+    assert head[0] == 0x52504E4C and head[1] == 2 and head[10] == nl.num_insts and head[12] == nl.num_pins
+    assert nl2.pin_names == nl.pin_names and sum(1 for v in nl.pin_names if v) == 54
 
 
 def test_generator_workload_shapes():

@@ -252,6 +252,13 @@ def test_info_values_match_golden_logs():
         assert f"({info.bin_size_x}, {info.bin_size_y})" == g["BinSize"]
         assert str(info.num_gcells) == g["FillerInit: NumGCells"]
         assert str(info.std_insts_area) == g["StdInstsArea"]
+        assert str(info.num_gnets) == g["FillerInit: NumGNets"]
+        assert str(info.num_gpins) == g["FillerInit: NumGPins"]
+        assert f"{info.target_density:.6f}" == g["TargetDensity"]
+        assert str(info.average_place_inst_area) == g["AveragePlaceInstArea"]
+        assert str(info.ideal_bin_area) == g["IdealBinArea"]
+        assert str(info.ideal_bin_count) == g["IdealBinCnt"]
+        assert str(info.total_bin_area) == g["TotalBinArea"]
 
 
 # --------------------------------------------------------------------------
