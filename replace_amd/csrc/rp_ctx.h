@@ -11,13 +11,28 @@
 #include "../../include/replace/rp_capi.h"
 #include "rp_dev_struct.h"
 
+// Device allocations come from the device's stream-ordered memory pool
+// (hipMallocAsync / hipFreeAsync on the null stream, which holds no work of
+// this library), kept from one allocation to the next.  Unlike hipFree,
+// releasing a buffer does not wait for the whole device, so placement
+// contexts set up and torn down on host threads do not stall each other's
+// queued kernels.  A buffer is released only when no queued work uses it:
+// its owner synchronises the streams that use it first.
+hipError_t rp_dev_malloc(void** p, size_t bytes);
+void rp_dev_free(void* p);
+// Pinned host staging blocks of the contexts (states, records, totals) come
+// from a process-wide cache by size class: hipHostFree waits for the whole
+// device, so a context's teardown hands its blocks back instead.
+hipError_t rp_pinned_alloc(void** p, size_t bytes);
+void rp_pinned_free(void* p);
+
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) rp_dev_free(p);
     p = nullptr;
     n = 0;
   }
@@ -162,7 +177,7 @@ template <typename T>
 static inline hipError_t dalloc(DevBuf<T>& b, size_t n) {
   b.release();
   if (n == 0) n = 1;
-  hipError_t e = hipMalloc((void**)&b.p, n * sizeof(T));
+  hipError_t e = rp_dev_malloc((void**)&b.p, n * sizeof(T));
   if (e == hipSuccess) b.n = n;
   return e;
 }
@@ -192,7 +207,7 @@ struct RpIpAsm {
   int* h_tot = nullptr;
   std::vector<int> h_pin_k, pos;  // pin -> row (locality order), movable index -> row
   ~RpIpAsm() {
-    if (h_tot) (void)hipHostFree(h_tot);
+    if (h_tot) rp_pinned_free(h_tot);
   }
 };
 int rp_ip_asm_init(rp_ctx* c, RpIpAsm& a);

@@ -19,6 +19,8 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
+#include <unordered_map>
 #include <numeric>
 #include <random>
 #include <string>
@@ -46,6 +48,68 @@ static inline int idiv_upper(int r, int s) {
 // ===========================================================================
 // PlacerBase
 // ===========================================================================
+namespace {
+std::mutex g_pool_mu;
+bool g_pool_ready[64];
+}  // namespace
+
+hipError_t rp_dev_malloc(void** p, size_t bytes) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev >= 0 && dev < 64) {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    if (!g_pool_ready[dev]) {
+      // keep freed pool memory for later allocations instead of returning it
+      hipMemPool_t pool;
+      if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+      }
+      g_pool_ready[dev] = true;
+    }
+  }
+  return hipMallocAsync(p, bytes, nullptr);
+}
+
+void rp_dev_free(void* p) { (void)hipFreeAsync(p, nullptr); }
+
+namespace {
+std::mutex g_pin_mu;
+std::vector<void*> g_pin_free[48];  // cached blocks by size class (2^k bytes)
+std::unordered_map<void*, int> g_pin_class;
+int pin_class(size_t bytes) {
+  int k = 8;  // 256 bytes at least
+  while (((size_t)1 << k) < bytes) k++;
+  return k;
+}
+}  // namespace
+
+hipError_t rp_pinned_alloc(void** p, size_t bytes) {
+  const int k = pin_class(bytes);
+  if (k >= 48) return hipErrorInvalidValue;
+  {
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    if (!g_pin_free[k].empty()) {
+      *p = g_pin_free[k].back();
+      g_pin_free[k].pop_back();
+      return hipSuccess;
+    }
+  }
+  hipError_t e = hipHostMalloc(p, (size_t)1 << k, hipHostMallocDefault);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lk(g_pin_mu);
+  g_pin_class[*p] = k;
+  return hipSuccess;
+}
+
+void rp_pinned_free(void* p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(g_pin_mu);
+  auto it = g_pin_class.find(p);
+  if (it != g_pin_class.end()) g_pin_free[it->second].push_back(p);
+}
+
 extern "C" void rp_params_default(rp_params* p) {
   // Replace::Replace, src/replace.cpp:13-33 and NesterovPlaceVars / NesterovBaseVars constants
   p->initial_place_max_iter = 20;
@@ -117,7 +181,7 @@ extern "C" int rp_create(const rp_netlist* nl, const rp_params* prm, rp_ctx** ou
     *out = c.release();
     return fail(*out, RP_ERR_HIP, "hipStreamCreate failed");
   }
-  if (hipHostMalloc((void**)&c->h_state, sizeof(RpState), hipHostMallocDefault) != hipSuccess) {
+  if (rp_pinned_alloc((void**)&c->h_state, sizeof(RpState)) != hipSuccess) {
     *out = c.release();
     return fail(*out, RP_ERR_HIP, "hipHostMalloc failed");
   }
@@ -323,7 +387,7 @@ extern "C" int rp_get_pb_info(rp_ctx* c, rp_pb_info* o) {
 extern "C" void rp_destroy(rp_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  if (c->h_state) (void)hipHostFree(c->h_state);
+  if (c->h_state) rp_pinned_free(c->h_state);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -307,7 +307,7 @@ int rp_ip_asm_init(rp_ctx* c, RpIpAsm& a) {
     HIPCHK(c, dalloc(a.rowp[ax], (size_t)n + 1));
     HIPCHK(c, dalloc(a.b[ax], n));
   }
-  if (!a.h_tot) HIPCHK(c, hipHostMalloc((void**)&a.h_tot, 4 * sizeof(int), hipHostMallocDefault));
+  if (!a.h_tot) HIPCHK(c, rp_pinned_alloc((void**)&a.h_tot, 4 * sizeof(int)));
   HIPCHK(c, hipStreamSynchronize(st));
   return 0;
 }
@@ -1305,7 +1305,8 @@ struct RpIpRun {
   DevBuf<float> keep[2];
   DevBuf<BcsCarry> carry;
   ~RpIpRun() {
-    if (h_rec) (void)hipHostFree(h_rec);
+    if (st) (void)hipStreamSynchronize(st);  // no queued work on the buffers released below
+    if (h_rec) rp_pinned_free(h_rec);
   }
 };
 
@@ -1406,7 +1407,7 @@ int rp_ipr_begin(rp_ctx* c, hipStream_t run_stream, bool host_asm, RpIpRun** out
     HIPCHK(c, dalloc(R->parts[ax], (size_t)BC_SLOTS * R->nparts));
   }
   HIPCHK(c, dalloc(R->d_rec, 1));
-  HIPCHK(c, hipHostMalloc((void**)&R->h_rec, sizeof(IpRec), hipHostMallocDefault));
+  HIPCHK(c, rp_pinned_alloc((void**)&R->h_rec, sizeof(IpRec)));
   R->mv = IpMove{n, nn, np, A.icx.p, A.icy.p, R->d_dlx.p, R->d_dly.p, R->d_hx.p, R->d_hy.p, R->d_mw.p, R->d_mh.p,
                  A.net_start.p, A.pin_k.p, R->d_offx.p, R->d_offy.p, A.pin_cx.p, A.pin_cy.p, R->d_sbox.p,
                  R->d_hparts.p, R->vec[0][0].p, R->vec[1][0].p};
