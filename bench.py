@@ -14,6 +14,12 @@ Besides the contract fields the JSON line carries
                 own Ooura FFT when oracle/_ref is built) pinned to one core:
                 seconds per gradient evaluation on a bounded sample x the
                 GPU run's evaluation count
+  batched       (default workload) 256 uneven instances through the
+                placement service against the same jobs back to back on the
+                plain path (--workload c3 prints the full record)
+  served        (default workload) a Poisson stream with 5% large jobs
+                through the service against a FIFO replay on the plain path
+                (--workload c4 prints the full record)
 """
 import argparse
 import json
@@ -215,7 +221,7 @@ def bench_batch(args):
         "tiles": TILES[0],
     }
     out.update(_parity(res, plain, jobs))
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_serve(args):
@@ -258,7 +264,7 @@ def bench_serve(args):
         "netgen_s": round(gen_s, 2), "timeline": _timeline(res), "batched_ip": args.batched_ip,
     }
     out.update(_parity(res, plain, jobs))
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_precision(args):
@@ -324,6 +330,8 @@ def main():
     ap.add_argument("--cpu-sample-evals", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--jobs", type=int, default=None, help="c3/c4: number of jobs")
+    ap.add_argument("--no-service", action="store_true",
+                    help="c1: leave out the batched (c3) and served (c4) summaries of the default line")
     ap.add_argument("--rate", type=float, default=27.0,
                     help="c4: arrival rate, jobs/s (default: ~80%% of the measured 34/s batched capacity)")
     ap.add_argument("--steps-per-poll", type=int, default=8, help="c3/c4: batched steps per graph replay")
@@ -346,7 +354,7 @@ def main():
     if args.workload in ("c3", "c4"):
         import torch
         torch.cuda.set_device(0)
-        (bench_batch if args.workload == "c3" else bench_serve)(args)
+        print(json.dumps((bench_batch if args.workload == "c3" else bench_serve)(args)), flush=True)
         return
 
     import torch
@@ -361,7 +369,9 @@ def main():
     t = time.perf_counter()
     ip = placer.initial_place()
     ip_s = time.perf_counter() - t
+    t = time.perf_counter()
     info = placer.setup()
+    setup_s = time.perf_counter() - t
     for _ in range(args.warmup):
         placer.nesterov_place()
     torch.cuda.synchronize()
@@ -393,14 +403,21 @@ def main():
     dom = max((k for k in kt if k != "sort_refresh"), key=lambda k: kt[k][0])
     avg_ms = kt[dom][0] / max(1, kt[dom][1])
     achieved = abytes[dom] / (avg_ms * 1e-3) / 1e9
-    traffic = None
+    traffic, traffic_note = None, "no counter profile for this workload"
     tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tpath):
-        name = ROCPROF_NAME.get(dom, "k_" + dom)
-        kern = json.load(open(tpath))["kernels"]
-        # the kernel, or the template instance of it this workload launches
-        ent = kern.get(name) or next((v for k, v in kern.items() if k.startswith(name + "<")), None)
-        traffic = ent["hbm_bytes_per_launch"] if ent else None
+        from replace_amd.profiling import kernel_source_hash
+        doc = json.load(open(tpath))
+        if doc.get("source_sha16") != kernel_source_hash():
+            traffic_note = (f"{os.path.basename(tpath)} was measured on other kernel sources "
+                            f"({doc.get('source_sha16')} != {kernel_source_hash()}): not used")
+        else:
+            name = ROCPROF_NAME.get(dom, "k_" + dom)
+            kern = doc["kernels"]
+            # the kernel, or the template instance of it this workload launches
+            ent = kern.get(name) or next((v for k, v in kern.items() if k.startswith(name + "<")), None)
+            traffic = ent["hbm_bytes_per_launch"] if ent else None
+            traffic_note = f"{os.path.basename(tpath)}, kernel sources {doc['source_sha16']}"
     code, iters, evals, ovf, hpwl, dev_s, fin = runs[-1]
     result = {
         "metric": METRIC,
@@ -426,12 +443,12 @@ def main():
         },
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "algorithmic_bytes_per_launch": abytes[dom],
-                     "avg_launch_us": round(avg_ms * 1e3, 2)},
+                     "traffic": traffic, "traffic_source": traffic_note,
+                     "algorithmic_bytes_per_launch": abytes[dom], "avg_launch_us": round(avg_ms * 1e3, 2)},
         "run": {"code": code, "finished": fin, "iterations": iters, "evals": evals,
                 "final_overflow": round(ovf, 5), "final_hpwl": hpwl, "device_s": round(dev_s, 4),
                 "initial_place_s": round(ip_s, 3), "initial_place_iters": ip.iterations,
-                "netgen_s": round(gen_s, 2)},
+                "nesterov_setup_s": round(setup_s, 3), "netgen_s": round(gen_s, 2)},
         "kernels": kernels,
     }
     if not args.no_cpu_baseline:
@@ -441,6 +458,19 @@ def main():
         except Exception as e:  # reported, never fatal for the GPU number
             result["cpu_baseline"] = {"value": None, "unit": "s", "cores": 1, "kind": "port",
                                       "sample": f"failed: {e}"[:300]}
+    placer.close()
+    if args.workload == "c1" and not args.no_service:
+        # the batching and serving forms on their workloads, summarised
+        b = bench_batch(args)
+        result["batched"] = {k: b[k] for k in ("metric", "value", "unit", "speedup_vs_plain", "wall_s",
+                                                "all_finished", "max_final_overflow", "hpwl_max_rel_diff_vs_plain")}
+        result["batched"]["plain_back_to_back"] = {k: b["plain_back_to_back"][k] for k in ("value", "wall_s")}
+        result["batched"]["config"] = b["config"]["workload"]
+        v = bench_serve(args)
+        result["served"] = {k: v[k] for k in ("metric", "value", "unit", "speedup_vs_plain", "latency_s",
+                                              "all_finished", "max_final_overflow", "hpwl_max_rel_diff_vs_plain")}
+        result["served"]["plain_back_to_back"] = v["plain_back_to_back"]
+        result["served"]["config"] = v["config"]["workload"]
     print(json.dumps(result), flush=True)
 
 

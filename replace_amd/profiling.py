@@ -101,6 +101,21 @@ def pmc_means(d: str, counter: str, regex: str = ""):
     return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
 
 
+KERNEL_SOURCES = ("rp_kernels.hip", "rp_dev_struct.h", "rp_device.h")
+
+
+def kernel_source_hash() -> str:
+    """First 16 hex digits of the SHA-256 over the placement kernels' sources:
+    a counter profile is valid for the kernels it was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    base = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc")
+    for name in KERNEL_SOURCES:
+        with open(os.path.join(base, name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
 def calibrated_traffic(fetch_dir, write_dir, copyf_dir, copyw_dir, copy_mb):
     """{kernel: (bytes per launch, fetch scale, write scale)} from two --pmc
     runs of the workload and two of the calibration copy."""
@@ -147,7 +162,8 @@ def main(argv=None):
     if a.cmd == "traffic":
         import json
         k, fs, ws = calibrated_traffic(a.fetch_dir, a.write_dir, a.copyf_dir, a.copyw_dir, a.copy_mb)
-        doc = {"workload": a.workload, "fetch_scale": round(fs, 4), "write_scale": round(ws, 4),
+        doc = {"workload": a.workload, "source_sha16": kernel_source_hash(),
+               "fetch_scale": round(fs, 4), "write_scale": round(ws, 4),
                "source": "rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE, KiB per dispatch, scaled by "
                          f"the same counters on a {a.copy_mb} MiB device copy", "kernels": k}
         txt = json.dumps(doc, indent=1)

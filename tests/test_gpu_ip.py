@@ -37,10 +37,13 @@ def _cases():
     # 14000 movable rows (grid-wide solver)
     nl = netgen.generate(14000, 14700, seed=78, name="ip14k")
     out.append((nl, netgen.params_default(target_density=0.7, initial_place_max_iter=5)))
+    # 30000 movable rows (grid-wide solver, several 1024-row partial blocks)
+    nl = netgen.generate(30000, 31500, seed=79, name="ip30k")
+    out.append((nl, netgen.params_default(target_density=0.7, initial_place_max_iter=5)))
     return out
 
 
-@pytest.mark.parametrize("case", range(7))
+@pytest.mark.parametrize("case", range(8))
 def test_device_assembly_matches_host_assembly(case):
     nl, p = _cases()[case]
     d = _ip(nl, p, host=False)
