@@ -179,9 +179,17 @@ def _parity(res, plain, jobs):
                    "overflow": round(float(r.stats.final_overflow), 4), "cells": int(jobs[k][0].num_insts),
                    "td": round(float(jobs[k][1].target_density), 3)}
                   for k, r in enumerate(res) if not (r.done == 1 and r.stats.finished == 1)]
+    fin = [float(r.stats.final_overflow) for r in res if r.done == 1 and r.stats.finished == 1]
     return {"hpwl_max_rel_diff_vs_plain": max(diffs),
-            "max_final_overflow": max(float(r.stats.final_overflow) for r in res),
-            "all_finished": not unfinished, "unfinished": unfinished[:20]}
+            "max_final_overflow_finished": max(fin) if fin else None,
+            "jobs_finished": len(fin), "all_finished": not unfinished, "unfinished": unfinished[:20]}
+
+
+def _finished(res):
+    """Jobs that reached the overflow target (the placements counted in the
+    throughput; jobs that stop on the reference's divergence rules, the same
+    way on the plain path, are listed separately)."""
+    return sum(1 for r in res if r.done == 1 and r.stats.finished == 1)
 
 
 def _timeline(res):
@@ -205,14 +213,15 @@ def bench_batch(args):
     plain_s = sum(svc)
     cells = sum(nl.num_insts for nl, _ in jobs)
     out = {
-        "metric": "placements/s, 256 uneven instances (batched service)",
-        "value": round(n / wall, 3), "unit": "placements/s", "n_gpus": 1, "higher_is_better": True,
+        "metric": "placements/s, 256 uneven instances (batched service; jobs reaching overflow 0.10)",
+        "value": round(_finished(res) / wall, 3), "unit": "placements/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{n} instances, cells log-uniform 500-50k (sum {cells}), td U[0.6,0.9], "
                                "bins clamp(nextPow2(ceil(sqrt(Nc)))/2,32,256)",
                    "steps_per_poll": args.steps_per_poll},
         "wall_s": round(wall, 3),
-        "plain_back_to_back": {"value": round(n / plain_s, 3), "unit": "placements/s", "wall_s": round(plain_s, 3),
+        "plain_back_to_back": {"value": round(_finished(res) / plain_s, 3), "unit": "placements/s",
+                               "wall_s": round(plain_s, 3),
                                "phases_s": {k: round(v, 3) for k, v in _PHASES.items()}},
         "speedup_vs_plain": round(plain_s / wall, 2),
         "batched_steps": int(stats.steps), "table_rebuilds": int(stats.rebuilds),
@@ -246,8 +255,8 @@ def bench_serve(args):
     plat = np.array(plat)
     makespan = max(r.done_s for r in res)
     out = {
-        "metric": "served placements/s + p99 latency (Poisson arrivals, 5% large jobs)",
-        "value": round(n / makespan, 3), "unit": "placements/s", "n_gpus": 1, "higher_is_better": True,
+        "metric": "served placements/s + p99 latency (Poisson arrivals, 5% large jobs; jobs reaching overflow 0.10)",
+        "value": round(_finished(res) / makespan, 3), "unit": "placements/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{n} jobs, Poisson rate {args.rate}/s (seed 7), 95% batched distribution, "
                                "5% large (200k-1M cells, 512x512 bins)",
@@ -255,7 +264,7 @@ def bench_serve(args):
                    "steps_per_poll": args.steps_per_poll},
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
                       "max": round(float(lat.max()), 4)},
-        "plain_back_to_back": {"value": round(n / (done), 3), "unit": "placements/s",
+        "plain_back_to_back": {"value": round(_finished(res) / done, 3), "unit": "placements/s",
                                "latency_s": {"p50": round(float(np.percentile(plat, 50)), 4),
                                              "p99": round(float(np.percentile(plat, 99)), 4)},
                                "service_s_total": round(sum(svc), 3)},
@@ -462,13 +471,15 @@ def main():
     if args.workload == "c1" and not args.no_service:
         # the batching and serving forms on their workloads, summarised
         b = bench_batch(args)
-        result["batched"] = {k: b[k] for k in ("metric", "value", "unit", "speedup_vs_plain", "wall_s",
-                                                "all_finished", "max_final_overflow", "hpwl_max_rel_diff_vs_plain")}
+        result["batched"] = {k: b[k] for k in ("metric", "value", "unit", "speedup_vs_plain", "wall_s", "jobs_finished",
+                                                "unfinished", "max_final_overflow_finished",
+                                                "hpwl_max_rel_diff_vs_plain")}
         result["batched"]["plain_back_to_back"] = {k: b["plain_back_to_back"][k] for k in ("value", "wall_s")}
         result["batched"]["config"] = b["config"]["workload"]
         v = bench_serve(args)
-        result["served"] = {k: v[k] for k in ("metric", "value", "unit", "speedup_vs_plain", "latency_s",
-                                              "all_finished", "max_final_overflow", "hpwl_max_rel_diff_vs_plain")}
+        result["served"] = {k: v[k] for k in ("metric", "value", "unit", "speedup_vs_plain", "latency_s", "jobs_finished",
+                                              "unfinished", "max_final_overflow_finished",
+                                              "hpwl_max_rel_diff_vs_plain")}
         result["served"]["plain_back_to_back"] = v["plain_back_to_back"]
         result["served"]["config"] = v["config"]["workload"]
     print(json.dumps(result), flush=True)
