@@ -84,3 +84,52 @@ def test_mfma_full_run_tracks_fft_path():
         out[mode] = st.final_hpwl
         g.close()
     assert abs(out[1] - out[0]) <= 0.005 * out[0], out
+
+
+def test_c2_gradients_against_fp64():
+    """The 1M-cell netlist on 2048 x 2048 bins (the precision workload),
+    cells starting at the core centre: the gradients of both GPU transforms
+    at the states of the matrix-core run after 0, 150 and 400 iterations,
+    evaluated from the same SLP coordinates, penalty and wire-length
+    coefficients as the oracle's FP64 FFT solve.  Normwise bars: the field
+    (2e-4 for split BF16, the stage bar above; 2e-5 for FP32 FFT), the
+    density gradient (cell-area-weighted field: the same bars), and the
+    whole preconditioner input wl + lambda density (1e-4 / 1e-5); the
+    wire-length gradient is the same FP32 arithmetic on all three (1e-6).
+    The measured errors are printed."""
+    nl, p = netgen.workload("c1")
+    p.bin_grid_cnt_x = p.bin_grid_cnt_y = 2048
+    p.initial_place_max_iter = 0
+    o = oracle.Oracle(nl, p, poisson="fft64")
+    o.initial_place()
+    o.setup()
+    gs = {}
+    for mode in (0, 1):
+        q = netgen.params_default(**{k: getattr(p, k) for k, _ in p._fields_})
+        q.poisson_transform = mode
+        gs[mode] = Placer(nl, q)
+        gs[mode].initial_place()
+        gs[mode].setup()
+    run = gs[1]
+    assert run.nesterov_init() == 0
+    done = 0
+    bars = {0: (2e-5, 1e-5), 1: (2e-4, 1e-4)}
+    for k in (0, 150, 400):
+        if k > done:
+            run.nesterov_iterate(k - done)
+            done = k
+        s = run.get_state()
+        cx = np.ascontiguousarray(s["cur_slp"][0::2])
+        cy = np.ascontiguousarray(s["cur_slp"][1::2])
+        lam, wlx, wly = s["density_penalty"], s["wl_coef_x"], s["wl_coef_y"]
+        ref = o.eval_gradient(cx, cy, lam, wlx, wly)
+        for mode in (0, 1):
+            got = gs[mode].eval_gradient(cx, cy, lam, wlx, wly)
+            e = {key: rel_err(got[key], ref[key]) for key in ("ex", "ey", "dens_grad", "wl_grad", "sum_grad")}
+            print(f"c2 iteration {k} mode {mode} rel err vs FP64: {e}")
+            f_bar, s_bar = bars[mode]
+            assert max(e["ex"], e["ey"], e["dens_grad"]) < f_bar, (k, mode, e)
+            assert e["sum_grad"] < s_bar and e["wl_grad"] < 1e-6, (k, mode, e)
+    for g in gs.values():
+        g.close()
+    o.close()
