@@ -23,17 +23,21 @@
 // A.B ~ Ahi.Bhi + Ahi.Blo + Alo.Bhi (the lo.lo term is below the FP32
 // rounding of the accumulation).  That is ONE plain bf16 GEMM with FP32
 // accumulation over concatenated operands, A' = [Ahi | Ahi | Alo] and
-// B' = [Bhi | Blo | Bhi] (K' = 3K), run by hipBLASLt on the matrix cores.  The
-// constant A' are built once per set-up; the data operands B' are written by
-// this file's kernels between the GEMMs (split, spectral scaling, the scatter
-// of the last products into phi and the field).  Per evaluation at nx = ny = N
-// the products are N^3 + N^3 + 2 N^3 + N^3 + N^3 = 6 N^3 multiply-adds (7 N^3
-// with phi), each over the 3 split components.  For each GEMM
-// the set-up times every algorithm hipBLASLt's heuristic offers and keeps the
-// fastest.
+// B' = [Bhi | Blo | Bhi] (K' = 3K), run by this file's k_gemm_nt on the
+// matrix cores.  The constant A' are built once per set-up; the data
+// operands B' are written by this file's kernels between the GEMMs (split,
+// spectral scaling, the scatter of the last products into phi and the field),
+// which also add the split-K partials of the product before them.  Per
+// evaluation at nx = ny = N the products are N^3 + N^3 + 2 N^3 + N^3 + N^3 =
+// 6 N^3 multiply-adds (7 N^3 with phi), each over the 3 split components.
+// RP_MFMA_GEMM=hipblaslt runs the same products through hipBLASLt instead
+// (the set-up times every algorithm its heuristic offers and keeps the
+// fastest), as a reference point for k_gemm_nt.
 #include <hipblaslt/hipblaslt.h>
 
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 
 #include "rp_ctx.h"
 
@@ -52,10 +56,13 @@ struct Gemm {
   hipblasLtMatrixLayout_t lb = nullptr, la = nullptr, lc = nullptr;
   hipblasLtMatmulAlgo_t algo{};
   int M = 0, N = 0, K = 0;  // C[M][N] = sum over 3K of A'[M][3K] B'[N][3K]
-  float us = 0;             // measured time of the chosen algorithm
+  float us = 0;             // measured time of the chosen algorithm (hipBLASLt)
+  int tile = 0;             // k_gemm_nt tile: 256 (256 x 256, 8 waves), 128 or 64 (4 waves)
+  int splitk = 1;           // k slices of k_gemm_nt (partial products summed by the consumer)
 };
 
 struct Plan {
+  bool lt = false;  // products through hipBLASLt (RP_MFMA_GEMM=hipblaslt) instead of k_gemm_nt
   hipblasLtHandle_t h = nullptr;
   Gemm g[NGEMM];
   DevBuf<uint16_t> a[5];                  // A' of G1, G2, G3, G4, G5 (row-major M x 3K)
@@ -87,6 +94,138 @@ __device__ __forceinline__ void put_split3(rp_bf16* row, int k, int i, float v) 
   row[i] = h;
   row[k + i] = l;
   row[2 * k + i] = h;
+}
+
+// ---------------------------------------------------------------------------
+// Split-BF16 products on the matrix cores: C[m][n] = sum_k A'[m][k] B'[n][k]
+// (both operands row-major along k), FP32 accumulation, v_mfma_f32_32x32x16_bf16.
+//
+// A workgroup of NW waves computes a BM x BN tile; the waves form a
+// 2 x (NW / 2) grid of (BM / 2) x (BN / (NW / 2)) sub-tiles of 32 x 32 MFMA
+// tiles.  K is walked in BK = 64 slices through two LDS stages filled by
+// LDS-DMA loads (global_load_lds_dwordx4: each lane's 16 bytes land at
+// M0 + 16 lane, so the lanes fetch the global chunks in the order of the
+// swizzled LDS layout): the next slice streams in while the current one is
+// multiplied.  A row of a slice is 8 chunks of 16 bytes, stored at chunk
+// kc ^ (row % 8), so the 32 rows a fragment load touches spread over all
+// banks.  The fragment of an MFMA operand is 8 consecutive k of one row
+// (lane l: row l % 32, k-half l / 32), read with one ds_read_b128.
+// With split-K the launch's blocks cover (k slice, tile) pairs, k slice
+// outermost, and write their partial products to C + ks * M * N; the
+// consumer adds the partials in ks order.  A tile's blocks follow an
+// XCD-aware order (hardware block b runs on XCD b % 8): XCD x takes a
+// (tiles_m / 4) x (tiles_n / 2) sub-grid, so its blocks share A and B slices
+// in its L2.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int GBK = 64;  // k slice
+
+__device__ __forceinline__ int gswz(int row, int kc) { return row * 8 + (kc ^ (row & 7)); }
+
+// vmcnt <= 0 (expcnt, lgkmcnt not waited)
+__device__ __forceinline__ void wait_vm0() { __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xF << 8)); }
+
+template <int BM, int BN, int NW>
+__global__ void __launch_bounds__(64 * NW) k_gemm_nt(const rp_bf16* __restrict__ A, const rp_bf16* __restrict__ B,
+                                                     float* __restrict__ C, int M, int N, int K, int splitk) {
+  constexpr int WM = BM / 2, WN = BN / (NW / 2), TM = WM / 32, TN = WN / 32;
+  constexpr int QA = BM * 8 / 64 / NW, QB = BN * 8 / 64 / NW;  // LDS-DMA loads per wave and slice
+  static_assert(TM >= 1 && TN >= 1 && QA >= 1 && QB >= 1, "tile too small for the wave grid");
+  __shared__ uint4 lds[2][(BM + BN) * 8];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tm = M / BM, tn = N / BN, T = tm * tn;
+  const int ks = blockIdx.x / T, t = blockIdx.x % T;
+  int bm, bn;
+  if (tm % 4 == 0 && tn % 2 == 0 && T % 8 == 0) {
+    const int x = t % 8, j = t / 8, sm = tm / 4, sn = tn / 2;
+    bm = (x / 2) * sm + j / sn;
+    bn = (x % 2) * sn + j % sn;
+  } else {
+    bm = t / tn;
+    bn = t % tn;
+  }
+  const int KT = K / GBK / splitk;
+  const rp_bf16* Ab = A + (size_t)bm * BM * K + (size_t)ks * KT * GBK;
+  const rp_bf16* Bb = B + (size_t)bn * BN * K + (size_t)ks * KT * GBK;
+  f32x16 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; i++)
+#pragma unroll
+    for (int j = 0; j < TN; j++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[i][j][r] = 0.0f;
+  const int wm = (wave / (NW / 2)) * WM, wn = (wave % (NW / 2)) * WN;
+  // LDS chunk p of a stage (A rows, then B rows) holds global chunk
+  // (row p / 8, kc) with gswz(row, kc) = p
+  auto issue = [&](int kt, int stage) {
+#pragma unroll
+    for (int i = 0; i < QA; i++) {
+      const int q = i * NW + wave, p = q * 64 + lane, row = p >> 3, kc = (p & 7) ^ (row & 7);
+      __builtin_amdgcn_global_load_lds(
+          (const __attribute__((address_space(1))) void*)(Ab + (size_t)row * K + kt * GBK + kc * 8),
+          (__attribute__((address_space(3))) void*)&lds[stage][q * 64], 16, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < QB; i++) {
+      const int q = i * NW + wave, p = q * 64 + lane, row = p >> 3, kc = (p & 7) ^ (row & 7);
+      __builtin_amdgcn_global_load_lds(
+          (const __attribute__((address_space(1))) void*)(Bb + (size_t)row * K + kt * GBK + kc * 8),
+          (__attribute__((address_space(3))) void*)&lds[stage][BM * 8 + q * 64], 16, 0, 0);
+    }
+  };
+#define GFRAG(FA, FB, buf, s)                                                                  \
+  {                                                                                            \
+    const int kc = 2 * (s) + (lane >> 5);                                                      \
+    _Pragma("unroll") for (int i = 0; i < TM; i++)                                             \
+      FA[i] = __builtin_bit_cast(bf16x8, lds[buf][gswz(wm + 32 * i + (lane & 31), kc)]);      \
+    _Pragma("unroll") for (int j = 0; j < TN; j++)                                             \
+      FB[j] = __builtin_bit_cast(bf16x8, lds[buf][gswz(BM + wn + 32 * j + (lane & 31), kc)]); \
+  }
+#define GMF(FA, FB)                                                                            \
+  _Pragma("unroll") for (int i = 0; i < TM; i++)                                               \
+    _Pragma("unroll") for (int j = 0; j < TN; j++)                                             \
+      acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(FA[i], FB[j], acc[i][j], 0, 0, 0);
+  if (KT > 0) issue(0, 0);
+  for (int kt = 0; kt < KT; kt++) {
+    const int buf = kt & 1;
+    wait_vm0();  // this wave's loads of slice kt have landed in LDS
+    // a barrier without the release fence of __syncthreads (that fence would
+    // also wait for loads still in flight); the empty asm keeps the LDS
+    // accesses on their side of it
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (kt + 1 < KT) issue(kt + 1, buf ^ 1);  // the other stage was read by slice kt - 1, before the barrier
+    bf16x8 xa[TM], xb[TN], ya[TM], yb[TN];
+    GFRAG(xa, xb, buf, 0)
+    GFRAG(ya, yb, buf, 1)
+    GMF(xa, xb)
+    GFRAG(xa, xb, buf, 2)
+    GMF(ya, yb)
+    GFRAG(ya, yb, buf, 3)
+    GMF(xa, xb)
+    GMF(ya, yb)
+  }
+#undef GFRAG
+#undef GMF
+  float* Cp = C + (size_t)ks * M * N;
+#pragma unroll
+  for (int i = 0; i < TM; i++)
+#pragma unroll
+    for (int j = 0; j < TN; j++) {
+      const int m0 = bm * BM + wm + 32 * i, n0 = bn * BN + wn + 32 * j;
+#pragma unroll
+      for (int r = 0; r < 16; r++)
+        Cp[(size_t)(m0 + (r / 4) * 8 + (lane / 32) * 4 + r % 4) * N + n0 + (lane & 31)] = acc[i][j][r];
+    }
+}
+
+// sum of the split-K partials of element k, in partial order
+__device__ __forceinline__ float psum_c(const float* __restrict__ c, size_t k, int nsplit, size_t stride) {
+  float v = c[k];
+  for (int s = 1; s < nsplit; s++) v += c[k + (size_t)s * stride];
+  return v;
 }
 
 // Mean of rho (fixed-order FP64 partials, then one block).  Subtracting a
@@ -131,11 +270,11 @@ __global__ void __launch_bounds__(256) k_split_rho(const float* __restrict__ x, 
 // FFT::doFFT first: w(u,v) / (wx_u^2 + wy_v^2), m = v, n = u, 0 at u = v = 0.
 __global__ void __launch_bounds__(256) k_split_c(const float* __restrict__ c, rp_bf16* __restrict__ out, int M,
                                                  int N, int spectral, const float* __restrict__ wx2,
-                                                 const float* __restrict__ wy2, float wscale) {
+                                                 const float* __restrict__ wy2, float wscale, int nsplit) {
   size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (size_t)M * N) return;
   int m = (int)(k / N), n = (int)(k % N);
-  float v = c[k];
+  float v = psum_c(c, k, nsplit, (size_t)M * N);
   if (spectral) {
     float s = 0.0f;
     if (m != 0 || n != 0) {
@@ -151,19 +290,22 @@ __global__ void __launch_bounds__(256) k_split_c(const float* __restrict__ c, rp
 
 // G4's [phi | Ex][j][c] (columns from col0) and G5's Ey[j][i] into phi and the field
 __global__ void __launch_bounds__(256) k_phi_ex(const float* __restrict__ c, float* __restrict__ phi,
-                                                float2* __restrict__ field, int nx, int ny, int ncol, int col0) {
+                                                float2* __restrict__ field, int nx, int ny, int ncol, int col0,
+                                                int nsplit) {
   size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (size_t)ny * ncol) return;
   int j = (int)(k / ncol), cc = col0 + (int)(k % ncol);
+  const float v = psum_c(c, k, nsplit, (size_t)ny * ncol);
   if (cc < nx)
-    phi[(size_t)j * nx + cc] = c[k];
+    phi[(size_t)j * nx + cc] = v;
   else
-    field[(size_t)j * nx + (cc - nx)].x = c[k];
+    field[(size_t)j * nx + (cc - nx)].x = v;
 }
 
-__global__ void __launch_bounds__(256) k_ey(const float* __restrict__ c, float2* __restrict__ field, size_t n) {
+__global__ void __launch_bounds__(256) k_ey(const float* __restrict__ c, float2* __restrict__ field, size_t n,
+                                            int nsplit) {
   size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) field[k].y = c[k];
+  if (k < n) field[k].y = psum_c(c, k, nsplit, n);
 }
 
 // bf16 round-to-nearest-even of a float (host)
@@ -284,6 +426,49 @@ int tune_gemm(rp_ctx* c, Plan& P, Gemm& g, const uint16_t* a, const uint16_t* b,
 
 inline dim3 grid_of(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
+// k_gemm_nt shape of a product: the largest tile that divides M and N, and
+// k slices (1, 2 or 4, dividing K' / 64, at most cmax / (M N) of them) until
+// the launch has at least 256 blocks (one per CU)
+int plan_hand(rp_ctx* c, Gemm& g, size_t cmax) {
+  const int K3 = 3 * g.K;
+  g.tile = g.M % 256 == 0 && g.N % 256 == 0 ? 256 : g.M % 128 == 0 && g.N % 128 == 0 ? 128
+           : g.M % 64 == 0 && g.N % 64 == 0 ? 64 : 0;
+  if (!g.tile || K3 % GBK)
+    return fail(c, RP_ERR_ARG, "matrix-core transform: bin grids of at least 64 bins a side (multiples of 64)");
+  const long tiles = (long)(g.M / g.tile) * (g.N / g.tile);
+  const int kt = K3 / GBK;
+  g.splitk = 1;
+  while (tiles * g.splitk < 256 && g.splitk < 4 && kt % (2 * g.splitk) == 0 &&
+         (size_t)2 * g.splitk * g.M * g.N <= cmax)
+    g.splitk *= 2;
+  return 0;
+}
+
+hipError_t run_hand(const Gemm& g, const uint16_t* a, const uint16_t* b, float* out, hipStream_t s) {
+  const int K3 = 3 * g.K;
+  const unsigned blocks = (unsigned)((g.M / g.tile) * (g.N / g.tile) * g.splitk);
+  const rp_bf16* A = reinterpret_cast<const rp_bf16*>(a);
+  const rp_bf16* B = reinterpret_cast<const rp_bf16*>(b);
+  if (g.tile == 256)
+    hipLaunchKernelGGL((k_gemm_nt<256, 256, 8>), dim3(blocks), dim3(512), 0, s, A, B, out, g.M, g.N, K3, g.splitk);
+  else if (g.tile == 128)
+    hipLaunchKernelGGL((k_gemm_nt<128, 128, 4>), dim3(blocks), dim3(256), 0, s, A, B, out, g.M, g.N, K3, g.splitk);
+  else
+    hipLaunchKernelGGL((k_gemm_nt<64, 64, 4>), dim3(blocks), dim3(256), 0, s, A, B, out, g.M, g.N, K3, g.splitk);
+  return hipGetLastError();
+}
+
+// one product, through k_gemm_nt or hipBLASLt; nsplit: partials in out
+hipError_t product(const Plan& P, const Gemm& g, const uint16_t* a, const uint16_t* b, float* out, hipStream_t s,
+                   int* nsplit) {
+  if (!P.lt) {
+    *nsplit = g.splitk;
+    return run_hand(g, a, b, out, s);
+  }
+  *nsplit = 1;
+  return run_gemm(P, g, a, b, out, s) == HIPBLAS_STATUS_SUCCESS ? hipSuccess : hipErrorUnknown;
+}
+
 }  // namespace
 
 RpMfma::~RpMfma() { delete static_cast<Plan*>(plan); }
@@ -298,7 +483,9 @@ int rp_mfma_setup(rp_ctx* c) {
   m.ready = false;
   Plan* P = new Plan();
   m.plan = P;
-  BLT(c, hipblasLtCreate(&P->h));
+  const char* impl = getenv("RP_MFMA_GEMM");
+  P->lt = impl && strcmp(impl, "hipblaslt") == 0;
+  if (P->lt) BLT(c, hipblasLtCreate(&P->h));
   const double asp = (double)c->bsy / (double)c->bsx;
   auto put = [&](const std::vector<double>& v, int M, int K, DevBuf<uint16_t>& dst) -> int {
     HIPCHK(c, dupload(dst, split3_const(v, M, K), st));
@@ -344,28 +531,37 @@ int rp_mfma_setup(rp_ctx* c) {
   HIPCHK(c, dalloc(P->t13, 3 * B));
   HIPCHK(c, dalloc(P->p3, 3 * B));
   HIPCHK(c, dalloc(P->r3, 6 * B));
-  HIPCHK(c, dalloc(P->c, 2 * B));
+  // the FP32 products: 2 B values (G3 / G4 with phi), or the split-K partials
+  const size_t cmax = 4 * B;
+  HIPCHK(c, dalloc(P->c, cmax));
   HIPCHK(c, dalloc(P->sums, MEAN_PARTS + 1));
-  HIPCHK(c, dalloc(P->ws, kWorkspace));
   for (auto* buf : {&P->rho3, &P->t13, &P->p3, &P->r3})
     HIPCHK(c, hipMemsetAsync(buf->p, 0, buf->n * sizeof(uint16_t), st));
-  if (int r = make_gemm(c, *P, P->g[G1], nx, ny, nx)) return r;
-  if (int r = make_gemm(c, *P, P->g[G2], ny, nx, ny)) return r;
-  if (int r = make_gemm(c, *P, P->g[G3], 2 * nx, ny, nx)) return r;
-  if (int r = make_gemm(c, *P, P->g[G4_PHI], ny, 2 * nx, ny)) return r;
-  if (int r = make_gemm(c, *P, P->g[G4], ny, nx, ny)) return r;
-  if (int r = make_gemm(c, *P, P->g[G5], ny, nx, ny)) return r;
-  // keep each product's fastest algorithm (timed on zero data of the real shapes)
-  if (int r = tune_gemm(c, *P, P->g[G1], P->a[0].p, P->rho3.p, st)) return r;
-  if (int r = tune_gemm(c, *P, P->g[G2], P->a[1].p, P->t13.p, st)) return r;
-  if (int r = tune_gemm(c, *P, P->g[G3], P->a[2].p, P->p3.p, st)) return r;
-  if (int r = tune_gemm(c, *P, P->g[G4_PHI], P->a[3].p, P->r3.p, st)) return r;
-  if (int r = tune_gemm(c, *P, P->g[G4], P->a[3].p, P->r3.p + 3 * B, st)) return r;
-  if (int r = tune_gemm(c, *P, P->g[G5], P->a[4].p, P->r3.p, st)) return r;
+  const int shapes[NGEMM][3] = {{nx, ny, nx}, {ny, nx, ny}, {2 * nx, ny, nx}, {ny, 2 * nx, ny}, {ny, nx, ny},
+                                {ny, nx, ny}};
+  for (int k = 0; k < NGEMM; k++) {
+    Gemm& g = P->g[k];
+    g.M = shapes[k][0], g.N = shapes[k][1], g.K = shapes[k][2];
+    if (P->lt) {
+      if (int r = make_gemm(c, *P, g, g.M, g.N, g.K)) return r;
+    } else if (int r = plan_hand(c, g, cmax)) {
+      return r;
+    }
+  }
+  if (P->lt) {
+    HIPCHK(c, dalloc(P->ws, kWorkspace));
+    // keep each product's fastest algorithm (timed on zero data of the real shapes)
+    if (int r = tune_gemm(c, *P, P->g[G1], P->a[0].p, P->rho3.p, st)) return r;
+    if (int r = tune_gemm(c, *P, P->g[G2], P->a[1].p, P->t13.p, st)) return r;
+    if (int r = tune_gemm(c, *P, P->g[G3], P->a[2].p, P->p3.p, st)) return r;
+    if (int r = tune_gemm(c, *P, P->g[G4_PHI], P->a[3].p, P->r3.p, st)) return r;
+    if (int r = tune_gemm(c, *P, P->g[G4], P->a[3].p, P->r3.p + 3 * B, st)) return r;
+    if (int r = tune_gemm(c, *P, P->g[G5], P->a[4].p, P->r3.p, st)) return r;
+  }
   HIPCHK(c, hipStreamSynchronize(st));
   size_t bytes = 0;
   for (auto& x : P->a) bytes += x.n * 2;
-  bytes += (P->rho3.n + P->t13.n + P->p3.n + P->r3.n) * 2 + P->c.n * 4 + kWorkspace;
+  bytes += (P->rho3.n + P->t13.n + P->p3.n + P->r3.n) * 2 + P->c.n * 4 + P->ws.n;
   m.nbytes = bytes;
   m.ready = true;
   return 0;
@@ -376,35 +572,36 @@ hipError_t rp_launch_poisson_mfma(const RpDev& d, const RpMfmaDev& mv, hipStream
   const Plan& P = *static_cast<const Plan*>(mv.plan);
   const int nx = d.nbx, ny = d.nby;
   const size_t B = (size_t)nx * ny;
-  auto blt = [](hipblasStatus_t st) { return st == HIPBLAS_STATUS_SUCCESS ? hipSuccess : hipErrorUnknown; };
+  int ns = 1;
   hipLaunchKernelGGL(k_rho_parts, dim3(MEAN_PARTS), dim3(256), 0, s, (const float*)d.rho, B, P.sums.p);
   hipLaunchKernelGGL(k_rho_mean, dim3(1), dim3(MEAN_PARTS), 0, s, (const double*)P.sums.p, B, P.sums.p + MEAN_PARTS);
   hipLaunchKernelGGL(k_split_rho, grid_of(B), dim3(256), 0, s, (const float*)d.rho,
                      (const double*)(P.sums.p + MEAN_PARTS), (rp_bf16*)P.rho3.p, nx, ny);
   // G1: T1t[u][j] = Cx[u][i] . rho[j][i]
-  hipError_t e = blt(run_gemm(P, P.g[G1], P.a[0].p, P.rho3.p, P.c.p, s));
+  hipError_t e = product(P, P.g[G1], P.a[0].p, P.rho3.p, P.c.p, s, &ns);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_split_c, grid_of(B), dim3(256), 0, s, (const float*)P.c.p, (rp_bf16*)P.t13.p, nx, ny, 0,
-                     (const float*)nullptr, (const float*)nullptr, 0.0f);
+                     (const float*)nullptr, (const float*)nullptr, 0.0f, ns);
   // G2: P[v][u] = Cy[v][j] . T1t[u][j], scaled
-  if ((e = blt(run_gemm(P, P.g[G2], P.a[1].p, P.t13.p, P.c.p, s))) != hipSuccess) return e;
+  if ((e = product(P, P.g[G2], P.a[1].p, P.t13.p, P.c.p, s, &ns)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_split_c, grid_of(B), dim3(256), 0, s, (const float*)P.c.p, (rp_bf16*)P.p3.p, ny, nx, 1,
-                     (const float*)P.wx2.p, (const float*)P.wy2.p, (float)(4.0 / nx / ny));
+                     (const float*)P.wx2.p, (const float*)P.wy2.p, (float)(4.0 / nx / ny), ns);
   // G3: [R1t; R2t][i][v] = [Cxt; wx Sxt][i][u] . P[v][u]
-  if ((e = blt(run_gemm(P, P.g[G3], P.a[2].p, P.p3.p, P.c.p, s))) != hipSuccess) return e;
+  if ((e = product(P, P.g[G3], P.a[2].p, P.p3.p, P.c.p, s, &ns)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_split_c, grid_of(2 * B), dim3(256), 0, s, (const float*)P.c.p, (rp_bf16*)P.r3.p, 2 * nx, ny,
-                     0, (const float*)nullptr, (const float*)nullptr, 0.0f);
+                     0, (const float*)nullptr, (const float*)nullptr, 0.0f, ns);
   // G4: [phi | Ex][j][.] = Cyt[j][v] . [R1t; R2t][.][v]  (Ex only without phi)
   if (want_phi) {
-    if ((e = blt(run_gemm(P, P.g[G4_PHI], P.a[3].p, P.r3.p, P.c.p, s))) != hipSuccess) return e;
+    if ((e = product(P, P.g[G4_PHI], P.a[3].p, P.r3.p, P.c.p, s, &ns)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_phi_ex, grid_of(2 * B), dim3(256), 0, s, (const float*)P.c.p, d.phi, d.field, nx, ny,
-                       2 * nx, 0);
+                       2 * nx, 0, ns);
   } else {
-    if ((e = blt(run_gemm(P, P.g[G4], P.a[3].p, P.r3.p + 3 * B, P.c.p, s))) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_phi_ex, grid_of(B), dim3(256), 0, s, (const float*)P.c.p, d.phi, d.field, nx, ny, nx, nx);
+    if ((e = product(P, P.g[G4], P.a[3].p, P.r3.p + 3 * B, P.c.p, s, &ns)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_phi_ex, grid_of(B), dim3(256), 0, s, (const float*)P.c.p, d.phi, d.field, nx, ny, nx, nx,
+                       ns);
   }
   // G5: Ey[j][i] = (wy Syt)[j][v] . R1t[i][v]
-  if ((e = blt(run_gemm(P, P.g[G5], P.a[4].p, P.r3.p, P.c.p, s))) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_ey, grid_of(B), dim3(256), 0, s, (const float*)P.c.p, d.field, B);
+  if ((e = product(P, P.g[G5], P.a[4].p, P.r3.p, P.c.p, s, &ns)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_ey, grid_of(B), dim3(256), 0, s, (const float*)P.c.p, d.field, B, ns);
   return hipGetLastError();
 }
