@@ -292,9 +292,9 @@ def bench_precision(args):
         if mode == "lt":
             os.environ["RP_MFMA_GEMM"] = "hipblaslt"
         g = Placer(nl, p)
-        os.environ.pop("RP_MFMA_GEMM", None)
         g.initial_place()
-        info = g.setup()
+        info = g.setup()  # the matrix-core set-up reads RP_MFMA_GEMM
+        os.environ.pop("RP_MFMA_GEMM", None)
         for _ in range(args.warmup):
             g.nesterov_place()
         torch.cuda.synchronize()

@@ -45,7 +45,7 @@ typedef __bf16 rp_bf16;
 
 namespace {
 
-constexpr int MEAN_PARTS = 256;
+constexpr int MEAN_PARTS = 1024;  // blocks of the mean's first pass
 constexpr size_t kWorkspace = (size_t)64 << 20;
 constexpr int kAlgos = 8;  // heuristic candidates timed per GEMM
 
@@ -253,7 +253,9 @@ __global__ void __launch_bounds__(256) k_rho_parts(const float* __restrict__ x, 
 
 __global__ void __launch_bounds__(256) k_rho_mean(const double* parts, size_t n, double* mean) {
   __shared__ double sh[4];
-  double s = block_sum(parts[threadIdx.x], sh);
+  double s = 0.0;
+  for (int k = threadIdx.x; k < MEAN_PARTS; k += blockDim.x) s += parts[k];
+  s = block_sum(s, sh);
   if (threadIdx.x == 0) mean[0] = s / (double)n;
 }
 
@@ -574,7 +576,7 @@ hipError_t rp_launch_poisson_mfma(const RpDev& d, const RpMfmaDev& mv, hipStream
   const size_t B = (size_t)nx * ny;
   int ns = 1;
   hipLaunchKernelGGL(k_rho_parts, dim3(MEAN_PARTS), dim3(256), 0, s, (const float*)d.rho, B, P.sums.p);
-  hipLaunchKernelGGL(k_rho_mean, dim3(1), dim3(MEAN_PARTS), 0, s, (const double*)P.sums.p, B, P.sums.p + MEAN_PARTS);
+  hipLaunchKernelGGL(k_rho_mean, dim3(1), dim3(256), 0, s, (const double*)P.sums.p, B, P.sums.p + MEAN_PARTS);
   hipLaunchKernelGGL(k_split_rho, grid_of(B), dim3(256), 0, s, (const float*)d.rho,
                      (const double*)(P.sums.p + MEAN_PARTS), (rp_bf16*)P.rho3.p, nx, ny);
   // G1: T1t[u][j] = Cx[u][i] . rho[j][i]

@@ -7,6 +7,10 @@
       markdown summary of a rocprofv3 --kernel-trace --stats run
   python -m replace_amd.profiling pmc DIR COUNTER [--regex R]
       mean of one counter per dispatch and kernel from a --pmc run
+  python -m replace_amd.profiling gemms DIR --n 2048
+      the five products of the matrix-core Poisson solve (nx = ny = n): mean
+      time and TFLOP/s of each product position, for the library's own GEMM
+      kernel (k_gemm_nt) and for hipBLASLt's (Cijk_*), from the kernel trace
   python -m replace_amd.profiling traffic FETCH_DIR WRITE_DIR COPYF_DIR COPYW_DIR --copy-mb 512 --out F
       HBM bytes per launch and kernel: FETCH_SIZE and WRITE_SIZE (KiB per
       dispatch) each scaled by what the same counter reads for a device copy
@@ -137,6 +141,40 @@ def calibrated_traffic(fetch_dir, write_dir, copyf_dir, copyw_dir, copy_mb):
     return out, fs, ws
 
 
+def gemm_positions(d: str, n: int) -> str:
+    """Per product position (G1..G5 of one solve, phi not computed): mean time
+    and TFLOP/s, for k_gemm_nt and for hipBLASLt's kernels.  Launches are taken
+    in start order and assigned to positions cyclically, after dropping the
+    launches of hipBLASLt's set-up timing (a multiple of 5 is kept from the end)."""
+    rows = []
+    for path in _find(d, "*kernel_trace.csv"):
+        with open(path) as f:
+            rows += list(csv.DictReader(f))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    k3 = 3 * n
+    flops = [2 * n * n * k3, 2 * n * n * k3, 4 * n * n * k3, 2 * n * n * k3, 2 * n * n * k3]
+    out = ["| kernel | product | launches | mean us | TFLOP/s |", "|---|---|---|---|---|"]
+    for label, key in (("k_gemm_nt", "k_gemm_nt"), ("hipBLASLt", "Cijk")):
+        sel = [r for r in rows if key in r["Kernel_Name"]]
+        if label == "hipBLASLt":
+            # the set-up's timing launches come first; the solves' five per solve follow
+            lt_solves = len(sel) // 5
+            sel = sel[len(sel) - 5 * lt_solves:]
+        if not sel:
+            continue
+        pos = defaultdict(list)
+        for k, r in enumerate(sel):
+            pos[k % 5].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+        total = 0.0
+        for k in range(5):
+            m = sum(pos[k]) / len(pos[k])
+            total += m
+            if label == "k_gemm_nt":  # hipBLASLt's set-up launches leave its positions unaligned
+                out.append(f"| {label} | G{k + 1} | {len(pos[k])} | {m:.2f} | {flops[k] / (m * 1e-6) / 1e12:.0f} |")
+        out.append(f"| {label} | all five | | {total:.2f} | {sum(flops) / (total * 1e-6) / 1e12:.0f} |")
+    return "\n".join(out)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -154,6 +192,9 @@ def main(argv=None):
     s = sub.add_parser("stats")
     s.add_argument("dir")
     s.add_argument("--out")
+    gp = sub.add_parser("gemms")
+    gp.add_argument("dir")
+    gp.add_argument("--n", type=int, default=2048)
     p = sub.add_parser("pmc")
     p.add_argument("dir")
     p.add_argument("counter")
@@ -170,6 +211,8 @@ def main(argv=None):
         if a.out:
             open(a.out, "w").write(txt + "\n")
         print(txt)
+    elif a.cmd == "gemms":
+        print(gemm_positions(a.dir, a.n))
     elif a.cmd == "copy":
         copy_probe(a.mb, a.reps)
     elif a.cmd == "stats":
