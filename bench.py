@@ -48,8 +48,10 @@ def algorithmic_bytes(nc, npin, nn, nb, sort_bits=16):
                      the same launch the pin centres: 28 B/pin (cell id,
                      offset, gathered centre; write the pin centre)
       pin_pos        the pin centres alone (batched launches; 0 here)
-      binfin_rowdct  48 B/bin (read inst, filler, nonPlace int64; write rho,
-                     the row transform, and the zeroed accumulators)
+      binfin_rowdct  40 B/bin (read the inst and filler int64 accumulators and
+                     the float nonPlace area; write the row transform and the
+                     zeroed accumulators; rho itself is stored only for the
+                     matrix-core transform)
       col_spectral   12 B/bin (read the row transform, write Y1, Y2)
       row_inverse    16 B/bin (read Y1, Y2; write Ex, Ey)
       mfma_poisson   the matrix-core solve (flops, see the precision leg)
@@ -65,7 +67,7 @@ def algorithmic_bytes(nc, npin, nn, nb, sort_bits=16):
         "step": 56 * nc,
         "scatter": 25 * nc + 16 * nb + 28 * npin,
         "pin_pos": 28 * npin,
-        "binfin_rowdct": 48 * nb,
+        "binfin_rowdct": 40 * nb,
         "mfma_poisson": 0,
         "col_spectral": 12 * nb,
         "row_inverse": 16 * nb,
