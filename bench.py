@@ -35,7 +35,7 @@ sys.path.insert(0, ROOT)
 METRIC = "s to overflow≤0.10 at 1M cells; served placements/s + p99 at ≤0.5% HPWL"
 HBM_PEAK_GBS = 8000.0
 # timing slot -> kernel name in rocprofv3 output (profiles/traffic_*.json)
-ROCPROF_NAME = {"scatter": "k_scatter_pins", "control": "k_control_main"}
+ROCPROF_NAME = {"scatter": "k_scatter", "control": "k_control_main"}
 
 
 def algorithmic_bytes(nc, npin, nn, nb, sort_bits=16):
@@ -44,10 +44,9 @@ def algorithmic_bytes(nc, npin, nn, nb, sort_bits=16):
       step           56 B/cell (read cur SLP, gradient, coordinate, half
                      size; write next coordinate, next SLP, integer centre)
       scatter        25 B/cell (order index, integer centre, half size,
-                     scale, kind) + 16 B/bin (two int64 accumulators), and in
-                     the same launch the pin centres: 28 B/pin (cell id,
-                     offset, gathered centre; write the pin centre)
-      pin_pos        the pin centres alone (batched launches; 0 here)
+                     scale, kind) + 16 B/bin (two int64 accumulators)
+      pin_pos        reserved slot (0: the wirelength kernel computes the
+                     pin centres)
       binfin_rowdct  40 B/bin (read the inst and filler int64 accumulators and
                      the float nonPlace area; write the row transform and the
                      zeroed accumulators; rho itself is stored only for the
@@ -55,7 +54,9 @@ def algorithmic_bytes(nc, npin, nn, nb, sort_bits=16):
       col_spectral   12 B/bin (read the row transform, write Y1, Y2)
       row_inverse    16 B/bin (read Y1, Y2; write Ex, Ey)
       mfma_poisson   the matrix-core solve (flops, see the precision leg)
-      wa             16 B/pin (pin centre, per-pin gradient) + 4 B/net
+      wa             20 B/pin (cell id, offset, per-pin gradient out) +
+                     8 B/cell (the integer centres the pin centres are made
+                     of) + 4 B/net
       grad           64 B/cell + 12 B/pin (pin index, per-pin gradient)
                      + 8 B/bin (field)
       control        5 doubles per gradient block
@@ -65,13 +66,13 @@ def algorithmic_bytes(nc, npin, nn, nb, sort_bits=16):
     """
     return {
         "step": 56 * nc,
-        "scatter": 25 * nc + 16 * nb + 28 * npin,
-        "pin_pos": 28 * npin,
+        "scatter": 25 * nc + 16 * nb,
+        "pin_pos": 0,
         "binfin_rowdct": 40 * nb,
         "mfma_poisson": 0,
         "col_spectral": 12 * nb,
         "row_inverse": 16 * nb,
-        "wa": 16 * npin + 4 * nn,
+        "wa": 20 * npin + 8 * nc + 4 * nn,
         "grad": 64 * nc + 12 * npin + 8 * nb,
         "control": 8 * 5 * ((nc + 255) // 256),
         "sort_refresh": (12 + 16 * ((sort_bits + 7) // 8)) * nc,

@@ -293,8 +293,8 @@ int rp_get_eval_records(rp_ctx* ctx, rp_eval_record* out, int32_t capacity, int3
  * accumulates the elapsed time (enabling clears the totals, disabling keeps
  * them; the events themselves slow the run, so time whole placements with
  * timing off).  rp_get_kernel_timing fills 11 entries:
- * for the 10 kernel slots in launch order (step, density scatter together
- * with the pin centres, pin centres alone (batched launches only: 0 here),
+ * for the 10 kernel slots in launch order (step, density scatter, a
+ * reserved slot (always 0: the WA kernel computes the pin centres),
  * bin finalize + row DCT, matrix-core Poisson solve (0 with the FFT
  * transform), column spectral pass, row inverse (both 0 with the
  * matrix-core transform), WA, gradient, controller) the summed

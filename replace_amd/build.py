@@ -43,13 +43,17 @@ def _run(cmd):
 
 
 def build_lib() -> str:
-    objs = []
-    for src in LIB_SOURCES:
-        path = os.path.join(CSRC, src)
-        obj = os.path.join(CSRC, src + ".o")
-        lang = ["-x", "hip"]
-        _run([HIPCC, *COMMON, *lang, "-c", path, "-o", obj])
-        objs.append(obj)
+    """Compiles the sources in parallel (one hipcc per file, at most
+    RP_BUILD_JOBS at once, default 8), then links."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    objs = [os.path.join(CSRC, src + ".o") for src in LIB_SOURCES]
+    cmds = [[HIPCC, *COMMON, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+            for src, obj in zip(LIB_SOURCES, objs)]
+    jobs = max(1, int(os.environ.get("RP_BUILD_JOBS", "8")))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for _ in ex.map(_run, cmds):
+            pass
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", LIB, *objs, "-lhipblaslt"])
     for o in objs:
         os.remove(o)

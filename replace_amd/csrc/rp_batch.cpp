@@ -543,7 +543,7 @@ static int rebuild(rp_server* s) {
     cnt[RP_K_SCATTER].push_back(0);  // launched per tile shape below
     for (int v = 0; v < RP_SC_VARIANTS; v++) cnt_sc[v].push_back(v == vs ? (d.nc + (256 << v) - 1) / (256 << v) : 0);
     for (int v = 0; v < RP_WA_VARIANTS; v++) cnt_wa[v].push_back(v == vw ? d.nwa : 0);
-    cnt[RP_K_PIN_POS].push_back((d.np + RP_BLOCK - 1) / RP_BLOCK);
+    cnt[RP_K_PIN_POS].push_back(0);  // pin centres: computed by the wirelength kernel
     cnt[RP_K_BINFIN_ROWDCT].push_back(d.nby / 2);
     cnt[RP_K_MFMA_POISSON].push_back(0);  // batched instances use the FFT solve
     cnt[RP_K_COL_SPECTRAL].push_back(d.nbx / 2);

@@ -96,7 +96,7 @@ struct rp_ctx {
   RpSortBufs sortb{};
   DevBuf<int> d_cp_start, d_cp_pin, d_nstart, d_pcell;
   DevBuf<int4> d_wa_rng[RP_WA_VARIANTS];
-  DevBuf<int2> d_ppos, d_ppin;
+  DevBuf<int2> d_ppos;
   DevBuf<float2> d_pgrad;
   DevBuf<int4> d_nbox;
   DevBuf<float4> d_nsum;
@@ -143,7 +143,7 @@ struct rp_ctx {
     size_t b = 0;
     auto add = [&b](const auto& buf) { b += buf.n * sizeof(*buf.p); };
     add(d_dhalf); add(d_scale); add(d_area); add(d_npin); add(d_kind); add(d_slp); add(d_sumg); add(d_co);
-    add(d_ic); add(d_cp_start); add(d_cp_pin); add(d_nstart); add(d_pcell); add(d_ppos); add(d_ppin); add(d_pgrad);
+    add(d_ic); add(d_cp_start); add(d_cp_pin); add(d_nstart); add(d_pcell); add(d_ppos); add(d_pgrad);
     add(d_nbox); add(d_nsum); add(d_bin_inst); add(d_bin_filler); add(d_bin_npf); add(d_rho); add(d_phi);
     add(d_t); add(d_y1); add(d_y2); add(d_field); add(d_twx); add(d_tw2x);
     add(d_twy); add(d_tw2y); add(d_wx); add(d_wx2); add(d_wy); add(d_wy2); add(d_part_grad); add(d_part_bin);

@@ -35,7 +35,6 @@ struct RpDev {
   const int4* wa_rng_v[RP_WA_VARIANTS];
   const int* pcell;
   const int2* ppos;
-  int2* ppin;        // absolute pin centres, rewritten every evaluation (k_pin_pos)
   int4* nbox;
   float4* nsum;
   unsigned long long* bin_inst;

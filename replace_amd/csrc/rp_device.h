@@ -51,7 +51,7 @@
 enum {
   RP_K_STEP = 0,  // single instance: step + integer centres (batched: fused into RP_K_SCATTER)
   RP_K_SCATTER,   // density scatter (single instance: spatial cell order)
-  RP_K_PIN_POS,
+  RP_K_PIN_POS,   // reserved (0): the wirelength kernel computes the pin centres as it loads them
   RP_K_BINFIN_ROWDCT,
   RP_K_MFMA_POISSON,  // matrix-core Poisson solve (replaces the next two)
   RP_K_COL_SPECTRAL,

@@ -1001,7 +1001,6 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   const int nwa_max = *std::max_element(nwa_v, nwa_v + RP_WA_VARIANTS);
   HIPCHK(c, dupload(c->d_pcell, pcell, s));
   HIPCHK(c, dupload(c->d_ppos, ppos, s));
-  HIPCHK(c, dalloc(c->d_ppin, std::max(1, npn)));
   HIPCHK(c, dalloc(c->d_nbox, nn));
   HIPCHK(c, dalloc(c->d_nsum, (size_t)2 * nn));
   HIPCHK(c, dalloc(c->d_bin_inst, nb));
@@ -1090,7 +1089,6 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   d.wa_rng = c->d_wa_rng[RP_WA_DEFAULT_VARIANT].p;
   d.pcell = c->d_pcell.p;
   d.ppos = c->d_ppos.p;
-  d.ppin = c->d_ppin.p;
   d.nbox = c->d_nbox.p;
   d.nsum = c->d_nsum.p;
   d.bin_inst = c->d_bin_inst.p;

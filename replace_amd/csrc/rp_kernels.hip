@@ -4,11 +4,11 @@
 // One evaluation (reference src/nesterovPlace.cpp:338-377) is the launch
 // sequence
 //   k_step              step + clamp + int centre (A14, A5)
-//   k_scatter_pins      density scatter in spatial cell order (A6) and pin centres (A5)
+//   k_scatter           density scatter in spatial cell order (A5, A6)
 //   k_binfin_rowdct     bin density, overflow partials, DCT-II along x (A6 tail, A8)
 //   k_col_spectral      DCT-II along y, spectral divide, DCT-III / DST-III along y (A8)
 //   k_row_inverse       DST-III / DCT-III along x -> field (Ex, Ey) (A8/A9)
-//   k_wa                net bbox, WA exponent sums, HPWL partials (A10, A16)
+//   k_wa                pin centres, net bbox, WA exponent sums, HPWL partials (A5, A10, A16)
 //   k_grad              WL gradient, field gather, preconditioner, partial sums (A11-A13, A15)
 //   k_control_main      reductions + back-tracking / schedules / stop test (A15, A16, A18)
 // Every kernel reads the controller state first and returns when the run
@@ -600,20 +600,17 @@ __device__ void sorted_scatter_body(const RpDev& d, int blk, const RpSortBufs& s
 
 // ---------------------------------------------------------------------------
 // 1b. pin centres (GPin::updateDensityLocation, src/nesterovBase.cpp:411-415):
-// integer density centre + offset, one thread per pin in the wirelength
-// kernel's pin order (coalesced stores; the centre gather follows the
-// locality order).  Fixed and port pins keep their absolute position.
-__device__ void pinpos_body(const RpDev& d, int blk) {
-  if (d.st->phase != 0) return;
-  int q = blk * RP_BLOCK + threadIdx.x;
-  if (q >= d.np) return;
-  int c = d.pcell[q];
+// integer density centre + offset; fixed and port pins (no gcell) keep their
+// absolute position.  The wirelength kernel computes them as it loads its
+// pins, so they are never stored.
+__device__ __forceinline__ int2 pin_centre(const RpDev& d, int q) {
+  const int c = d.pcell[q];
   int2 o = d.ppos[q];
   if (c >= 0) {
-    int2 icc = d.ic[c];
+    const int2 icc = d.ic[c];
     o = make_int2(icc.x + o.x, icc.y + o.y);
   }
-  d.ppin[q] = o;
+  return o;
 }
 
 // ---------------------------------------------------------------------------
@@ -1242,7 +1239,7 @@ __device__ __forceinline__ float2 wa_pin_grad(int2 p, float4 v, const float4& sa
 // parallel.  The workgroup takes the nets d.wa_rng[blk] (consecutive, at
 // most RP_WA_NETS nets and CAP pins, or one larger net alone; the packing
 // for each of the RP_WA_VARIANTS capacities is built at set-up):
-//   1. pin centres into LDS (coalesced);
+//   1. pin centres (cell centre + offset, pin_centre) into LDS;
 //   2. one lane per net: pin bbox, the pin -> net map, HPWL term;
 //   3. one lane per pin: the four exponentials;
 //   4. one lane per net: the eight sums in pin order (the reference's float
@@ -1280,11 +1277,25 @@ __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
       t = d.nstart[e + 1] - p0;
     }
     {
+      // pin centres (pin_centre): cell ids and offsets of all of a thread's
+      // pins first, then the cell centre gathers
+      int pc[PER];
       int2 lp[PER];
 #pragma unroll
       for (int r = 0; r < PER; r++) {
         const int j = tid + r * RP_BLOCK;
-        if (j < npw) lp[r] = d.ppin[p0 + j];
+        if (j < npw) {
+          pc[r] = d.pcell[p0 + j];
+          lp[r] = d.ppos[p0 + j];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < PER; r++) {
+        const int j = tid + r * RP_BLOCK;
+        if (j < npw && pc[r] >= 0) {
+          const int2 icc = d.ic[pc[r]];
+          lp[r] = make_int2(icc.x + lp[r].x, icc.y + lp[r].y);
+        }
       }
 #pragma unroll
       for (int r = 0; r < PER; r++) {
@@ -1394,7 +1405,7 @@ __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
     const int e = rng.x;
     int4 box = make_int4(INT_MAX, INT_MAX, INT_MIN, INT_MIN);
     for (int q = tid; q < npw; q += blockDim.x) {
-      int2 p = d.ppin[p0 + q];
+      int2 p = pin_centre(d, p0 + q);
       box.x = min(p.x, box.x);
       box.y = min(p.y, box.y);
       box.z = max(p.x, box.z);
@@ -1421,7 +1432,7 @@ __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
       const int cnt = min(CAP, npw - c0);
       __syncthreads();  // previous chunk consumed (and s_box read by everyone)
       for (int j = tid; j < cnt; j += blockDim.x) {
-        int2 p = d.ppin[p0 + c0 + j];
+        int2 p = pin_centre(d, p0 + c0 + j);
         s_pos[j] = p;
         s_ex[j] = wa_exps(p, box, cx, cy, bar);
       }
@@ -1442,7 +1453,7 @@ __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
     __syncthreads();
     const float4 sa = s_sum[0], sb = s_sum[1];
     for (int q = tid; q < npw; q += blockDim.x) {
-      int2 p = d.ppin[p0 + q];
+      int2 p = pin_centre(d, p0 + q);
       d.pgrad[p0 + q] = wa_pin_grad(p, wa_exps(p, box, cx, cy, bar), sa, sb, cx, cy);
     }
   }
@@ -1952,33 +1963,26 @@ __device__ void control_main_body(const RpDev& d) {
 // its result is bit-identical alone or in a batch.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(RP_BLOCK) k_step(RpDev d) { step_body(d, xcd_block(blockIdx.x, gridDim.x)); }
-// Density scatter (blocks [0, nss)) and pin centres (the rest) in one
-// launch: both read only the integer centres the step wrote.  nss is a
-// multiple of the XCD count, so both ranges keep the XCD-contiguous order.
+// Density scatter in spatial cell order.  The block count is a multiple of
+// the XCD count, so xcd_block keeps each XCD on a contiguous range of the
+// order.
 template <int PER>
-__global__ void __launch_bounds__(RP_BLOCK) k_scatter_pins(RpDev d, RpSortBufs sb, int count, int nss) {
-  const int b = blockIdx.x;
-  if (b < nss)
-    sorted_scatter_body<PER>(d, xcd_block(b, nss), sb, count);
-  else
-    pinpos_body(d, xcd_block(b - nss, gridDim.x - nss));
+__global__ void __launch_bounds__(RP_BLOCK) k_scatter(RpDev d, RpSortBufs sb, int count) {
+  sorted_scatter_body<PER>(d, xcd_block(blockIdx.x, gridDim.x), sb, count);
 }
-static int pin_blocks(const RpDev& d) { return std::max(1, (d.np + RP_BLOCK - 1) / RP_BLOCK); }
 static int scatter_per_thread(const RpDev& d) { return d.scatter_per == 2 ? 2 : 1; }
-static int scatter_pins_blocks(const RpDev& d) {
+static int scatter_blocks(const RpDev& d) {
   const int cells = RP_BLOCK * scatter_per_thread(d);
   int n = (d.nc + cells - 1) / cells;
   return (n + RP_NUM_XCD - 1) / RP_NUM_XCD * RP_NUM_XCD;
 }
-static void launch_scatter_pins(const RpDev& d, hipStream_t s, const RpSortBufs& sb, int count) {
-  const int nss = scatter_pins_blocks(d);
-  const dim3 grid(nss + pin_blocks(d));
+static void launch_scatter(const RpDev& d, hipStream_t s, const RpSortBufs& sb, int count) {
+  const dim3 grid(scatter_blocks(d));
   if (scatter_per_thread(d) == 2)
-    hipLaunchKernelGGL(k_scatter_pins<2>, grid, dim3(RP_BLOCK), 0, s, d, sb, count, nss);
+    hipLaunchKernelGGL(k_scatter<2>, grid, dim3(RP_BLOCK), 0, s, d, sb, count);
   else
-    hipLaunchKernelGGL(k_scatter_pins<1>, grid, dim3(RP_BLOCK), 0, s, d, sb, count, nss);
+    hipLaunchKernelGGL(k_scatter<1>, grid, dim3(RP_BLOCK), 0, s, d, sb, count);
 }
-__global__ void __launch_bounds__(RP_BLOCK) k_pin_pos(RpDev d) { pinpos_body(d, xcd_block(blockIdx.x, gridDim.x)); }
 __global__ void __launch_bounds__(RP_BLOCK) k_binfin_rowdct(RpDev d, int from_rho) {
   RP_FFT_DISPATCH(d.nbx, binfin_body, d, blockIdx.x, from_rho)
 }
@@ -2004,10 +2008,6 @@ template <int PER, int BITS>
 __global__ void __launch_bounds__(RP_BLOCK) kb_step_scatter(const RpDev* devs, const int2* tab) {
   int2 e = tab[blockIdx.x];
   if (e.x >= 0) scatter_body<PER, BITS>(devs[e.x], e.y);
-}
-__global__ void __launch_bounds__(RP_BLOCK) kb_pin_pos(const RpDev* devs, const int2* tab) {
-  int2 e = tab[blockIdx.x];
-  if (e.x >= 0) pinpos_body(devs[e.x], e.y);
 }
 __global__ void __launch_bounds__(RP_BLOCK) kb_binfin_rowdct(const RpDev* devs, const int2* tab) {
   int2 e = tab[blockIdx.x];
@@ -2078,7 +2078,7 @@ hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_
   hipLaunchKernelGGL(k_step, dim3(nb_c), dim3(RP_BLOCK), 0, s, d);
   end(RP_K_STEP);
   beg(RP_K_SCATTER);
-  launch_scatter_pins(d, s, sb ? *sb : RpSortBufs{}, sb ? 1 : 0);
+  launch_scatter(d, s, sb ? *sb : RpSortBufs{}, sb ? 1 : 0);
   end(RP_K_SCATTER);
   beg(RP_K_BINFIN_ROWDCT);
   hipLaunchKernelGGL(k_binfin_rowdct, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), rp_lds_binfin(d.nbx), s, d, 0);
@@ -2118,7 +2118,7 @@ hipError_t rp_launch_ctrl_init(const RpDev& d, hipStream_t s, int stage) {
 // Split stages for the stage-level test entry points.
 hipError_t rp_launch_scatter(const RpDev& d, hipStream_t s) {
   hipLaunchKernelGGL(k_step, dim3((d.nc + RP_BLOCK - 1) / RP_BLOCK), dim3(RP_BLOCK), 0, s, d);
-  launch_scatter_pins(d, s, RpSortBufs{}, 0);
+  launch_scatter(d, s, RpSortBufs{}, 0);
   return hipGetLastError();
 }
 
@@ -2233,8 +2233,6 @@ hipError_t rp_launch_eval_batch(const RpBatchLaunch& b, hipStream_t s) {
   const int* n = b.nblk;
   for (int v = 0; v < RP_SC_VARIANTS; v++)
     if (b.nblk_sc[v]) (void)rp_launch_batch_variant(RP_K_SCATTER, v, b.devs, b.tab_sc[v], b.nblk_sc[v], s);
-  if (n[RP_K_PIN_POS])
-    hipLaunchKernelGGL(kb_pin_pos, dim3(n[RP_K_PIN_POS]), dim3(RP_BLOCK), 0, s, b.devs, b.tab[RP_K_PIN_POS]);
   if (n[RP_K_BINFIN_ROWDCT])
     hipLaunchKernelGGL(kb_binfin_rowdct, dim3(n[RP_K_BINFIN_ROWDCT]), dim3(RP_BLOCK), b.lds_binfin, s, b.devs,
                        b.tab[RP_K_BINFIN_ROWDCT]);
