@@ -485,20 +485,55 @@ def main():
                                       "sample": f"failed: {e}"[:300]}
     placer.close()
     if args.workload == "c1" and not args.no_service:
-        # the batching and serving forms on their workloads, summarised
-        b = bench_batch(args)
-        result["batched"] = {k: b[k] for k in ("metric", "value", "unit", "speedup_vs_plain", "wall_s", "jobs_finished",
-                                                "unfinished", "max_final_overflow_finished",
-                                                "hpwl_max_rel_diff_vs_plain")}
-        result["batched"]["plain_back_to_back"] = {k: b["plain_back_to_back"][k] for k in ("value", "wall_s")}
-        result["batched"]["config"] = b["config"]["workload"]
-        v = bench_serve(args)
-        result["served"] = {k: v[k] for k in ("metric", "value", "unit", "speedup_vs_plain", "latency_s", "jobs_finished",
-                                              "unfinished", "max_final_overflow_finished",
-                                              "hpwl_max_rel_diff_vs_plain")}
-        result["served"]["plain_back_to_back"] = v["plain_back_to_back"]
-        result["served"]["config"] = v["config"]["workload"]
+        # the batching and serving forms on their workloads, summarised; each
+        # runs in a child process, so a failure there is reported in its key
+        # and never costs the headline line
+        b = _service_leg("c3", args)
+        if "error" in b:
+            result["batched"] = b
+        else:
+            _summarise_batch(result, b)
+        v = _service_leg("c4", args)
+        if "error" in v:
+            result["served"] = v
+        else:
+            _summarise_serve(result, v)
     print(json.dumps(result), flush=True)
+
+
+def _service_leg(workload, args):
+    """`bench.py --workload c3|c4` as a child process; its JSON line, or
+    {"error": ...} when it fails."""
+    import subprocess
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--workload", workload,
+           "--steps-per-poll", str(args.steps_per_poll), "--setup-threads", str(args.setup_threads),
+           "--batched-ip", str(args.batched_ip), "--rate", str(args.rate)]
+    if args.jobs:
+        cmd += ["--jobs", str(args.jobs)]
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+    except subprocess.TimeoutExpired:
+        return {"error": f"{workload}: no result within 900 s"}
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": f"{workload}: exit status {r.returncode}", "stderr_tail": r.stderr[-600:]}
+    return json.loads(lines[-1])
+
+
+def _summarise_batch(result, b):
+    result["batched"] = {k: b[k] for k in ("metric", "value", "unit", "speedup_vs_plain", "wall_s", "jobs_finished",
+                                            "unfinished", "max_final_overflow_finished",
+                                            "hpwl_max_rel_diff_vs_plain")}
+    result["batched"]["plain_back_to_back"] = {k: b["plain_back_to_back"][k] for k in ("value", "wall_s")}
+    result["batched"]["config"] = b["config"]["workload"]
+
+
+def _summarise_serve(result, v):
+    result["served"] = {k: v[k] for k in ("metric", "value", "unit", "speedup_vs_plain", "latency_s", "jobs_finished",
+                                          "unfinished", "max_final_overflow_finished",
+                                          "hpwl_max_rel_diff_vs_plain")}
+    result["served"]["plain_back_to_back"] = v["plain_back_to_back"]
+    result["served"]["config"] = v["config"]["workload"]
 
 
 if __name__ == "__main__":
