@@ -11,6 +11,7 @@ struct RpDev {
   int nc, nn, np, nbx, nby, bsx, bsy;
   int mfma;  // Poisson on matrix cores: the bin kernel only produces rho
   int scatter_per;  // cells per thread of the spatial-order scatter (1 or 2)
+  int fuse_scatter_wa;  // single instance: density scatter and wirelength in one launch
   int core_lx, core_ly, core_ux, core_uy;
   float td;
   float bar;          // minWireLengthForceBar

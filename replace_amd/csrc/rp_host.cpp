@@ -1061,6 +1061,7 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   d.core_uy = c->core_uy;
   d.td = c->td;
   d.scatter_per = ng >= RP_SS_WIDE_CELLS ? 2 : 1;
+  d.fuse_scatter_wa = getenv("RP_NO_FUSE") == nullptr ? 1 : 0;  // RP_NO_FUSE=1: two launches
   if (const char* e = getenv("RP_SCATTER_PER")) {
     const int v = atoi(e);
     if (v == 1 || v == 2) d.scatter_per = v;

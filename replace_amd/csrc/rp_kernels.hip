@@ -531,10 +531,16 @@ __device__ void step_body(const RpDev& d, int blk) {
 // kept in sb.rank.  rp_launch_sort_refresh turns counts and ranks into the
 // new order.
 #define RP_TILE_SLOTS 256
+struct SortScatterLds {
+  WinTable t;
+  int key[RP_TILE_SLOTS], cnt[RP_TILE_SLOTS], base[RP_TILE_SLOTS];
+};
 template <int PER>
-__device__ void sorted_scatter_body(const RpDev& d, int blk, const RpSortBufs& sb, int count) {
-  __shared__ WinTable t;
-  __shared__ int t_key[RP_TILE_SLOTS], t_cnt[RP_TILE_SLOTS], t_base[RP_TILE_SLOTS];
+__device__ void sorted_scatter_body(const RpDev& d, int blk, const RpSortBufs& sb, int count, SortScatterLds& L) {
+  WinTable& t = L.t;
+  int* t_key = L.key;
+  int* t_cnt = L.cnt;
+  int* t_base = L.base;
   const RpState* st = d.st;
   if (st->phase != 0) return;
   int cell[PER], slot[PER], lrank[PER];
@@ -596,6 +602,13 @@ __device__ void sorted_scatter_body(const RpDev& d, int blk, const RpSortBufs& s
     for (int r = 0; r < PER; r++)
       if (cell[r] >= 0 && slot[r] >= 0) sb.rank[cell[r]] = t_base[slot[r]] + lrank[r];
   }
+}
+
+// sorted_scatter_body with its own LDS
+template <int PER>
+__device__ void sorted_scatter_body(const RpDev& d, int blk, const RpSortBufs& sb, int count) {
+  __shared__ SortScatterLds L;
+  sorted_scatter_body<PER>(d, blk, sb, count, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -1249,13 +1262,22 @@ __device__ __forceinline__ float2 wa_pin_grad(int2 p, float4 v, const float4& sa
 // The net bbox and sums are stored only when store_nets is set (stage-level
 // checks).
 template <int CAP>
-__device__ void wa_body(const RpDev& d, int blk, int store_nets) {
-  __shared__ int2 s_pos[CAP];
-  __shared__ float4 s_ex[CAP];
-  __shared__ uint8_t s_net[CAP];  // net index within the workgroup (< RP_WA_NETS = 256)
-  __shared__ int4 s_box[RP_WA_NETS];
-  __shared__ float4 s_sum[2 * RP_WA_NETS];
-  __shared__ long long sh_red[RP_BLOCK / 64];
+struct WaLds {
+  float4 ex[CAP];
+  float4 sum[2 * RP_WA_NETS];
+  int4 box[RP_WA_NETS];
+  int2 pos[CAP];
+  long long red[RP_BLOCK / 64];
+  uint8_t net[CAP];  // net index within the workgroup (< RP_WA_NETS = 256)
+};
+template <int CAP>
+__device__ void wa_body(const RpDev& d, int blk, int store_nets, WaLds<CAP>& W) {
+  int2* s_pos = W.pos;
+  float4* s_ex = W.ex;
+  uint8_t* s_net = W.net;
+  int4* s_box = W.box;
+  float4* s_sum = W.sum;
+  long long* sh_red = W.red;
   const RpState* st = d.st;
   if (st->phase != 0) return;
   const float cx = st->wl_coef_x, cy = st->wl_coef_y, bar = d.bar;
@@ -1459,6 +1481,13 @@ __device__ void wa_body(const RpDev& d, int blk, int store_nets) {
   }
   long long tot = block_sum_ll(h, sh_red);
   if (threadIdx.x == 0) d.part_net[blk] = tot;
+}
+
+// wa_body with its own LDS (the wirelength kernels of one kind)
+template <int CAP>
+__device__ void wa_body(const RpDev& d, int blk, int store_nets) {
+  __shared__ WaLds<CAP> W;
+  wa_body<CAP>(d, blk, store_nets, W);
 }
 
 // ---------------------------------------------------------------------------
@@ -2001,6 +2030,33 @@ __global__ void __launch_bounds__(RP_BLOCK) k_row_inverse(RpDev d, int want_phi)
 __global__ void __launch_bounds__(RP_BLOCK) k_wa(RpDev d, int store_nets) {
   wa_body<RP_WA_CAP>(d, xcd_block(blockIdx.x, gridDim.x), store_nets);
 }
+// Density scatter and wirelength in one launch (single instance).  Both read
+// only the integer centres the step wrote and write disjoint outputs; the
+// scatter waits on gathers and memory-side atomics, the wirelength blocks on
+// VALU, so sharing the CUs overlaps the two, and one launch gap goes.  The
+// LDS is one union of the two tables.  Hardware block b runs on XCD
+// b mod 8; in each XCD's sequence of blocks the ns8 scatter blocks are
+// spread evenly among the wirelength blocks (a scatter block first), and
+// each kind covers the XCD-contiguous logical range xcd_block gives it in a
+// launch of its own, so every logical block computes exactly what it does
+// there (results are identical).
+union ScatterWaLds {
+  SortScatterLds s;
+  WaLds<RP_WA_CAP> w;
+};
+template <int PER>
+__global__ void __launch_bounds__(RP_BLOCK) k_scatter_wa(RpDev d, RpSortBufs sb, int count, int ns8) {
+  __shared__ ScatterWaLds L;
+  const int x = blockIdx.x % RP_NUM_XCD, j = blockIdx.x / RP_NUM_XCD;
+  const int nwa = max(1, d.nwa), perw = nwa / RP_NUM_XCD, remw = nwa % RP_NUM_XCD;
+  const int cnt = ns8 + perw + (x < remw);  // blocks of this XCD
+  // scatter blocks among positions [0, j) and [0, j]
+  const int c0 = (j * ns8 + cnt - 1) / cnt, c1 = ((j + 1) * ns8 + cnt - 1) / cnt;
+  if (c1 > c0)
+    sorted_scatter_body<PER>(d, x * ns8 + c0, sb, count, L.s);
+  else
+    wa_body<RP_WA_CAP>(d, x * perw + min(x, remw) + (j - c1), 0, L.w);
+}
 __global__ void __launch_bounds__(RP_BLOCK) k_grad(RpDev d) { grad_body(d, xcd_block(blockIdx.x, gridDim.x)); }
 __global__ void __launch_bounds__(RP_CTRL_BLOCK) k_control_main(RpDev d) { control_main_body(d); }
 
@@ -2063,6 +2119,15 @@ static void launch_rowinv(const RpDev& d, hipStream_t s, int want_phi) {
 // One evaluation slot.  ev (optional) holds 2 * RP_NUM_EVAL_KERNELS events:
 // ev[2q] and ev[2q + 1] bracket kernel q when it is launched on this path;
 // *launched (optional) receives the mask of the kernels that were.
+static void launch_scatter_wa(const RpDev& d, hipStream_t s, const RpSortBufs& sb, int count) {
+  const int ns = scatter_blocks(d);  // a multiple of the XCD count
+  const dim3 grid(ns + std::max(1, d.nwa));
+  if (scatter_per_thread(d) == 2)
+    hipLaunchKernelGGL(k_scatter_wa<2>, grid, dim3(RP_BLOCK), 0, s, d, sb, count, ns / RP_NUM_XCD);
+  else
+    hipLaunchKernelGGL(k_scatter_wa<1>, grid, dim3(RP_BLOCK), 0, s, d, sb, count, ns / RP_NUM_XCD);
+}
+
 hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_t* ev, const RpMfmaDev* mf,
                           const RpSortBufs* sb, unsigned* launched) {
   const int nb_c = (d.nc + RP_BLOCK - 1) / RP_BLOCK;
@@ -2078,7 +2143,10 @@ hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_
   hipLaunchKernelGGL(k_step, dim3(nb_c), dim3(RP_BLOCK), 0, s, d);
   end(RP_K_STEP);
   beg(RP_K_SCATTER);
-  launch_scatter(d, s, sb ? *sb : RpSortBufs{}, sb ? 1 : 0);
+  if (d.fuse_scatter_wa)  // the wirelength kernel's work goes into this launch
+    launch_scatter_wa(d, s, sb ? *sb : RpSortBufs{}, sb ? 1 : 0);
+  else
+    launch_scatter(d, s, sb ? *sb : RpSortBufs{}, sb ? 1 : 0);
   end(RP_K_SCATTER);
   beg(RP_K_BINFIN_ROWDCT);
   hipLaunchKernelGGL(k_binfin_rowdct, dim3(d.nby / 2), dim3(fft_threads(d.nbx)), rp_lds_binfin(d.nbx), s, d, 0);
@@ -2097,9 +2165,11 @@ hipError_t rp_launch_eval(const RpDev& d, hipStream_t s, int want_phi, hipEvent_
     launch_rowinv(d, s, want_phi);
     end(RP_K_ROW_INVERSE);
   }
-  beg(RP_K_WA);
-  hipLaunchKernelGGL(k_wa, dim3(std::max(1, d.nwa)), dim3(RP_BLOCK), 0, s, d, 0);
-  end(RP_K_WA);
+  if (!d.fuse_scatter_wa) {
+    beg(RP_K_WA);
+    hipLaunchKernelGGL(k_wa, dim3(std::max(1, d.nwa)), dim3(RP_BLOCK), 0, s, d, 0);
+    end(RP_K_WA);
+  }
   beg(RP_K_GRAD);
   hipLaunchKernelGGL(k_grad, dim3(nb_c), dim3(RP_BLOCK), 0, s, d);
   end(RP_K_GRAD);
