@@ -32,9 +32,11 @@
 // fastExp, reference src/nesterovBase.cpp:1568-1582.  Its first step,
 // (float)(1.0 + (double)a / 1024.0), is computed in float: a / 1024 is exact,
 // and the double sum is exact whenever it can differ from 1.0f, so one float
-// rounding gives the same value (checked for every finite float).
+// rounding gives the same value (checked for every finite float).  As one
+// FMA: the product by 2^-10 is exact (or, below 2^-116, too small to move
+// 1.0f), so fma(a, 2^-10, 1) rounds the same sum once, like the add.
 __device__ __forceinline__ float fast_exp(float a) {
-  a = 1.0f + a * 0.0009765625f;
+  a = __builtin_fmaf(a, 0.0009765625f, 1.0f);
   a = a * a; a = a * a; a = a * a; a = a * a; a = a * a;
   a = a * a; a = a * a; a = a * a; a = a * a; a = a * a;
   return a;
@@ -1504,17 +1506,28 @@ __device__ void grad_body(const RpDev& d, int blk) {
   if (c < d.nc) {
     float lam = st->density_penalty;
     int dst = st->dst, ref = st->ring.cur;
-    int2 icc = d.ic[c];
+    // Every load that needs only the cell index is issued first.  The pin
+    // and bin gathers below stay one small batch or one bin at a time:
+    // prefetching a 3 x 3 field box for every cell (up to 9 gathers instead
+    // of the ~6 a box covers) measured slower, the gathers' issue rate and
+    // not their latency being what this kernel waits on.
+    const int2 icc = d.ic[c];
+    const int2 h = d.dhalf[c];
+    const float sc = d.scale[c];
+    const float np = d.npin[c], ar = d.area[c];
+    const int k0 = d.cp_start[c], k1 = d.cp_start[c + 1];
     // the distance terms' operands, loaded before the gradient store below
     // (which the compiler cannot tell apart from them)
     const float2 a = d.slp[(size_t)ref * d.nc + c];
     const float2 b = d.slp[(size_t)dst * d.nc + c];
     const float2 ga = d.sumg[(size_t)ref * d.nc + c];
-    // wire length
+    const int dlx = icc.x - h.x, dux = icc.x + h.x, dly = icc.y - h.y, duy = icc.y + h.y;
+    int x0, x1, y0, y1;
+    idx_range(dlx, dux, d.core_lx, d.bsx, d.nbx, x0, x1);
+    idx_range(dly, duy, d.core_ly, d.bsy, d.nby, y0, y1);
+    // wire length: per-pin WA gradients (written by k_wa), summed in the
+    // cell's pin order (getWireLengthGradientWA, src/nesterovBase.cpp:1363-1375)
     float gx = 0.0f, gy = 0.0f;
-    int k0 = d.cp_start[c], k1 = d.cp_start[c + 1];
-    // per-pin WA gradients (written by k_wa), summed in the cell's pin order
-    // (getWireLengthGradientWA, src/nesterovBase.cpp:1363-1375)
     for (int k = k0; k < k1; k += 4) {
       int qq[4];
       float2 pg[4];
@@ -1532,12 +1545,6 @@ __device__ void grad_body(const RpDev& d, int blk) {
         }
     }
     // field gather (getDensityGradient, src/nesterovBase.cpp:1458-1478)
-    int2 h = d.dhalf[c];
-    int dlx = icc.x - h.x, dux = icc.x + h.x, dly = icc.y - h.y, duy = icc.y + h.y;
-    int x0, x1, y0, y1;
-    idx_range(dlx, dux, d.core_lx, d.bsx, d.nbx, x0, x1);
-    idx_range(dly, duy, d.core_ly, d.bsy, d.nby, y0, y1);
-    float sc = d.scale[c];
     float fx = 0.0f, fy = 0.0f;
     for (int i = x0; i < x1; i++) {
       int blx, bux;
@@ -1557,7 +1564,6 @@ __device__ void grad_body(const RpDev& d, int blk) {
     // sum gradient and preconditioner
     float sx = gx + lam * fx;
     float sy = gy + lam * fy;
-    float np = d.npin[c], ar = d.area[c];
     float ppx = np + lam * ar;
     float ppy = np + lam * ar;
     if (ppx <= d.min_precond) ppx = d.min_precond;
