@@ -60,6 +60,7 @@ struct RpDev {
   double* part_bin;
   long long* part_net;
   RpState* st;
+  int npart_grad, npart_bin;  // partial sums per evaluation (RpState's num_partials_grad / _bin)
   RpIterRec* rec;
   float2* dbg_wl;
   float2* dbg_dens;

@@ -1112,6 +1112,8 @@ extern "C" int rp_nesterov_setup(rp_ctx* c, rp_nb_info* info) {
   d.wy2 = c->d_wy2.p;
   d.part_grad = c->d_part_grad.p;
   d.part_bin = c->d_part_bin.p;
+  d.npart_grad = npg;
+  d.npart_bin = npb;
   d.part_net = c->d_part_net.p;
   d.st = c->d_state.p;
   d.rec = c->d_rec.p;

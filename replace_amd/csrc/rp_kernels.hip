@@ -1659,19 +1659,21 @@ struct Totals {
 // Totals of the gradient, bin and net partials; valid in thread 0.
 __device__ void reduce_all(const RpDev& d, Totals& t) {
   __shared__ long long shl[RP_CTRL_BLOCK / 64];
-  const RpState* st = d.st;
   // Each thread sums a fixed strided subset of the gradient, bin (overflow)
   // and net (HPWL) partials, batch by batch in index order; the first batch
   // of all three arrays is loaded before any of them is added.
   double v[RP_P_COUNT] = {0, 0, 0, 0, 0};
   double vb = 0.0;
   long long vh = 0;
-  const int ng = st->num_partials_grad;
-  const int nbp = st->num_partials_bin, nnp = d.nwa;  // one HPWL partial per wirelength workgroup
+  const int ng = d.npart_grad;
+  const int nbp = d.npart_bin, nnp = d.nwa;  // one HPWL partial per wirelength workgroup
   constexpr int GB = 4;  // gradient iterations whose loads are issued together
   const int T = blockDim.x, k0 = threadIdx.x;
+  // the net partials (one per wirelength workgroup: 8252 at 1M cells) in
+  // batches of 16 per thread, so the usual sizes take one load round
+  constexpr int NB = 16;
   double xg[GB][RP_P_COUNT], xb[RP_RED_BATCH];
-  long long xn[RP_RED_BATCH];
+  long long xn[NB];
 #pragma unroll
   for (int u = 0; u < GB; u++) {
     const int k = k0 + u * T;
@@ -1683,6 +1685,10 @@ __device__ void reduce_all(const RpDev& d, Totals& t) {
   for (int u = 0; u < RP_RED_BATCH; u++) {
     const int k = k0 + u * T;
     xb[u] = k < nbp ? d.part_bin[k] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < NB; u++) {
+    const int k = k0 + u * T;
     xn[u] = k < nnp ? d.part_net[k] : 0;
   }
   for (int kb = k0;;) {  // gradient partials
@@ -1713,13 +1719,13 @@ __device__ void reduce_all(const RpDev& d, Totals& t) {
       xb[u] = k < nbp ? d.part_bin[k] : 0.0;
     }
   }
-  for (int kb = k0;;) {  // net partials
+  for (int kb = k0;;) {  // net partials (integers: any order)
 #pragma unroll
-    for (int u = 0; u < RP_RED_BATCH; u++) vh += xn[u];
-    kb += RP_RED_BATCH * T;
+    for (int u = 0; u < NB; u++) vh += xn[u];
+    kb += NB * T;
     if (kb >= nnp) break;
 #pragma unroll
-    for (int u = 0; u < RP_RED_BATCH; u++) {
+    for (int u = 0; u < NB; u++) {
       const int k = kb + u * T;
       xn[u] = k < nnp ? d.part_net[k] : 0;
     }
@@ -1888,14 +1894,8 @@ __global__ void __launch_bounds__(RP_CTRL_BLOCK) k_control_init(RpDev d, int sta
   begin_iteration(s);
 }
 
-__device__ void control_main_body(const RpDev& d) {
-  if (d.st->phase != 0) return;
-  Totals t;
-  reduce_all(d, t);
-  TraceTotals tt = {0.0, 0.0};
-  if (d.evrec) reduce_trace(d, tt);
-  if (threadIdx.x != 0) return;
-  RpState* s = d.st;
+// The controller's scalar logic (thread 0) on the state s, after the totals.
+__device__ void control_logic(const RpDev& d, RpState* s, const Totals& t, const TraceTotals& tt) {
   s->evals++;
   s->wl_grad_sum = t.wl;
   s->dens_grad_sum = t.dens;
@@ -1986,6 +1986,25 @@ __device__ void control_main_body(const RpDev& d) {
     return;
   }
   begin_iteration(s);
+}
+
+// One workgroup: the state comes into LDS in one round (a 4-byte word per
+// thread, issued beside the first partial loads), thread 0 runs the logic on
+// that copy, and the state goes back in one round, so the scalar logic's
+// dependent reads and writes are LDS accesses, not memory round trips.
+__device__ void control_main_body(const RpDev& d) {
+  __shared__ RpState ls;
+  static_assert(sizeof(RpState) % 4 == 0 && sizeof(RpState) / 4 <= RP_CTRL_BLOCK, "state words");
+  constexpr int W = sizeof(RpState) / 4;
+  if (threadIdx.x < W) reinterpret_cast<int*>(&ls)[threadIdx.x] = reinterpret_cast<const int*>(d.st)[threadIdx.x];
+  Totals t;
+  reduce_all(d, t);  // its barriers also publish ls
+  if (ls.phase != 0) return;  // the same for every thread: the kernels of a stopped run are no-ops
+  TraceTotals tt = {0.0, 0.0};
+  if (d.evrec) reduce_trace(d, tt);
+  if (threadIdx.x == 0) control_logic(d, &ls, t, tt);
+  __syncthreads();
+  if (threadIdx.x < W) reinterpret_cast<int*>(d.st)[threadIdx.x] = reinterpret_cast<const int*>(&ls)[threadIdx.x];
 }
 
 
