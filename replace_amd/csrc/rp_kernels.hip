@@ -113,6 +113,15 @@ __device__ __forceinline__ float clamp_in(float c, int h, int lo, int hi) {
 // walks a contiguous range of logical blocks (neighbouring cells / nets in
 // the locality order share its L2).  A bijection on [0, nblocks).
 #define RP_NUM_XCD 8
+// The run state's phase through a vector load: a scalar load of it would be
+// waited for together with the kernel's later scalar loads (arguments), which
+// puts its memory round trip in front of the kernel's first data loads.
+__device__ __forceinline__ int run_phase(const RpState* st) {
+  // a relaxed atomic load is never made scalar, and is not waited for at once
+  // (the state was written by an earlier kernel: the launch boundary orders it)
+  return __hip_atomic_load(&st->phase, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 __device__ __forceinline__ int xcd_block(int b, int nblocks) {
   int per = nblocks / RP_NUM_XCD, rem = nblocks % RP_NUM_XCD;
   int x = b % RP_NUM_XCD, j = b / RP_NUM_XCD;
@@ -145,24 +154,57 @@ __device__ __forceinline__ float2 mul_dir_i(float2 z) {
 }
 
 // Stage twiddle tables of an N-point transform from the global half circle
-// tw[m] = e^{-2 pi i m / N}, m < N/2 (e^{-2 pi i m / N} = -tw[m - N/2] above).
+// tw[m] = e^{-2 pi i m / N}, m < N/2 (e^{-2 pi i m / N} = -tw[m - N/2] above):
+// for every radix-4 stage with ns > 1, w^r for r = 1..3 and k < ns (entry
+// (r - 1) ns + k of the stage's table, the stages' tables one after the
+// other).  In two steps: the global loads into registers (NT threads, entry
+// g = tid + r NT), then the LDS stores, so the loads go out beside the
+// kernel's data loads with no wait between them.
+constexpr int stage_tw_entries(int n) {
+  int e = 0;
+  for (int ns = (ilog2c(n) & 1) ? 2 : 4; ns < n; ns *= 4) e += 3 * ns;
+  return e;
+}
 template <int N>
-__device__ __forceinline__ void load_stage_twiddles(float2* TW, const float2* __restrict__ tw) {
+__device__ __forceinline__ int stage_tw_m(int g) {  // the circle index m of entry g
   constexpr int LOG = ilog2c(N);
-  int off = 0;
+  int off = 0, m = 0;
 #pragma unroll
   for (int ns = (LOG & 1) ? 2 : 4; ns < N; ns *= 4) {
-    const int t = N / (4 * ns);
-    for (int e = threadIdx.x; e < 3 * ns; e += blockDim.x) {
-      const int r = e / ns + 1, k = e - (r - 1) * ns;
-      const int m = r * k * t;
-      float2 w = m < N / 2 ? tw[m] : tw[m - N / 2];
-      if (m >= N / 2) w = make_float2(-w.x, -w.y);
-      TW[off + e] = w;
+    if (g >= off && g < off + 3 * ns) {
+      const int e = g - off, r = e / ns + 1, k = e - (r - 1) * ns;
+      m = r * k * (N / (4 * ns));
     }
     off += 3 * ns;
   }
+  return m;
 }
+template <int N, int NT>
+struct StageTw {
+  static constexpr int E = stage_tw_entries(N);
+  static constexpr int R = (E + NT - 1) / NT;
+  float2 v[R > 0 ? R : 1];
+  unsigned neg = 0;  // bit r: entry r is the negated circle value
+  // unconditional loads (an entry past the tables repeats the last one and
+  // is not stored); the sign is applied at the store
+  __device__ __forceinline__ void load(const float2* __restrict__ tw, int tid) {
+    if constexpr (E > 0) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int m = stage_tw_m<N>(min(tid + r * NT, E - 1));
+        v[r] = tw[m < N / 2 ? m : m - N / 2];
+        neg |= (m >= N / 2 ? 1u : 0u) << r;
+      }
+    }
+  }
+  __device__ __forceinline__ void store(float2* TW, int tid) const {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int g = tid + r * NT;
+      if (g < E) TW[g] = (neg >> r & 1) ? make_float2(-v[r].x, -v[r].y) : v[r];
+    }
+  }
+};
 
 // One radix-4 stage of stride ns (twiddles at TW + off), in -> out.
 template <int N, int NS, int DIR>
@@ -637,7 +679,8 @@ template <int N>
 __device__ void binfin_body(const RpDev& d, int blk, int from_rho) {
   extern __shared__ float2 smem[];
   __shared__ double sh_red[RP_BLOCK / 64];
-  if (d.st->phase != 0) return;
+  // the run state is checked once the loads below are in flight (a stopped
+  // run's kernels return before any store)
   float2* A = smem;
   float2* B = smem + fft_buf(N);
   float2* TW = smem + 2 * fft_buf(N);
@@ -647,15 +690,18 @@ __device__ void binfin_body(const RpDev& d, int blk, int from_rho) {
   // the DCT-II post-twiddles of the thread's outputs k = threadIdx.x + r
   // blockDim.x, loaded now rather than after the FFT
   float2 tw[MAXC];
+  StageTw<N, RP_BLOCK> stw;
   if (!d.mfma) {
 #pragma unroll
     for (int r = 0; r < MAXC; r++) {
       const int k = threadIdx.x + r * blockDim.x;
       if (k < N) tw[r] = d.tw2x[k];
     }
-    load_stage_twiddles<N>(TW, d.twx);
+    stw.load(d.twx, threadIdx.x);
   }
   if (from_rho) {  // Poisson-only entry: density given
+    if (run_phase(d.st) != 0) return;
+    if (!d.mfma) stw.store(TW, threadIdx.x);
     for (int i = threadIdx.x; i < N; i += blockDim.x)
       A[lpad(mk_pos(i, N))] = make_float2(d.rho[(size_t)ja * N + i], d.rho[(size_t)jb * N + i]);
   } else {
@@ -675,6 +721,8 @@ __device__ void binfin_body(const RpDev& d, int blk, int from_rho) {
           v_np[r][t] = d.bin_npf[b];
         }
     }
+    if (run_phase(d.st) != 0) return;
+    if (!d.mfma) stw.store(TW, threadIdx.x);
 #pragma unroll
     for (int t = 0; t < MAXC; t++) {
       const int i = threadIdx.x + t * blockDim.x;
@@ -727,7 +775,8 @@ __device__ void binfin_body(const RpDev& d, int blk, int from_rho) {
 template <int N>
 __device__ void col_body(const RpDev& d, int blk) {
   extern __shared__ float2 smem[];
-  if (d.st->phase != 0) return;
+  // the run state is checked once the loads are in flight (a stopped run's
+  // kernels return before any store)
   float2* A = smem;
   float2* B = smem + fft_buf(N);
   float2* TW = smem + 2 * fft_buf(N);
@@ -756,7 +805,10 @@ __device__ void col_body(const RpDev& d, int blk) {
     }
   }
   const float wx2u[2] = {d.wx2[ua], d.wx2[ub]};
-  load_stage_twiddles<N>(TW, d.twy);
+  StageTw<N, RP_BLOCK> stw;
+  stw.load(d.twy, threadIdx.x);
+  if (run_phase(d.st) != 0) return;
+  stw.store(TW, threadIdx.x);
 #pragma unroll
   for (int r = 0; r < MAXC; r++) {
     const int v = threadIdx.x + r * blockDim.x;
@@ -831,7 +883,8 @@ __device__ void col_body(const RpDev& d, int blk) {
 template <int N>
 __device__ void rowinv_body(const RpDev& d, int blk, int want_phi) {
   extern __shared__ float2 smem[];
-  if (d.st->phase != 0) return;
+  // the run state is checked once the loads are in flight (a stopped run's
+  // kernels return before any store)
   float2* A = smem;
   float2* B = smem + fft_buf(N);
   float2* TW = smem + 2 * fft_buf(N);
@@ -857,7 +910,10 @@ __device__ void rowinv_body(const RpDev& d, int blk, int want_phi) {
       tw[r] = d.tw2x[u];
     }
   }
-  load_stage_twiddles<N>(TW, d.twx);
+  StageTw<N, RP_BLOCK> stw;
+  stw.load(d.twx, threadIdx.x);
+  if (run_phase(d.st) != 0) return;
+  stw.store(TW, threadIdx.x);
   // Ex: DST-III of (wx[u] * Y1[j][u]); Ey: DCT-III of Y2 rows
 #pragma unroll
   for (int r = 0; r < MAXC; r++) {
@@ -954,7 +1010,8 @@ inline size_t rp_lds_rinv2(int n) { return rp_lds_col4(n) - sizeof(float) * 4 * 
 template <int N>
 __device__ void col4_body(const RpDev& d, int blk) {
   extern __shared__ float2 smem[];
-  if (d.st->phase != 0) return;
+  // the run state is checked once the loads are in flight (a stopped run's
+  // kernels return before any store)
   const int h = threadIdx.x / RP_BLOCK, lt = threadIdx.x % RP_BLOCK;
   float2* A = smem + 2 * h * fft_buf(N);
   float2* B = A + fft_buf(N);
@@ -986,7 +1043,10 @@ __device__ void col4_body(const RpDev& d, int blk) {
     }
   }
   const float wx2u[2] = {d.wx2[ua], d.wx2[ub]};
-  load_stage_twiddles<N>(TW, d.twy);
+  StageTw<N, RP_WIDE> stw;
+  stw.load(d.twy, threadIdx.x);
+  if (run_phase(d.st) != 0) return;
+  stw.store(TW, threadIdx.x);
   float2* A0 = smem;
   float2* A1 = smem + 2 * fft_buf(N);
 #pragma unroll
@@ -1071,7 +1131,8 @@ __device__ void col4_body(const RpDev& d, int blk) {
 template <int N>
 __device__ void rowinv2_body(const RpDev& d, int blk, int want_phi) {
   extern __shared__ float2 smem[];
-  if (d.st->phase != 0) return;
+  // the run state is checked once the loads are in flight (a stopped run's
+  // kernels return before any store)
   const int h = threadIdx.x / RP_BLOCK, lt = threadIdx.x % RP_BLOCK;
   float2* A = smem + 2 * h * fft_buf(N);
   float2* B = A + fft_buf(N);
@@ -1101,7 +1162,10 @@ __device__ void rowinv2_body(const RpDev& d, int blk, int want_phi) {
     const int k = lt + r * RP_BLOCK;
     if (k < N) tw[r] = d.tw2x[k];
   }
-  load_stage_twiddles<N>(TW, d.twx);
+  StageTw<N, RP_WIDE> stw;
+  stw.load(d.twx, threadIdx.x);
+  if (run_phase(d.st) != 0) return;
+  stw.store(TW, threadIdx.x);
 #pragma unroll
   for (int r = 0; r < MAXW; r++) {
     const int u = threadIdx.x + r * RP_WIDE;
