@@ -475,14 +475,51 @@ __device__ double psum(const BcAxis& a, int slot, int nparts, double* sh) {
   return bsum(v, sh);
 }
 
-// y_i = sum_k val_k x_col_k over BC_G lanes (strided entries, FP64), then a
-// fixed xor tree inside the lane group; every lane of the group gets y_i.
-__device__ __forceinline__ float spmv_row(const BcAxis& a, int i, const float* __restrict__ x) {
+// y_i = sum_k val_k x_col_k over the BC_G lanes of a group (lane l takes
+// entries l, l + BC_G, ... in order, FP64), then a fixed xor tree inside the
+// group; every lane of the group gets y_i.  U rows at once (rows ri[u], valid
+// when below n): every row's bounds, then its first two entries per lane,
+// then their x gathers, so a wave has U rows' loads in flight; a longer row
+// continues its lanes' sums in order.
+template <int U>
+__device__ __forceinline__ void spmv_rows(const BcAxis& a, const int (&ri)[U], int n, const float* __restrict__ x,
+                                          float (&y)[U]) {
   const int lane = threadIdx.x & (BC_G - 1);
-  double s = 0.0;
-  for (int k = a.rowp[i] + lane; k < a.rowp[i + 1]; k += BC_G) s += (double)a.val[k] * (double)x[a.col[k]];
-  for (int o = BC_G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  return (float)s;
+  int b0[U], b1[U], c[U][2];
+  float v[U][2], xv[U][2];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    b0[u] = b1[u] = 0;
+    if (ri[u] < n) {
+      b0[u] = a.rowp[ri[u]];
+      b1[u] = a.rowp[ri[u] + 1];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++)
+#pragma unroll
+    for (int m = 0; m < 2; m++) {
+      const int k = b0[u] + lane + m * BC_G;
+      if (k < b1[u]) {
+        c[u][m] = a.col[k];
+        v[u][m] = a.val[k];
+      }
+    }
+#pragma unroll
+  for (int u = 0; u < U; u++)
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+      if (b0[u] + lane + m * BC_G < b1[u]) xv[u][m] = x[c[u][m]];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    double sum = 0.0;
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+      if (b0[u] + lane + m * BC_G < b1[u]) sum += (double)v[u][m] * (double)xv[u][m];
+    for (int k = b0[u] + lane + 2 * BC_G; k < b1[u]; k += BC_G) sum += (double)a.val[k] * (double)x[a.col[k]];
+    for (int o = BC_G / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+    y[u] = (float)sum;
+  }
 }
 
 // Kernel bodies take (axis, block), so that one body serves a launch per
@@ -493,19 +530,31 @@ __device__ __forceinline__ float spmv_row(const BcAxis& a, int i, const float* _
 // launched with those alone; the r.r and r0.r partials of the blocks past them
 // are the 0.0 that init writes.
 #define BC_ROWS(i) for (int i = blk * RP_BLOCK + (int)threadIdx.x; i < g.n; i += g.nparts * RP_BLOCK)
-// one row per BC_G-lane group; all lanes of a group run the same trip count
-#define BC_GROUP_ROWS(i)                                                                  \
-  for (int i = (blk * RP_BLOCK + (int)threadIdx.x) / BC_G; i < g.n;                        \
-       i += g.nparts * RP_BLOCK / BC_G)
 #define BC_LEAD ((threadIdx.x & (BC_G - 1)) == 0)
+// One row per BC_G-lane group (rows (blk RP_BLOCK + tid) / BC_G + m S, S the
+// grid's group count; all lanes of a group run the same trip count), four
+// rows at a time, with their products from spmv_rows: f(i, y_i) per row in
+// row order
+template <typename F>
+__device__ __forceinline__ void bc_group_rows4(const BcArgs& g, const BcAxis& a, int blk, const float* __restrict__ x,
+                                               F&& f) {
+  const int S = g.nparts * RP_BLOCK / BC_G;
+  for (int i0 = (blk * RP_BLOCK + (int)threadIdx.x) / BC_G; i0 < g.n; i0 += 4 * S) {
+    const int ri[4] = {i0, i0 + S, i0 + 2 * S, i0 + 3 * S};
+    float y[4];
+    spmv_rows<4>(a, ri, g.n, x, y);
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (ri[u] < g.n) f(ri[u], y[u]);
+  }
+}
 
 // r = b - A x ; r0 = r ; v = p = 0 ; partials r.r and b.b
 __device__ __forceinline__ void bc_init_body(const BcArgs& g, int axis, int blk) {
   __shared__ double sh[RP_BLOCK / 64];
   const BcAxis& a = g.ax[axis];
   double d0 = 0.0, d1 = 0.0;
-  BC_GROUP_ROWS(i) {
-    float t = spmv_row(a, i, a.x);
+  bc_group_rows4(g, a, blk, a.x, [&](int i, float t) {
     if (BC_LEAD) {
       float r = (float)((double)a.b[i] + -1.0 * (double)t);
       a.r[i] = r;
@@ -515,7 +564,7 @@ __device__ __forceinline__ void bc_init_body(const BcArgs& g, int axis, int blk)
       d0 += (double)r * (double)r;
       d1 += (double)a.b[i] * (double)a.b[i];
     }
-  }
+  });
   d0 = bsum(d0, sh);
   d1 = bsum(d1, sh);
   if (threadIdx.x == 0) {
@@ -577,15 +626,14 @@ __device__ __forceinline__ void bc_restart_body(const BcArgs& g, int axis, int b
   const BcState& S = *a.st;
   if (S.done || !S.restart) return;
   double d0 = 0.0;
-  BC_GROUP_ROWS(i) {
-    float t = spmv_row(a, i, a.x);
+  bc_group_rows4(g, a, blk, a.x, [&](int i, float t) {
     if (BC_LEAD) {
       float r = (float)((double)a.b[i] + -1.0 * (double)t);
       a.r[i] = r;
       a.r0[i] = r;
       d0 += (double)r * (double)r;
     }
-  }
+  });
   d0 = bsum(d0, sh);
   if (threadIdx.x == 0) a.parts[BC_RST * g.nparts + blk] = d0;
 }
@@ -615,13 +663,12 @@ __device__ __forceinline__ void bc_v_body(const BcArgs& g, int axis, int blk) {
   const BcAxis& a = g.ax[axis];
   if (a.st->done) return;
   double d = 0.0;
-  BC_GROUP_ROWS(i) {
-    float v = spmv_row(a, i, a.p);
+  bc_group_rows4(g, a, blk, a.p, [&](int i, float v) {
     if (BC_LEAD) {
       a.v[i] = v;
       d += (double)a.r0[i] * (double)v;
     }
-  }
+  });
   d = bsum(d, sh);
   if (threadIdx.x == 0) a.parts[BC_R0V * g.nparts + blk] = d;
 }
@@ -644,14 +691,13 @@ __device__ __forceinline__ void bc_t_body(const BcArgs& g, int axis, int blk) {
   const BcAxis& a = g.ax[axis];
   if (a.st->done) return;
   double tt = 0.0, ts = 0.0;
-  BC_GROUP_ROWS(i) {
-    float t = spmv_row(a, i, a.s);
+  bc_group_rows4(g, a, blk, a.s, [&](int i, float t) {
     if (BC_LEAD) {
       a.t[i] = t;
       tt += (double)t * (double)t;
       ts += (double)t * (double)a.s[i];
     }
-  }
+  });
   tt = bsum(tt, sh);
   ts = bsum(ts, sh);
   if (threadIdx.x == 0) {
