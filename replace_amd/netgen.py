@@ -61,7 +61,10 @@ def _filler_area_ok(core_area, nonplace_area, std_area, td):
 
 
 def generate(num_cells: int, num_nets: int, seed: int, util: float = 0.70, td: float = 0.70,
-             num_macros: int = 0, macro_total: float = 0.40, name: str = "synthetic") -> Netlist:
+             num_macros: int = 0, macro_total: float = 0.40, name: str = "synthetic",
+             slots_out: list | None = None) -> Netlist:
+    """The netlist; with `slots_out` (a list) the virtual slot grid side and
+    every movable cell's slot are appended to it (for rent_profile)."""
     rng = np.random.default_rng(seed)
     widths = rng.integers(1, 9, size=num_cells).astype(np.int64) * SITE_W
     heights = np.full(num_cells, SITE_H, np.int64)
@@ -118,6 +121,8 @@ def generate(num_cells: int, num_nets: int, seed: int, util: float = 0.70, td: f
     slot_of = rng.permutation(g * g)[:num_cells]
     cell_at = np.full(g * g, -1, np.int64)
     cell_at[slot_of] = np.arange(num_cells)
+    if slots_out is not None:
+        slots_out.extend([g, slot_of])
     net_start = np.zeros(num_nets + 1, np.int64)
     np.cumsum(deg, out=net_start[1:])
     npin = int(net_start[-1])
@@ -174,6 +179,37 @@ def generate(num_cells: int, num_nets: int, seed: int, util: float = 0.70, td: f
         pin_box=None,
         name=name,
     )
+
+
+def rent_profile(nl: Netlist, g: int, slot_of: np.ndarray, sides=(2, 4, 8, 16, 32, 64)):
+    """Rent's rule on the generator's virtual slot grid: for square blocks of
+    side L slots (G cells each), T = the nets with pins both inside and
+    outside the block, averaged over the blocks; the exponent p is the slope
+    of log T against log G (least squares over the sides).  Returns
+    (p, [(L, mean G, mean T), ...])."""
+    ns = np.asarray(nl.net_pin_start, np.int64)
+    pin_cell = np.asarray(nl.pin_inst, np.int64)
+    net_of_pin = np.repeat(np.arange(len(ns) - 1), np.diff(ns))
+    n_mov = len(slot_of)
+    keep = pin_cell < n_mov
+    pc, pn = pin_cell[keep], net_of_pin[keep]
+    sx, sy = slot_of[pc] % g, slot_of[pc] // g
+    pts = []
+    for L in sides:
+        nb = (g + L - 1) // L
+        blk = (sy // L) * nb + (sx // L)
+        # nets touching a block (each (net, block) pair once), and the blocks each net touches
+        pair = np.unique(pn * (nb * nb) + blk)
+        net_blocks = np.bincount(pair // (nb * nb), minlength=len(ns) - 1)
+        cut = net_blocks[pair // (nb * nb)] > 1   # the net leaves this block
+        t_per_block = np.bincount(pair % (nb * nb), weights=cut, minlength=nb * nb)
+        cells = np.bincount((slot_of // g // L) * nb + (slot_of % g) // L, minlength=nb * nb)
+        full = cells > 0
+        pts.append((L, float(cells[full].mean()), float(t_per_block[full].mean())))
+    x = np.log([q[1] for q in pts])
+    y = np.log([q[2] for q in pts])
+    p = float(np.polyfit(x, y, 1)[0])
+    return p, pts
 
 
 def next_pow2(x: int) -> int:

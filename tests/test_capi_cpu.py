@@ -124,6 +124,25 @@ def test_generator_workload_shapes():
     assert p.bin_grid_cnt_x == 128 and p.target_density == np.float32(0.7)
 
 
+def test_generator_rent_exponent():
+    """The 20k-cell workload's connectivity follows Rent's rule with p near
+    0.6 (the configuration's clustered, Rent-like netlist): on the
+    generator's virtual slot grid, blocks of 4 to ~2200 cells have on average
+    T = t G^p external nets with p in [0.5, 0.7], and T grows monotonically
+    with G."""
+    out = []
+    nl = netgen.generate(20000, 21000, seed=1, name="c0", slots_out=out)
+    ref, _ = netgen.workload("c0")
+    assert np.array_equal(nl.pin_inst, ref.pin_inst)  # the workload itself
+    p, pts = netgen.rent_profile(nl, out[0], out[1])
+    assert 0.5 <= p <= 0.7, (p, pts)
+    ts = [t for _, _, t in pts]
+    assert all(a < b for a, b in zip(ts, ts[1:]))
+    # a block of one slot side 2 holds 4 cells with ~4 pins each, nearly all
+    # on nets that leave it
+    assert 10.0 < pts[0][2] < 17.0
+
+
 def test_generator_macros_do_not_overlap():
     nl = netgen.generate(3000, 3150, seed=2, num_macros=40)
     m = nl.inst_box[nl.inst_fixed == 1]
