@@ -81,6 +81,27 @@ def algorithmic_bytes(nc, npin, nn, nb, sort_bits=16):
     }
 
 
+def copy_bandwidth_gbs(mib=1024, reps=10):
+    """Achievable HBM bandwidth on this box: a device-to-device copy of `mib`
+    MiB (read + write counted), best of `reps`, timed with events."""
+    import torch
+    a = torch.empty(mib << 20, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    best = None
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    del a, b
+    torch.cuda.empty_cache()
+    return round(2 * (mib << 20) / (best * 1e-3) / 1e9, 1)
+
+
 def cpu_baseline(workload, gpu_evals, sample_evals, core):
     code = f"""
 import json, os, sys, time
@@ -99,7 +120,7 @@ t1 = time.perf_counter()
 o.nesterov_iterate({sample_evals})
 t2 = time.perf_counter()
 c, st = o.nesterov_iterate(0)
-print(json.dumps(dict(init_s=t1 - t0, iter_s=t2 - t1, evals=st.evals, poisson=o.poisson)))
+print(json.dumps(dict(init_s=t1 - t0, iter_s=t2 - t1, evals=st.evals, poisson=o.poisson, stages=o.stage_times())))
 """
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=900)
     if out.returncode != 0:
@@ -116,11 +137,15 @@ print(json.dumps(dict(init_s=t1 - t0, iter_s=t2 - t1, evals=st.evals, poisson=o.
                 break
     except OSError:
         pass
+    # where the restatement's time goes, per evaluation of the sample (its own
+    # per-stage timers: the two init evaluations and the loop's)
+    split = {k: round(v / max(1, r["evals"]), 5) for k, v in r["stages"].items()}
     return {
         "value": round(est, 3),
         "unit": "s",
         "cores": 1,
         "kind": "port",
+        "split_s_per_eval": split,
         "sample": (f"oracle {workload}: init (2 evals) {r['init_s']:.2f}s + {loop_evals} loop evals "
                    f"{r['iter_s']:.2f}s = {per_eval:.3f}s/eval on 1 core ({cpu}); value = init + "
                    f"s/eval x {gpu_evals - 2} GPU loop evals; Poisson: {r['poisson']}"),
@@ -435,6 +460,7 @@ def main():
         kernels[k] = {"avg_us": round(avg * 1e3, 2), "launches": n,
                       "GBs": round(abytes[k] / (avg * 1e-3) / 1e9, 1) if avg > 0 and abytes[k] else None}
     dom = max((k for k in kt if k != "sort_refresh"), key=lambda k: kt[k][0])
+    copy_gbs = copy_bandwidth_gbs()
     avg_ms = kt[dom][0] / max(1, kt[dom][1])
     achieved = abytes[dom] / (avg_ms * 1e-3) / 1e9
     traffic, traffic_note = None, "no counter profile for this workload"
@@ -478,10 +504,12 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "traffic_source": traffic_note,
-                     "algorithmic_bytes_per_launch": abytes[dom], "avg_launch_us": round(avg_ms * 1e3, 2)},
+                     "algorithmic_bytes_per_launch": abytes[dom], "avg_launch_us": round(avg_ms * 1e3, 2),
+                     "copy_gbs": copy_gbs},
         "run": {"code": code, "finished": fin, "iterations": iters, "evals": evals,
                 "final_overflow": round(ovf, 5), "final_hpwl": hpwl, "device_s": round(dev_s, 4),
                 "initial_place_s": round(ip_s, 3), "initial_place_iters": ip.iterations,
+                "with_initial_place_s": round(ms_per_step / 1e3 + ip_s, 4),
                 "nesterov_setup_s": round(setup_s, 3), "netgen_s": round(gen_s, 2)},
         "kernels": kernels,
     }

@@ -60,6 +60,7 @@ def lib() -> ctypes.CDLL:
         L.ro_nesterov_init.argtypes = [vp]
         L.ro_nesterov_iterate.argtypes = [vp, ctypes.c_int, ctypes.POINTER(RpNesterovStats)]
         L.ro_get_state.argtypes = [vp, ctypes.POINTER(RpNesterovState)]
+        L.ro_get_stage_times.argtypes = [vp, vp]
         L.ro_set_state.argtypes = [vp, ctypes.POINTER(RpNesterovState)]
         L.ro_get_iter_records.argtypes = [vp, ctypes.POINTER(RpIterRecord), ctypes.c_int32,
                                           ctypes.POINTER(ctypes.c_int32)]
@@ -221,6 +222,15 @@ class Oracle:
         st = RpNesterovStats()
         code = lib().ro_nesterov_iterate(self.h, n, ctypes.byref(st))
         return code, st
+
+    STAGES = ("step", "scatter", "poisson", "wa", "grad", "step_length")
+
+    def stage_times(self) -> dict:
+        """Seconds per Nesterov-loop stage so far (the restatement's own
+        instrumentation; see ro_get_stage_times)."""
+        out = np.zeros(len(self.STAGES), np.float64)
+        lib().ro_get_stage_times(self.h, _p(out))
+        return dict(zip(self.STAGES, out.tolist()))
 
     def get_state(self) -> dict:
         st, bufs = nesterov_state_buffers(self.info.num_gcells)

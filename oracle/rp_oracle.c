@@ -17,6 +17,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #define RO_SQRT2 1.414213562373095048801L /* src/nesterovBase.cpp:13 */
 #define RO_PI 3.141592653589793238462L    /* src/fft.cpp:9 */
@@ -79,6 +80,8 @@ float ro_fast_exp(float a) {
 /* ------------------------------------------------------------------ */
 struct ro_ctx {
   rp_params p;
+  /* wall time per stage of the Nesterov loop (instrumentation only) */
+  double t_stage[RO_STAGES];
   /* die / core, src/placerBase.cpp:614-626 */
   int core_lx, core_ly, core_ux, core_uy;
   int site_w, site_h;
@@ -1604,11 +1607,26 @@ static float overflow_of(ro_ctx* c) {
   return (float)c->overflow_area / ((float)c->std_area + (float)c->macro_area * c->td);
 }
 
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
 static void full_eval(ro_ctx* c, const float* xy, float* sumG, float* wlG, float* dG) {
+  double t0 = now_s();
   update_density_center(c, xy);
+  double t1 = now_s();
   density_force_bin(c);
+  double t2 = now_s();
   wa_force(c, c->wlCoefX, c->wlCoefY);
+  double t3 = now_s();
   update_gradients(c, sumG, wlG, dG, NULL);
+  double t4 = now_s();
+  c->t_stage[RO_T_SCATTER] += t1 - t0;
+  c->t_stage[RO_T_POISSON] += t2 - t1;
+  c->t_stage[RO_T_WA] += t3 - t2;
+  c->t_stage[RO_T_GRAD] += t4 - t3;
   c->evals++;
 }
 
@@ -1714,6 +1732,7 @@ int ro_nesterov_iterate(ro_ctx* c, int n_iters, rp_nesterov_stats* st) {
     float coeff = (float)(((double)prevA - 1.0) / (double)c->curA);
     int bt;
     for (bt = 0; bt < c->p.max_back_track; bt++) {
+      double ts0 = now_s();
       for (int k = 0; k < ng; k++) {
         float nx = c->curSLP[2 * k] + c->stepLength * c->curSLPSum[2 * k];
         float ny = c->curSLP[2 * k + 1] + c->stepLength * c->curSLPSum[2 * k + 1];
@@ -1724,9 +1743,12 @@ int ro_nesterov_iterate(ro_ctx* c, int n_iters, rp_nesterov_stats* st) {
         c->nextSLP[2 * k] = clamp_x(c, k, sx);
         c->nextSLP[2 * k + 1] = clamp_y(c, k, sy);
       }
+      c->t_stage[RO_T_STEP] += now_s() - ts0;
       full_eval(c, c->nextSLP, c->nextSLPSum, c->nextSLPWL, c->nextSLPD);
       if (c->isDiverged) break;
+      double tc0 = now_s();
       float ns = step_length(c, c->curSLP, c->curSLPSum, c->nextSLP, c->nextSLPSum);
+      c->t_stage[RO_T_STEPLEN] += now_s() - tc0;
       if ((double)ns > (double)c->stepLength * 0.95) {
         c->stepLength = ns;
         break;
@@ -1891,6 +1913,11 @@ int ro_set_positions(ro_ctx* c, const int32_t* lx, const int32_t* ly) {
     c->dbly[i] = ly[i];
     inst_update_pins(c, i);
   }
+  return 0;
+}
+
+int ro_get_stage_times(ro_ctx* c, double* out) {
+  for (int k = 0; k < RO_STAGES; k++) out[k] = c->t_stage[k];
   return 0;
 }
 

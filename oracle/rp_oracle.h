@@ -53,6 +53,12 @@ ro_ctx* ro_create(const rp_netlist* nl, const rp_params* p, int* code);
 void ro_destroy(ro_ctx* c);
 void ro_set_poisson(ro_ctx* c, ro_poisson_fn fn, void* user);
 int ro_set_verbose_trace(ro_ctx* c, int on);
+/* Seconds spent per stage of the Nesterov loop since the context was made
+ * (instrumentation for the CPU-baseline split; it changes no result):
+ * step + clamp, centres + density scatter + bin density, Poisson solve,
+ * WA sums, cell gradients + preconditioner, step length. */
+enum { RO_T_STEP = 0, RO_T_SCATTER, RO_T_POISSON, RO_T_WA, RO_T_GRAD, RO_T_STEPLEN, RO_STAGES };
+int ro_get_stage_times(ro_ctx* c, double* out);
 
 int ro_initial_place(ro_ctx* c, float* err, int64_t* hpwl, int cap, int* n_iter);
 
