@@ -577,14 +577,13 @@ __device__ void step_body(const RpDev& d, int blk) {
 #define RP_TILE_SLOTS 256
 struct SortScatterLds {
   WinTable t;
-  int key[RP_TILE_SLOTS], cnt[RP_TILE_SLOTS], base[RP_TILE_SLOTS];
+  int key[RP_TILE_SLOTS], cnt[RP_TILE_SLOTS];  // cnt becomes the tile's base rank after the flush
 };
 template <int PER>
 __device__ void sorted_scatter_body(const RpDev& d, int blk, const RpSortBufs& sb, int count, SortScatterLds& L) {
   WinTable& t = L.t;
   int* t_key = L.key;
   int* t_cnt = L.cnt;
-  int* t_base = L.base;
   const RpState* st = d.st;
   if (st->phase != 0) return;
   int cell[PER], slot[PER], lrank[PER];
@@ -640,11 +639,11 @@ __device__ void sorted_scatter_body(const RpDev& d, int blk, const RpSortBufs& s
   win_flush(d, t);
   if (count) {
     for (int k = threadIdx.x; k < RP_TILE_SLOTS; k += blockDim.x)
-      if (t_key[k] >= 0) t_base[k] = atomicAdd(&sb.cnt[t_key[k]], t_cnt[k]);
+      if (t_key[k] >= 0) t_cnt[k] = atomicAdd(&sb.cnt[t_key[k]], t_cnt[k]);  // count -> base rank
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < PER; r++)
-      if (cell[r] >= 0 && slot[r] >= 0) sb.rank[cell[r]] = t_base[slot[r]] + lrank[r];
+      if (cell[r] >= 0 && slot[r] >= 0) sb.rank[cell[r]] = t_cnt[slot[r]] + lrank[r];
   }
 }
 
@@ -1330,8 +1329,10 @@ __device__ __forceinline__ float2 wa_pin_grad(int2 p, float4 v, const float4& sa
 template <int CAP>
 struct WaLds {
   float4 ex[CAP];
-  float4 sum[2 * RP_WA_NETS];
-  int4 box[RP_WA_NETS];
+  union {  // the net boxes are dead once the exponentials are written, the sums come after
+    float4 sum[2 * RP_WA_NETS];
+    int4 box[RP_WA_NETS];
+  };
   int2 pos[CAP];
   long long red[RP_BLOCK / 64];
   uint8_t net[CAP];  // net index within the workgroup (< RP_WA_NETS = 256)
@@ -2134,7 +2135,8 @@ union ScatterWaLds {
   WaLds<RP_WA_CAP> w;
 };
 template <int PER>
-__global__ void __launch_bounds__(RP_BLOCK) k_scatter_wa(RpDev d, RpSortBufs sb, int count, int ns8) {
+__global__ void __launch_bounds__(RP_BLOCK) __attribute__((amdgpu_waves_per_eu(7)))
+k_scatter_wa(RpDev d, RpSortBufs sb, int count, int ns8) {
   __shared__ ScatterWaLds L;
   const int x = blockIdx.x % RP_NUM_XCD, j = blockIdx.x / RP_NUM_XCD;
   const int nwa = max(1, d.nwa), perw = nwa / RP_NUM_XCD, remw = nwa % RP_NUM_XCD;
