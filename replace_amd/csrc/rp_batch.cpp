@@ -113,6 +113,7 @@ constexpr int kTuneJobs = 16;   // jobs per bucket the tuner times
 constexpr int kTuneMinJobs = 4;  // running jobs a bucket needs before it is tuned
 constexpr int kTuneReps = 5;     // timed launches per shape (the fastest counts: other
                                  // streams' set-up kernels can share the GPU meanwhile)
+constexpr int kIpDrivers = 2;     // batched initial place: driver threads, each with its own rounds
 constexpr int kIpAux = 4;        // batched initial place: streams (and host threads) for the
                                  // per-run assembly and records (the hardware queues per process)
 
@@ -125,6 +126,20 @@ int size_bucket(int ncells) {
 }
 
 }  // namespace
+
+// One driver of the batched initial place: its queue of runs, host thread,
+// solve stream, aux streams and the tables of its shared solves.
+struct IpDriver {
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Job*> queue;
+  std::thread th;
+  hipStream_t stream = nullptr;
+  hipStream_t aux[kIpAux] = {};
+  hipEvent_t asm_done[kIpAux] = {};
+  hipEvent_t solved = nullptr;
+  RpIpBatch* ipb = nullptr;
+};
 
 struct rp_server {
   rp_server_opts opt{};
@@ -153,18 +168,13 @@ struct rp_server {
   std::deque<int> setup_queue;
   bool stop = false;
   std::vector<std::thread> workers;
-  // batched initial place: set-up workers hand their runs to one driver; a
-  // run's assembly and record work go on ip_aux[job % kIpAux] (one host
-  // thread per stream issues them), the shared solves on ip_stream
-  std::mutex ip_mu;
-  std::condition_variable ip_cv;
-  std::deque<Job*> ip_queue;
-  std::thread ip_thread;
-  hipStream_t ip_stream = nullptr;
-  hipStream_t ip_aux[kIpAux] = {};
-  hipEvent_t ip_asm_done[kIpAux] = {};
-  hipEvent_t ip_solved = nullptr;
-  RpIpBatch* ipb = nullptr;
+  // batched initial place: set-up workers hand their runs to the drivers
+  // (job % kIpDrivers); a driver's run has its assembly and record work on
+  // aux[job / kIpDrivers % kIpAux] (one host thread per stream issues them)
+  // and the driver's shared solves on its stream.  Two drivers keep the GPU
+  // busy with one's assemblies while the other solves.
+  IpDriver ipd[kIpDrivers];
+  bool batched_ip = false;
   rp_server_stats stats{};
 };
 
@@ -298,14 +308,15 @@ static void setup_worker(rp_server* s) {
     } else {
       r = rp_create(&j.nl, &j.prm, &c);
       // batched_ip 2: only the systems of the grid-wide solver go to the rounds
-      if (r == 0 && s->ip_stream && (s->opt.batched_ip != 2 || !rp_ip_one_workgroup((int)c->place.size()))) {
+      if (r == 0 && s->batched_ip && (s->opt.batched_ip != 2 || !rp_ip_one_workgroup((int)c->place.size()))) {
+        IpDriver& d = s->ipd[j.id % kIpDrivers];
         RpIpRun* run = nullptr;
-        r = rp_initial_place_begin(c, s->ip_aux[j.id % kIpAux], &run);
+        r = rp_initial_place_begin(c, d.aux[j.id / kIpDrivers % kIpAux], &run);
         if (r == 0 && run) {  // the driver iterates it and queues the job here again
           j.ip_run = run;
-          std::lock_guard<std::mutex> lk(s->ip_mu);
-          s->ip_queue.push_back(&j);
-          s->ip_cv.notify_one();
+          std::lock_guard<std::mutex> lk(d.mu);
+          d.queue.push_back(&j);
+          d.cv.notify_one();
           continue;
         }
         if (r == 0) r = rp_initial_place_end(c, nullptr);
@@ -331,22 +342,23 @@ static void setup_worker(rp_server* s) {
 // records after one wait), so a round costs about what one job's iteration
 // costs while all jobs advance.  A finished job, its positions written to the
 // host model, goes back to the set-up queue for the rest of its set-up.
-static void ip_worker(rp_server* s) {
+static void ip_worker(rp_server* s, IpDriver* dp) {
+  IpDriver& d = *dp;
   (void)hipSetDevice(s->device);
   std::vector<Job*> active;
   const bool timing = getenv("RP_IP_TIMING") != nullptr;
   auto ms = [](clk::time_point a) { return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
   for (int round = 0;; round++) {
     {
-      std::unique_lock<std::mutex> lk(s->ip_mu);
-      if (active.empty()) s->ip_cv.wait(lk, [&] { return s->stop || !s->ip_queue.empty(); });
-      if (active.empty() && s->ip_queue.empty()) return;  // stopping
-      while (!s->ip_queue.empty()) {
-        active.push_back(s->ip_queue.front());
-        s->ip_queue.pop_front();
+      std::unique_lock<std::mutex> lk(d.mu);
+      if (active.empty()) d.cv.wait(lk, [&] { return s->stop || !d.queue.empty(); });
+      if (active.empty() && d.queue.empty()) return;  // stopping
+      while (!d.queue.empty()) {
+        active.push_back(d.queue.front());
+        d.queue.pop_front();
       }
     }
-    hipStream_t st = s->ip_stream;
+    hipStream_t st = d.stream;
     const auto t0 = clk::now();
     // fn(job) for every active job: one host thread per aux stream issues the
     // launches of that stream's runs
@@ -356,13 +368,13 @@ static void ip_worker(rp_server* s) {
         th[h] = std::thread([&, h] {
           (void)hipSetDevice(s->device);
           for (Job* j : active)
-            if (j->id % kIpAux == h) fn(j);
+            if (j->id / kIpDrivers % kIpAux == h) fn(j);
         });
       for (auto& t : th) t.join();
     };
     auto sync_aux = [&] {
       bool ok = true;
-      for (int h = 0; h < kIpAux; h++) ok = hipStreamSynchronize(s->ip_aux[h]) == hipSuccess && ok;
+      for (int h = 0; h < kIpAux; h++) ok = hipStreamSynchronize(d.aux[h]) == hipSuccess && ok;
       return ok;
     };
     per_stream([](Job* j) {
@@ -385,14 +397,14 @@ static void ip_worker(rp_server* s) {
     // the shared solves wait for every run's assembly; the records for the solves
     bool evok = true;
     for (int h = 0; h < kIpAux; h++) {
-      evok = hipEventRecord(s->ip_asm_done[h], s->ip_aux[h]) == hipSuccess && evok;
-      evok = hipStreamWaitEvent(st, s->ip_asm_done[h], 0) == hipSuccess && evok;
+      evok = hipEventRecord(d.asm_done[h], d.aux[h]) == hipSuccess && evok;
+      evok = hipStreamWaitEvent(st, d.asm_done[h], 0) == hipSuccess && evok;
     }
     const double t_b = ms(t0);
-    if (int r = evok ? rp_ipr_solve(live.data(), (int)live.size(), st, s->ipb) : RP_ERR_HIP)
+    if (int r = evok ? rp_ipr_solve(live.data(), (int)live.size(), st, d.ipb) : RP_ERR_HIP)
       for (Job* j : lj) j->ip_code = r;
-    evok = hipEventRecord(s->ip_solved, st) == hipSuccess;
-    for (int h = 0; h < kIpAux; h++) evok = hipStreamWaitEvent(s->ip_aux[h], s->ip_solved, 0) == hipSuccess && evok;
+    evok = hipEventRecord(d.solved, st) == hipSuccess;
+    for (int h = 0; h < kIpAux; h++) evok = hipStreamWaitEvent(d.aux[h], d.solved, 0) == hipSuccess && evok;
     const double t_s = ms(t0);
     per_stream([&](Job* j) {
       if (!evok && !j->ip_code) j->ip_code = RP_ERR_HIP;
@@ -449,17 +461,23 @@ extern "C" int rp_server_create(const rp_server_opts* opt, rp_server** out) {
   tiles_default(s.get());
   rp_server* raw = s.get();
   if (s->opt.batched_ip) {
-    bool ok = hipStreamCreateWithFlags(&s->ip_stream, hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreateWithFlags(&s->ip_solved, hipEventDisableTiming) == hipSuccess;
-    for (int h = 0; h < kIpAux && ok; h++)
-      ok = hipStreamCreateWithFlags(&s->ip_aux[h], hipStreamNonBlocking) == hipSuccess &&
-           hipEventCreateWithFlags(&s->ip_asm_done[h], hipEventDisableTiming) == hipSuccess;
+    bool ok = true;
+    for (IpDriver& d : s->ipd) {
+      ok = ok && hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&d.solved, hipEventDisableTiming) == hipSuccess;
+      for (int h = 0; h < kIpAux && ok; h++)
+        ok = hipStreamCreateWithFlags(&d.aux[h], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&d.asm_done[h], hipEventDisableTiming) == hipSuccess;
+    }
     if (!ok) {
       *out = s.release();
       return sfail(*out, RP_ERR_HIP, "hipStreamCreate / hipEventCreate failed");
     }
-    s->ipb = rp_ipb_create();
-    s->ip_thread = std::thread(ip_worker, raw);
+    s->batched_ip = true;
+    for (IpDriver& d : s->ipd) {
+      d.ipb = rp_ipb_create();
+      d.th = std::thread(ip_worker, raw, &d);
+    }
   }
   int nt = std::max(1, s->opt.setup_threads);
   for (int k = 0; k < nt; k++) s->workers.emplace_back(setup_worker, raw);
@@ -812,20 +830,22 @@ extern "C" void rp_server_destroy(rp_server* s) {
   }
   s->cv.notify_all();
   for (auto& t : s->workers) t.join();
-  if (s->ip_thread.joinable()) {
-    {
-      std::lock_guard<std::mutex> lk(s->ip_mu);  // the workers are gone: no run is waiting
+  for (IpDriver& d : s->ipd) {
+    if (d.th.joinable()) {
+      {
+        std::lock_guard<std::mutex> lk(d.mu);  // the workers are gone: no run is waiting
+      }
+      d.cv.notify_all();
+      d.th.join();
     }
-    s->ip_cv.notify_all();
-    s->ip_thread.join();
+    if (d.ipb) rp_ipb_free(d.ipb);
+    if (d.stream) (void)hipStreamDestroy(d.stream);
+    for (int h = 0; h < kIpAux; h++) {
+      if (d.aux[h]) (void)hipStreamDestroy(d.aux[h]);
+      if (d.asm_done[h]) (void)hipEventDestroy(d.asm_done[h]);
+    }
+    if (d.solved) (void)hipEventDestroy(d.solved);
   }
-  if (s->ipb) rp_ipb_free(s->ipb);
-  if (s->ip_stream) (void)hipStreamDestroy(s->ip_stream);
-  for (int h = 0; h < kIpAux; h++) {
-    if (s->ip_aux[h]) (void)hipStreamDestroy(s->ip_aux[h]);
-    if (s->ip_asm_done[h]) (void)hipEventDestroy(s->ip_asm_done[h]);
-  }
-  if (s->ip_solved) (void)hipEventDestroy(s->ip_solved);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (auto& j : s->jobs)
     if (j->ctx) rp_destroy(j->ctx);

@@ -426,8 +426,9 @@ int rp_ip_assemble_b(rp_ctx* c, RpIpAsm& a, hipStream_t st) {
 namespace {
 
 constexpr int BC_MAX_PARTS = 1024;
-constexpr int BC_G = 8;  // lanes per CSR row in the SpMV kernels
 constexpr int BC_SLOTS = 8;  // partial-sum slots per axis
+constexpr int GS_WIN = 1024;  // rows of a sorting window of the sliced-ELL copy (one workgroup)
+constexpr int GS_SMAX = GS_WIN + 8;  // slices of a window at most (64 / g rows each, 7 classes of g)
 enum { BC_RR0 = 0, BC_BB, BC_R0V, BC_TT, BC_TS, BC_RR, BC_R0R, BC_RST };
 
 struct BcState {
@@ -436,10 +437,31 @@ struct BcState {
   float err;
 };
 
+// One axis of a grid-wide solve.  The solve runs on the row-sorted system
+// P A P^T (P x) = P b (rows sorted by length, longest first, within windows of
+// GS_WIN consecutive rows, so the locality order survives at window scale):
+// b, x and the work vectors are indexed by sorted position q; bo and xo are
+// the right-hand side and solution in row order.  One workgroup of GS_WIN
+// threads serves one window.  The matrix is a sliced ELL (after SELL-C-sigma,
+// Kreutzer et al., SIAM J. Sci. Comput. 36(5), 2014) whose slices give a row
+// g lanes by its length (g = 1 up to 16 entries, 2 up to 32, ... 64 above
+// 512), so a slice holds 64 / g rows and no lane walks more than 16 slots
+// (rows above 1024 entries excepted): lane k of a row sums entries k, k + g,
+// ... in order in FP64, then the g lanes add by a fixed xor tree.  Slot m of
+// lane l of a slice sits at soff + (m / 4) 256 + 4 l + m % 4, so a wave reads
+// 64 x 16 contiguous bytes of values and of column positions per four slots.
 struct BcAxis {
-  const int* rowp;
+  const int* rowp;  // CSR in row order (the assembly's output)
   const int* col;
   const float* val;
+  const float* bo;
+  float* xo;
+  const int* perm;   // sorted position -> row
+  const int* slen;   // sorted position -> row length
+  const int4* sinfo;  // per window, GS_SMAX slices: (first row, g | rows << 8, slots per lane, first slot)
+  const int* nsl;     // per window: slices
+  const float* sval;
+  const int* scol;   // sorted column positions (pads: 0, never summed)
   const float* b;
   float *x, *r, *r0, *v, *p, *s, *t;
   double* parts;  // BC_SLOTS x nparts
@@ -475,95 +497,111 @@ __device__ double psum(const BcAxis& a, int slot, int nparts, double* sh) {
   return bsum(v, sh);
 }
 
-// y_i = sum_k val_k x_col_k over the BC_G lanes of a group (lane l takes
-// entries l, l + BC_G, ... in order, FP64), then a fixed xor tree inside the
-// group; every lane of the group gets y_i.  U rows at once (rows ri[u], valid
-// when below n): every row's bounds, then its first two entries per lane,
-// then their x gathers, so a wave has U rows' loads in flight; a longer row
-// continues its lanes' sums in order.
-template <int U>
-__device__ __forceinline__ void spmv_rows(const BcAxis& a, const int (&ri)[U], int n, const float* __restrict__ x,
-                                          float (&y)[U]) {
-  const int lane = threadIdx.x & (BC_G - 1);
-  int b0[U], b1[U], c[U][2];
-  float v[U][2], xv[U][2];
+// y_q = sum_k val_k x_col_k for the rows of one slice (wave-uniform k of
+// window win): lane l takes entries l % g, l % g + g, ... of row q0 + l / g,
+// eight slots (two quads) loaded and gathered before they are summed, slots
+// past the lane's count dropped by a select; then the row's g lanes add by a
+// fixed xor tree.  Returns true in the lane that holds y_q (l % g == 0).
+__device__ __forceinline__ bool gsell_slice(const BcAxis& a, int win, int k, const float* __restrict__ x, int& q,
+                                            float& y) {
+  const int4 si = a.sinfo[(size_t)win * GS_SMAX + k];
+  const int q0 = __builtin_amdgcn_readfirstlane(si.x);
+  const int gl = __builtin_amdgcn_readfirstlane(si.y);
+  const int w = __builtin_amdgcn_readfirstlane(si.z);
+  const unsigned soff = (unsigned)__builtin_amdgcn_readfirstlane(si.w);
+  const int g = gl & 0xFF, rows = gl >> 8;
+  const int lane = threadIdx.x & 63;
+  const int rl = lane / g, kk = lane & (g - 1);
+  const bool valid = rl < rows;
+  q = q0 + rl;
+  const int L = valid ? a.slen[q] : 0;
+  const int cnt = L > kk ? (L - kk + g - 1) / g : 0;
+  const float* __restrict__ pv = a.sval + soff + (unsigned)lane * 4u;
+  const int* __restrict__ pc = a.scol + soff + (unsigned)lane * 4u;
+  double acc = 0.0;
+  if (valid) {
+    int m = 0;
+    for (; m + 8 <= w; m += 8) {
+      const float4 v0 = *reinterpret_cast<const float4*>(pv + (unsigned)m * 64u);
+      const float4 v1 = *reinterpret_cast<const float4*>(pv + (unsigned)(m + 4) * 64u);
+      const int4 c0 = *reinterpret_cast<const int4*>(pc + (unsigned)m * 64u);
+      const int4 c1 = *reinterpret_cast<const int4*>(pc + (unsigned)(m + 4) * 64u);
+      const float vv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+      const float xx[8] = {x[c0.x], x[c0.y], x[c0.z], x[c0.w], x[c1.x], x[c1.y], x[c1.z], x[c1.w]};
 #pragma unroll
-  for (int u = 0; u < U; u++) {
-    b0[u] = b1[u] = 0;
-    if (ri[u] < n) {
-      b0[u] = a.rowp[ri[u]];
-      b1[u] = a.rowp[ri[u] + 1];
-    }
-  }
-#pragma unroll
-  for (int u = 0; u < U; u++)
-#pragma unroll
-    for (int m = 0; m < 2; m++) {
-      const int k = b0[u] + lane + m * BC_G;
-      if (k < b1[u]) {
-        c[u][m] = a.col[k];
-        v[u][m] = a.val[k];
+      for (int u = 0; u < 8; u++) {
+        const double t = acc + (double)vv[u] * (double)xx[u];
+        acc = m + u < cnt ? t : acc;
       }
     }
+    if (m < w) {  // a last quad
+      const float4 v0 = *reinterpret_cast<const float4*>(pv + (unsigned)m * 64u);
+      const int4 c0 = *reinterpret_cast<const int4*>(pc + (unsigned)m * 64u);
+      const float vv[4] = {v0.x, v0.y, v0.z, v0.w};
+      const float xx[4] = {x[c0.x], x[c0.y], x[c0.z], x[c0.w]};
 #pragma unroll
-  for (int u = 0; u < U; u++)
-#pragma unroll
-    for (int m = 0; m < 2; m++)
-      if (b0[u] + lane + m * BC_G < b1[u]) xv[u][m] = x[c[u][m]];
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    double sum = 0.0;
-#pragma unroll
-    for (int m = 0; m < 2; m++)
-      if (b0[u] + lane + m * BC_G < b1[u]) sum += (double)v[u][m] * (double)xv[u][m];
-    for (int k = b0[u] + lane + 2 * BC_G; k < b1[u]; k += BC_G) sum += (double)a.val[k] * (double)x[a.col[k]];
-    for (int o = BC_G / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
-    y[u] = (float)sum;
+      for (int u = 0; u < 4; u++) {
+        const double t = acc + (double)vv[u] * (double)xx[u];
+        acc = m + u < cnt ? t : acc;
+      }
+    }
   }
+  for (int o = g >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  y = (float)acc;
+  return valid && kk == 0;
 }
 
 // Kernel bodies take (axis, block), so that one body serves a launch per
-// context and a launch over many contexts (a job table); g.nparts blocks of
-// RP_BLOCK threads cover an axis either way.  The row stride is
-// g.nparts * RP_BLOCK, so the one-thread-per-row kernels (p, s, xr, zero_x)
-// have rows only in their first min(nparts, ceil(n / RP_BLOCK)) blocks and are
-// launched with those alone; the r.r and r0.r partials of the blocks past them
-// are the 0.0 that init writes.
-#define BC_ROWS(i) for (int i = blk * RP_BLOCK + (int)threadIdx.x; i < g.n; i += g.nparts * RP_BLOCK)
-#define BC_LEAD ((threadIdx.x & (BC_G - 1)) == 0)
-// One row per BC_G-lane group (rows (blk RP_BLOCK + tid) / BC_G + m S, S the
-// grid's group count; all lanes of a group run the same trip count), four
-// rows at a time, with their products from spmv_rows: f(i, y_i) per row in
-// row order
+// context and a launch over many contexts (a job table); g.nparts blocks
+// cover an axis either way, block b the row windows b, b + nparts, ..., so
+// every dot product has one partial per block.
+// f(q, y_q) for every sorted row of the block's windows, the windows' slices
+// dealt to its waves in turn
 template <typename F>
-__device__ __forceinline__ void bc_group_rows4(const BcArgs& g, const BcAxis& a, int blk, const float* __restrict__ x,
-                                               F&& f) {
-  const int S = g.nparts * RP_BLOCK / BC_G;
-  for (int i0 = (blk * RP_BLOCK + (int)threadIdx.x) / BC_G; i0 < g.n; i0 += 4 * S) {
-    const int ri[4] = {i0, i0 + S, i0 + 2 * S, i0 + 3 * S};
-    float y[4];
-    spmv_rows<4>(a, ri, g.n, x, y);
+__device__ __forceinline__ void gsell_rows(const BcArgs& g, const BcAxis& a, int blk, const float* __restrict__ x,
+                                           F&& f) {
+  const int nwin = (g.n + GS_WIN - 1) / GS_WIN, wave = threadIdx.x >> 6;
+  for (int win = blk; win < nwin; win += g.nparts) {
+    const int ns = a.nsl[win];
+    for (int k = wave; k < ns; k += GS_WIN / 64) {
+      int q;
+      float y;
+      if (gsell_slice(a, win, k, x, q, y)) f(q, y);
+    }
+  }
+}
+
+// The row-wise kernels (p, s, xr, unperm) run RP_BLOCK threads a window,
+// four rows a thread: ld(q, u) for the four rows first, then st(q, u), so a
+// thread has its four rows' loads in flight at once.
+static_assert(GS_WIN == 4 * RP_BLOCK, "four rows a thread");
+template <typename Ld, typename St>
+__device__ __forceinline__ void bc_rows4(const BcArgs& g, int blk, Ld&& ld, St&& st) {
+  const int nwin = (g.n + GS_WIN - 1) / GS_WIN;
+  for (int win = blk; win < nwin; win += g.nparts) {
+    const int q0 = win * GS_WIN + (int)threadIdx.x;
 #pragma unroll
     for (int u = 0; u < 4; u++)
-      if (ri[u] < g.n) f(ri[u], y[u]);
+      if (q0 + u * RP_BLOCK < g.n) ld(q0 + u * RP_BLOCK, u);
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (q0 + u * RP_BLOCK < g.n) st(q0 + u * RP_BLOCK, u);
   }
 }
 
 // r = b - A x ; r0 = r ; v = p = 0 ; partials r.r and b.b
 __device__ __forceinline__ void bc_init_body(const BcArgs& g, int axis, int blk) {
-  __shared__ double sh[RP_BLOCK / 64];
+  __shared__ double sh[GS_WIN / 64];
   const BcAxis& a = g.ax[axis];
   double d0 = 0.0, d1 = 0.0;
-  bc_group_rows4(g, a, blk, a.x, [&](int i, float t) {
-    if (BC_LEAD) {
-      float r = (float)((double)a.b[i] + -1.0 * (double)t);
-      a.r[i] = r;
-      a.r0[i] = r;
-      a.v[i] = 0.0f;
-      a.p[i] = 0.0f;
-      d0 += (double)r * (double)r;
-      d1 += (double)a.b[i] * (double)a.b[i];
-    }
+  gsell_rows(g, a, blk, a.x, [&](int i, float t) {
+    float r = (float)((double)a.b[i] + -1.0 * (double)t);
+    a.r[i] = r;
+    a.r0[i] = r;
+    a.v[i] = 0.0f;
+    a.p[i] = 0.0f;
+    d0 += (double)r * (double)r;
+    d1 += (double)a.b[i] * (double)a.b[i];
   });
   d0 = bsum(d0, sh);
   d1 = bsum(d1, sh);
@@ -621,18 +659,16 @@ __device__ __forceinline__ void bc_head_body(const BcArgs& g, int axis, int blk)
 
 // restart: r = b - A x ; r0 = r ; partial r.r
 __device__ __forceinline__ void bc_restart_body(const BcArgs& g, int axis, int blk) {
-  __shared__ double sh[RP_BLOCK / 64];
+  __shared__ double sh[GS_WIN / 64];
   const BcAxis& a = g.ax[axis];
   const BcState& S = *a.st;
   if (S.done || !S.restart) return;
   double d0 = 0.0;
-  bc_group_rows4(g, a, blk, a.x, [&](int i, float t) {
-    if (BC_LEAD) {
-      float r = (float)((double)a.b[i] + -1.0 * (double)t);
-      a.r[i] = r;
-      a.r0[i] = r;
-      d0 += (double)r * (double)r;
-    }
+  gsell_rows(g, a, blk, a.x, [&](int i, float t) {
+    float r = (float)((double)a.b[i] + -1.0 * (double)t);
+    a.r[i] = r;
+    a.r0[i] = r;
+    d0 += (double)r * (double)r;
   });
   d0 = bsum(d0, sh);
   if (threadIdx.x == 0) a.parts[BC_RST * g.nparts + blk] = d0;
@@ -640,7 +676,7 @@ __device__ __forceinline__ void bc_restart_body(const BcArgs& g, int axis, int b
 
 // p = r + beta (p - w v)   (after a restart, rho = r0_sq = r.r)
 __device__ __forceinline__ void bc_p_body(const BcArgs& g, int axis, int blk) {
-  __shared__ double sh[RP_BLOCK / 64];
+  __shared__ double sh[GS_WIN / 64];
   const BcAxis& a = g.ax[axis];
   BcState& S = *a.st;
   if (S.done) return;
@@ -654,20 +690,26 @@ __device__ __forceinline__ void bc_p_body(const BcArgs& g, int axis, int blk) {
     }
   }
   const double beta = (rho / S.rho_old) * (S.alpha / S.w), w = S.w;
-  BC_ROWS(i) a.p[i] = (float)((double)a.r[i] + beta * ((double)a.p[i] - w * (double)a.v[i]));
+  float rv[4], pv[4], vv[4];
+  bc_rows4(
+      g, blk,
+      [&](int i, int u) {
+        rv[u] = a.r[i];
+        pv[u] = a.p[i];
+        vv[u] = a.v[i];
+      },
+      [&](int i, int u) { a.p[i] = (float)((double)rv[u] + beta * ((double)pv[u] - w * (double)vv[u])); });
 }
 
 // v = A p ; partial r0.v
 __device__ __forceinline__ void bc_v_body(const BcArgs& g, int axis, int blk) {
-  __shared__ double sh[RP_BLOCK / 64];
+  __shared__ double sh[GS_WIN / 64];
   const BcAxis& a = g.ax[axis];
   if (a.st->done) return;
   double d = 0.0;
-  bc_group_rows4(g, a, blk, a.p, [&](int i, float v) {
-    if (BC_LEAD) {
-      a.v[i] = v;
-      d += (double)a.r0[i] * (double)v;
-    }
+  gsell_rows(g, a, blk, a.p, [&](int i, float v) {
+    a.v[i] = v;
+    d += (double)a.r0[i] * (double)v;
   });
   d = bsum(d, sh);
   if (threadIdx.x == 0) a.parts[BC_R0V * g.nparts + blk] = d;
@@ -675,28 +717,33 @@ __device__ __forceinline__ void bc_v_body(const BcArgs& g, int axis, int blk) {
 
 // alpha = rho / (r0.v) ; s = r - alpha v
 __device__ __forceinline__ void bc_s_body(const BcArgs& g, int axis, int blk) {
-  __shared__ double sh[RP_BLOCK / 64];
+  __shared__ double sh[GS_WIN / 64];
   const BcAxis& a = g.ax[axis];
   BcState& S = *a.st;
   if (S.done) return;
   const double alpha = S.rho / psum(a, BC_R0V, g.nparts, sh);
   __syncthreads();
   if (blk == 0 && threadIdx.x == 0) S.alpha = alpha;
-  BC_ROWS(i) a.s[i] = (float)((double)a.r[i] + -alpha * (double)a.v[i]);
+  float rv[4], vv[4];
+  bc_rows4(
+      g, blk,
+      [&](int i, int u) {
+        rv[u] = a.r[i];
+        vv[u] = a.v[i];
+      },
+      [&](int i, int u) { a.s[i] = (float)((double)rv[u] + -alpha * (double)vv[u]); });
 }
 
 // t = A s ; partials t.t and t.s
 __device__ __forceinline__ void bc_t_body(const BcArgs& g, int axis, int blk) {
-  __shared__ double sh[RP_BLOCK / 64];
+  __shared__ double sh[GS_WIN / 64];
   const BcAxis& a = g.ax[axis];
   if (a.st->done) return;
   double tt = 0.0, ts = 0.0;
-  bc_group_rows4(g, a, blk, a.s, [&](int i, float t) {
-    if (BC_LEAD) {
-      a.t[i] = t;
-      tt += (double)t * (double)t;
-      ts += (double)t * (double)a.s[i];
-    }
+  gsell_rows(g, a, blk, a.s, [&](int i, float t) {
+    a.t[i] = t;
+    tt += (double)t * (double)t;
+    ts += (double)t * (double)a.s[i];
   });
   tt = bsum(tt, sh);
   ts = bsum(ts, sh);
@@ -708,7 +755,7 @@ __device__ __forceinline__ void bc_t_body(const BcArgs& g, int axis, int blk) {
 
 // w = (t.s)/(t.t) ; x += alpha p + w s ; r = s - w t ; partials r.r and r0.r
 __device__ __forceinline__ void bc_xr_body(const BcArgs& g, int axis, int blk) {
-  __shared__ double sh[RP_BLOCK / 64];
+  __shared__ double sh[GS_WIN / 64];
   const BcAxis& a = g.ax[axis];
   BcState& S = *a.st;
   if (S.done) return;
@@ -721,13 +768,23 @@ __device__ __forceinline__ void bc_xr_body(const BcArgs& g, int axis, int blk) {
     S.pending = 1;
   }
   double d0 = 0.0, d1 = 0.0;
-  BC_ROWS(i) {
-    a.x[i] = (float)((double)a.x[i] + alpha * (double)a.p[i] + w * (double)a.s[i]);
-    float r = (float)((double)a.s[i] - w * (double)a.t[i]);
-    a.r[i] = r;
-    d0 += (double)r * (double)r;
-    d1 += (double)a.r0[i] * (double)r;
-  }
+  float xv[4], pv[4], sv[4], tv[4], r0v[4];
+  bc_rows4(
+      g, blk,
+      [&](int i, int u) {
+        xv[u] = a.x[i];
+        pv[u] = a.p[i];
+        sv[u] = a.s[i];
+        tv[u] = a.t[i];
+        r0v[u] = a.r0[i];
+      },
+      [&](int i, int u) {
+        a.x[i] = (float)((double)xv[u] + alpha * (double)pv[u] + w * (double)sv[u]);
+        float r = (float)((double)sv[u] - w * (double)tv[u]);
+        a.r[i] = r;
+        d0 += (double)r * (double)r;
+        d1 += (double)r0v[u] * (double)r;
+      });
   d0 = bsum(d0, sh);
   d1 = bsum(d1, sh);
   if (threadIdx.x == 0) {
@@ -736,11 +793,19 @@ __device__ __forceinline__ void bc_xr_body(const BcArgs& g, int axis, int blk) {
   }
 }
 
-// a zero right-hand side gives x = 0
-__device__ __forceinline__ void bc_zero_x_body(const BcArgs& g, int axis, int blk) {
+// the solution back in row order (a zero right-hand side gives x = 0)
+__device__ __forceinline__ void bc_unperm_body(const BcArgs& g, int axis, int blk) {
   const BcAxis& a = g.ax[axis];
-  if (!a.st->zero) return;
-  BC_ROWS(i) a.x[i] = 0.0f;
+  const bool zero = a.st->zero != 0;
+  float xv[4];
+  int pm[4];
+  bc_rows4(
+      g, blk,
+      [&](int i, int u) {
+        xv[u] = a.x[i];
+        pm[u] = a.perm[i];
+      },
+      [&](int i, int u) { a.xo[pm[u]] = zero ? 0.0f : xv[u]; });
 }
 
 // one launch per context (blockIdx.y = axis) and one over many contexts
@@ -753,15 +818,184 @@ __device__ __forceinline__ void bc_zero_x_body(const BcArgs& g, int axis, int bl
     const int2 e = tab[blockIdx.x];                                                      \
     bc_##NAME##_body(gs[e.x], blockIdx.y, e.y);                                          \
   }
-BC_KERNELS(init, RP_BLOCK)
+BC_KERNELS(init, GS_WIN)
 BC_KERNELS(head, 1024)
-BC_KERNELS(restart, RP_BLOCK)
+BC_KERNELS(restart, GS_WIN)
 BC_KERNELS(p, RP_BLOCK)
-BC_KERNELS(v, RP_BLOCK)
+BC_KERNELS(v, GS_WIN)
 BC_KERNELS(s, RP_BLOCK)
-BC_KERNELS(t, RP_BLOCK)
+BC_KERNELS(t, GS_WIN)
 BC_KERNELS(xr, RP_BLOCK)
-BC_KERNELS(zero_x, RP_BLOCK)
+BC_KERNELS(unperm, RP_BLOCK)
+
+// lanes per row of a row of L entries: no lane walks more than 16 slots
+__device__ __forceinline__ int gsell_lanes(int L) {
+  int g = 1;
+  while (g < 64 && L > 16 * g) g <<= 1;
+  return g;
+}
+
+// Sliced-ELL copy of one axis' CSR for the grid-wide solve, part 1, one
+// workgroup per window of GS_WIN rows: the window's rows sorted by length,
+// longest first, ties in row order (a bitonic sort of length-major keys that
+// hold the row's place in the window, so the order is unique; lengths above
+// 65534 count as equal), the sorted position -> row map and its inverse, the
+// sorted lengths, and the window's slices: consecutive sorted rows of one
+// lane class g, 64 / g of them (fewer at a class's end), each as wide as its
+// longest row needs (slots per lane rounded up to a multiple of 4).  Slice
+// slot offsets are relative to the window here; part 2 adds the window's base.
+__global__ void __launch_bounds__(GS_WIN) k_gsell_order(const int* __restrict__ rowp, int n, int* __restrict__ perm,
+                                                       int* __restrict__ iperm, int* __restrict__ slen,
+                                                       int* __restrict__ qslice, int4* __restrict__ sinfo,
+                                                       int* __restrict__ nsl, int* __restrict__ wtot) {
+  __shared__ unsigned key[GS_WIN];
+  __shared__ int len_s[GS_WIN];
+  const int tid = threadIdx.x, w0 = blockIdx.x * GS_WIN;
+  const int nw = min(GS_WIN, n - w0);
+  const int i = w0 + tid;
+  const int len = i < n ? rowp[i + 1] - rowp[i] : 0;
+  key[tid] = ((i < n ? 0xFFFFu - (unsigned)min(len, 0xFFFF) : 0x10000u) << 10) | (unsigned)tid;
+  __syncthreads();
+  for (int k = 2; k <= GS_WIN; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int o = tid ^ j;
+      if (o > tid) {
+        const unsigned x = key[tid], y = key[o];
+        if (((tid & k) == 0) == (x > y)) {
+          key[tid] = y;
+          key[o] = x;
+        }
+      }
+      __syncthreads();
+    }
+  const int q = w0 + tid;
+  const int src = w0 + (int)(key[tid] & (GS_WIN - 1));
+  int l = 0;
+  if (tid < nw) {  // rows past n sort last
+    l = rowp[src + 1] - rowp[src];
+    perm[q] = src;
+    iperm[src] = q;
+    slen[q] = l;
+  }
+  // slices: within each lane class (contiguous, larger g first, as the rows
+  // are sorted by length) runs of 64 / g rows from the class's first row
+  __shared__ int cstart[8], sbase[8], scan[GS_WIN];
+  if (tid < 8) cstart[tid] = nw;
+  __syncthreads();
+  const int cg = tid < nw ? 31 - __clz(gsell_lanes(l)) : 0;  // log2 g
+  len_s[tid] = cg;
+  __syncthreads();
+  if (tid < nw && (tid == 0 || len_s[tid - 1] != cg)) cstart[cg] = tid;
+  __syncthreads();
+  if (tid == 0) {  // first slice of each class, classes in position order (g = 64 first)
+    int k = 0;
+    for (int c = 6; c >= 0; c--) {
+      sbase[c] = k;
+      if (cstart[c] < nw) {
+        int e = nw;  // the class ends where the next present class starts
+        for (int d = c - 1; d >= 0; d--)
+          if (cstart[d] < nw) {
+            e = cstart[d];
+            break;
+          }
+        const int h = 64 >> c;
+        k += (e - cstart[c] + h - 1) / h;
+      }
+    }
+    sbase[7] = k;  // slices of the window
+  }
+  __syncthreads();
+  int kslice = 0, slots = 0, rows = 0, w = 0;
+  bool head = false;
+  if (tid < nw) {
+    const int g = 1 << cg, h = 64 >> cg, loc = tid - cstart[cg];
+    int e = nw;
+    for (int d = cg - 1; d >= 0; d--)
+      if (cstart[d] < nw) {
+        e = cstart[d];
+        break;
+      }
+    kslice = sbase[cg] + loc / h;
+    head = loc % h == 0;
+    if (head) {  // the slice's first row is its longest
+      rows = min(h, e - tid);
+      w = ((l + g - 1) / g + 3) & ~3;
+      slots = 64 * w;
+    }
+  }
+  // exclusive scan of the slices' slots in position order (Hillis-Steele)
+  scan[tid] = slots;
+  __syncthreads();
+  for (int o = 1; o < GS_WIN; o <<= 1) {
+    const int v = tid >= o ? scan[tid - o] : 0;
+    __syncthreads();
+    scan[tid] += v;
+    __syncthreads();
+  }
+  if (head)
+    sinfo[(size_t)blockIdx.x * GS_SMAX + kslice] = make_int4(q, (1 << cg) | (rows << 8), w, scan[tid] - slots);
+  if (tid < nw) qslice[q] = kslice;
+  if (tid == 0) {
+    nsl[blockIdx.x] = sbase[7];
+    wtot[blockIdx.x] = scan[GS_WIN - 1];
+  }
+}
+
+// Part 2, one workgroup per window: the window's slot base (the windows
+// before it) added to its slices' offsets, the entries with sorted column
+// positions (a lane's slots past its count: value 0, column 0), and b and x in
+// sorted order.
+__global__ void __launch_bounds__(GS_WIN) k_gsell_fill(const int* __restrict__ rowp, const int* __restrict__ col,
+                                                      const float* __restrict__ val, const float* __restrict__ b,
+                                                      const float* __restrict__ x, const int* __restrict__ perm,
+                                                      const int* __restrict__ iperm, const int* __restrict__ slen,
+                                                      const int* __restrict__ qslice, const int* __restrict__ nsl,
+                                                      const int* __restrict__ wtot, int n, int4* __restrict__ sinfo,
+                                                      float* __restrict__ sval, int* __restrict__ scol,
+                                                      float* __restrict__ bs, float* __restrict__ xs) {
+  __shared__ int base;
+  const int tid = threadIdx.x, w0 = blockIdx.x * GS_WIN;
+  const int nw = min(GS_WIN, n - w0);
+  if (tid == 0) base = 0;
+  __syncthreads();
+  int part = 0;
+  for (int k = tid; k < (int)blockIdx.x; k += GS_WIN) part += wtot[k];
+  if (part) atomicAdd(&base, part);  // integer sum: any order gives the same total
+  __syncthreads();
+  int4* si = sinfo + (size_t)blockIdx.x * GS_SMAX;
+  const int ns = nsl[blockIdx.x];
+  int4 mine = make_int4(0, 0, 0, 0);
+  if (tid < nw) mine = si[qslice[w0 + tid]];
+  __syncthreads();  // every thread has read its slice before they are rebased
+  for (int k = tid; k < ns; k += GS_WIN) si[k].w += base;
+  if (tid >= nw) return;
+  const int q = w0 + tid;
+  const int g = mine.y & 0xFF, w = mine.z;
+  const int rl = q - mine.x;
+  const int soff = base + mine.w;
+  const int len = slen[q];
+  const int rb = rowp[perm[q]];
+  for (int kk = 0; kk < g; kk++) {
+    const int lane = rl * g + kk;
+    for (int m = 0; m < w; m += 4) {  // one quad: loads, then gathers, then 16-byte stores
+      float v4[4];
+      int c4[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int j = (m + u) * g + kk;  // entry of the row in this slot
+        v4[u] = j < len ? val[rb + j] : 0.0f;
+        c4[u] = j < len ? col[rb + j] : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) c4[u] = c4[u] >= 0 ? iperm[c4[u]] : 0;
+      const int e = soff + (m >> 2) * 256 + lane * 4;
+      *reinterpret_cast<float4*>(sval + e) = make_float4(v4[0], v4[1], v4[2], v4[3]);
+      *reinterpret_cast<int4*>(scol + e) = make_int4(c4[0], c4[1], c4[2], c4[3]);
+    }
+  }
+  bs[q] = b[perm[q]];
+  xs[q] = x[perm[q]];
+}
 
 // done flags of ng grid runs (both axes) into one array, read back in one copy
 __global__ void __launch_bounds__(RP_BLOCK) k_bc_done_gather(const BcArgs* gs, int ng, int* out) {
@@ -1212,6 +1446,8 @@ struct SellBufs {
   DevBuf<float> sval;
   DevBuf<uint16_t> scol;
   DevBuf<char> temp;
+  DevBuf<int> scol32, wtot, nsl, qslice;  // grid-wide solve: 32-bit columns, slots and slices per window, slice of a row
+  DevBuf<int4> sinfo;                     // grid-wide solve: slices of each window
 };
 
 // ---------------------------------------------------------------------------
@@ -1339,7 +1575,7 @@ struct RpIpRun {
   DevBuf<int> d_dlx, d_dly, d_hx, d_hy, d_mw, d_mh, d_offx, d_offy;
   DevBuf<int4> d_sbox;
   DevBuf<long long> d_hparts;
-  DevBuf<float> vec[2][7];
+  DevBuf<float> vec[2][9];  // x (row order), r, r0, v, p, s, t, x and b in sorted order
   DevBuf<double> parts[2];
   DevBuf<IpRec> d_rec;
   IpRec* h_rec = nullptr;
@@ -1446,8 +1682,8 @@ int rp_ipr_begin(rp_ctx* c, hipStream_t run_stream, bool host_asm, RpIpRun** out
   HIPCHK(c, hipMemcpyAsync(A.pin_cx.p, c->pin_cx.data(), sizeof(int) * np, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(A.pin_cy.p, c->pin_cy.data(), sizeof(int) * np, hipMemcpyHostToDevice, st));
   // solver buffers (both axes)
-  R->nparts = (int)std::max<long>(1, std::min<long>(BC_MAX_PARTS, ((long)n * BC_G + RP_BLOCK - 1) / RP_BLOCK));
-  R->eblocks = std::max(1, std::min(R->nparts, (n + RP_BLOCK - 1) / RP_BLOCK));  // blocks with rows at stride nparts
+  R->nparts = std::max(1, std::min(BC_MAX_PARTS, (n + GS_WIN - 1) / GS_WIN));  // one block per row window
+  R->eblocks = R->nparts;
   for (int ax = 0; ax < 2; ax++) {
     for (auto& v : R->vec[ax]) HIPCHK(c, dalloc(v, n));
     HIPCHK(c, dalloc(R->parts[ax], (size_t)BC_SLOTS * R->nparts));
@@ -1462,7 +1698,8 @@ int rp_ipr_begin(rp_ctx* c, hipStream_t run_stream, bool host_asm, RpIpRun** out
   g.nparts = R->nparts;
   for (int ax = 0; ax < 2; ax++) {
     BcAxis& a = g.ax[ax];
-    a.x = R->vec[ax][0].p;
+    a.xo = R->vec[ax][0].p;
+    a.x = R->vec[ax][7].p;
     a.r = R->vec[ax][1].p;
     a.r0 = R->vec[ax][2].p;
     a.v = R->vec[ax][3].p;
@@ -1499,6 +1736,19 @@ int rp_ipr_begin(rp_ctx* c, hipStream_t run_stream, bool host_asm, RpIpRun** out
       HIPCHK(c, dalloc(B.temp, std::max(t1, t2)));
     }
   }
+  if (!R->small_r) {
+    const int nwin = (n + GS_WIN - 1) / GS_WIN;
+    for (int ax = 0; ax < 2; ax++) {
+      SellBufs& B = R->sb[ax];
+      HIPCHK(c, dalloc(B.perm, n));
+      HIPCHK(c, dalloc(B.iperm, n));
+      HIPCHK(c, dalloc(B.slen, n));
+      HIPCHK(c, dalloc(B.qslice, n));
+      HIPCHK(c, dalloc(B.sinfo, (size_t)nwin * GS_SMAX));
+      HIPCHK(c, dalloc(B.nsl, nwin));
+      HIPCHK(c, dalloc(B.wtot, nwin));
+    }
+  }
   HIPCHK(c, hipStreamSynchronize(st));
   *out = R.release();
   return 0;
@@ -1525,7 +1775,8 @@ int rp_ipr_assemble_b(RpIpRun* R) {
     a.rowp = A.rowp[ax].p;
     a.col = A.col[ax].p;
     a.val = A.val[ax].p;
-    a.b = A.b[ax].p;
+    a.bo = A.b[ax].p;
+    a.b = R->vec[ax][8].p;
   }
   IpRec r0{};
   r0.st[0] = r0.st[1] = R->init;
@@ -1533,7 +1784,38 @@ int rp_ipr_assemble_b(RpIpRun* R) {
   HIPCHK(c, hipMemcpyAsync(R->d_rec.p, R->h_rec, sizeof(IpRec), hipMemcpyHostToDevice, st));
   const dim3 gn((n + RP_BLOCK - 1) / RP_BLOCK), blk(RP_BLOCK);
   hipLaunchKernelGGL(k_ip_warm, gn, blk, 0, st, R->mv);
-  if (!R->small_r) return 0;
+  if (!R->small_r) {  // sliced-ELL copy of each axis' matrix for the grid-wide solve
+    const int nwin = (n + GS_WIN - 1) / GS_WIN;
+    for (int ax = 0; ax < 2; ax++) {
+      SellBufs& B = R->sb[ax];
+      // slots: within a window, the one-lane slices hold at most the
+      // nonzeros, 64 x 19 for the first one and 3 rounding slots a row; a full
+      // slice of g >= 2 lanes a row holds at most twice its nonzeros, and a
+      // window has at most six partial ones of 1024 slots
+      const size_t cap = 2 * A.col[ax].n + 3 * (size_t)n + 8192 * (size_t)nwin;
+      if (cap >= (size_t)INT_MAX) return fail(c, RP_ERR_ARG, "initial place: matrix too large for 32-bit offsets");
+      if (B.sval.n < cap) {
+        HIPCHK(c, dalloc(B.sval, cap));
+        HIPCHK(c, dalloc(B.scol32, cap));
+      }
+      BcAxis& a = g.ax[ax];
+      hipLaunchKernelGGL(k_gsell_order, dim3(nwin), dim3(GS_WIN), 0, st, (const int*)A.rowp[ax].p, n, B.perm.p,
+                         B.iperm.p, B.slen.p, B.qslice.p, B.sinfo.p, B.nsl.p, B.wtot.p);
+      hipLaunchKernelGGL(k_gsell_fill, dim3(nwin), dim3(GS_WIN), 0, st, (const int*)A.rowp[ax].p,
+                         (const int*)A.col[ax].p, (const float*)A.val[ax].p, (const float*)A.b[ax].p,
+                         (const float*)a.xo, (const int*)B.perm.p, (const int*)B.iperm.p, (const int*)B.slen.p,
+                         (const int*)B.qslice.p, (const int*)B.nsl.p, (const int*)B.wtot.p, n, B.sinfo.p,
+                         B.sval.p, B.scol32.p, R->vec[ax][8].p, a.x);
+      HIPCHK(c, hipGetLastError());
+      a.perm = B.perm.p;
+      a.slen = B.slen.p;
+      a.sinfo = B.sinfo.p;
+      a.nsl = B.nsl.p;
+      a.sval = B.sval.p;
+      a.scol = B.scol32.p;
+    }
+    return 0;
+  }
   // sorted SELL copy of each axis' matrix
   for (int ax = 0; ax < 2; ax++) {
     SellBufs& B = R->sb[ax];
@@ -1558,8 +1840,8 @@ int rp_ipr_assemble_b(RpIpRun* R) {
     a.rowp = g.ax[ax].rowp;
     a.col = g.ax[ax].col;
     a.val = g.ax[ax].val;
-    a.b = g.ax[ax].b;
-    a.x = g.ax[ax].x;
+    a.b = g.ax[ax].bo;
+    a.x = g.ax[ax].xo;
     a.perm = B.perm.p;
     a.slen = B.slen.p;
     a.width = B.wslots.p;
@@ -1617,7 +1899,7 @@ void launch_small1(int r, size_t lds, hipStream_t st, const SellArgs& g, int fir
 int rp_ipr_solve(RpIpRun* const* runs, int k, hipStream_t st, RpIpBatch* bt) {
   if (k <= 0) return 0;
   rp_ctx* c0 = runs[0]->c;
-  const dim3 blk(RP_BLOCK);
+  const dim3 blk(RP_BLOCK), gblk(GS_WIN);
   if (k == 1 || !bt) {  // one run: launches of its own
     for (int j = 0; j < k; j++) {
       RpIpRun* R = runs[j];
@@ -1631,20 +1913,24 @@ int rp_ipr_solve(RpIpRun* const* runs, int k, hipStream_t st, RpIpBatch* bt) {
       const BcArgs& g = R->g;
       const dim3 gb(R->nparts, 2), ge(R->eblocks, 2), g1(1, 2);
       const int cap = 2 * std::max(0, R->init.max_iter) + 1;  // i restarts from 0 at most once
-      hipLaunchKernelGGL(k_bc_init, gb, blk, 0, st, g);
+      hipLaunchKernelGGL(k_bc_init, gb, gblk, 0, st, g);
       int done_iters = 0;
       int* h_done = &R->h_rec->st[0].done;  // scratch: the record is rewritten by finish_a
       for (;;) {
         for (int i = 0; i < BC_CHUNK; i++) {
           hipLaunchKernelGGL(k_bc_head, g1, dim3(1024), 0, st, g);
-          hipLaunchKernelGGL(k_bc_restart, gb, blk, 0, st, g);
+          hipLaunchKernelGGL(k_bc_restart, gb, gblk, 0, st, g);
           hipLaunchKernelGGL(k_bc_p, ge, blk, 0, st, g);
-          hipLaunchKernelGGL(k_bc_v, gb, blk, 0, st, g);
+          hipLaunchKernelGGL(k_bc_v, gb, gblk, 0, st, g);
           hipLaunchKernelGGL(k_bc_s, ge, blk, 0, st, g);
-          hipLaunchKernelGGL(k_bc_t, gb, blk, 0, st, g);
+          hipLaunchKernelGGL(k_bc_t, gb, gblk, 0, st, g);
           hipLaunchKernelGGL(k_bc_xr, ge, blk, 0, st, g);
         }
         done_iters += BC_CHUNK;
+        // no solve ends before max_iter iterations unless it converges to
+        // float epsilon, so the first poll waits for them (the kernels of a
+        // finished solve return at once; polling changes no result)
+        if (done_iters < std::min(cap, R->init.max_iter)) continue;
         hipLaunchKernelGGL(k_bc_head, g1, dim3(1024), 0, st, g);  // closes the chunk's last iteration
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipMemcpyAsync(&h_done[0], &R->d_rec.p->st[0].done, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1653,7 +1939,7 @@ int rp_ipr_solve(RpIpRun* const* runs, int k, hipStream_t st, RpIpBatch* bt) {
         HIPCHK(c, hipStreamSynchronize(st));
         if ((R->h_rec->st[0].done && R->h_rec->st[1].done) || done_iters >= cap) break;
       }
-      hipLaunchKernelGGL(k_bc_zero_x, ge, blk, 0, st, g);
+      hipLaunchKernelGGL(k_bc_unperm, ge, blk, 0, st, g);
       HIPCHK(c, hipGetLastError());
     }
     return 0;
@@ -1693,11 +1979,12 @@ int rp_ipr_solve(RpIpRun* const* runs, int k, hipStream_t st, RpIpBatch* bt) {
   // blocks a run) and the thread-per-row kernels (eblocks), (run, 0) for the head
   const int ng = (int)grid.size();
   size_t ntab = 0, netab = 0;
-  int cap = 0;
+  int cap = 0, first_poll = INT_MAX;
   for (RpIpRun* R : grid) {
     ntab += R->nparts;
     netab += R->eblocks;
     cap = std::max(cap, 2 * std::max(0, R->init.max_iter) + 1);
+    first_poll = std::min(first_poll, R->init.max_iter);
   }
   if (int r = grow_pinned(c0, bt->h_bc, bt->d_bc, bt->cap_bc, ng)) return r;
   if (int r = grow_pinned(c0, bt->h_tab, bt->d_tab, bt->cap_tab, ntab + ng + netab)) return r;
@@ -1719,30 +2006,64 @@ int rp_ipr_solve(RpIpRun* const* runs, int k, hipStream_t st, RpIpBatch* bt) {
   const int2* htab = bt->d_tab.p + ntab;
   const int2* etab = bt->d_tab.p + ntab + ng;
   const dim3 gb((unsigned)ntab, 2), ge((unsigned)netab, 2), g1((unsigned)ng, 2);
-  hipLaunchKernelGGL(kb_bc_init, gb, blk, 0, st, gsd, tab);
-  int done_iters = 0;
+  // RP_IP_TIMING: event times of the launches of one iteration (the tenth)
+  // and of the whole solve
+  const bool timing = getenv("RP_IP_TIMING") != nullptr;
+  hipEvent_t ev[10] = {};
+  if (timing) {
+    for (auto& e : ev) (void)hipEventCreate(&e);
+    (void)hipEventRecord(ev[8], st);
+  }
+  hipLaunchKernelGGL(kb_bc_init, gb, gblk, 0, st, gsd, tab);
+  int done_iters = 0, polls = 0;
+  const auto th0 = std::chrono::steady_clock::now();
+  double t_enq = 0.0;
   for (;;) {
     for (int i = 0; i < BC_CHUNK; i++) {
+      const bool tk = timing && done_iters + i == 9;
+      if (tk) (void)hipEventRecord(ev[0], st);
       hipLaunchKernelGGL(kb_bc_head, g1, dim3(1024), 0, st, gsd, htab);
-      hipLaunchKernelGGL(kb_bc_restart, gb, blk, 0, st, gsd, tab);
+      if (tk) (void)hipEventRecord(ev[1], st);
+      hipLaunchKernelGGL(kb_bc_restart, gb, gblk, 0, st, gsd, tab);
+      if (tk) (void)hipEventRecord(ev[2], st);
       hipLaunchKernelGGL(kb_bc_p, ge, blk, 0, st, gsd, etab);
-      hipLaunchKernelGGL(kb_bc_v, gb, blk, 0, st, gsd, tab);
+      if (tk) (void)hipEventRecord(ev[3], st);
+      hipLaunchKernelGGL(kb_bc_v, gb, gblk, 0, st, gsd, tab);
+      if (tk) (void)hipEventRecord(ev[4], st);
       hipLaunchKernelGGL(kb_bc_s, ge, blk, 0, st, gsd, etab);
-      hipLaunchKernelGGL(kb_bc_t, gb, blk, 0, st, gsd, tab);
+      if (tk) (void)hipEventRecord(ev[5], st);
+      hipLaunchKernelGGL(kb_bc_t, gb, gblk, 0, st, gsd, tab);
+      if (tk) (void)hipEventRecord(ev[6], st);
       hipLaunchKernelGGL(kb_bc_xr, ge, blk, 0, st, gsd, etab);
+      if (tk) (void)hipEventRecord(ev[7], st);
     }
     done_iters += BC_CHUNK;
+    if (done_iters < std::min(cap, first_poll)) continue;  // as for one run
+    if (!polls) t_enq = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count();
     hipLaunchKernelGGL(kb_bc_head, g1, dim3(1024), 0, st, gsd, htab);
     HIPCHK(c0, hipGetLastError());
     hipLaunchKernelGGL(k_bc_done_gather, dim3((2 * ng + RP_BLOCK - 1) / RP_BLOCK), blk, 0, st, gsd, ng,
                        bt->d_done.p);
     HIPCHK(c0, hipMemcpyAsync(bt->h_done, bt->d_done.p, sizeof(int) * 2 * ng, hipMemcpyDeviceToHost, st));
     HIPCHK(c0, hipStreamSynchronize(st));
+    polls++;
     bool all = true;
     for (int j = 0; j < 2 * ng; j++) all = all && bt->h_done[j];
     if (all || done_iters >= cap) break;
   }
-  hipLaunchKernelGGL(kb_bc_zero_x, ge, blk, 0, st, gsd, etab);
+  hipLaunchKernelGGL(kb_bc_unperm, ge, blk, 0, st, gsd, etab);
+  if (timing) {
+    (void)hipEventRecord(ev[9], st);
+    (void)hipEventSynchronize(ev[9]);
+    float t[8] = {};
+    for (int k = 0; k < 7; k++) (void)hipEventElapsedTime(&t[k], ev[k], ev[k + 1]);
+    (void)hipEventElapsedTime(&t[7], ev[8], ev[9]);
+    fprintf(stderr, "[rp] grid solves of %d runs (%zu blocks), %d iterations, %d polls, %.2f ms on the stream, "
+            "%.2f ms to enqueue the first %d: head %.1f restart %.1f p %.1f v %.1f s %.1f t %.1f xr %.1f us\n", ng,
+            ntab, done_iters, polls, t[7], t_enq, std::min(cap, first_poll),
+            1e3 * t[0], 1e3 * t[1], 1e3 * t[2], 1e3 * t[3], 1e3 * t[4], 1e3 * t[5], 1e3 * t[6]);
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  }
   HIPCHK(c0, hipGetLastError());
   return 0;
 }
