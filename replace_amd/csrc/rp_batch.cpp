@@ -175,6 +175,7 @@ struct rp_server {
   // busy with one's assemblies while the other solves.
   IpDriver ipd[kIpDrivers];
   bool batched_ip = false;
+  clk::time_point t0{};  // start of rp_server_run (log times)
   rp_server_stats stats{};
 };
 
@@ -412,9 +413,10 @@ static void ip_worker(rp_server* s, IpDriver* dp) {
     });
     const bool synced2 = sync_aux();
     if (timing)
-      fprintf(stderr, "[rp] ip round %d: %zu runs, assembly launches %.1f ms, counts %.1f ms, CSR %.1f ms, "
-              "solves %.1f ms, records %.1f ms\n", round, live.size(), t_la, t_a - t_la, t_b - t_a, t_s - t_b,
-              ms(t0) - t_s);
+      fprintf(stderr, "[rp] ip driver %d round %d at %.3f s: %zu runs, assembly launches %.1f ms, counts %.1f ms, "
+              "CSR %.1f ms, solves %.1f ms, records %.1f ms\n", (int)(dp - s->ipd), round,
+              std::chrono::duration<double>(t0 - s->t0).count(), live.size(), t_la, t_a - t_la, t_b - t_a,
+              t_s - t_b, ms(t0) - t_s);
     std::vector<Job*> keep, back;
     for (Job* j : active) {
       bool stop = false;
@@ -525,11 +527,21 @@ extern "C" int rp_server_submit(rp_server* s, const rp_netlist* nl, const rp_par
 
 static int rebuild(rp_server* s) {
   const int n = (int)s->running.size();
+  const bool timing = getenv("RP_IP_TIMING") != nullptr;
+  auto tr0 = clk::now();
+  auto lap = [&](const char* what) {
+    const double t = std::chrono::duration<double, std::milli>(clk::now() - tr0).count();
+    if (timing && t > 10.0)
+      fprintf(stderr, "[rp] slow rebuild step at %.3f s: %s %.1f ms (%d jobs)\n",
+              std::chrono::duration<double>(clk::now() - s->t0).count(), what, t, n);
+    tr0 = clk::now();
+  };
   std::vector<RpDev> devs(n);
   std::vector<int> cnt[RP_NUM_EVAL_KERNELS];
   std::vector<int> cnt_sc[RP_SC_VARIANTS], cnt_wa[RP_WA_VARIANTS];
   size_t lb = 0, lc = 0, lr = 0;
   SCHK(s, hipStreamSynchronize(s->stream));
+  lap("stream sync");
   if (s->opt.autotune) {
     std::vector<int> in_bucket[RP_NUM_SIZE_BUCKETS];
     for (int i = 0; i < n; i++) {
@@ -549,6 +561,7 @@ static int rebuild(rp_server* s) {
       if (!s->tiles[b].tuned && !in_bucket[b].empty() &&
           ((int)in_bucket[b].size() >= kTuneMinJobs || waiting[b] == 0))
         if (int r = tune_bucket(s, b, in_bucket[b], devs)) return r;
+    lap("tuner");
   }
   for (int i = 0; i < n; i++) {
     RpDev d = s->jobs[s->running[i]]->ctx->dev;
@@ -621,6 +634,7 @@ static int rebuild(rp_server* s) {
     SCHK(s, hipHostMalloc((void**)&s->h_phase, sizeof(int) * s->h_phase_cap, hipHostMallocDefault));
   }
   SCHK(s, hipStreamSynchronize(s->stream));
+  lap("tables and uploads");
   if (s->opt.use_graph && n > 0) {
     SCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
     for (int k = 0; k < std::max(1, s->opt.steps_per_poll); k++) {
@@ -634,6 +648,7 @@ static int rebuild(rp_server* s) {
     }
     SCHK(s, hipStreamEndCapture(s->stream, &s->graph));
     SCHK(s, hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
+    lap("graph capture");
   }
   s->stats.rebuilds++;
   return RP_OK;
@@ -642,6 +657,7 @@ static int rebuild(rp_server* s) {
 extern "C" int rp_server_run(rp_server* s) {
   if (!s) return RP_ERR_ARG;
   const auto t0 = clk::now();
+  s->t0 = t0;
   auto now_s = [&] { return std::chrono::duration<double>(clk::now() - t0).count(); };
   std::vector<int> order(s->jobs.size());
   for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
@@ -719,6 +735,7 @@ extern "C" int rp_server_run(rp_server* s) {
     }
     // 3. batched steps
     const int K = std::max(1, s->opt.steps_per_poll);
+    const auto tst = clk::now();
     if (s->gexec) {
       SCHK(s, hipGraphLaunch(s->gexec, s->stream));
     } else {
@@ -730,6 +747,10 @@ extern "C" int rp_server_run(rp_server* s) {
     SCHK(s, rp_launch_gather_phase(s->d_devs.p, n, s->d_phase.p, s->stream));
     SCHK(s, hipMemcpyAsync(s->h_phase, s->d_phase.p, sizeof(int) * n, hipMemcpyDeviceToHost, s->stream));
     SCHK(s, hipStreamSynchronize(s->stream));
+    if (getenv("RP_IP_TIMING")) {
+      const double t = std::chrono::duration<double, std::milli>(clk::now() - tst).count();
+      if (t > 30.0) fprintf(stderr, "[rp] slow batched steps at %.3f s: %.1f ms (%d jobs)\n", now_s(), t, n);
+    }
     std::vector<int> keep;
     for (int i = 0; i < n; i++) {
       Job& j = *s->jobs[s->running[i]];
@@ -748,7 +769,12 @@ extern "C" int rp_server_run(rp_server* s) {
       j.lx.resize(j.nl.num_insts);
       j.ly.resize(j.nl.num_insts);
       rp_get_positions(j.ctx, j.lx.data(), j.ly.data());
+      const auto tdd = clk::now();
       rp_destroy(j.ctx);
+      if (getenv("RP_IP_TIMING")) {
+        const double t = std::chrono::duration<double, std::milli>(clk::now() - tdd).count();
+        if (t > 10.0) fprintf(stderr, "[rp] slow context release at %.3f s: %.1f ms\n", now_s(), t);
+      }
       j.ctx = nullptr;
       s->reserved -= j.reserved;
       j.done_s = td;

@@ -20,6 +20,9 @@
 // its owner synchronises the streams that use it first.
 hipError_t rp_dev_malloc(void** p, size_t bytes);
 void rp_dev_free(void* p);
+// streams of placement contexts, reused across contexts (rp_host.cpp)
+hipError_t rp_stream_acquire(int dev, hipStream_t* st);
+void rp_stream_release(int dev, hipStream_t st);
 // Pinned host staging blocks of the contexts (states, records, totals) come
 // from a process-wide cache by size class: hipHostFree waits for the whole
 // device, so a context's teardown hands its blocks back instead.

@@ -74,6 +74,37 @@ hipError_t rp_dev_malloc(void** p, size_t bytes) {
 
 void rp_dev_free(void* p) { (void)hipFreeAsync(p, nullptr); }
 
+// Streams of placement contexts are kept for reuse once a context is
+// released: creating and destroying a stream per job costs milliseconds and,
+// in a service, stalls the other streams' work while it runs.
+namespace {
+std::mutex g_stream_mu;
+std::vector<hipStream_t> g_free_streams[64];
+}  // namespace
+
+hipError_t rp_stream_acquire(int dev, hipStream_t* st) {
+  if (dev >= 0 && dev < 64) {
+    std::lock_guard<std::mutex> lk(g_stream_mu);
+    if (!g_free_streams[dev].empty()) {
+      *st = g_free_streams[dev].back();
+      g_free_streams[dev].pop_back();
+      return hipSuccess;
+    }
+  }
+  return hipStreamCreateWithFlags(st, hipStreamNonBlocking);
+}
+
+// the stream must be idle (its work synchronised) when it is handed back
+void rp_stream_release(int dev, hipStream_t st) {
+  if (!st) return;
+  if (dev >= 0 && dev < 64) {
+    std::lock_guard<std::mutex> lk(g_stream_mu);
+    g_free_streams[dev].push_back(st);
+    return;
+  }
+  (void)hipStreamDestroy(st);
+}
+
 namespace {
 std::mutex g_pin_mu;
 std::vector<void*> g_pin_free[48];  // cached blocks by size class (2^k bytes)
@@ -177,7 +208,7 @@ extern "C" int rp_create(const rp_netlist* nl, const rp_params* prm, rp_ctx** ou
     return fail(*out, RP_ERR_NO_DEVICE, "no HIP device visible");
   }
   (void)hipGetDevice(&c->device);
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (rp_stream_acquire(c->device, &c->stream) != hipSuccess) {
     *out = c.release();
     return fail(*out, RP_ERR_HIP, "hipStreamCreate failed");
   }
@@ -388,8 +419,10 @@ extern "C" void rp_destroy(rp_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->h_state) rp_pinned_free(c->h_state);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
-  delete c;
+  const hipStream_t st = c->stream;
+  const int dev = c->device;
+  delete c;  // its device buffers go back to the pool, ordered after the synchronised stream's work
+  rp_stream_release(dev, st);
 }
 
 

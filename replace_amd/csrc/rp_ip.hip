@@ -347,6 +347,7 @@ int rp_ip_assemble_b(rp_ctx* c, RpIpAsm& a, hipStream_t st) {
            c->prm.initial_place_min_diff_length, c->prm.initial_place_max_fanout,
            c->prm.initial_place_net_weight_scale};
   int mt[2] = {a.h_tot[0], a.h_tot[1]}, mr[2] = {a.h_tot[2], a.h_tot[3]};
+  const auto te = std::chrono::steady_clock::now();
   for (int ax = 0; ax < 2; ax++) {
     if (mt[ax] < 0 || mr[ax] < 0) return fail(c, RP_ERR_NOMEM, "initial place: B2B system exceeds 2^31 entries");
     HIPCHK(c, ensure(a.tk[ax], mt[ax]));
@@ -378,6 +379,10 @@ int rp_ip_assemble_b(rp_ctx* c, RpIpAsm& a, hipStream_t st) {
     tneed = std::max(tneed, std::max(t1, std::max(t2, t3)));
   }
   HIPCHK(c, ensure(a.temp, tneed));
+  {
+    const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - te).count();
+    if (t > 10.0 && getenv("RP_IP_TIMING")) fprintf(stderr, "[rp] slow assembly buffers: %.1f ms (%d rows)\n", t, n);
+  }
   IpOut o{a.off.p, {a.tk[0].p, a.tk[1].p}, {a.tv[0].p, a.tv[1].p}, {a.rk[0].p, a.rk[1].p},
           {a.rv[0].p, a.rv[1].p}, a.cb};
   hipLaunchKernelGGL(k_ip_emit, grid_of(np), dim3(RP_BLOCK), 0, st, v, (const int*)a.pin_net.p,
@@ -1764,11 +1769,15 @@ int rp_ipr_assemble_b(RpIpRun* R) {
   hipStream_t st = R->st;
   RpIpAsm& A = R->A;
   const int n = R->n;
+  const auto ta = std::chrono::steady_clock::now();
   if (R->host_asm) {
     if (int r = rp_ip_host_assemble(c, A)) return r;
   } else {
     if (int r = rp_ip_assemble_b(c, A, st)) return r;
   }
+  const double t_asm = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta).count();
+  if (t_asm > 20.0 && getenv("RP_IP_TIMING"))
+    fprintf(stderr, "[rp] slow assembly enqueue: %.1f ms (%d rows)\n", t_asm, n);
   BcArgs& g = R->g;
   for (int ax = 0; ax < 2; ax++) {
     BcAxis& a = g.ax[ax];
