@@ -211,14 +211,38 @@ __global__ void k_ip_heads(const K* __restrict__ key, int m, int* __restrict__ f
 }
 
 // one (row, col) run -> one CSR entry, summed in triplet order from 0.0f
+// Sum of the run of equal keys starting at i, in order from 0.0f: eight
+// keys and values are loaded before they are added (a long run, such as the
+// diagonal entry of a cell on many nets, is walked eight entries a round).
+template <typename K>
+__device__ __forceinline__ float run_sum(const K* __restrict__ key, const float* __restrict__ val, int m, int i,
+                                         K k) {
+  float s = 0.0f;
+  for (int j = i; j < m; j += 8) {
+    K kk[8];
+    float vv[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int q = min(j + u, m - 1);
+      kk[u] = key[q];
+      vv[u] = val[q];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      if (j + u >= m || kk[u] != k) return s;
+      s += vv[u];
+    }
+  }
+  return s;
+}
+
 __global__ void k_ip_segsum(const unsigned long long* __restrict__ key, const float* __restrict__ val, int m,
                             const int* __restrict__ flag, const int* __restrict__ seg, int cb,
                             int* __restrict__ col, float* __restrict__ out, int* __restrict__ srow) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m || !flag[i]) return;
   const unsigned long long k = key[i];
-  float s = 0.0f;
-  for (int j = i; j < m && key[j] == k; j++) s += val[j];
+  const float s = run_sum(key, val, m, i, k);
   int at = seg[i];
   col[at] = (int)(k & ((1ull << cb) - 1));
   srow[at] = (int)(k >> cb);
@@ -251,9 +275,7 @@ __global__ void k_ip_rhs(const unsigned* __restrict__ key, const float* __restri
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m || (i > 0 && key[i] == key[i - 1])) return;
   const unsigned k = key[i];
-  float s = 0.0f;
-  for (int j = i; j < m && key[j] == k; j++) s += val[j];
-  b[k] = s;
+  b[k] = run_sum(key, val, m, i, k);
 }
 
 inline dim3 grid_of(long n) { return dim3((unsigned)std::max<long>(1, (n + RP_BLOCK - 1) / RP_BLOCK)); }
