@@ -413,8 +413,8 @@ typedef struct rp_server_opts {
                                 bucket's first jobs and keep the fastest */
   int32_t batched_ip;        /* 2 (default): the initial place of jobs of more
                                 than 8192 movable instances (the grid-wide
-                                solver) runs in rounds of shared launches (one
-                                driver thread), smaller jobs' on their set-up
+                                solver) runs in rounds of shared launches (two
+                                driver threads), smaller jobs' on their set-up
                                 worker; 1: every job's in the rounds; 0: every
                                 job's on its set-up worker.  Each job's result
                                 is the same in every mode */
